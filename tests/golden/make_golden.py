@@ -1,0 +1,239 @@
+"""Write the golden fixtures of tests/golden/*.npz from the reference's own code.
+
+Runs only where the reference source is present: it imports the reference's
+pybind11 extension as compiled from its sources by `make -C oracle ref` into
+oracle/_ref/ (recipe: oracle/Makefile). The fixtures hold input digests and
+per-codeword outputs (parity CRCs, repair status, repaired-codeword CRCs); the
+small edge sets hold their inputs and outputs explicitly.
+
+    python tests/golden/make_golden.py
+"""
+import importlib.util
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, HERE)
+
+import cases as C  # noqa: E402
+from libffrs_amd import workloads as W  # noqa: E402
+
+
+def load_reference():
+    import sysconfig
+    path = os.path.join(ROOT, "oracle", "_ref", "libffrs" + sysconfig.get_config_var("EXT_SUFFIX"))
+    if not os.path.exists(path):
+        raise SystemExit("build the reference first: make -C oracle ref")
+    spec = importlib.util.spec_from_file_location("libffrs", path)
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    return mod
+
+
+def ref_repair_rs256(ref, cws, k):
+    status = np.zeros(len(cws), np.uint8)
+    out = cws.copy()
+    for i in range(len(cws)):
+        m = bytearray(cws[i, :k].tobytes())
+        e = bytearray(cws[i, k:].tobytes())
+        status[i] = 1 if ref.repair(m, e) else 0
+        out[i, :k] = np.frombuffer(m, np.uint8)
+        out[i, k:] = np.frombuffer(e, np.uint8)
+    return status, out
+
+
+def ref_repair_rsi16(ref, cws, k, positions=None):
+    status = np.zeros(len(cws), np.uint8)
+    out = cws.copy()
+    for i in range(len(cws)):
+        m = bytearray(cws[i, :k].tobytes())
+        e = bytearray(cws[i, k:].tobytes())
+        try:
+            if positions is None:
+                status[i] = int(ref.repair(m, e))
+            else:
+                status[i] = int(ref.repair(m, e, [int(p) for p in positions[i] if p >= 0]))
+        except RuntimeError:
+            status[i] = 255
+            continue
+        out[i, :k] = np.frombuffer(m, np.uint16)
+        out[i, k:] = np.frombuffer(e, np.uint16)
+    return status, out
+
+
+def main():
+    L = load_reference()
+    out = {}
+
+    # ---------------------------------------------------------------- RS256 plain
+    for name in C.RS256_PLAIN_CASES:
+        ecc, msgs, counts, s = C.rs256_plain_case(name)
+        k = 255 - ecc
+        ref = L.RS256(ecc_len=ecc)
+        par = np.frombuffer(ref.encode(msgs.tobytes()), np.uint8).reshape(-1, ecc)
+        cws = np.concatenate([msgs, par], 1)
+        W.rs256_corrupt(cws, counts, s)
+        st, rep = ref_repair_rs256(ref, cws, k)
+        out[f"rs256_{name}"] = dict(in_sha=C.sha(msgs, counts), corrupt_sha=C.sha(cws),
+                                   parity_crc=C.row_crc(par), status=st, repaired_crc=C.row_crc(rep))
+        print(name, "status true:", int(st.sum()), "/", len(st))
+
+    # ---------------------------------------------------------------- RS256 shapes
+    for (bl, ecc) in C.RS256_SHAPES:
+        k = bl - ecc
+        buf, rep_msgs, counts, s = C.rs256_shape_case(bl, ecc)
+        ref = L.RS256(block_len=bl, ecc_len=ecc)
+        par_all = np.frombuffer(ref.encode(buf.tobytes()), np.uint8)
+        par = np.frombuffer(ref.encode(rep_msgs.tobytes()), np.uint8).reshape(-1, ecc)
+        cws = np.concatenate([rep_msgs, par], 1)
+        W.rs256_corrupt(cws, counts, s)
+        st, rep = ref_repair_rs256(ref, cws, k)
+        synd = np.frombuffer(ref._synds(cws[0].tobytes()), np.uint8)
+        out[f"rs256_shape_{bl}_{ecc}"] = dict(
+            in_sha=C.sha(buf, rep_msgs, counts), parity_all=par_all, corrupt_sha=C.sha(cws), status=st,
+            repaired_crc=C.row_crc(rep), synds0=synd, generator=np.frombuffer(ref.generator, np.uint8))
+
+    # ---------------------------------------------------------------- RS256 mixed
+    eccs, lens, counts, msgs, s = C.rs256_mixed_case()
+    refs = {}
+    par_crc = np.zeros(len(eccs), np.uint32)
+    st = np.zeros(len(eccs), np.uint8)
+    rep_crc = np.zeros(len(eccs), np.uint32)
+    corrupt = []
+    for i, (e, l) in enumerate(zip(eccs, lens)):
+        e, l = int(e), int(l)
+        ref = refs.setdefault((l, e), L.RS256(block_len=l + e, ecc_len=e))
+        par = np.frombuffer(ref.encode(msgs[i].tobytes()), np.uint8)
+        par_crc[i] = C.row_crc(par[None])[0]
+        cw = np.concatenate([msgs[i], par])[None].copy()
+        W.rs256_corrupt(cw, counts[i:i + 1], s)
+        corrupt.append(cw[0].copy())
+        s1, r1 = ref_repair_rs256(ref, cw, l)
+        st[i] = s1[0]
+        rep_crc[i] = C.row_crc(r1)[0]
+    out["rs256_mixed"] = dict(in_sha=C.sha(eccs, lens, counts, *msgs), corrupt_sha=C.sha(*corrupt),
+                              parity_crc=par_crc, status=st, repaired_crc=rep_crc)
+
+    # ---------------------------------------------------------------- RSi16 plain
+    for name in C.RSI16_PLAIN_CASES:
+        (n, ecc), msgs, counts, s = C.rsi16_plain_case(name)
+        k = n - ecc
+        ref = L.RSi16(n, ecc)
+        par = np.frombuffer(ref.encode(msgs.tobytes()), np.uint16).reshape(-1, ecc)
+        refs_ = L.RSi16(n, ecc, simd_x4=False, simd_x8=False, simd_x16=False)
+        par_s = np.frombuffer(refs_.encode(msgs.tobytes()), np.uint16).reshape(-1, ecc)
+        cws = np.concatenate([msgs, par], 1)
+        W.rsi16_corrupt(cws, counts, s)
+        st, rep = ref_repair_rsi16(ref, cws, k)
+        d = dict(in_sha=C.sha(msgs, counts), corrupt_sha=C.sha(cws), parity_crc=C.row_crc(par),
+                 parity_scalar_crc=C.row_crc(par_s), status=st, repaired_crc=C.row_crc(rep),
+                 root=np.array([ref.root]), ntt_len=np.array([ref.ntt_len]))
+        # syndromes and Sugiyama outputs of the first 128 codewords (debug surface)
+        syn = np.zeros((128, ecc), np.uint32)
+        sug = np.zeros(128, np.uint32)
+        for i in range(min(128, len(cws))):
+            syn[i] = np.array(ref._synd(cws[i, :k].tobytes(), cws[i, k:].tobytes()), np.uint32)
+            if syn[i].any():
+                loc, ev = ref._sugiyama([int(v) for v in syn[i]])
+                sug[i] = C.row_crc(np.array(list(loc) + list(ev), np.uint32)[None])[0]
+        d["synds_head"] = syn
+        d["sugiyama_crc_head"] = sug
+        out[f"rsi16_{name}"] = d
+        print(name, "status counts:", np.bincount(st, minlength=256)[[0, 2, 4, 255]])
+
+    # ---------------------------------------------------------------- RSi16 mixed
+    ns, eccs, counts, msgs, s = C.rsi16_mixed_case()
+    par_crc = np.zeros(len(ns), np.uint32)
+    st = np.zeros(len(ns), np.uint8)
+    rep_crc = np.zeros(len(ns), np.uint32)
+    corrupt = []
+    refs = {}
+    for i, (n, e) in enumerate(zip(ns, eccs)):
+        n, e = int(n), int(e)
+        ref = refs.setdefault((n, e), L.RSi16(n, e))
+        par = np.frombuffer(ref.encode(msgs[i].tobytes()), np.uint16)
+        par_crc[i] = C.row_crc(par[None])[0]
+        cw = np.concatenate([msgs[i], par])[None].copy()
+        W.rsi16_corrupt(cw, counts[i:i + 1], s)
+        corrupt.append(cw[0].copy())
+        s1, r1 = ref_repair_rsi16(ref, cw, n - e)
+        st[i] = s1[0]
+        rep_crc[i] = C.row_crc(r1)[0]
+    out["rsi16_mixed"] = dict(in_sha=C.sha(ns, eccs, counts, *msgs), corrupt_sha=C.sha(*corrupt),
+                              parity_crc=par_crc, status=st, repaired_crc=rep_crc)
+
+    # ---------------------------------------------------------------- RSi16 edges
+    n, ecc, k = 256, 32, 224
+    ref = L.RSi16(n, ecc)
+    # (a) messages whose true parity contains 65536 (stored as 0 by the API)
+    s = W.Stream(13)
+    trunc_msgs = []
+    for _ in range(400):
+        m = s.u16(4096 * k).reshape(4096, k)
+        par = np.frombuffer(ref.encode(m.tobytes()), np.uint16).reshape(-1, ecc)
+        for i in range(len(m)):
+            cw_m = bytearray(m[i].tobytes())
+            cw_e = bytearray(par[i].tobytes())
+            if int(ref.repair(cw_m, cw_e)) != 0:   # a clean codeword that is not error-free
+                trunc_msgs.append(m[i].copy())
+        if len(trunc_msgs) >= 48:
+            break
+    trunc_msgs = np.array(trunc_msgs[:48], np.uint16)
+    par = np.frombuffer(ref.encode(trunc_msgs.tobytes()), np.uint16).reshape(-1, ecc)
+    cws = np.concatenate([trunc_msgs, par], 1)
+    st, rep = ref_repair_rsi16(ref, cws, k)
+    out["rsi16_edge_trunc"] = dict(msgs=trunc_msgs, parity=par, status=st, repaired_crc=C.row_crc(rep))
+    print("truncation codewords:", len(trunc_msgs), "statuses", np.bincount(st))
+
+    # (b) error patterns with last syndrome zero
+    roots = []
+    w = ref.root
+    for i in range(ref.ntt_len):
+        roots.append(pow(w, i, 65537))
+    pats = C.rsi16_last_synd_zero(roots, n, ecc, 96)
+    s = W.Stream(14)
+    m = s.u16(len(pats) * k).reshape(len(pats), k)
+    par = np.frombuffer(ref.encode(m.tobytes()), np.uint16).reshape(-1, ecc)
+    cws = np.concatenate([m, par], 1).astype(np.int64)
+    keep = []
+    for i, (pos, val) in enumerate(pats):
+        nv = (cws[i, pos] + val) % 65537
+        if (nv == 65536).any():
+            continue
+        cws[i, pos] = nv
+        keep.append(i)
+    cws = cws[keep].astype(np.uint16)
+    st, rep = ref_repair_rsi16(ref, cws, k)
+    out["rsi16_edge_lastsynd"] = dict(cws=cws, status=st, repaired_crc=C.row_crc(rep))
+    print("last-synd-zero codewords:", len(cws), "statuses", np.bincount(st))
+
+    # (c) constant messages
+    const = np.array([np.zeros(k), np.full(k, 65535), np.full(k, 1), np.arange(k) * 293 % 65536], np.uint16)
+    par = np.frombuffer(ref.encode(const.tobytes()), np.uint16).reshape(-1, ecc)
+    par_s = np.frombuffer(L.RSi16(n, ecc, simd_x4=False, simd_x8=False, simd_x16=False)
+                          .encode(const.tobytes()), np.uint16).reshape(-1, ecc)
+    out["rsi16_edge_const"] = dict(msgs=const, parity=par, parity_scalar=par_s)
+
+    # (d) known-position repair: the corrupted positions plus up to 4 clean ones
+    msgs, counts, s = C.rsi16_known_case()
+    par = np.frombuffer(ref.encode(msgs.tobytes()), np.uint16).reshape(-1, ecc)
+    cws = np.concatenate([msgs, par], 1)
+    pos = W.rsi16_corrupt(cws, counts, s)
+    allpos = C.rsi16_known_positions(pos, s)
+    st, rep = ref_repair_rsi16(ref, cws, k, positions=allpos)
+    out["rsi16_known"] = dict(in_sha=C.sha(msgs, counts), corrupt_sha=C.sha(cws), positions=allpos, status=st,
+                              repaired_crc=C.row_crc(rep))
+    print("known-position statuses", np.bincount(st, minlength=5))
+
+    for key, d in out.items():
+        np.savez_compressed(os.path.join(HERE, key + ".npz"), **d)
+    total = sum(os.path.getsize(os.path.join(HERE, key + ".npz")) for key in out)
+    print("wrote", len(out), "fixtures,", total, "bytes")
+
+
+if __name__ == "__main__":
+    main()

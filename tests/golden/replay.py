@@ -1,0 +1,185 @@
+"""Replays the golden fixtures against a codec backend (the oracle or the HIP path).
+
+A backend provides:
+  rs256_encode(block_len, ecc, buf) -> bytes            (RS256.encode semantics, ragged tail)
+  rs256_encode_batch(ecc, msgs[N,k]) -> uint8[N,ecc]
+  rs256_repair_batch(ecc, msgs, parity) -> uint8 status (1 True / 0 False), in place
+  rs256_synds(block_len, ecc, codeword) -> uint8[ecc]
+  rsi16_encode_batch(n, ecc, msgs, vector) -> uint16[N,ecc]
+  rsi16_repair_batch(n, ecc, msgs, parity) -> uint8 status (255 = reference raises), in place
+  rsi16_synds(n, ecc, msg, parity) -> uint32[ecc]
+  rsi16_sugiyama(n, ecc, synds) -> (locator, evaluator) or None where the reference raises
+  rsi16_repair_known(n, ecc, msg, parity, positions) -> status, in place
+"""
+import os
+
+import numpy as np
+
+import cases as C
+from libffrs_amd import workloads as W
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def fixture(name):
+    with np.load(os.path.join(HERE, name + ".npz")) as f:
+        return {k: f[k] for k in f.files}
+
+
+def _mismatch(a, b):
+    idx = np.nonzero(np.asarray(a) != np.asarray(b))[0]
+    return f"{idx.size} mismatching rows, first {idx[:8].tolist()}"
+
+
+def check_rs256_plain(be, name):
+    fx = fixture(f"rs256_{name}")
+    ecc, msgs, counts, s = C.rs256_plain_case(name)
+    assert (C.sha(msgs, counts) == fx["in_sha"]).all(), "input generator drift"
+    par = be.rs256_encode_batch(ecc, msgs)
+    crc = C.row_crc(par)
+    assert (crc == fx["parity_crc"]).all(), "parity: " + _mismatch(crc, fx["parity_crc"])
+    cws = np.concatenate([msgs, par], 1)
+    W.rs256_corrupt(cws, counts, s)
+    assert (C.sha(cws) == fx["corrupt_sha"]).all()
+    k = 255 - ecc
+    m = np.ascontiguousarray(cws[:, :k])
+    p = np.ascontiguousarray(cws[:, k:])
+    st = be.rs256_repair_batch(ecc, m, p)
+    assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
+    rep = C.row_crc(np.concatenate([m, p], 1))
+    assert (rep == fx["repaired_crc"]).all(), "repaired: " + _mismatch(rep, fx["repaired_crc"])
+
+
+def check_rs256_shape(be, block_len, ecc):
+    fx = fixture(f"rs256_shape_{block_len}_{ecc}")
+    buf, rep_msgs, counts, s = C.rs256_shape_case(block_len, ecc)
+    assert (C.sha(buf, rep_msgs, counts) == fx["in_sha"]).all()
+    out = np.frombuffer(bytes(be.rs256_encode(block_len, ecc, buf.tobytes())), np.uint8)
+    assert out.shape == fx["parity_all"].shape and (out == fx["parity_all"]).all(), "tail encode"
+    k = block_len - ecc
+    par = be.rs256_encode_batch(ecc, rep_msgs)
+    cws = np.concatenate([rep_msgs, par], 1)
+    W.rs256_corrupt(cws, counts, s)
+    assert (C.sha(cws) == fx["corrupt_sha"]).all()
+    syn = be.rs256_synds(block_len, ecc, cws[0])
+    assert (np.asarray(syn, np.uint8) == fx["synds0"]).all(), "syndromes"
+    m = np.ascontiguousarray(cws[:, :k])
+    p = np.ascontiguousarray(cws[:, k:])
+    st = be.rs256_repair_batch(ecc, m, p)
+    assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
+    rep = C.row_crc(np.concatenate([m, p], 1))
+    assert (rep == fx["repaired_crc"]).all(), "repaired: " + _mismatch(rep, fx["repaired_crc"])
+
+
+def check_rs256_mixed(be):
+    fx = fixture("rs256_mixed")
+    eccs, lens, counts, msgs, s = C.rs256_mixed_case()
+    assert (C.sha(eccs, lens, counts, *msgs) == fx["in_sha"]).all()
+    par = be.rs256_mixed_encode(eccs, lens, msgs)
+    crc = np.array([C.row_crc(p[None])[0] for p in par], np.uint32)
+    assert (crc == fx["parity_crc"]).all(), "parity: " + _mismatch(crc, fx["parity_crc"])
+    cws = []
+    for i in range(len(eccs)):
+        cw = np.concatenate([msgs[i], par[i]])[None].copy()
+        W.rs256_corrupt(cw, counts[i:i + 1], s)
+        cws.append(cw[0])
+    assert (C.sha(*cws) == fx["corrupt_sha"]).all()
+    st = be.rs256_mixed_repair(eccs, lens, cws)
+    assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
+    rep = np.array([C.row_crc(c[None])[0] for c in cws], np.uint32)
+    assert (rep == fx["repaired_crc"]).all(), "repaired: " + _mismatch(rep, fx["repaired_crc"])
+
+
+def check_rsi16_plain(be, name):
+    fx = fixture(f"rsi16_{name}")
+    (n, ecc), msgs, counts, s = C.rsi16_plain_case(name)
+    assert (C.sha(msgs, counts) == fx["in_sha"]).all(), "input generator drift"
+    k = n - ecc
+    par = be.rsi16_encode_batch(n, ecc, msgs, True)
+    crc = C.row_crc(par)
+    assert (crc == fx["parity_crc"]).all(), "parity: " + _mismatch(crc, fx["parity_crc"])
+    par_s = be.rsi16_encode_batch(n, ecc, msgs, False)
+    crc = C.row_crc(par_s)
+    assert (crc == fx["parity_scalar_crc"]).all(), "scalar parity: " + _mismatch(crc, fx["parity_scalar_crc"])
+    cws = np.concatenate([msgs, par], 1)
+    W.rsi16_corrupt(cws, counts, s)
+    assert (C.sha(cws) == fx["corrupt_sha"]).all()
+    heads = min(128, len(cws))
+    for i in range(heads):
+        syn = np.asarray(be.rsi16_synds(n, ecc, cws[i, :k], cws[i, k:]), np.uint32)
+        assert (syn == fx["synds_head"][i]).all(), f"syndromes of codeword {i}"
+        if syn.any():
+            r = be.rsi16_sugiyama(n, ecc, syn)
+            if r is not None:
+                crc = C.row_crc(np.array(list(r[0]) + list(r[1]), np.uint32)[None])[0]
+                assert crc == fx["sugiyama_crc_head"][i], f"sugiyama of codeword {i}"
+    m = np.ascontiguousarray(cws[:, :k])
+    p = np.ascontiguousarray(cws[:, k:])
+    st = be.rsi16_repair_batch(n, ecc, m, p)
+    assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
+    rep = C.row_crc(np.concatenate([m, p], 1))
+    ok = fx["status"] != 255
+    assert (rep[ok] == fx["repaired_crc"][ok]).all(), "repaired: " + _mismatch(rep[ok], fx["repaired_crc"][ok])
+
+
+def check_rsi16_mixed(be):
+    fx = fixture("rsi16_mixed")
+    ns, eccs, counts, msgs, s = C.rsi16_mixed_case()
+    assert (C.sha(ns, eccs, counts, *msgs) == fx["in_sha"]).all()
+    par = be.rsi16_mixed_encode(ns, eccs, msgs)
+    crc = np.array([C.row_crc(p[None])[0] for p in par], np.uint32)
+    assert (crc == fx["parity_crc"]).all(), "parity: " + _mismatch(crc, fx["parity_crc"])
+    cws = []
+    for i in range(len(ns)):
+        cw = np.concatenate([msgs[i], par[i]])[None].copy()
+        W.rsi16_corrupt(cw, counts[i:i + 1], s)
+        cws.append(cw[0])
+    assert (C.sha(*cws) == fx["corrupt_sha"]).all()
+    st = be.rsi16_mixed_repair(ns, eccs, cws)
+    assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
+    rep = np.array([C.row_crc(c[None])[0] for c in cws], np.uint32)
+    assert (rep == fx["repaired_crc"]).all(), "repaired: " + _mismatch(rep, fx["repaired_crc"])
+
+
+def check_rsi16_edges(be):
+    n, ecc, k = 256, 32, 224
+    fx = fixture("rsi16_edge_trunc")
+    par = be.rsi16_encode_batch(n, ecc, fx["msgs"], True)
+    assert (par == fx["parity"]).all()
+    m = np.ascontiguousarray(fx["msgs"].copy())
+    p = np.ascontiguousarray(par.copy())
+    st = be.rsi16_repair_batch(n, ecc, m, p)
+    assert (st == fx["status"]).all()
+    assert (C.row_crc(np.concatenate([m, p], 1)) == fx["repaired_crc"]).all()
+
+    fx = fixture("rsi16_edge_lastsynd")
+    m = np.ascontiguousarray(fx["cws"][:, :k])
+    p = np.ascontiguousarray(fx["cws"][:, k:])
+    st = be.rsi16_repair_batch(n, ecc, m, p)
+    assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
+    assert (C.row_crc(np.concatenate([m, p], 1)) == fx["repaired_crc"]).all()
+
+    fx = fixture("rsi16_edge_const")
+    assert (be.rsi16_encode_batch(n, ecc, fx["msgs"], True) == fx["parity"]).all()
+    assert (be.rsi16_encode_batch(n, ecc, fx["msgs"], False) == fx["parity_scalar"]).all()
+
+
+def check_rsi16_known(be):
+    n, ecc, k = 256, 32, 224
+    fx = fixture("rsi16_known")
+    msgs, counts, s = C.rsi16_known_case()
+    assert (C.sha(msgs, counts) == fx["in_sha"]).all()
+    par = be.rsi16_encode_batch(n, ecc, msgs, True)
+    cws = np.concatenate([msgs, par], 1)
+    pos = W.rsi16_corrupt(cws, counts, s)
+    allpos = C.rsi16_known_positions(pos, s)
+    assert (C.sha(cws) == fx["corrupt_sha"]).all() and (allpos == fx["positions"]).all()
+    st = np.zeros(len(cws), np.uint8)
+    for i in range(len(cws)):
+        m = cws[i, :k].copy()
+        p = cws[i, k:].copy()
+        st[i] = be.rsi16_repair_known(n, ecc, m, p, [int(x) for x in allpos[i] if x >= 0])
+        cws[i, :k] = m
+        cws[i, k:] = p
+    assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
+    assert (C.row_crc(cws) == fx["repaired_crc"]).all()
