@@ -1,0 +1,231 @@
+"""Benchmark: RS256 encode + repair GB/s on one MI355X (plain GPU path at scale).
+
+Workload (default): RS256 ecc_len 32, n 255 (223 message bytes), 1,048,576 codewords
+(233,832,448 message bytes); every codeword is corrupted at 0-16 distinct bytes, 1% at
+17-20 (uncorrectable path). A step is one encode of all messages plus one repair of all
+corrupted codewords, with inputs resident in HBM. The corrupted input is restored from a
+pristine device copy before each step, outside the timed bracket.
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+
+Prints one JSON line; `value` is message GB/s of the whole step, `roofline` reports the
+dominant kernel (HIP events on the codec's stream), `cpu_baseline` times the reference's
+own CPU build (or the oracle port where it is absent) on a bounded sample.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "RS encode/decode GB/s on 1 MI355X, fused-batched vs plain; p99 request latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def ensure_built():
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("_ffrs_build", os.path.join(ROOT, "libffrs_amd", "build.py"))
+    b = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(b)
+    if not b.is_built():
+        log("building libffrs_amd ...")
+        b.build()
+
+
+def load_reference_cpu():
+    """The reference's own extension compiled from its sources (oracle/_ref), or None."""
+    import importlib.util
+    import sysconfig
+    path = os.path.join(ROOT, "oracle", "_ref", "libffrs" + sysconfig.get_config_var("EXT_SUFFIX"))
+    if not os.path.exists(path):
+        return None
+    try:
+        spec = importlib.util.spec_from_file_location("libffrs", path)
+        mod = importlib.util.module_from_spec(spec)
+        spec.loader.exec_module(mod)
+        return mod
+    except Exception as e:  # a host without the ISA it was built for, etc.
+        log("reference CPU build not loadable:", e)
+        return None
+
+
+def cpu_baseline(msgs, cws_corrupt, ecc, k, repair_sample):
+    """Reference CPU time of encode (whole message buffer, one call) and repair (one call
+    per codeword, as its API works) on a bounded sample; returns message GB/s of a step."""
+    ref = load_reference_cpu()
+    kind = "reference"
+    if ref is not None:
+        rs = ref.RS256(ecc_len=ecc)
+        t = time.perf_counter()
+        rs.encode(msgs.tobytes())
+        te = time.perf_counter() - t
+        mb = [bytearray(r[:k].tobytes()) for r in cws_corrupt[:repair_sample]]
+        pb = [bytearray(r[k:].tobytes()) for r in cws_corrupt[:repair_sample]]
+        t = time.perf_counter()
+        for m, p in zip(mb, pb):
+            rs.repair(m, p)
+        tr = time.perf_counter() - t
+    else:
+        kind = "port"
+        from oracle import oracle as O
+        orc = O.RS256(ecc)
+        t = time.perf_counter()
+        orc.encode_batch(msgs, threads=1)
+        te = time.perf_counter() - t
+        m = np.ascontiguousarray(cws_corrupt[:repair_sample, :k])
+        p = np.ascontiguousarray(cws_corrupt[:repair_sample, k:])
+        t = time.perf_counter()
+        orc.decode_batch(m, p, threads=1)
+        tr = time.perf_counter() - t
+    per_byte = te / msgs.size + tr / (repair_sample * k)
+    return {
+        "value": round(1.0 / per_byte / 1e9, 6),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"encode of all {msgs.shape[0]} messages in one call ({te:.2f} s) + repair of the first "
+                  f"{repair_sample} corrupted codewords, one call each ({tr:.2f} s); single thread",
+        "encode_gbps": round(msgs.size / te / 1e9, 4),
+        "repair_gbps": round(repair_sample * k / tr / 1e9, 6),
+    }
+
+
+def rocprof_traffic(kernel):
+    """HBM bytes per launch for `kernel` from the committed rocprofv3 --pmc summary, if any."""
+    path = os.path.join(ROOT, "profiles", "traffic_rs256_plain_scale.json")
+    try:
+        with open(path) as f:
+            return json.load(f).get(kernel)
+    except (OSError, ValueError):
+        return None
+
+
+def run_rs256_plain_scale(args):
+    from libffrs_amd import capi
+    from libffrs_amd import workloads as W
+
+    N, ecc = args.codewords, 32
+    k = 255 - ecc
+    t0 = time.perf_counter()
+    msgs, counts, s = W.rs256_plain(N, ecc, 16, seed=2, uncorrectable_frac=0.01)
+    log(f"generated {N} messages in {time.perf_counter() - t0:.1f} s")
+
+    ctx = capi.Context(0)
+    dev_name = ctx.info()[0]
+    codec = capi.RS256Codec(ctx, ecc)
+    d_msg = ctx.alloc(N * k)
+    d_par = ctx.alloc(N * ecc)
+    d_cm = ctx.alloc(N * k)     # pristine corrupted codewords
+    d_cp = ctx.alloc(N * ecc)
+    d_wm = ctx.alloc(N * k)     # working copy repaired in place
+    d_wp = ctx.alloc(N * ecc)
+    d_st = ctx.alloc(N)
+    d_msg.upload(msgs)
+    codec.encode_device(k, d_msg, N, d_par)
+    par = np.empty((N, ecc), np.uint8)
+    d_par.download(par)
+    cws = np.concatenate([msgs, par], 1)
+    t0 = time.perf_counter()
+    W.rs256_corrupt(cws, counts, s)
+    log(f"corrupted codewords in {time.perf_counter() - t0:.1f} s")
+    d_cm.upload(np.ascontiguousarray(cws[:, :k]))
+    d_cp.upload(np.ascontiguousarray(cws[:, k:]))
+
+    def step(timed):
+        d_wm.copy_from(d_cm)
+        d_wp.copy_from(d_cp)
+        ctx.sync()
+        t = time.perf_counter()
+        ctx.timer_begin()
+        codec.encode_device(k, d_msg, N, d_par)
+        te = ctx.timer_end()
+        ctx.timer_begin()
+        codec.repair_device(k, d_wm, d_wp, N, d_st)
+        tr = ctx.timer_end()
+        wall = time.perf_counter() - t
+        return wall, te, tr
+
+    for _ in range(args.warmup):
+        step(False)
+    walls, tes, trs = [], [], []
+    for _ in range(args.steps):
+        w, te, tr = step(True)
+        walls.append(w)
+        tes.append(te)
+        trs.append(tr)
+    st = np.empty(N, np.uint8)
+    d_st.download(st)
+    ms_step = 1e3 * float(np.mean(walls))
+    msg_bytes = N * k
+    value = msg_bytes / (ms_step * 1e-3) / 1e9
+    te_ms, tr_ms = float(np.mean(tes)), float(np.mean(trs))
+    # algorithmic bytes per launch: encode reads k + writes ecc per codeword; repair reads
+    # n per codeword and writes one status byte (corrections are sparse and left out)
+    enc_bytes = N * (k + ecc)
+    rep_bytes = N * (k + ecc) + N
+    if tr_ms >= te_ms:
+        dom, dom_ms, dom_bytes = "rs256_repair_kernel", tr_ms, rep_bytes
+    else:
+        dom, dom_ms, dom_bytes = "rs256_encode_kernel", te_ms, enc_bytes
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u8",
+        "data": "synthetic (SplitMix64 stream, seed 2)",
+        "config": {"workload": "rs256_plain_scale", "codewords": N, "ecc_len": ecc, "message_len": k,
+                   "errors": "0-16 per codeword, 1% with 17-20", "device": dev_name},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": rocprof_traffic(dom)},
+        "kernels": {"rs256_encode_kernel": {"ms": round(te_ms, 4), "gbps_msg": round(msg_bytes / te_ms / 1e6, 2)},
+                    "rs256_repair_kernel": {"ms": round(tr_ms, 4), "gbps_msg": round(msg_bytes / tr_ms / 1e6, 2),
+                                            "repaired_true": int(st.sum())}},
+    }
+    if not args.no_cpu_baseline:
+        log("timing the CPU baseline ...")
+        result["cpu_baseline"] = cpu_baseline(msgs, cws, ecc, k, min(N, args.cpu_repair_sample))
+    for b in (d_msg, d_par, d_cm, d_cp, d_wm, d_wp, d_st):
+        b.free()
+    codec.close()
+    ctx.close()
+    return result
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--codewords", type=int, default=1048576)
+    ap.add_argument("--cpu-repair-sample", type=int, default=262144)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
+              "(codewords are independent; a multi-GPU run would shard them with no collectives)",
+              file=sys.stderr)
+        sys.exit(2)
+    ensure_built()
+    res = run_rs256_plain_scale(args)
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

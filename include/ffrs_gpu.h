@@ -1,0 +1,141 @@
+/*
+ * ffrs_gpu.h -- C ABI of the MI355X (gfx950) Reed-Solomon encode / repair path.
+ *
+ * Plain pointers and sizes only. Every entry point returns an ffrs_status; on
+ * failure ffrs_last_error() describes it (thread-local). Host-pointer entry
+ * points stage through device memory owned by the context and return when the
+ * result is in the caller's buffers; *_device entry points take device pointers,
+ * are asynchronous on the context's stream and return after launch.
+ *
+ * Reference interfaces replaced (paths relative to the reference repository root):
+ *   RS256  libffrs/pyrs256.hpp:58-210  (PyRS256: ctor, py_encode, py_repair, py_synds)
+ *          libffrs/reed_solomon.hpp:55-88, 169-389, 412-453, 511-552, 555-744
+ *   RSi16  libffrs/pyrsi16.hpp:34-692 (PyRSi16: ctor, py_encode, py_repair, py_synd,
+ *          py_sugiyama, py_roots), libffrs/rsi16v_impl.hpp:68-636, libffrs/ntt.hpp:32-579
+ */
+#ifndef FFRS_GPU_H
+#define FFRS_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFRS_ABI_VERSION 1
+
+typedef enum ffrs_status {
+    FFRS_OK = 0,
+    FFRS_ERR_INVALID = -1, /* argument value rejected (reference: ValueError)            */
+    FFRS_ERR_SIZE = -2,    /* buffer size mismatch (reference: RuntimeError, py_assert)   */
+    FFRS_ERR_DEVICE = -3,  /* HIP runtime failure                                         */
+    FFRS_ERR_NOMEM = -4,   /* device or host allocation failed                            */
+    FFRS_ERR_PARAM = -5    /* codec parameters rejected (reference: RuntimeError)         */
+} ffrs_status;
+
+/* Per-codeword RSi16 status values written by the repair entry points: the
+ * RepairStatus enum of libffrs/rsi16v.hpp:32-38, plus FFRS_RSI16_RAISES for the
+ * inputs on which the reference's repair raises RuntimeError from a py_assert
+ * (libffrs/ntt.hpp:482). RS256 repair writes 1 (True) or 0 (False). */
+enum {
+    FFRS_RSI16_NO_ERRORS = 0,
+    FFRS_RSI16_NO_ERRORS_ZERO = 1,
+    FFRS_RSI16_REPAIR_OK = 2,
+    FFRS_RSI16_REPAIR_FAIL = 3,
+    FFRS_RSI16_ERROR_LOCATION_FAIL = 4,
+    FFRS_RSI16_RAISES = 255
+};
+
+typedef struct ffrs_ctx ffrs_ctx;
+typedef struct ffrs_rs256 ffrs_rs256;
+typedef struct ffrs_rsi16 ffrs_rsi16;
+
+int ffrs_abi_version(void);
+const char *ffrs_last_error(void);
+
+/* ---- context: one device, one stream, device staging memory --------------------- */
+int ffrs_ctx_create(int device, ffrs_ctx **out);
+void ffrs_ctx_destroy(ffrs_ctx *ctx);
+int ffrs_ctx_synchronize(ffrs_ctx *ctx);
+int ffrs_ctx_device_info(ffrs_ctx *ctx, char *name, size_t name_cap, int *cu_count, size_t *free_bytes,
+                         size_t *total_bytes);
+int ffrs_device_alloc(ffrs_ctx *ctx, size_t bytes, void **d_ptr);
+int ffrs_device_free(ffrs_ctx *ctx, void *d_ptr);
+int ffrs_memcpy_h2d(ffrs_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int ffrs_memcpy_d2h(ffrs_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
+int ffrs_memcpy_d2d(ffrs_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);
+/* HIP events recorded on the context stream; ffrs_timer_end waits and returns ms. */
+int ffrs_timer_begin(ffrs_ctx *ctx);
+int ffrs_timer_end(ffrs_ctx *ctx, float *ms);
+
+/* ---- RS256 over GF(2^8) ------------------------------------------------------------ */
+/* Codec for one ecc_len (1..254) and field (primitive, polynomial); the parity and
+ * syndrome tables are built on the host once and kept resident on the device.
+ * Replaces the RS256<GF> composition of libffrs/pyrs256.hpp:33-55, 82-86. */
+int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsigned polynomial,
+                      ffrs_rs256 **out);
+void ffrs_rs256_destroy(ffrs_rs256 *rs);
+/* generator[0..ecc_len], highest degree first (libffrs/reed_solomon.hpp:55-88) */
+int ffrs_rs256_generator(const ffrs_rs256 *rs, uint8_t *generator, uint8_t *roots);
+
+/* PyRS256::py_encode (libffrs/pyrs256.hpp:99-127): `size` bytes split into blocks of
+ * message_len plus one shorter tail block; writes ceil(size / message_len) * ecc_len
+ * parity bytes to out (out_size must be exactly that). */
+int ffrs_rs256_encode(ffrs_rs256 *rs, size_t message_len, const uint8_t *buf, size_t size, uint8_t *out,
+                      size_t out_size);
+/* `count` codewords; msg holds count * message_len bytes, ecc count * ecc_len bytes,
+ * both repaired in place; status[i] = 1 (True) / 0 (False). One call of
+ * PyRS256::py_repair (libffrs/pyrs256.hpp:129-131 -> reed_solomon.hpp:559-595) per codeword. */
+int ffrs_rs256_repair(ffrs_rs256 *rs, size_t message_len, uint8_t *msg, uint8_t *ecc, size_t count,
+                      uint8_t *status);
+/* PyRS256::py_synds (libffrs/pyrs256.hpp:133-140) for `count` codewords. */
+int ffrs_rs256_synds(ffrs_rs256 *rs, size_t message_len, const uint8_t *msg, const uint8_t *ecc, size_t count,
+                     uint8_t *synds);
+/* Device-pointer forms: inputs already resident in HBM, asynchronous on the stream. */
+int ffrs_rs256_encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *d_msg, size_t count,
+                             uint8_t *d_ecc);
+int ffrs_rs256_repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d_ecc, size_t count,
+                             uint8_t *d_status);
+
+/* ---- RSi16 over GF(65537) ---------------------------------------------------------- */
+/* Codec (block_len, ecc_len, primitive) with interleave 1: field, NTT and mix tables
+ * resident on the device (libffrs/pyrsi16.hpp:41-125, libffrs/ntt.hpp:55-123,
+ * libffrs/rsi16v_impl.hpp:68-86). Fails with FFRS_ERR_PARAM where the reference
+ * constructor raises. */
+int ffrs_rsi16_create(ffrs_ctx *ctx, size_t block_len, size_t ecc_len, uint32_t primitive, ffrs_rsi16 **out);
+void ffrs_rsi16_destroy(ffrs_rsi16 *rs);
+uint32_t ffrs_rsi16_root(const ffrs_rsi16 *rs);
+size_t ffrs_rsi16_ntt_len(const ffrs_rsi16 *rs);
+
+/* encode_blocks (libffrs/pyrsi16.hpp:128-140, 364-388 -> rsi16v_impl.hpp:93-119) over
+ * `count` messages of block_len - ecc_len symbols. vector_lanes != 0 reproduces the
+ * SIMD instantiations' lane arithmetic (W = 4/8/16, the reference's default on x86-64),
+ * 0 the scalar W = 1 one. Parity symbols are written as value & 0xFFFF. */
+int ffrs_rsi16_encode(ffrs_rsi16 *rs, const uint16_t *msg, size_t count, uint16_t *ecc, int vector_lanes);
+/* `count` calls of PyRSi16::py_repair without error positions (libffrs/pyrsi16.hpp:588-631
+ * -> rsi16v_impl.hpp:122-168); repaired in place, status[i] as above. */
+int ffrs_rsi16_repair(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, size_t count, uint8_t *status);
+/* py_repair with error positions (natural order, < block_len, at most ecc_len):
+ * rsi16v_impl.hpp:171-231. One codeword. */
+int ffrs_rsi16_repair_known(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, const uint32_t *positions,
+                            size_t npos, uint8_t *status);
+/* py_synd (libffrs/pyrsi16.hpp:633-651): ecc_len syndromes per codeword. */
+int ffrs_rsi16_synds(ffrs_rsi16 *rs, const uint16_t *msg, const uint16_t *ecc, size_t count, uint32_t *synds);
+/* py_sugiyama<1> (libffrs/pyrsi16.hpp:653-674): locator and evaluator (ecc_len values
+ * each) per syndrome vector; length[i] = locator length or 0xFFFFFFFF where the
+ * reference raises. */
+int ffrs_rsi16_sugiyama(ffrs_rsi16 *rs, const uint32_t *synds, size_t count, uint32_t *locator,
+                        uint32_t *evaluator, uint32_t *length);
+/* py_roots (libffrs/pyrsi16.hpp:676-691): is_root[m] = 1 where the partial inverse NTT
+ * of the ecc_len coefficients vanishes, m < block_len. */
+int ffrs_rsi16_roots(ffrs_rsi16 *rs, const uint32_t *poly, uint8_t *is_root);
+int ffrs_rsi16_encode_device(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc,
+                             int vector_lanes);
+int ffrs_rsi16_repair_device(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count,
+                             uint8_t *d_status);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

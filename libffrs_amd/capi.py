@@ -1,0 +1,180 @@
+"""ctypes binding of the C ABI in include/ffrs_gpu.h (libffrs_gpu.so).
+
+The pybind11 module covers the reference's host-buffer API; this binding also
+exposes the device-pointer entry points, device memory and stream timing, for
+callers whose data already sits in HBM (benchmarks, pipelines, other runtimes).
+"""
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libffrs_gpu.so")
+
+FFRS_OK = 0
+_lib = None
+
+c_vp = ctypes.c_void_p
+c_sz = ctypes.c_size_t
+c_int = ctypes.c_int
+c_uint = ctypes.c_uint
+c_u32 = ctypes.c_uint32
+c_u8p = ctypes.POINTER(ctypes.c_uint8)
+c_u16p = ctypes.POINTER(ctypes.c_uint16)
+c_u32p = ctypes.POINTER(ctypes.c_uint32)
+
+# (name, restype, argtypes) for every entry point of include/ffrs_gpu.h
+SIGNATURES = [
+    ("ffrs_abi_version", c_int, []),
+    ("ffrs_last_error", ctypes.c_char_p, []),
+    ("ffrs_ctx_create", c_int, [c_int, ctypes.POINTER(c_vp)]),
+    ("ffrs_ctx_destroy", None, [c_vp]),
+    ("ffrs_ctx_synchronize", c_int, [c_vp]),
+    ("ffrs_ctx_device_info", c_int, [c_vp, ctypes.c_char_p, c_sz, ctypes.POINTER(c_int), ctypes.POINTER(c_sz),
+                                     ctypes.POINTER(c_sz)]),
+    ("ffrs_device_alloc", c_int, [c_vp, c_sz, ctypes.POINTER(c_vp)]),
+    ("ffrs_device_free", c_int, [c_vp, c_vp]),
+    ("ffrs_memcpy_h2d", c_int, [c_vp, c_vp, c_vp, c_sz]),
+    ("ffrs_memcpy_d2h", c_int, [c_vp, c_vp, c_vp, c_sz]),
+    ("ffrs_memcpy_d2d", c_int, [c_vp, c_vp, c_vp, c_sz]),
+    ("ffrs_timer_begin", c_int, [c_vp]),
+    ("ffrs_timer_end", c_int, [c_vp, ctypes.POINTER(ctypes.c_float)]),
+    ("ffrs_rs256_create", c_int, [c_vp, c_uint, c_uint, c_uint, ctypes.POINTER(c_vp)]),
+    ("ffrs_rs256_destroy", None, [c_vp]),
+    ("ffrs_rs256_generator", c_int, [c_vp, c_u8p, c_u8p]),
+    ("ffrs_rs256_encode", c_int, [c_vp, c_sz, c_vp, c_sz, c_vp, c_sz]),
+    ("ffrs_rs256_repair", c_int, [c_vp, c_sz, c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_rs256_synds", c_int, [c_vp, c_sz, c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_rs256_encode_device", c_int, [c_vp, c_sz, c_vp, c_sz, c_vp]),
+    ("ffrs_rs256_repair_device", c_int, [c_vp, c_sz, c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_rsi16_create", c_int, [c_vp, c_sz, c_sz, c_u32, ctypes.POINTER(c_vp)]),
+    ("ffrs_rsi16_destroy", None, [c_vp]),
+    ("ffrs_rsi16_root", c_u32, [c_vp]),
+    ("ffrs_rsi16_ntt_len", c_sz, [c_vp]),
+    ("ffrs_rsi16_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_int]),
+    ("ffrs_rsi16_repair", c_int, [c_vp, c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_rsi16_repair_known", c_int, [c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_rsi16_synds", c_int, [c_vp, c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_rsi16_sugiyama", c_int, [c_vp, c_vp, c_sz, c_vp, c_vp, c_vp]),
+    ("ffrs_rsi16_roots", c_int, [c_vp, c_vp, c_vp]),
+    ("ffrs_rsi16_encode_device", c_int, [c_vp, c_vp, c_sz, c_vp, c_int]),
+    ("ffrs_rsi16_repair_device", c_int, [c_vp, c_vp, c_vp, c_sz, c_vp]),
+]
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in SIGNATURES:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+class FfrsError(RuntimeError):
+    pass
+
+
+def check(code, what):
+    if code != FFRS_OK:
+        raise FfrsError(f"{what} failed ({code}): {lib().ffrs_last_error().decode()}")
+
+
+class Context:
+    """One device and one stream (ffrs_ctx)."""
+
+    def __init__(self, device=0):
+        h = c_vp()
+        check(lib().ffrs_ctx_create(device, ctypes.byref(h)), "ffrs_ctx_create")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().ffrs_ctx_destroy(self.h)
+            self.h = None
+
+    def sync(self):
+        check(lib().ffrs_ctx_synchronize(self.h), "synchronize")
+
+    def info(self):
+        name = ctypes.create_string_buffer(256)
+        cus, fr, tot = c_int(), c_sz(), c_sz()
+        check(lib().ffrs_ctx_device_info(self.h, name, 256, ctypes.byref(cus), ctypes.byref(fr), ctypes.byref(tot)),
+              "device_info")
+        return name.value.decode(), cus.value, fr.value, tot.value
+
+    def alloc(self, nbytes):
+        p = c_vp()
+        check(lib().ffrs_device_alloc(self.h, nbytes, ctypes.byref(p)), "alloc")
+        return DeviceBuffer(self, p.value, nbytes)
+
+    def timer_begin(self):
+        check(lib().ffrs_timer_begin(self.h), "timer_begin")
+
+    def timer_end(self):
+        ms = ctypes.c_float()
+        check(lib().ffrs_timer_end(self.h, ctypes.byref(ms)), "timer_end")
+        return ms.value
+
+
+class DeviceBuffer:
+    def __init__(self, ctx, ptr, nbytes):
+        self.ctx, self.ptr, self.nbytes = ctx, ptr, nbytes
+
+    def free(self):
+        if self.ptr:
+            check(lib().ffrs_device_free(self.ctx.h, self.ptr), "free")
+            self.ptr = None
+
+    def upload(self, arr):
+        assert arr.nbytes <= self.nbytes and arr.flags.c_contiguous
+        check(lib().ffrs_memcpy_h2d(self.ctx.h, self.ptr, arr.ctypes.data, arr.nbytes), "h2d")
+
+    def download(self, arr):
+        assert arr.nbytes <= self.nbytes and arr.flags.c_contiguous
+        check(lib().ffrs_memcpy_d2h(self.ctx.h, arr.ctypes.data, self.ptr, arr.nbytes), "d2h")
+
+    def copy_from(self, other, nbytes=None):
+        n = other.nbytes if nbytes is None else nbytes
+        check(lib().ffrs_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, n), "d2d")
+
+
+class RS256Codec:
+    def __init__(self, ctx, ecc_len, primitive=2, polynomial=0x11D):
+        h = c_vp()
+        check(lib().ffrs_rs256_create(ctx.h, ecc_len, primitive, polynomial, ctypes.byref(h)), "ffrs_rs256_create")
+        self.h, self.ctx, self.ecc_len = h, ctx, ecc_len
+
+    def close(self):
+        if self.h:
+            lib().ffrs_rs256_destroy(self.h)
+            self.h = None
+
+    def encode_device(self, message_len, d_msg, count, d_ecc):
+        check(lib().ffrs_rs256_encode_device(self.h, message_len, d_msg.ptr, count, d_ecc.ptr), "rs256 encode")
+
+    def repair_device(self, message_len, d_msg, d_ecc, count, d_status):
+        check(lib().ffrs_rs256_repair_device(self.h, message_len, d_msg.ptr, d_ecc.ptr, count, d_status.ptr),
+              "rs256 repair")
+
+
+class RSi16Codec:
+    def __init__(self, ctx, block_len, ecc_len, primitive=3):
+        h = c_vp()
+        check(lib().ffrs_rsi16_create(ctx.h, block_len, ecc_len, primitive, ctypes.byref(h)), "ffrs_rsi16_create")
+        self.h, self.ctx = h, ctx
+        self.block_len, self.ecc_len, self.message_len = block_len, ecc_len, block_len - ecc_len
+
+    def close(self):
+        if self.h:
+            lib().ffrs_rsi16_destroy(self.h)
+            self.h = None
+
+    def encode_device(self, d_msg, count, d_ecc, vector_lanes=True):
+        check(lib().ffrs_rsi16_encode_device(self.h, d_msg.ptr, count, d_ecc.ptr, 1 if vector_lanes else 0),
+              "rsi16 encode")
+
+    def repair_device(self, d_msg, d_ecc, count, d_status):
+        check(lib().ffrs_rsi16_repair_device(self.h, d_msg.ptr, d_ecc.ptr, count, d_status.ptr), "rsi16 repair")

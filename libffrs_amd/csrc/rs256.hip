@@ -1,0 +1,873 @@
+// rs256.hip -- RS256 (GF(2^8)) systematic encode and repair on gfx950.
+//
+// Reference semantics (paths relative to the reference repository root):
+//   field       libffrs/galois.hpp:93-146, 624-667 (alpha = primitive, poly 0x11d default)
+//   generator   libffrs/reed_solomon.hpp:55-88
+//   encode      libffrs/reed_solomon.hpp:169-389, libffrs/pyrs256.hpp:99-127
+//   syndromes   libffrs/reed_solomon.hpp:412-453
+//   repair      libffrs/reed_solomon.hpp:555-744 (BM :655-702, roots :511-552, Forney :704-743)
+//
+// Kernels
+//   rs256_encode_kernel  persistent workgroups, one codeword per lane. The parity is the
+//                        remainder of x^ecc M(x) by g(x), advanced S bytes per step with
+//                        S slice tables T_s[b] = b x^(ecc+s) mod g held in LDS.
+//   rs256_repair_kernel  one codeword per lane: the same remainder over the message,
+//                        XOR the received parity -> R(x) = C(x) mod g(x); S_i = R(alpha^i);
+//                        Berlekamp-Massey with the reference's truncation and status rules;
+//                        Chien search over positions [0, n); Forney; in-place corrections.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace ffrs {
+
+// ---------------------------------------------------------------- host tables
+
+struct Gf256Host {
+    uint8_t exp_t[256];  // reference convention: exp[255] = 1
+    uint8_t log_t[256];  // reference convention: log[1] = 255, log[0] = 0
+    uint8_t mul(uint8_t a, uint8_t b) const
+    {
+        if (!a || !b)
+            return 0;
+        unsigned r = (unsigned)log_t[a] + log_t[b];
+        if (r >= 255)
+            r -= 255;
+        return exp_t[r];
+    }
+};
+
+// Shift-and-add product modulo the field polynomial (galois.hpp:624-651).
+static uint8_t gf256_mul_serial(uint8_t a, uint8_t b, uint8_t poly1)
+{
+    uint8_t r = 0;
+    for (int i = 7; i >= 0; --i) {
+        r = (r & 0x80) ? (uint8_t)((r << 1) ^ poly1) : (uint8_t)(r << 1);
+        if (a & (1u << i))
+            r ^= b;
+    }
+    return r;
+}
+
+static bool gf256_build(unsigned primitive, unsigned polynomial, Gf256Host &gf)
+{
+    uint8_t poly1 = (uint8_t)(polynomial & 0xff);
+    if (poly1 & 0x80)
+        return false;  // the reference's wide multiply asserts this (galois.hpp:715)
+    memset(gf.log_t, 0, sizeof gf.log_t);
+    uint8_t x = 1;
+    for (int i = 0; i < 256; ++i) {
+        gf.exp_t[i] = x;
+        gf.log_t[x] = (uint8_t)i;
+        x = gf256_mul_serial(x, (uint8_t)primitive, poly1);
+    }
+    bool seen[256] = {false};
+    for (int i = 0; i < 255; ++i) {
+        if (gf.exp_t[i] == 0 || seen[gf.exp_t[i]])
+            return false;  // primitive does not generate the multiplicative group
+        seen[gf.exp_t[i]] = true;
+    }
+    return true;
+}
+
+// Device-side field tables: EXP2[x] = alpha^(x mod 255) for x <= 508 and 0 above,
+// LOG2[a] = log_alpha(a) in [0, 254] with LOG2[0] = ZLOG (511). A product of logs
+// that involves a zero lands in the zero region of EXP2.
+constexpr int ZLOG = 511;
+constexpr int EXP2_LEN = 1024;
+
+// ---------------------------------------------------------------- device code
+
+struct Rs256Tables {
+    const uint32_t *lut;     // S * 256 * EW dwords, slice-major
+    const uint8_t *exp2;     // EXP2_LEN
+    const uint16_t *log2;    // 256
+};
+
+template <int EW, int S>
+__device__ inline void lut_step(uint32_t (&rem)[EW], uint32_t w, const uint32_t *__restrict__ s_lut)
+{
+    if constexpr (S == 4) {
+        uint32_t x = rem[0] ^ w;
+        const uint32_t *t3 = s_lut + (3 * 256 + (x & 0xff)) * EW;
+        const uint32_t *t2 = s_lut + (2 * 256 + ((x >> 8) & 0xff)) * EW;
+        const uint32_t *t1 = s_lut + (1 * 256 + ((x >> 16) & 0xff)) * EW;
+        const uint32_t *t0 = s_lut + (0 * 256 + (x >> 24)) * EW;
+        if constexpr (EW % 4 == 0) {
+#pragma unroll
+            for (int d = 0; d < EW; d += 4) {
+                uint4 a = *reinterpret_cast<const uint4 *>(t3 + d);
+                uint4 b = *reinterpret_cast<const uint4 *>(t2 + d);
+                uint4 c = *reinterpret_cast<const uint4 *>(t1 + d);
+                uint4 e = *reinterpret_cast<const uint4 *>(t0 + d);
+                uint32_t n0 = rem[d + 1];
+                uint32_t n1 = rem[d + 2];
+                uint32_t n2 = rem[d + 3];
+                uint32_t n3 = (d + 4 < EW) ? rem[(d + 4) % EW] : 0u;
+                rem[d + 0] = n0 ^ a.x ^ b.x ^ c.x ^ e.x;
+                rem[d + 1] = n1 ^ a.y ^ b.y ^ c.y ^ e.y;
+                rem[d + 2] = n2 ^ a.z ^ b.z ^ c.z ^ e.z;
+                rem[d + 3] = n3 ^ a.w ^ b.w ^ c.w ^ e.w;
+            }
+        } else {
+#pragma unroll
+            for (int d = 0; d < EW; ++d) {
+                uint32_t nx = (d + 1 < EW) ? rem[(d + 1) % EW] : 0u;
+                rem[d] = nx ^ t3[d] ^ t2[d] ^ t1[d] ^ t0[d];
+            }
+        }
+    } else if constexpr (S == 2) {
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            uint32_t x = rem[0] ^ (w >> (16 * h));
+            const uint32_t *t1 = s_lut + (256 + (x & 0xff)) * EW;
+            const uint32_t *t0 = s_lut + ((x >> 8) & 0xff) * EW;
+#pragma unroll
+            for (int d = 0; d < EW; ++d) {
+                uint32_t nx = (d + 1 < EW) ? rem[(d + 1) % EW] : 0u;
+                rem[d] = alignbyte(nx, rem[d], 2) ^ t1[d] ^ t0[d];
+            }
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint32_t f = (rem[0] ^ (w >> (8 * j))) & 0xff;
+            const uint32_t *t0 = s_lut + f * EW;
+#pragma unroll
+            for (int d = 0; d < EW; ++d) {
+                uint32_t nx = (d + 1 < EW) ? rem[(d + 1) % EW] : 0u;
+                rem[d] = alignbyte(nx, rem[d], 1) ^ t0[d];
+            }
+        }
+    }
+}
+
+// Remainder of x^ecc * M(x) by g(x) for the message of `k` bytes at byte offset
+// `msg_off` of the buffer behind `rsrc`. The message is fed in 4-byte words after
+// `pad` leading zero bytes (leading zeros do not change the remainder).
+template <int EW, int S>
+__device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, int64_t msg_off, uint32_t k,
+                                         const uint32_t *__restrict__ s_lut, uint32_t (&rem)[EW])
+{
+#pragma unroll
+    for (int d = 0; d < EW; ++d)
+        rem[d] = 0;
+    uint32_t P = (k + 3) & ~3u;
+    uint32_t pad = P - k;
+    int64_t start = msg_off - (int64_t)pad;
+    int64_t a = start >> 2;                 // floor division (start >= -3)
+    uint32_t sh = (uint32_t)(start - a * 4);
+    uint32_t off = (uint32_t)(a * 4);       // wraps below zero -> out of range -> reads 0
+    uint32_t headmask = pad ? ~((1u << (8 * pad)) - 1u) : 0xffffffffu;
+    uint32_t nw = P >> 2;
+    uint32_t prev = buf_load_u32(rsrc, off);
+    uint32_t t = 0;
+    for (; t + 4 <= nw; t += 4) {
+        uint32_t c0 = buf_load_u32(rsrc, off + 4 * (t + 1));
+        uint32_t c1 = buf_load_u32(rsrc, off + 4 * (t + 2));
+        uint32_t c2 = buf_load_u32(rsrc, off + 4 * (t + 3));
+        uint32_t c3 = buf_load_u32(rsrc, off + 4 * (t + 4));
+        uint32_t w0 = alignbyte(c0, prev, sh);
+        if (t == 0)
+            w0 &= headmask;
+        lut_step<EW, S>(rem, w0, s_lut);
+        lut_step<EW, S>(rem, alignbyte(c1, c0, sh), s_lut);
+        lut_step<EW, S>(rem, alignbyte(c2, c1, sh), s_lut);
+        lut_step<EW, S>(rem, alignbyte(c3, c2, sh), s_lut);
+        prev = c3;
+    }
+    for (; t < nw; ++t) {
+        uint32_t c = buf_load_u32(rsrc, off + 4 * (t + 1));
+        uint32_t w = alignbyte(c, prev, sh);
+        if (t == 0)
+            w &= headmask;
+        lut_step<EW, S>(rem, w, s_lut);
+        prev = c;
+    }
+}
+
+template <int EW>
+__device__ inline uint8_t rem_byte(const uint32_t (&rem)[EW], uint32_t j)
+{
+    uint32_t v = 0;
+#pragma unroll
+    for (int d = 0; d < EW; ++d)
+        v = ((j >> 2) == (uint32_t)d) ? rem[d] : v;
+    return (uint8_t)(v >> (8 * (j & 3)));
+}
+
+template <int EW, int S>
+__global__ __launch_bounds__(256) void rs256_encode_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
+                                                            uint32_t count, uint32_t k, uint8_t *__restrict__ out,
+                                                            uint32_t ecc, const uint32_t *__restrict__ lut)
+{
+    extern __shared__ uint32_t s_lut[];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * EW); i += blockDim.x)
+        s_lut[i] = lut[i];
+    __syncthreads();
+
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += stride) {
+        uint32_t rem[EW];
+        message_remainder<EW, S>(rsrc, (int64_t)cw * k, k, s_lut, rem);
+        uint8_t *o = out + (size_t)cw * ecc;
+        if ((ecc & 3) == 0) {
+            uint32_t *o4 = reinterpret_cast<uint32_t *>(o);
+#pragma unroll
+            for (int d = 0; d < EW; ++d)
+                if ((uint32_t)d * 4 < ecc)
+                    o4[d] = rem[d];
+        } else {
+            for (uint32_t j = 0; j < ecc; ++j)
+                o[j] = rem_byte<EW>(rem, j);
+        }
+    }
+}
+
+// ---- GF(2^8) helpers over the LDS copies of EXP2 / LOG2
+
+struct GfLds {
+    const uint8_t *exp2;
+    const uint16_t *log2;
+    __device__ inline uint32_t mul(uint32_t a, uint32_t b) const { return exp2[log2[a] + log2[b]]; }
+    __device__ inline uint32_t mul_log(uint32_t la, uint32_t b) const { return exp2[la + log2[b]]; }
+    // reference table conventions (galois.hpp:98-114): inv(0) = 1, div(a, 0) = a
+    __device__ inline uint32_t inv(uint32_t a) const { return a ? exp2[255 - log2[a]] : 1u; }
+    __device__ inline uint32_t div(uint32_t a, uint32_t b) const
+    {
+        if (!a)
+            return 0;
+        if (!b)
+            return a;
+        return exp2[log2[a] + 255 - log2[b]];
+    }
+};
+
+// S_i = R(alpha^i) for i < E, where R(x) = sum_j r_j x^(E-1-j) is held in remainder
+// order (r_0 = highest). Horner over j per syndrome; EXP2 is zero above index 508, so a
+// zero accumulator (LOG2 = ZLOG) contributes nothing.
+template <int EW>
+__device__ inline void syndromes_from_remainder(const uint32_t (&rem)[EW], uint32_t E, const GfLds &gf,
+                                                uint8_t *Sy)
+{
+    for (uint32_t i = 0; i < E; ++i) {
+        uint32_t acc = 0;
+#pragma unroll
+        for (int d = 0; d < EW; ++d) {
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                if ((uint32_t)(d * 4 + b) < E)
+                    acc = (uint32_t)gf.exp2[gf.log2[acc] + i] ^ ((rem[d] >> (8 * b)) & 0xffu);
+            }
+        }
+        Sy[i] = (uint8_t)acc;
+    }
+}
+
+// Reads E parity bytes of one codeword and XORs them into the remainder words.
+template <int EW>
+__device__ inline uint32_t xor_parity(uint32_t (&rem)[EW], const uint8_t *pe, uint32_t E)
+{
+    uint32_t nz = 0;
+    if ((E & 3) == 0) {
+        const uint32_t *p4 = reinterpret_cast<const uint32_t *>(pe);
+#pragma unroll
+        for (int d = 0; d < EW; ++d)
+            if ((uint32_t)d * 4 < E)
+                rem[d] ^= p4[d];
+    } else {
+        for (uint32_t j = 0; j < E; ++j) {
+            uint32_t v = (uint32_t)pe[j] << (8 * (j & 3));
+#pragma unroll
+            for (int d = 0; d < EW; ++d)
+                rem[d] ^= ((j >> 2) == (uint32_t)d) ? v : 0u;
+        }
+    }
+#pragma unroll
+    for (int d = 0; d < EW; ++d)
+        nz |= rem[d];
+    return nz;
+}
+
+// Per-lane scratch rows in LDS: bytes [0,E) syndromes (later the root positions),
+// [E,2E) locator, [2E,3E) B(x) (later Omega), then E u16 logs of the locator.
+// The row stride is an odd number of dwords so that lanes spread over the banks.
+__host__ __device__ inline uint32_t repair_row_stride(uint32_t ecc)
+{
+    uint32_t bytes = 3 * ecc + 1 + 2 * ecc;
+    uint32_t dw = (bytes + 3) / 4;
+    if ((dw & 1) == 0)
+        dw += 1;
+    return dw * 4;
+}
+
+template <int EW, int S, int WG>
+__global__ __launch_bounds__(WG) void rs256_repair_kernel(uint8_t *__restrict__ msgs, uint32_t msg_bytes,
+                                                           uint8_t *__restrict__ eccs, uint32_t count, uint32_t k,
+                                                           uint32_t ecc, uint8_t *__restrict__ status,
+                                                           Rs256Tables tab)
+{
+    extern __shared__ uint32_t s_mem[];
+    uint32_t *s_lut = s_mem;
+    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem + S * 256 * EW);
+    uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
+    uint8_t *s_rows = reinterpret_cast<uint8_t *>(s_log2 + 256);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * EW); i += blockDim.x)
+        s_lut[i] = tab.lut[i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
+        s_exp2[i] = tab.exp2[i];
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
+        s_log2[i] = tab.log2[i];
+    __syncthreads();
+    GfLds gf{s_exp2, s_log2};
+
+    const uint32_t E = ecc;
+    const uint32_t n = k + E;
+    uint8_t *row = s_rows + threadIdx.x * repair_row_stride(E);
+    uint8_t *Sy = row;
+    uint8_t *lam = row + E;
+    uint8_t *Bx = row + 2 * E;
+    uint16_t *llam = reinterpret_cast<uint16_t *>(row + ((3 * E + 1) & ~1u));
+
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    uint32_t stride = gridDim.x * blockDim.x;
+    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += stride) {
+        // ---- R(x) = C(x) mod g(x): remainder of the message XOR the received parity
+        uint32_t rem[EW];
+        message_remainder<EW, S>(rsrc, (int64_t)cw * k, k, s_lut, rem);
+        uint8_t *pe = eccs + (size_t)cw * E;
+        if (xor_parity<EW>(rem, pe, E) == 0) {  // all syndromes zero (reed_solomon.hpp:565-566)
+            status[cw] = 1;
+            continue;
+        }
+        syndromes_from_remainder<EW>(rem, E, gf, Sy);
+
+        // ---- Berlekamp-Massey (reed_solomon.hpp:655-702): locator capacity E, degree
+        //      terms >= E dropped, L returned. Natural order lam[i] = Lambda_i.
+        for (uint32_t i = 0; i < E; ++i) {
+            lam[i] = 0;
+            Bx[i] = 0;
+        }
+        lam[0] = 1;
+        Bx[0] = 1;
+        uint32_t L = 0, m = 1, b = 1;
+        for (uint32_t nn = 0; nn < E; ++nn) {
+            uint32_t d = Sy[nn];
+            for (uint32_t i = 1; i <= L; ++i)
+                d ^= gf.mul(lam[i], Sy[nn - i]);
+            if (d == 0) {
+                m += 1;
+            } else {
+                uint32_t c = gf.div(d, b);
+                uint32_t lc = gf.log2[c];
+                bool grow = 2 * L <= nn;
+                // lam <- lam - c x^m B ; if grow, B <- old lam (downwards keeps B[i-m] unread-after-write)
+                for (uint32_t ii = E; ii-- > 0;) {
+                    uint32_t old = lam[ii];
+                    uint32_t sb = ii >= m ? (uint32_t)Bx[ii - m] : 0u;
+                    lam[ii] = (uint8_t)(old ^ gf.mul_log(lc, sb));
+                    if (grow)
+                        Bx[ii] = (uint8_t)old;
+                }
+                if (grow) {
+                    L = nn + 1 - L;
+                    b = d;
+                    m = 1;
+                } else {
+                    m += 1;
+                }
+            }
+        }
+        if (L > E / 2) {
+            status[cw] = 0;
+            continue;
+        }
+
+        // ---- Omega = S(x) Lambda(x) mod x^E (reed_solomon.hpp:708-728); B(x) is dead now
+        for (uint32_t i = 0; i <= L; ++i)
+            llam[i] = gf.log2[lam[i]];
+        uint8_t *om = Bx;
+        for (uint32_t kk = 0; kk < E; ++kk) {
+            uint32_t v = 0;
+            uint32_t top = kk < L ? kk : L;
+            for (uint32_t i = 0; i <= top; ++i)
+                v ^= gf.exp2[llam[i] + gf.log2[Sy[kk - i]]];
+            om[kk] = (uint8_t)v;
+        }
+
+        // ---- Chien search over [0, n): Lambda(alpha^-p) == 0, ascending p
+        //      (reed_solomon.hpp:516-538); positions go to the dead syndrome row
+        uint8_t *pos = Sy;
+        uint32_t roots = 0;
+        for (uint32_t p = 0; p < n; ++p) {
+            uint32_t pinv = p ? 255 - p : 0;  // alpha^-p = alpha^(255 - p)
+            uint32_t e = 0, acc = 0;
+            for (uint32_t i = 0; i <= L; ++i) {
+                acc ^= gf.exp2[llam[i] + e];
+                e += pinv;
+                e = e >= 255 ? e - 255 : e;
+            }
+            if (acc == 0) {
+                if (roots < E)
+                    pos[roots] = (uint8_t)p;
+                roots++;
+            }
+        }
+        if (roots != L) {
+            status[cw] = 0;
+            continue;
+        }
+
+        // ---- Forney (reed_solomon.hpp:730-743): e_p = X Omega(X^-1) / Lambda'(X^-1),
+        //      X = alpha^p, Lambda' = odd terms of Lambda (characteristic 2)
+        for (uint32_t r = 0; r < L; ++r) {
+            uint32_t p = pos[r];
+            uint32_t pinv = p ? 255 - p : 0;
+            uint32_t num = 0, den = 0, e = 0;
+            for (uint32_t kk = 0; kk < E; ++kk) {
+                num ^= gf.exp2[gf.log2[om[kk]] + e];
+                if ((kk & 1) == 0 && kk + 1 <= L)
+                    den ^= gf.exp2[llam[kk + 1] + e];
+                e += pinv;
+                e = e >= 255 ? e - 255 : e;
+            }
+            uint32_t X = gf.exp2[p];
+            uint32_t mag = gf.mul(X, gf.div(num, den));
+            uint32_t at = n - 1 - p;
+            if (at < k)
+                msgs[(size_t)cw * k + at] ^= (uint8_t)mag;
+            else
+                pe[at - k] ^= (uint8_t)mag;
+        }
+        status[cw] = 1;
+    }
+}
+
+template <int EW, int S>
+__global__ __launch_bounds__(256) void rs256_synds_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
+                                                           const uint8_t *__restrict__ eccs, uint32_t count,
+                                                           uint32_t k, uint32_t ecc, uint8_t *__restrict__ synds,
+                                                           Rs256Tables tab)
+{
+    extern __shared__ uint32_t s_mem[];
+    uint32_t *s_lut = s_mem;
+    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem + S * 256 * EW);
+    uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * EW); i += blockDim.x)
+        s_lut[i] = tab.lut[i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
+        s_exp2[i] = tab.exp2[i];
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
+        s_log2[i] = tab.log2[i];
+    __syncthreads();
+    GfLds gf{s_exp2, s_log2};
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += gridDim.x * blockDim.x) {
+        uint32_t rem[EW];
+        message_remainder<EW, S>(rsrc, (int64_t)cw * k, k, s_lut, rem);
+        xor_parity<EW>(rem, eccs + (size_t)cw * ecc, ecc);
+        syndromes_from_remainder<EW>(rem, ecc, gf, synds + (size_t)cw * ecc);
+    }
+}
+
+}  // namespace ffrs
+
+// ---------------------------------------------------------------- C ABI
+
+using namespace ffrs;
+
+struct ffrs_rs256 {
+    ffrs_ctx *ctx = nullptr;
+    unsigned ecc = 0, EW = 0, S = 0;
+    Gf256Host gf;
+    uint8_t gen[256] = {0};
+    uint8_t roots[256] = {0};
+    uint32_t *d_lut = nullptr;
+    uint8_t *d_exp2 = nullptr;
+    uint16_t *d_log2 = nullptr;
+};
+
+namespace {
+
+unsigned ew_class(unsigned ecc)
+{
+    unsigned w = (ecc + 3) / 4;
+    unsigned c = 1;
+    while (c < w)
+        c <<= 1;
+    return c;
+}
+
+unsigned slices_for(unsigned EW) { return EW <= 8 ? 4 : (EW == 16 ? 2 : 1); }
+
+// Remainder-order bytes r_0..r_{E-1} (r_0 = coefficient of x^(E-1)) packed little-endian.
+void lfsr_zero_step(const ffrs_rs256 &rs, uint8_t *r)
+{
+    unsigned E = rs.ecc;
+    uint8_t f = r[0];
+    memmove(r, r + 1, E - 1);
+    r[E - 1] = 0;
+    if (f)
+        for (unsigned j = 0; j < E; ++j)
+            r[j] ^= rs.gf.mul(f, rs.gen[j + 1]);
+}
+
+std::vector<uint32_t> build_lut(const ffrs_rs256 &rs)
+{
+    unsigned E = rs.ecc, EW = rs.EW, S = rs.S;
+    std::vector<uint32_t> lut((size_t)S * 256 * EW, 0);
+    for (unsigned b = 0; b < 256; ++b) {
+        uint8_t r[256] = {0};
+        // b * x^E mod g: one LFSR step with feedback b
+        r[0] = (uint8_t)b;
+        lfsr_zero_step(rs, r);
+        for (unsigned s = 0; s < S; ++s) {
+            uint32_t *row = &lut[((size_t)s * 256 + b) * EW];
+            for (unsigned j = 0; j < E; ++j)
+                row[j / 4] |= (uint32_t)r[j] << (8 * (j % 4));
+            lfsr_zero_step(rs, r);
+        }
+    }
+    return lut;
+}
+
+template <int EW, int S>
+int launch_encode(ffrs_rs256 *rs, const uint8_t *d_msg, size_t count, uint32_t k, uint8_t *d_ecc)
+{
+    ffrs_ctx *ctx = rs->ctx;
+    size_t lds = (size_t)S * 256 * EW * 4;
+    size_t tiles = (count + 255) / 256;
+    int wgs_per_cu = lds <= 16384 ? 8 : (lds <= 32768 ? 4 : 2);
+    int grid = persistent_grid(ctx, wgs_per_cu, tiles);
+    if (int r = allow_lds(rs256_encode_kernel<EW, S>, lds))
+        return r;
+    hipLaunchKernelGGL((rs256_encode_kernel<EW, S>), dim3(grid), dim3(256), lds, ctx->stream, d_msg,
+                       (uint32_t)(count * k), (uint32_t)count, k, d_ecc, rs->ecc, rs->d_lut);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+template <int EW, int S, int WG>
+int launch_repair(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_t count, uint32_t k, uint8_t *d_status)
+{
+    ffrs_ctx *ctx = rs->ctx;
+    size_t lds = (size_t)S * 256 * EW * 4 + EXP2_LEN + 512 + (size_t)WG * repair_row_stride(rs->ecc);
+    if (lds > 160 * 1024)
+        return fail(FFRS_ERR_INVALID, "repair workspace of %zu bytes exceeds LDS", lds);
+    int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
+    int grid = persistent_grid(ctx, per_cu * 2, (count + WG - 1) / WG);
+    Rs256Tables tab{rs->d_lut, rs->d_exp2, rs->d_log2};
+    if (int r = allow_lds(rs256_repair_kernel<EW, S, WG>, lds))
+        return r;
+    hipLaunchKernelGGL((rs256_repair_kernel<EW, S, WG>), dim3(grid), dim3(WG), lds, ctx->stream, d_msg,
+                       (uint32_t)(count * k), d_ecc, (uint32_t)count, k, rs->ecc, d_status, tab);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+template <int EW, int S>
+int launch_synds(ffrs_rs256 *rs, const uint8_t *d_msg, const uint8_t *d_ecc, size_t count, uint32_t k,
+                 uint8_t *d_out)
+{
+    ffrs_ctx *ctx = rs->ctx;
+    size_t lds = (size_t)S * 256 * EW * 4 + EXP2_LEN + 512;
+    int grid = persistent_grid(ctx, 2, (count + 255) / 256);
+    Rs256Tables tab{rs->d_lut, rs->d_exp2, rs->d_log2};
+    if (int r = allow_lds(rs256_synds_kernel<EW, S>, lds))
+        return r;
+    hipLaunchKernelGGL((rs256_synds_kernel<EW, S>), dim3(grid), dim3(256), lds, ctx->stream, d_msg,
+                       (uint32_t)(count * k), d_ecc, (uint32_t)count, k, rs->ecc, d_out, tab);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+// Splits launches so that every launch addresses < 2 GiB of messages with 32-bit offsets.
+size_t max_batch(size_t k) { return std::max<size_t>(1, ((size_t)1 << 31) / std::max<size_t>(k, 1)) & ~(size_t)255; }
+
+int encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *d_msg, size_t count, uint8_t *d_ecc)
+{
+    if (count == 0)
+        return FFRS_OK;
+    uint32_t k = (uint32_t)message_len;
+    size_t step = max_batch(message_len);
+    for (size_t b = 0; b < count; b += step) {
+        size_t c = std::min(step, count - b);
+        const uint8_t *m = d_msg + b * message_len;
+        uint8_t *e = d_ecc + b * rs->ecc;
+        int r;
+        switch (rs->EW) {
+        case 1: r = launch_encode<1, 4>(rs, m, c, k, e); break;
+        case 2: r = launch_encode<2, 4>(rs, m, c, k, e); break;
+        case 4: r = launch_encode<4, 4>(rs, m, c, k, e); break;
+        case 8: r = launch_encode<8, 4>(rs, m, c, k, e); break;
+        case 16: r = launch_encode<16, 2>(rs, m, c, k, e); break;
+        case 32: r = launch_encode<32, 1>(rs, m, c, k, e); break;
+        default: r = launch_encode<64, 1>(rs, m, c, k, e); break;
+        }
+        if (r)
+            return r;
+    }
+    return FFRS_OK;
+}
+
+int repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d_ecc, size_t count,
+                  uint8_t *d_status)
+{
+    if (count == 0)
+        return FFRS_OK;
+    uint32_t k = (uint32_t)message_len;
+    size_t step = max_batch(message_len);
+    for (size_t b = 0; b < count; b += step) {
+        size_t c = std::min(step, count - b);
+        uint8_t *m = d_msg + b * message_len;
+        uint8_t *e = d_ecc + b * rs->ecc;
+        uint8_t *st = d_status + b;
+        int r;
+        switch (rs->EW) {
+        case 1: r = launch_repair<1, 4, 256>(rs, m, e, c, k, st); break;
+        case 2: r = launch_repair<2, 4, 256>(rs, m, e, c, k, st); break;
+        case 4: r = launch_repair<4, 4, 256>(rs, m, e, c, k, st); break;
+        case 8: r = launch_repair<8, 4, 256>(rs, m, e, c, k, st); break;
+        case 16: r = launch_repair<16, 2, 128>(rs, m, e, c, k, st); break;
+        case 32: r = launch_repair<32, 1, 64>(rs, m, e, c, k, st); break;
+        default: r = launch_repair<64, 1, 64>(rs, m, e, c, k, st); break;
+        }
+        if (r)
+            return r;
+    }
+    return FFRS_OK;
+}
+
+int check_codec(const ffrs_rs256 *rs, size_t message_len)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (message_len == 0 || message_len + rs->ecc > 255)
+        return fail(FFRS_ERR_INVALID, "message_len %zu invalid for ecc_len %u (block_len must be <= 255)",
+                    message_len, rs->ecc);
+    return FFRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsigned polynomial, ffrs_rs256 **out)
+{
+    if (!ctx || !out)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    if (ecc_len < 1 || ecc_len > 254)
+        return fail(FFRS_ERR_INVALID, "ecc_len must be in [1, 254], got %u", ecc_len);
+    auto *rs = new ffrs_rs256();
+    rs->ctx = ctx;
+    rs->ecc = ecc_len;
+    rs->EW = ew_class(ecc_len);
+    rs->S = slices_for(rs->EW);
+    if (!gf256_build(primitive & 0xff, polynomial, rs->gf)) {
+        delete rs;
+        return fail(FFRS_ERR_INVALID,
+                    "primitive %u does not generate GF(2^8) modulo polynomial 0x%x (unsupported field)", primitive,
+                    polynomial);
+    }
+    // g(x) = prod_{i<ecc} (x - alpha^i), highest degree first (reed_solomon.hpp:65-86)
+    uint8_t g[257] = {0};
+    unsigned len = 1;
+    g[0] = 1;
+    for (unsigned i = 0; i < ecc_len; ++i) {
+        uint8_t root = rs->gf.exp_t[i];
+        rs->roots[i] = root;
+        uint8_t ng[257] = {0};
+        for (unsigned a = 0; a < len; ++a) {
+            ng[a] ^= g[a];
+            ng[a + 1] ^= rs->gf.mul(g[a], root);
+        }
+        len++;
+        memcpy(g, ng, len);
+    }
+    memcpy(rs->gen, g, ecc_len + 1);
+
+    std::vector<uint32_t> lut = build_lut(*rs);
+    uint8_t exp2[EXP2_LEN];
+    uint16_t log2[256];
+    for (int x = 0; x < EXP2_LEN; ++x)
+        exp2[x] = x <= 508 ? rs->gf.exp_t[x % 255] : 0;
+    log2[0] = ZLOG;
+    for (int a = 1; a < 256; ++a)
+        log2[a] = rs->gf.log_t[a] % 255;
+
+    int r = ctx_set_device(ctx);
+    if (r) {
+        delete rs;
+        return r;
+    }
+    hipError_t e = hipMalloc(&rs->d_lut, lut.size() * 4);
+    if (e == hipSuccess)
+        e = hipMalloc(&rs->d_exp2, EXP2_LEN);
+    if (e == hipSuccess)
+        e = hipMalloc(&rs->d_log2, 512);
+    if (e == hipSuccess)
+        e = hipMemcpy(rs->d_lut, lut.data(), lut.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(rs->d_exp2, exp2, EXP2_LEN, hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(rs->d_log2, log2, 512, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        ffrs_rs256_destroy(rs);
+        return fail(FFRS_ERR_DEVICE, "table upload failed: %s", hipGetErrorString(e));
+    }
+    *out = rs;
+    return FFRS_OK;
+}
+
+void ffrs_rs256_destroy(ffrs_rs256 *rs)
+{
+    if (!rs)
+        return;
+    (void)hipSetDevice(rs->ctx->device);
+    if (rs->d_lut)
+        (void)hipFree(rs->d_lut);
+    if (rs->d_exp2)
+        (void)hipFree(rs->d_exp2);
+    if (rs->d_log2)
+        (void)hipFree(rs->d_log2);
+    delete rs;
+}
+
+int ffrs_rs256_generator(const ffrs_rs256 *rs, uint8_t *generator, uint8_t *roots)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (generator)
+        memcpy(generator, rs->gen, rs->ecc + 1);
+    if (roots)
+        memcpy(roots, rs->roots, rs->ecc);
+    return FFRS_OK;
+}
+
+int ffrs_rs256_encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *d_msg, size_t count, uint8_t *d_ecc)
+{
+    if (int r = check_codec(rs, message_len))
+        return r;
+    if (int r = ctx_set_device(rs->ctx))
+        return r;
+    return encode_device(rs, message_len, d_msg, count, d_ecc);
+}
+
+int ffrs_rs256_repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d_ecc, size_t count,
+                             uint8_t *d_status)
+{
+    if (int r = check_codec(rs, message_len))
+        return r;
+    if (int r = ctx_set_device(rs->ctx))
+        return r;
+    return repair_device(rs, message_len, d_msg, d_ecc, count, d_status);
+}
+
+int ffrs_rs256_encode(ffrs_rs256 *rs, size_t message_len, const uint8_t *buf, size_t size, uint8_t *out,
+                      size_t out_size)
+{
+    if (int r = check_codec(rs, message_len))
+        return r;
+    size_t full = size / message_len, tail = size - full * message_len;
+    size_t need = (full + (tail ? 1 : 0)) * rs->ecc;
+    if (out_size != need)
+        return fail(FFRS_ERR_SIZE, "output holds %zu bytes, %zu needed", out_size, need);
+    if (size == 0)
+        return FFRS_OK;
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    void *d_in, *d_out;
+    if (int r = stage_reserve(ctx, 0, size, &d_in))
+        return r;
+    if (int r = stage_reserve(ctx, 1, need, &d_out))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(d_in, buf, size, hipMemcpyHostToDevice, ctx->stream));
+    if (int r = encode_device(rs, message_len, (const uint8_t *)d_in, full, (uint8_t *)d_out))
+        return r;
+    if (tail)
+        if (int r = encode_device(rs, tail, (const uint8_t *)d_in + full * message_len, 1,
+                                  (uint8_t *)d_out + full * rs->ecc))
+            return r;
+    FFRS_HIP(hipMemcpyAsync(out, d_out, need, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rs256_repair(ffrs_rs256 *rs, size_t message_len, uint8_t *msg, uint8_t *ecc, size_t count,
+                      uint8_t *status)
+{
+    if (int r = check_codec(rs, message_len))
+        return r;
+    if (count == 0)
+        return FFRS_OK;
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    size_t mb = count * message_len, eb = count * rs->ecc;
+    void *d_m, *d_e, *d_s;
+    if (int r = stage_reserve(ctx, 0, mb, &d_m))
+        return r;
+    if (int r = stage_reserve(ctx, 1, eb, &d_e))
+        return r;
+    if (int r = stage_reserve(ctx, 2, count, &d_s))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(d_m, msg, mb, hipMemcpyHostToDevice, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(d_e, ecc, eb, hipMemcpyHostToDevice, ctx->stream));
+    if (int r = repair_device(rs, message_len, (uint8_t *)d_m, (uint8_t *)d_e, count, (uint8_t *)d_s))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(msg, d_m, mb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(ecc, d_e, eb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(status, d_s, count, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rs256_synds(ffrs_rs256 *rs, size_t message_len, const uint8_t *msg, const uint8_t *ecc, size_t count,
+                     uint8_t *synds)
+{
+    if (int r = check_codec(rs, message_len))
+        return r;
+    if (count == 0)
+        return FFRS_OK;
+    if (count > max_batch(message_len))
+        return fail(FFRS_ERR_INVALID, "too many codewords in one syndrome call");
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    size_t mb = count * message_len, eb = count * rs->ecc;
+    void *d_m, *d_e, *d_o;
+    if (int r = stage_reserve(ctx, 0, mb, &d_m))
+        return r;
+    if (int r = stage_reserve(ctx, 1, eb, &d_e))
+        return r;
+    if (int r = stage_reserve(ctx, 2, eb, &d_o))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(d_m, msg, mb, hipMemcpyHostToDevice, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(d_e, ecc, eb, hipMemcpyHostToDevice, ctx->stream));
+    uint32_t k = (uint32_t)message_len;
+    int r;
+    const uint8_t *m = (const uint8_t *)d_m, *e = (const uint8_t *)d_e;
+    uint8_t *o = (uint8_t *)d_o;
+    switch (rs->EW) {
+    case 1: r = launch_synds<1, 4>(rs, m, e, count, k, o); break;
+    case 2: r = launch_synds<2, 4>(rs, m, e, count, k, o); break;
+    case 4: r = launch_synds<4, 4>(rs, m, e, count, k, o); break;
+    case 8: r = launch_synds<8, 4>(rs, m, e, count, k, o); break;
+    case 16: r = launch_synds<16, 2>(rs, m, e, count, k, o); break;
+    case 32: r = launch_synds<32, 1>(rs, m, e, count, k, o); break;
+    default: r = launch_synds<64, 1>(rs, m, e, count, k, o); break;
+    }
+    if (r)
+        return r;
+    FFRS_HIP(hipMemcpyAsync(synds, d_o, eb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,1167 @@
+// rsi16.hip -- RSi16 (GF(65537), NTT-based) systematic encode and repair on gfx950.
+//
+// Reference semantics (paths relative to the reference repository root):
+//   field ops     libffrs/galois.hpp:93-146, 168-216, 258-395 (scalar and SIMD-lane branches)
+//   NTT tables    libffrs/ntt.hpp:55-123; butterflies :225-326; nttr/inttr :569-591
+//   encode        libffrs/rsi16v_impl.hpp:93-119 via NTT::pntt_message_residue (ntt.hpp:180-199)
+//   repair        libffrs/rsi16v_impl.hpp:122-168 (_repair), :271-440 (_sugiyama),
+//                 :443-512 (_find_roots_ntt, _forney), ntt.hpp:133-152 (pntt),
+//                 ntt.hpp:343-564 (reversal, Newton inverse, NTT-domain division)
+//   known pos.    libffrs/rsi16v_impl.hpp:171-231, 532-563
+//
+// Layout: every kernel runs in single-wave workgroups. A wave is split into lane
+// groups of g = min(64, ecc/2) lanes; each group owns one codeword and keeps its
+// working polynomials in LDS. Butterfly stages are spread over the group's lanes and
+// ordered with wavefront-scope fences (a group never waits on another group, so the
+// groups of one wave may follow different control flow).
+//
+// Exactness: the encode path reproduces the reference's lazy residue arithmetic
+// word for word (the u32 values, not just their residues), because the parity depends
+// on them in corner cases; `VEC` selects the SIMD-lane or the scalar variant of
+// mod_p / mul / mul_residue / sub. The repair path uses the scalar full-reduction
+// operations, whose outputs are canonical field elements.
+#include "common.hpp"
+
+#include <algorithm>
+#include <cstring>
+
+namespace ffrs {
+
+constexpr uint32_t P16 = 0x10001u;
+constexpr uint32_t RSI16_MAX_ECC = 4096;
+
+// ---------------------------------------------------------------- field operations
+
+struct ScalarOps {  // integral branches
+    __host__ __device__ static inline uint32_t add(uint32_t a, uint32_t b)
+    {
+        uint32_t r = a + b;
+        return r >= P16 ? r - P16 : r;
+    }
+    __host__ __device__ static inline uint32_t sub(uint32_t a, uint32_t b)
+    {
+        uint32_t r = a - b;
+        return r >= 0x80000000u ? r + P16 : r;
+    }
+    __host__ __device__ static inline uint32_t mul(uint32_t a, uint32_t b)
+    {
+        uint32_t r = a * b;
+        if (r == 0 && a && b)
+            return 1;
+        r = (r & 0xffff) - (r >> 16);
+        return r >= 0x80000000u ? r + P16 : r;
+    }
+    __host__ __device__ static inline uint32_t mul_residue(uint32_t a, uint32_t b)
+    {
+        uint32_t r = a * b;
+        if (r == 0 && a && b)
+            return 1;
+        return (r & 0xffff) - (r >> 16);
+    }
+    __host__ __device__ static inline uint32_t mod_p(uint32_t x)
+    {
+        uint32_t r = (x & 0xffff) - (x >> 16) - (x >> 31);
+        return r >= 0x80000000u ? r + P16 : r;
+    }
+};
+
+struct VectorOps {  // SIMD-lane branches
+    __host__ __device__ static inline uint32_t add(uint32_t a, uint32_t b)
+    {
+        uint32_t r = a + b;
+        return r - (r >= P16 ? P16 : 0u);
+    }
+    __host__ __device__ static inline uint32_t sub(uint32_t a, uint32_t b) { return a - b + (b > a ? P16 : 0u); }
+    __host__ __device__ static inline uint32_t mul(uint32_t a, uint32_t b)
+    {
+        uint32_t r = a * b;
+        uint32_t ovf = (r == 0 && (a & b) != 0) ? 1u : 0u;
+        uint32_t lo = r & 0xffff, hi = r >> 16;
+        return lo - hi + ovf + (hi > lo ? P16 : 0u);
+    }
+    __host__ __device__ static inline uint32_t mul_residue(uint32_t a, uint32_t b)
+    {
+        uint32_t r = a * b;
+        uint32_t ovf = (r == 0 && (a & b) != 0) ? 1u : 0u;
+        return (r & 0xffff) - (r >> 16) + ovf;
+    }
+    __host__ __device__ static inline uint32_t mod_p(uint32_t x)
+    {
+        uint32_t lo = x & 0xffff, hi = x >> 16;
+        return lo - hi - (x >> 31) + (hi >= lo ? P16 : 0u);
+    }
+};
+
+using S = ScalarOps;
+
+__host__ __device__ inline uint32_t s_neg(uint32_t a) { return a == 0 ? 0u : P16 - a; }
+
+// a^e with canonical scalar products
+__host__ __device__ inline uint32_t s_pow(uint32_t a, uint32_t e)
+{
+    uint32_t r = 1;
+    while (e) {
+        if (e & 1)
+            r = S::mul(r, a);
+        a = S::mul(a, a);
+        e >>= 1;
+    }
+    return r;
+}
+
+// Table conventions of galois.hpp:98-114 over GF(65537): inv(0) = 1, div(0, b) = 0,
+// div(a, 0) = a; otherwise the field inverse / quotient (a^65535 = a^-1).
+__host__ __device__ inline uint32_t g_inv(uint32_t a) { return a == 0 ? 1u : s_pow(a, 65535u); }
+__host__ __device__ inline uint32_t g_div(uint32_t a, uint32_t b)
+{
+    if (a == 0)
+        return 0;
+    if (b == 0)
+        return a;
+    return S::mul(a, g_inv(b));
+}
+
+__host__ __device__ inline uint32_t rbo_bits(uint32_t b, unsigned bits)
+{
+    return bits ? (__builtin_bitreverse32(b) >> (32 - bits)) : 0u;
+}
+
+// ---------------------------------------------------------------- device tables
+
+struct Rsi16Dev {
+    uint32_t n, k, E, N, nbits;
+    uint32_t ecc_inv, ecc2_inv;
+    const uint32_t *roots_ecc, *roots_i_ecc;     // E
+    const uint32_t *roots_ecc2, *roots_i_ecc2;   // 2E
+    const uint32_t *pntt_shift;                  // n
+    const uint32_t *pintt_shift;                 // N
+    const uint32_t *roots_ntt;                   // N
+    const uint32_t *mix;                         // E
+    const uint32_t *rbo_ecc;                     // E
+};
+
+__device__ inline void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
+struct Grp {
+    uint32_t gl;  // lane within the group
+    uint32_t g;   // group size
+};
+
+// ---- butterflies over x[0..len), spread over the group's lanes
+
+// Cooperative CT butterflies (RBO in, natural out), ntt.hpp:225-249, full reduction.
+__device__ inline void ct_full(uint32_t *x, uint32_t len, const uint32_t *roots, Grp G)
+{
+    for (uint32_t s = 1, ef = len >> 1; s < len; s <<= 1, ef >>= 1) {
+        for (uint32_t b = G.gl; b < len / 2; b += G.g) {
+            uint32_t j = b & (s - 1);
+            uint32_t i = ((b - j) << 1) + j;
+            uint32_t a = x[i], c = x[i + s];
+            if (j) {
+                c = S::mul(c, roots[ef * j]);
+            }
+            x[i] = S::add(a, c);
+            x[i + s] = S::sub(a, c);
+        }
+        wave_sync();
+    }
+}
+
+// Cooperative CT butterflies with the lazy residue arithmetic, ntt.hpp:252-275.
+template <class O>
+__device__ inline void ct_residue(uint32_t *x, uint32_t len, const uint32_t *roots, Grp G)
+{
+    for (uint32_t s = 1, ef = len >> 1; s < len; s <<= 1, ef >>= 1) {
+        for (uint32_t b = G.gl; b < len / 2; b += G.g) {
+            uint32_t j = b & (s - 1);
+            uint32_t i = ((b - j) << 1) + j;
+            uint32_t a = x[i], c = x[i + s];
+            if (j) {
+                uint32_t m = O::mul_residue(O::mod_p(c), roots[ef * j]);
+                x[i] = a + m;
+                x[i + s] = a - m;
+            } else {
+                x[i] = a + c;
+                x[i + s] = a - c;
+            }
+        }
+        wave_sync();
+    }
+}
+
+// Cooperative GS butterflies (natural in, RBO out), ntt.hpp:280-302.
+template <class O>
+__device__ inline void gs_full(uint32_t *x, uint32_t len, const uint32_t *roots, Grp G)
+{
+    for (uint32_t s = len >> 1, ef = 0; s > 0; s >>= 1, ++ef) {
+        for (uint32_t b = G.gl; b < len / 2; b += G.g) {
+            uint32_t j = b & (s - 1);
+            uint32_t i = ((b - j) << 1) + j;
+            uint32_t a = x[i], c = x[i + s];
+            x[i] = O::add(a, c);
+            uint32_t d = O::sub(a, c);
+            x[i + s] = j ? O::mul(d, roots[j << ef]) : d;
+        }
+        wave_sync();
+    }
+}
+
+__device__ inline void nttr(const Rsi16Dev &T, uint32_t *x, Grp G) { gs_full<S>(x, T.E, T.roots_ecc, G); }
+
+__device__ inline void inttr(const Rsi16Dev &T, uint32_t *x, Grp G)
+{
+    ct_full(x, T.E, T.roots_i_ecc, G);
+    for (uint32_t j = G.gl; j < T.E; j += G.g)
+        x[j] = S::mul(x[j], T.ecc_inv);
+    wave_sync();
+}
+
+// degree + 1 of r[0..max) (simd.hpp:234-242); every lane of the group scans.
+__device__ inline uint32_t poly_len(const uint32_t *r, uint32_t max)
+{
+    while (max > 0 && r[max - 1] == 0)
+        --max;
+    return max;
+}
+
+// ---------------------------------------------------------------- encode
+
+// One codeword per lane group: pntt_message_residue + _mix_ecc_residue
+// (rsi16v_impl.hpp:93-119), values written as & 0xFFFF.
+template <class O>
+__global__ __launch_bounds__(64) void rsi16_encode_kernel(const uint16_t *__restrict__ msgs, uint32_t count,
+                                                           uint16_t *__restrict__ out, Rsi16Dev T, uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E;
+    Grp G{threadIdx.x % g, g};
+    uint32_t grp = threadIdx.x / g;
+    uint32_t cpw = 64 / g;
+    uint32_t cw = blockIdx.x * cpw + grp;
+    if (cw >= count)
+        return;
+    uint32_t *buf = s_mem + grp * 2 * E;
+    uint32_t *acc = buf + E;
+    const uint16_t *m = msgs + (size_t)cw * T.k;
+    uint32_t nb = T.n / E;
+    for (uint32_t c = 0; c + 1 < nb; ++c) {
+        for (uint32_t j = G.gl; j < E; j += G.g)
+            buf[j] = m[c * E + j];
+        wave_sync();
+        ct_residue<O>(buf, E, T.roots_ecc, G);
+        for (uint32_t j = G.gl; j < E; j += G.g) {
+            uint32_t v = O::mul_residue(O::mod_p(buf[j]), T.pntt_shift[c * E + j]);
+            acc[j] = c ? acc[j] + v : v;
+        }
+        wave_sync();
+    }
+    for (uint32_t j = G.gl; j < E; j += G.g)
+        acc[j] = O::mul(O::mod_p(acc[j]), T.mix[j]);
+    wave_sync();
+    gs_full<O>(acc, E, T.roots_i_ecc, G);
+    uint16_t *o = out + (size_t)cw * E;
+    for (uint32_t j = G.gl; j < E; j += G.g)
+        o[j] = (uint16_t)(acc[j] & 0xffff);
+}
+
+// ---------------------------------------------------------------- syndromes
+
+// NTT::pntt (ntt.hpp:133-152) of the codeword msg || ecc into syn[0..E); buf is E scratch.
+__device__ inline void pntt_codeword(const Rsi16Dev &T, const uint16_t *m, const uint16_t *e, uint32_t *buf,
+                                     uint32_t *syn, Grp G)
+{
+    const uint32_t E = T.E;
+    uint32_t nb = T.n / E;
+    for (uint32_t c = 0; c < nb; ++c) {
+        for (uint32_t j = G.gl; j < E; j += G.g) {
+            uint32_t idx = c * E + j;
+            buf[j] = idx < T.k ? (uint32_t)m[idx] : (uint32_t)e[idx - T.k];
+        }
+        wave_sync();
+        ct_full(buf, E, T.roots_ecc, G);
+        for (uint32_t j = G.gl; j < E; j += G.g) {
+            uint32_t v = S::mul(buf[j], T.pntt_shift[c * E + j]);
+            syn[j] = c ? S::add(syn[j], v) : v;
+        }
+        wave_sync();
+    }
+}
+
+// ---------------------------------------------------------------- Sugiyama
+
+// NTT::_reverse_ntt_vec (ntt.hpp:344-354) in gather form: the sequential swaps form an
+// involution k(i) = rbo_ecc[E - rbo_ecc[i]], so v'[i] = v[k(i)] * roots_i_ecc[j] * roots_ecc[shift j].
+__device__ inline void reverse_ntt(const Rsi16Dev &T, uint32_t *v, uint32_t shift, uint32_t *tmp, Grp G)
+{
+    const uint32_t E = T.E, mask = E - 1;
+    for (uint32_t i = G.gl; i < E; i += G.g)
+        tmp[i] = v[i];
+    wave_sync();
+    for (uint32_t i = G.gl; i < E; i += G.g) {
+        if (i == 0)
+            continue;
+        uint32_t j = T.rbo_ecc[i];
+        uint32_t k = T.rbo_ecc[E - j];
+        uint32_t x = tmp[k];
+        x = S::mul(x, T.roots_i_ecc[j]);
+        x = S::mul(x, T.roots_ecc[(shift * j) & mask]);
+        v[i] = x;
+    }
+    wave_sync();
+}
+
+// NTT::poly_inv_mod_xn_rev (ntt.hpp:363-469), scalar instantiation. Returns false where
+// the reference's py_assert fires.
+__device__ inline bool poly_inv_mod_xn_rev(const Rsi16Dev &T, uint32_t *g, uint32_t g_len, uint32_t g0,
+                                           uint32_t a_len, uint32_t *a, Grp G)
+{
+    const uint32_t E = T.E;
+    uint32_t a0 = g_inv(g0);
+    for (uint32_t i = G.gl; i < E; i += G.g)
+        a[i] = a0;
+    wave_sync();
+    uint32_t l = 2;
+    uint32_t a_max = a_len;
+    while (l < a_len) {
+        if (g_len + l - 2 > E)
+            return false;
+        for (uint32_t i = G.gl; i < E; i += G.g) {
+            uint32_t ai = a[i];
+            uint32_t ag = S::mul(ai, g[i]);
+            a[i] = S::mul(ai, S::sub(2, ag));
+        }
+        wave_sync();
+        inttr(T, a, G);
+        for (uint32_t i = G.gl; i < E; i += G.g)
+            if (i >= l)
+                a[i] = 0;
+        wave_sync();
+        nttr(T, a, G);
+        l *= 2;
+    }
+    for (uint32_t i = G.gl; i < E; i += G.g) {
+        uint32_t ag = S::mul(a[i], g[i]);
+        g[i] = S::mul(a[i], S::sub(2, ag));
+    }
+    wave_sync();
+    inttr(T, g, G);
+    // a = g % x^l, then reverse a[0..a_max), then zero a[a_len..E)
+    for (uint32_t i = G.gl; i < E; i += G.g) {
+        uint32_t src = i < a_max ? a_max - 1 - i : i;
+        uint32_t v = src < l ? g[src] : 0u;
+        a[i] = i < a_len ? v : 0u;
+    }
+    wave_sync();
+    nttr(T, a, G);
+    return true;
+}
+
+// NTT::poly_div_ntt (ntt.hpp:471-564), scalar instantiation. q_len via *q_len_out;
+// returns false where the reference raises.
+__device__ inline bool poly_div_ntt(const Rsi16Dev &T, const uint32_t *f, uint32_t f_len, const uint32_t *g,
+                                    uint32_t g_len, uint32_t f0, uint32_t g0, uint32_t *t, uint32_t *q,
+                                    uint32_t *tmp, uint32_t *q_len_out, Grp G)
+{
+    const uint32_t E = T.E;
+    uint32_t q_len = f_len - g_len + 1;
+    if (g_len == 0)
+        return false;
+    if (f_len < g_len) {
+        for (uint32_t i = G.gl; i < E; i += G.g)
+            q[i] = 0;
+        wave_sync();
+        *q_len_out = 0;
+        return true;
+    }
+    for (uint32_t i = G.gl; i < E; i += G.g)
+        t[i] = g[i];
+    wave_sync();
+    reverse_ntt(T, t, g_len, tmp, G);
+    if (!poly_inv_mod_xn_rev(T, t, g_len, g0, q_len, q, G))
+        return false;
+    for (uint32_t j = G.gl; j < E; j += G.g)
+        t[j] = S::mul(S::sub(f[j], f0), q[j]);
+    wave_sync();
+    inttr(T, t, G);
+    uint32_t right_n = q_len < E - g_len ? q_len : E - g_len;
+    for (uint32_t i = G.gl; i < E; i += G.g) {
+        uint32_t v;
+        if (i < right_n)
+            v = t[g_len + i];
+        else if (i < q_len)
+            v = t[i - right_n];
+        else
+            v = 0;
+        q[i] = v;
+    }
+    wave_sync();
+    nttr(T, q, G);
+    *q_len_out = q_len;
+    return true;
+}
+
+// RSi16v<1>::_sugiyama (rsi16v_impl.hpp:271-440). r1 holds the syndromes on entry and
+// the normalised evaluator on exit; w holds 5 E scratch words. Returns the locator
+// length, or 0xFFFFFFFF where the reference raises.
+__device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1, uint32_t *r1, uint32_t *w, Grp G)
+{
+    const uint32_t E = T.E, mask = E - 1;
+    uint32_t *q = w, *t = w + E, *a2 = w + 2 * E, *r2 = w + 3 * E, *tmp = w + 4 * E;
+    for (uint32_t i = G.gl; i < E; i += G.g) {
+        q[i] = 0;
+        t[i] = 0;
+        a1[i] = 0;
+        a2[i] = 1;
+        r2[i] = r1[i];
+    }
+    wave_sync();
+    uint32_t q_len = 0, t_len = 0, a2_len = 1, a1_len;
+    uint32_t r1_len = poly_len(r1, E);
+    uint32_t r2_len = r1_len;
+    uint32_t r2l = r2[0];
+    uint32_t lanes = r1_len != 0 ? 0xffffffffu : 0u;
+    {
+        uint32_t r1h = r1[(r1_len - 1) & mask];
+        uint32_t r1h2 = r1[(r1_len - 2) & mask];
+        uint32_t a11 = s_neg(g_inv(r1h));
+        uint32_t a10 = S::mul(r1h2, S::mul(a11, a11));
+        wave_sync();
+        if (G.gl == 0) {
+            a1[0] = a10;
+            a1[1] = a11;
+        }
+        a1_len = 2 & lanes;
+        for (uint32_t i = G.gl; i < E; i += G.g) {
+            uint32_t v = S::mul(a10, r2[i]);  // r2 still holds the original r1
+            if (i >= 1)
+                v = S::add(v, S::mul(a11, r2[i - 1]));
+            tmp[i] = v;
+        }
+        wave_sync();
+        r1_len = poly_len(tmp, E - 1) & lanes;
+        for (uint32_t i = G.gl; i < E; i += G.g)
+            r1[i] = i < r1_len ? tmp[i] : 0u;
+        wave_sync();
+        nttr(T, a1, G);
+        nttr(T, r2, G);
+    }
+    bool cond = r1_len >= E / 2;
+    if (cond) {
+        for (uint32_t it = 1; it < E / 2; ++it) {
+            uint32_t r1h = r1[(r1_len - 1) & mask];
+            uint32_t r1l = r1[0];
+            wave_sync();
+            nttr(T, r1, G);
+            if (!poly_div_ntt(T, r2, r2_len, r1, r1_len, r2l, r1h, t, q, tmp, &q_len, G))
+                return 0xffffffffu;
+            for (uint32_t i = G.gl; i < E; i += G.g) {
+                uint32_t qi = q[i];
+                t[i] = S::sub(a2[i], S::mul(qi, a1[i]));
+                q[i] = S::sub(r2[i], S::mul(qi, r1[i]));
+            }
+            wave_sync();
+            t_len = q_len + a1_len - 1;
+            t_len = a2_len > t_len ? a2_len : t_len;
+            uint32_t nq_len = q_len + r1_len - 1;
+            nq_len = r2_len > nq_len ? r2_len : nq_len;
+            for (uint32_t i = G.gl; i < E; i += G.g) {
+                a2[i] = a1[i];
+                r2[i] = r1[i];
+            }
+            wave_sync();
+            a2_len = a1_len;
+            r2_len = r1_len;
+            r2l = r1l;
+            if (!(r1_len >= E / 2)) {
+                inttr(T, r1, G);
+                break;
+            }
+            for (uint32_t i = G.gl; i < E; i += G.g) {
+                a1[i] = t[i];
+                r1[i] = q[i];
+            }
+            wave_sync();
+            a1_len = t_len;
+            r1_len = nq_len;
+            inttr(T, r1, G);
+            r1_len = poly_len(r1, r1_len);
+        }
+    }
+    inttr(T, a1, G);
+    a1_len = poly_len(a1, a1_len & lanes);
+    uint32_t inv0 = g_inv(a1[0]);
+    wave_sync();
+    for (uint32_t i = G.gl; i < E; i += G.g) {
+        a1[i] = S::mul(a1[i], inv0);
+        r1[i] = S::mul(r1[i], inv0);
+    }
+    wave_sync();
+    return a1_len & lanes;
+}
+
+// RSi16v::_eval (rsi16v_impl.hpp:628-636), Horner over r[len-1 .. 0]
+__device__ inline uint32_t eval_poly(const uint32_t *r, uint32_t len, uint32_t x)
+{
+    uint32_t s = 0;
+    for (uint32_t i = len; i-- > 0;)
+        s = S::add(S::mul(s, x), r[i]);
+    return s;
+}
+
+// ---------------------------------------------------------------- repair
+
+// Per group LDS: syn/evaluator E, locator E, derivative E, roots chunk E, Sugiyama 5E.
+constexpr uint32_t REPAIR_WORDS = 9;
+
+__global__ __launch_bounds__(64) void rsi16_repair_kernel(uint16_t *__restrict__ msgs, uint16_t *__restrict__ eccs,
+                                                           uint32_t count, uint8_t *__restrict__ status, Rsi16Dev T,
+                                                           uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E;
+    Grp G{threadIdx.x % g, g};
+    uint32_t grp = threadIdx.x / g;
+    uint32_t cpw = 64 / g;
+    uint32_t cw = blockIdx.x * cpw + grp;
+    if (cw >= count)
+        return;
+    uint32_t *base = s_mem + grp * REPAIR_WORDS * E;
+    uint32_t *syn = base, *loc = base + E, *der = base + 2 * E, *chunk = base + 3 * E, *w = base + 4 * E;
+    uint16_t *m = msgs + (size_t)cw * T.k;
+    uint16_t *e = eccs + (size_t)cw * E;
+
+    pntt_codeword(T, m, e, chunk, syn, G);
+    bool any = false;
+    for (uint32_t j = 0; j < E; ++j)
+        any |= syn[j] != 0;
+    if (!any) {
+        if (G.gl == 0)
+            status[cw] = FFRS_RSI16_NO_ERRORS;
+        return;
+    }
+    uint32_t L = sugiyama(T, loc, syn, w, G);
+    if (L == 0xffffffffu) {
+        if (G.gl == 0)
+            status[cw] = FFRS_RSI16_RAISES;
+        return;
+    }
+    uint32_t eval_len = E - L + 1;
+    // _deriv (rsi16v_impl.hpp:608-616)
+    for (uint32_t i = G.gl; i < E; i += G.g)
+        der[i] = i + 1 < E ? S::mul(loc[i + 1], i + 1) : 0u;
+    wave_sync();
+    uint32_t der_len = poly_len(der, E);
+    if (L > E) {  // py_assert at rsi16v_impl.hpp:454
+        if (G.gl == 0)
+            status[cw] = FFRS_RSI16_RAISES;
+        return;
+    }
+    uint32_t lanes = L != 0 ? 0xffffffffu : 0u;
+    uint32_t dmax = der_len & lanes, emax = eval_len & lanes;
+    uint32_t nb = T.n / E;
+    uint32_t found = 0;
+    // NTT::pintt_ecc (ntt.hpp:206-220), one chunk of E positions at a time
+    for (uint32_t c = 0; c < nb; ++c) {
+        for (uint32_t j = G.gl; j < E; j += G.g)
+            chunk[j] = S::mul(loc[j], T.pintt_shift[c * E + j]);
+        wave_sync();
+        gs_full<S>(chunk, E, T.roots_i_ecc, G);
+        for (uint32_t j = G.gl; j < E; j += G.g) {
+            uint32_t pos = c * E + j;
+            if (chunk[j] == 0 && lanes) {
+                found++;
+                uint32_t ri = rbo_bits(pos, T.nbits);
+                uint32_t x = T.roots_ntt[ri];
+                uint32_t xi = T.roots_ntt[(T.N - ri) & (T.N - 1)];
+                uint32_t num = eval_poly(syn, emax, xi);
+                uint32_t den = eval_poly(der, dmax, xi);
+                uint32_t err = S::mul(g_div(num, den), x);
+                uint16_t *at = pos < T.k ? &m[pos] : &e[pos - T.k];
+                *at = (uint16_t)(S::add((uint32_t)*at, err) & 0xffff);
+            }
+        }
+        wave_sync();
+    }
+    // group-wide root count: lanes publish their counts through the chunk row
+    chunk[G.gl] = found;
+    wave_sync();
+    uint32_t total = 0;
+    for (uint32_t j = 0; j < G.g; ++j)
+        total += chunk[j];
+    if (G.gl == 0)
+        status[cw] = total > 0 ? FFRS_RSI16_REPAIR_OK : (lanes ? FFRS_RSI16_ERROR_LOCATION_FAIL : FFRS_RSI16_NO_ERRORS);
+}
+
+// Syndromes only (py_synd).
+__global__ __launch_bounds__(64) void rsi16_synds_kernel(const uint16_t *__restrict__ msgs,
+                                                          const uint16_t *__restrict__ eccs, uint32_t count,
+                                                          uint32_t *__restrict__ out, Rsi16Dev T, uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E;
+    Grp G{threadIdx.x % g, g};
+    uint32_t grp = threadIdx.x / g;
+    uint32_t cw = blockIdx.x * (64 / g) + grp;
+    if (cw >= count)
+        return;
+    uint32_t *syn = s_mem + grp * 2 * E, *chunk = syn + E;
+    pntt_codeword(T, msgs + (size_t)cw * T.k, eccs + (size_t)cw * E, chunk, syn, G);
+    for (uint32_t j = G.gl; j < E; j += G.g)
+        out[(size_t)cw * E + j] = syn[j];
+}
+
+// Sugiyama on given syndromes (py_sugiyama<1>).
+__global__ __launch_bounds__(64) void rsi16_sugiyama_kernel(const uint32_t *__restrict__ synds, uint32_t count,
+                                                             uint32_t *__restrict__ loc_out,
+                                                             uint32_t *__restrict__ ev_out,
+                                                             uint32_t *__restrict__ len_out, Rsi16Dev T, uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E;
+    Grp G{threadIdx.x % g, g};
+    uint32_t grp = threadIdx.x / g;
+    uint32_t cw = blockIdx.x * (64 / g) + grp;
+    if (cw >= count)
+        return;
+    uint32_t *r1 = s_mem + grp * 7 * E, *a1 = r1 + E, *w = r1 + 2 * E;
+    for (uint32_t j = G.gl; j < E; j += G.g)
+        r1[j] = synds[(size_t)cw * E + j];
+    wave_sync();
+    uint32_t L = sugiyama(T, a1, r1, w, G);
+    for (uint32_t j = G.gl; j < E; j += G.g) {
+        loc_out[(size_t)cw * E + j] = a1[j];
+        ev_out[(size_t)cw * E + j] = r1[j];
+    }
+    if (G.gl == 0)
+        len_out[cw] = L;
+}
+
+// py_roots: zero positions of the partial inverse NTT of E coefficients.
+__global__ __launch_bounds__(64) void rsi16_roots_kernel(const uint32_t *__restrict__ poly, uint8_t *__restrict__ hit,
+                                                          Rsi16Dev T, uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E;
+    if (threadIdx.x >= g)
+        return;
+    Grp G{threadIdx.x, g};
+    uint32_t *chunk = s_mem;
+    for (uint32_t c = 0; c < T.n / E; ++c) {
+        for (uint32_t j = G.gl; j < E; j += G.g)
+            chunk[j] = S::mul(poly[j], T.pintt_shift[c * E + j]);
+        wave_sync();
+        gs_full<S>(chunk, E, T.roots_i_ecc, G);
+        for (uint32_t j = G.gl; j < E; j += G.g)
+            hit[c * E + j] = chunk[j] == 0;
+        wave_sync();
+    }
+}
+
+// Known-position repair, rsi16v_impl.hpp:171-231 (one codeword, one group).
+__global__ __launch_bounds__(64) void rsi16_repair_known_kernel(uint16_t *__restrict__ m, uint16_t *__restrict__ e,
+                                                                 const uint32_t *__restrict__ pos, uint32_t npos,
+                                                                 uint8_t *__restrict__ status, Rsi16Dev T, uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E;
+    if (threadIdx.x >= g)
+        return;
+    Grp G{threadIdx.x, g};
+    uint32_t *ev = s_mem, *loc = s_mem + 2 * E, *tmp = s_mem + 3 * E, *der = s_mem + 5 * E, *chunk = s_mem + 6 * E;
+    pntt_codeword(T, m, e, chunk, ev, G);
+    bool any = false;
+    for (uint32_t j = 0; j < E; ++j)
+        any |= ev[j] != 0;
+    if (!any) {
+        if (G.gl == 0)
+            *status = FFRS_RSI16_NO_ERRORS;
+        return;
+    }
+    // _error_locator (rsi16v_impl.hpp:532-542): coefficients c_1..c_npos of prod (1 - x_j z)
+    if (G.gl == 0) {
+        for (uint32_t i = 0; i < E; ++i)
+            loc[i] = 0;
+        for (uint32_t j = 0; j < npos; ++j) {
+            uint32_t x = T.roots_ntt[rbo_bits(pos[j], T.nbits)];
+            for (uint32_t i = npos - 1; i > 0; --i)
+                loc[i] = S::sub(loc[i], S::mul(loc[i - 1], x));
+            loc[0] = S::sub(loc[0], x);
+        }
+    }
+    wave_sync();
+    // _error_evaluator (rsi16v_impl.hpp:546-563): Omega = S(z) * (1 + sum c_i z^i) mod z^E via length-2E NTTs
+    for (uint32_t i = G.gl; i < 2 * E; i += G.g) {
+        if (i >= E)
+            ev[i] = 0;
+        tmp[i] = i == 0 ? 1u : (i <= npos ? loc[i - 1] : 0u);
+    }
+    wave_sync();
+    gs_full<S>(ev, 2 * E, T.roots_ecc2, G);
+    gs_full<S>(tmp, 2 * E, T.roots_ecc2, G);
+    for (uint32_t i = G.gl; i < 2 * E; i += G.g)
+        ev[i] = S::mul(ev[i], tmp[i]);
+    wave_sync();
+    ct_full(ev, 2 * E, T.roots_i_ecc2, G);
+    for (uint32_t i = G.gl; i < 2 * E; i += G.g)
+        ev[i] = i < E ? S::mul(ev[i], T.ecc2_inv) : 0u;
+    // _deriv_shifted (rsi16v_impl.hpp:619-626)
+    for (uint32_t i = G.gl; i < E; i += G.g)
+        der[i] = S::mul(loc[i], i + 1);
+    wave_sync();
+    uint32_t dlen = poly_len(der, E);
+    for (uint32_t r = G.gl; r < npos; r += G.g) {
+        uint32_t p = pos[r];
+        uint32_t ri = rbo_bits(p, T.nbits);
+        uint32_t x = T.roots_ntt[ri];
+        uint32_t xi = T.roots_ntt[(T.N - ri) & (T.N - 1)];
+        uint32_t num = eval_poly(ev, E, xi);
+        uint32_t den = eval_poly(der, dlen, xi);
+        uint32_t err = S::mul(g_div(num, den), x);
+        uint16_t *at = p < T.k ? &m[p] : &e[p - T.k];
+        *at = (uint16_t)(S::add((uint32_t)*at, err) & 0xffff);
+    }
+    if (G.gl == 0)
+        *status = FFRS_RSI16_REPAIR_OK;
+}
+
+}  // namespace ffrs
+
+// ---------------------------------------------------------------- host side + C ABI
+
+using namespace ffrs;
+
+struct ffrs_rsi16 {
+    ffrs_ctx *ctx = nullptr;
+    uint32_t n = 0, k = 0, E = 0, N = 0, nbits = 0, primitive = 0, root = 0;
+    uint32_t *d_tables = nullptr;
+    Rsi16Dev dev{};
+};
+
+namespace {
+
+// Field tables as the reference builds them (galois.hpp:133-144 with gf_mul_mod):
+// exp over i = 0..65536 inclusive, so log[1] = 65536 and log[0] = 0.
+struct Gfi16Host {
+    std::vector<uint32_t> exp_t, log_t;
+    explicit Gfi16Host(uint32_t primitive) : exp_t(65537), log_t(65537, 0)
+    {
+        uint32_t x = 1;
+        for (uint32_t i = 0; i < 65537; ++i) {
+            exp_t[i] = x;
+            log_t[x] = i;
+            uint32_t r;
+            if (__builtin_mul_overflow(x, primitive, &r))
+                r = 1;
+            x = r % P16;
+        }
+    }
+    uint32_t inv(uint32_t a) const { return exp_t[65536 - log_t[a]]; }
+    uint32_t div(uint32_t a, uint32_t b) const
+    {
+        if (a == 0)
+            return 0;
+        uint32_t r = log_t[a] + 65536 - log_t[b];
+        if (r >= 65536)
+            r -= 65536;
+        return exp_t[r];
+    }
+    uint32_t pow(uint32_t a, uint32_t e) const { return exp_t[(uint32_t)(log_t[a] * e) % 65536u]; }
+};
+
+uint32_t group_size(uint32_t E) { return std::min<uint32_t>(64, std::max<uint32_t>(1, E / 2)); }
+
+int check_count(size_t count)
+{
+    if (count > 0x7fffffffu)
+        return fail(FFRS_ERR_INVALID, "too many codewords in one call");
+    return FFRS_OK;
+}
+
+int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vec)
+{
+    if (count == 0)
+        return FFRS_OK;
+    uint32_t g = group_size(rs->E), cpw = 64 / g;
+    size_t lds = (size_t)cpw * 2 * rs->E * 4;
+    dim3 grid((unsigned)((count + cpw - 1) / cpw));
+    if (int r = allow_lds(rsi16_encode_kernel<VectorOps>, lds))
+        return r;
+    if (int r = allow_lds(rsi16_encode_kernel<ScalarOps>, lds))
+        return r;
+    if (vec)
+        hipLaunchKernelGGL(rsi16_encode_kernel<VectorOps>, grid, dim3(64), lds, rs->ctx->stream, d_msg,
+                           (uint32_t)count, d_ecc, rs->dev, g);
+    else
+        hipLaunchKernelGGL(rsi16_encode_kernel<ScalarOps>, grid, dim3(64), lds, rs->ctx->stream, d_msg,
+                           (uint32_t)count, d_ecc, rs->dev, g);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+int launch_repair(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count, uint8_t *d_status)
+{
+    if (count == 0)
+        return FFRS_OK;
+    uint32_t g = group_size(rs->E), cpw = 64 / g;
+    size_t lds = (size_t)cpw * REPAIR_WORDS * rs->E * 4;
+    dim3 grid((unsigned)((count + cpw - 1) / cpw));
+    if (int r = allow_lds(rsi16_repair_kernel, lds))
+        return r;
+    hipLaunchKernelGGL(rsi16_repair_kernel, grid, dim3(64), lds, rs->ctx->stream, d_msg, d_ecc, (uint32_t)count,
+                       d_status, rs->dev, g);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffrs_rsi16_create(ffrs_ctx *ctx, size_t block_len, size_t ecc_len, uint32_t primitive, ffrs_rsi16 **out)
+{
+    if (!ctx || !out)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    if (ecc_len < 2 || (ecc_len & (ecc_len - 1)))
+        return fail(FFRS_ERR_PARAM, "ecc_len must be a power of 2: %zu", ecc_len);
+    if (block_len % ecc_len || block_len <= ecc_len)
+        return fail(FFRS_ERR_PARAM, "block_len must be a multiple of ecc_len larger than it (%zu, %zu)", block_len,
+                    ecc_len);
+    if (block_len > 65536)
+        return fail(FFRS_ERR_PARAM, "ntt root of unity sanity check failed (block_len > 65536)");
+    if (ecc_len > RSI16_MAX_ECC)
+        return fail(FFRS_ERR_PARAM, "ecc_len %zu above this build's limit of %u", ecc_len, RSI16_MAX_ECC);
+    if (primitive < 2 || primitive > 65536)
+        return fail(FFRS_ERR_PARAM, "%u is not a primitive element of GF(65537)", primitive);
+    Gfi16Host gf(primitive);
+    if (!(gf.pow(primitive, 32768) == 65536 && gf.pow(primitive, 65536) == 1))
+        return fail(FFRS_ERR_PARAM, "%u is not a primitive element of GF(65537)", primitive);
+
+    // ntt.hpp:55-123
+    uint32_t n = (uint32_t)block_len, E = (uint32_t)ecc_len;
+    uint32_t N = 2;
+    while (N < n)
+        N <<= 1;
+    uint32_t nbits = (uint32_t)__builtin_ctz(N);
+    uint32_t root = gf.pow(primitive, 65536 / N);
+    if (gf.pow(root, N) != 1 || gf.pow(root, N / 2) != 65536)
+        return fail(FFRS_ERR_PARAM, "ntt root of unity sanity check failed");
+    uint32_t root_i = gf.inv(root);
+    uint32_t er = gf.pow(primitive, 65536 / E), er_i = gf.inv(er);
+    uint32_t e2 = gf.pow(primitive, 65536 / (2 * E)), e2_i = gf.inv(e2);
+    if (gf.pow(er, E) != 1 || gf.pow(er, E / 2) != 65536 || gf.pow(e2, 2 * E) != 1 || gf.pow(e2, E) != 65536)
+        return fail(FFRS_ERR_PARAM, "ntt root of unity sanity check failed");
+
+    std::vector<uint32_t> roots_ecc(E), roots_i_ecc(E), roots_ecc2(2 * E), roots_i_ecc2(2 * E), pntt_shift(n),
+        pintt_shift(N), roots_ntt(N), mix(E), rbo_ecc(E);
+    for (uint32_t i = 0; i < N; ++i)
+        roots_ntt[i] = gf.pow(root, i);
+    for (uint32_t i = 0; i < E; ++i) {
+        roots_ecc[i] = gf.pow(er, i);
+        roots_i_ecc[i] = gf.pow(er_i, i);
+        rbo_ecc[i] = rbo_bits(i * (N / E), nbits);
+    }
+    for (uint32_t i = 0; i < 2 * E; ++i) {
+        roots_ecc2[i] = gf.pow(e2, i);
+        roots_i_ecc2[i] = gf.pow(e2_i, i);
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t blk = (i / E) * E, j = i - blk;
+        pntt_shift[i] = gf.pow(root, j * rbo_bits(blk, nbits));
+    }
+    for (uint32_t i = 0; i < N; ++i) {
+        uint32_t blk = (i / E) * E, j = i - blk;
+        pintt_shift[i] = gf.div(gf.pow(root_i, j * rbo_bits(blk, nbits)), N);
+    }
+    // rsi16v_impl.hpp:68-86
+    uint32_t mw = roots_ntt[rbo_bits(n - E, nbits)];
+    uint32_t mw_i = gf.inv(mw);
+    for (uint32_t i = 0; i < E; ++i)
+        mix[i] = s_neg(gf.div(gf.pow(mw_i, i), E));
+
+    auto *rs = new ffrs_rsi16();
+    rs->ctx = ctx;
+    rs->n = n;
+    rs->k = n - E;
+    rs->E = E;
+    rs->N = N;
+    rs->nbits = nbits;
+    rs->primitive = primitive;
+    rs->root = root;
+    size_t words = 6 * (size_t)E + 2 * (size_t)E * 2 + n + 2 * (size_t)N;
+    if (int r = ctx_set_device(ctx)) {
+        delete rs;
+        return r;
+    }
+    hipError_t herr = hipMalloc(&rs->d_tables, words * 4);
+    if (herr != hipSuccess) {
+        delete rs;
+        return fail(FFRS_ERR_NOMEM, "table allocation failed: %s", hipGetErrorString(herr));
+    }
+    std::vector<uint32_t> host;
+    host.reserve(words);
+    auto put = [&](const std::vector<uint32_t> &v) {
+        size_t off = host.size();
+        host.insert(host.end(), v.begin(), v.end());
+        return rs->d_tables + off;
+    };
+    Rsi16Dev &d = rs->dev;
+    d.n = n;
+    d.k = n - E;
+    d.E = E;
+    d.N = N;
+    d.nbits = nbits;
+    d.ecc_inv = gf.inv(E);
+    d.ecc2_inv = gf.inv(2 * E);
+    d.roots_ecc = put(roots_ecc);
+    d.roots_i_ecc = put(roots_i_ecc);
+    d.roots_ecc2 = put(roots_ecc2);
+    d.roots_i_ecc2 = put(roots_i_ecc2);
+    d.pntt_shift = put(pntt_shift);
+    d.pintt_shift = put(pintt_shift);
+    d.roots_ntt = put(roots_ntt);
+    d.mix = put(mix);
+    d.rbo_ecc = put(rbo_ecc);
+    herr = hipMemcpy(rs->d_tables, host.data(), host.size() * 4, hipMemcpyHostToDevice);
+    if (herr != hipSuccess) {
+        ffrs_rsi16_destroy(rs);
+        return fail(FFRS_ERR_DEVICE, "table upload failed: %s", hipGetErrorString(herr));
+    }
+    *out = rs;
+    return FFRS_OK;
+}
+
+void ffrs_rsi16_destroy(ffrs_rsi16 *rs)
+{
+    if (!rs)
+        return;
+    (void)hipSetDevice(rs->ctx->device);
+    if (rs->d_tables)
+        (void)hipFree(rs->d_tables);
+    delete rs;
+}
+
+uint32_t ffrs_rsi16_root(const ffrs_rsi16 *rs) { return rs ? rs->root : 0; }
+size_t ffrs_rsi16_ntt_len(const ffrs_rsi16 *rs) { return rs ? rs->N : 0; }
+
+int ffrs_rsi16_encode_device(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vector_lanes)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (int r = check_count(count))
+        return r;
+    if (int r = ctx_set_device(rs->ctx))
+        return r;
+    return launch_encode(rs, d_msg, count, d_ecc, vector_lanes);
+}
+
+int ffrs_rsi16_repair_device(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count, uint8_t *d_status)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (int r = check_count(count))
+        return r;
+    if (int r = ctx_set_device(rs->ctx))
+        return r;
+    return launch_repair(rs, d_msg, d_ecc, count, d_status);
+}
+
+int ffrs_rsi16_encode(ffrs_rsi16 *rs, const uint16_t *msg, size_t count, uint16_t *ecc, int vector_lanes)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (int r = check_count(count))
+        return r;
+    if (count == 0)
+        return FFRS_OK;
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    size_t mb = count * rs->k * 2, eb = count * rs->E * 2;
+    void *d_m, *d_e;
+    if (int r = stage_reserve(ctx, 0, mb, &d_m))
+        return r;
+    if (int r = stage_reserve(ctx, 1, eb, &d_e))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(d_m, msg, mb, hipMemcpyHostToDevice, ctx->stream));
+    if (int r = launch_encode(rs, (const uint16_t *)d_m, count, (uint16_t *)d_e, vector_lanes))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(ecc, d_e, eb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_repair(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, size_t count, uint8_t *status)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (int r = check_count(count))
+        return r;
+    if (count == 0)
+        return FFRS_OK;
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    size_t mb = count * rs->k * 2, eb = count * rs->E * 2;
+    void *d_m, *d_e, *d_s;
+    if (int r = stage_reserve(ctx, 0, mb, &d_m))
+        return r;
+    if (int r = stage_reserve(ctx, 1, eb, &d_e))
+        return r;
+    if (int r = stage_reserve(ctx, 2, count, &d_s))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(d_m, msg, mb, hipMemcpyHostToDevice, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(d_e, ecc, eb, hipMemcpyHostToDevice, ctx->stream));
+    if (int r = launch_repair(rs, (uint16_t *)d_m, (uint16_t *)d_e, count, (uint8_t *)d_s))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(msg, d_m, mb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(ecc, d_e, eb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(status, d_s, count, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_repair_known(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, const uint32_t *positions, size_t npos,
+                            uint8_t *status)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (npos == 0) {
+        *status = FFRS_RSI16_NO_ERRORS;
+        return FFRS_OK;
+    }
+    if (npos > rs->E)
+        return fail(FFRS_ERR_SIZE, "%zu error positions exceed ecc_len %u", npos, rs->E);
+    for (size_t i = 0; i < npos; ++i)
+        if (positions[i] >= rs->n)
+            return fail(FFRS_ERR_INVALID, "error position %u >= block_len %u", positions[i], rs->n);
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    size_t mb = rs->k * 2, eb = rs->E * 2;
+    void *d_m, *d_e, *d_p;
+    if (int r = stage_reserve(ctx, 0, mb, &d_m))
+        return r;
+    if (int r = stage_reserve(ctx, 1, eb + 64, &d_e))
+        return r;
+    if (int r = stage_reserve(ctx, 2, npos * 4 + 64, &d_p))
+        return r;
+    uint8_t *d_s = (uint8_t *)d_e + eb;
+    FFRS_HIP(hipMemcpyAsync(d_m, msg, mb, hipMemcpyHostToDevice, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(d_e, ecc, eb, hipMemcpyHostToDevice, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(d_p, positions, npos * 4, hipMemcpyHostToDevice, ctx->stream));
+    uint32_t g = group_size(rs->E);
+    size_t lds = (size_t)7 * rs->E * 4;
+    if (int r = allow_lds(rsi16_repair_known_kernel, lds))
+        return r;
+    hipLaunchKernelGGL(rsi16_repair_known_kernel, dim3(1), dim3(64), lds, ctx->stream, (uint16_t *)d_m,
+                       (uint16_t *)d_e, (const uint32_t *)d_p, (uint32_t)npos, d_s, rs->dev, g);
+    FFRS_HIP(hipGetLastError());
+    FFRS_HIP(hipMemcpyAsync(msg, d_m, mb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(ecc, d_e, eb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(status, d_s, 1, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_synds(ffrs_rsi16 *rs, const uint16_t *msg, const uint16_t *ecc, size_t count, uint32_t *synds)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (int r = check_count(count))
+        return r;
+    if (count == 0)
+        return FFRS_OK;
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    size_t mb = count * rs->k * 2, eb = count * rs->E * 2, ob = count * rs->E * 4;
+    void *d_m, *d_e, *d_o;
+    if (int r = stage_reserve(ctx, 0, mb, &d_m))
+        return r;
+    if (int r = stage_reserve(ctx, 1, eb, &d_e))
+        return r;
+    if (int r = stage_reserve(ctx, 2, ob, &d_o))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(d_m, msg, mb, hipMemcpyHostToDevice, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(d_e, ecc, eb, hipMemcpyHostToDevice, ctx->stream));
+    uint32_t g = group_size(rs->E), cpw = 64 / g;
+    size_t lds = (size_t)cpw * 2 * rs->E * 4;
+    if (int r = allow_lds(rsi16_synds_kernel, lds))
+        return r;
+    hipLaunchKernelGGL(rsi16_synds_kernel, dim3((unsigned)((count + cpw - 1) / cpw)), dim3(64), lds, ctx->stream,
+                       (const uint16_t *)d_m, (const uint16_t *)d_e, (uint32_t)count, (uint32_t *)d_o, rs->dev, g);
+    FFRS_HIP(hipGetLastError());
+    FFRS_HIP(hipMemcpyAsync(synds, d_o, ob, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_sugiyama(ffrs_rsi16 *rs, const uint32_t *synds, size_t count, uint32_t *locator, uint32_t *evaluator,
+                        uint32_t *length)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (int r = check_count(count))
+        return r;
+    if (count == 0)
+        return FFRS_OK;
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    size_t sb = count * rs->E * 4;
+    void *d_s, *d_o, *d_l;
+    if (int r = stage_reserve(ctx, 0, sb, &d_s))
+        return r;
+    if (int r = stage_reserve(ctx, 1, 2 * sb, &d_o))
+        return r;
+    if (int r = stage_reserve(ctx, 2, count * 4, &d_l))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(d_s, synds, sb, hipMemcpyHostToDevice, ctx->stream));
+    uint32_t g = group_size(rs->E), cpw = 64 / g;
+    size_t lds = (size_t)cpw * 7 * rs->E * 4;
+    uint32_t *lo = (uint32_t *)d_o, *eo = lo + count * rs->E;
+    if (int r = allow_lds(rsi16_sugiyama_kernel, lds))
+        return r;
+    hipLaunchKernelGGL(rsi16_sugiyama_kernel, dim3((unsigned)((count + cpw - 1) / cpw)), dim3(64), lds, ctx->stream,
+                       (const uint32_t *)d_s, (uint32_t)count, lo, eo, (uint32_t *)d_l, rs->dev, g);
+    FFRS_HIP(hipGetLastError());
+    FFRS_HIP(hipMemcpyAsync(locator, lo, sb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(evaluator, eo, sb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(length, d_l, count * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_roots(ffrs_rsi16 *rs, const uint32_t *poly, uint8_t *is_root)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    void *d_p, *d_h;
+    if (int r = stage_reserve(ctx, 0, rs->E * 4, &d_p))
+        return r;
+    if (int r = stage_reserve(ctx, 1, rs->n, &d_h))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(d_p, poly, rs->E * 4, hipMemcpyHostToDevice, ctx->stream));
+    uint32_t g = group_size(rs->E);
+    hipLaunchKernelGGL(rsi16_roots_kernel, dim3(1), dim3(64), (size_t)rs->E * 4, ctx->stream, (const uint32_t *)d_p,
+                       (uint8_t *)d_h, rs->dev, g);
+    FFRS_HIP(hipGetLastError());
+    FFRS_HIP(hipMemcpyAsync(is_root, d_h, rs->n, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+}  // extern "C"

@@ -56,25 +56,39 @@ class Stream:
         w = self.u64((n + 3) // 4)
         return w.view(np.uint16)[:n].copy()
 
-    def distinct_positions(self, counts, n_positions, chunk=65536):
+    def distinct_positions(self, counts, n_positions):
         """For each row i, counts[i] distinct positions in [0, n_positions).
 
-        Returns an int16/int32 array [rows, max(counts)] padded with -1. Positions are
-        the indices of the counts[i] smallest keys of an independent key per position.
+        Returns an int32 array [rows, max(counts)] padded with -1. Positions are drawn
+        uniformly with replacement; every later duplicate within a row is redrawn until
+        the row holds distinct values (sampling without replacement).
         """
         counts = np.asarray(counts, np.int64)
         rows = counts.size
         cmax = int(counts.max()) if rows else 0
-        out = np.full((rows, max(cmax, 1)), -1, np.int32)
         if cmax == 0:
-            return out[:, :0]
-        for b in range(0, rows, chunk):
-            e = min(rows, b + chunk)
-            keys = self.u64((e - b) * n_positions).reshape(e - b, n_positions)
-            order = np.argsort(keys, axis=1, kind="stable")[:, :cmax].astype(np.int32)
-            mask = np.arange(cmax)[None, :] < counts[b:e, None]
-            out[b:e, :cmax] = np.where(mask, order, -1)
-        return out[:, :cmax]
+            return np.full((rows, 0), -1, np.int32)
+        if cmax > n_positions:
+            raise ValueError("more errors than positions")
+        valid = np.arange(cmax)[None, :] < counts[:, None]
+        pos = self.ints(0, n_positions - 1, rows * cmax).reshape(rows, cmax)
+        pos = np.where(valid, pos, -1 - np.arange(cmax)[None, :])  # padding never collides
+        todo = np.arange(rows)
+        while todo.size:
+            sub = pos[todo]
+            order = np.argsort(sub, axis=1, kind="stable")
+            srt = np.take_along_axis(sub, order, axis=1)
+            dup_sorted = np.zeros_like(srt, dtype=bool)
+            dup_sorted[:, 1:] = srt[:, 1:] == srt[:, :-1]
+            dup = np.zeros_like(dup_sorted)
+            np.put_along_axis(dup, order, dup_sorted, axis=1)
+            r, c = np.nonzero(dup)
+            if r.size == 0:
+                break
+            sub[r, c] = self.ints(0, n_positions - 1, r.size)
+            pos[todo] = sub
+            todo = todo[np.unique(r)]
+        return np.where(valid, pos, -1).astype(np.int32)
 
 
 def rs256_corrupt(cws, counts, stream):

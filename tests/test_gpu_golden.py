@@ -1,0 +1,44 @@
+"""The HIP path against the golden fixtures written from the reference's own code."""
+import pytest
+
+import cases as C
+import replay as R
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from backends import GpuBackend
+    return GpuBackend()
+
+
+@pytest.mark.parametrize("name", C.RS256_PLAIN_CASES)
+def test_rs256_plain(gpu, name):
+    R.check_rs256_plain(gpu, name)
+
+
+@pytest.mark.parametrize("shape", C.RS256_SHAPES, ids=lambda s: f"{s[0]}_{s[1]}")
+def test_rs256_shapes(gpu, shape):
+    R.check_rs256_shape(gpu, *shape)
+
+
+def test_rs256_mixed(gpu):
+    R.check_rs256_mixed(gpu)
+
+
+@pytest.mark.parametrize("name", C.RSI16_PLAIN_CASES)
+def test_rsi16_plain(gpu, name):
+    R.check_rsi16_plain(gpu, name)
+
+
+def test_rsi16_mixed(gpu):
+    R.check_rsi16_mixed(gpu)
+
+
+def test_rsi16_edges(gpu):
+    R.check_rsi16_edges(gpu)
+
+
+def test_rsi16_known(gpu):
+    R.check_rsi16_known(gpu)
