@@ -71,9 +71,10 @@ static bool gf256_build(unsigned primitive, unsigned polynomial, Gf256Host &gf)
     return true;
 }
 
-// Device-side field tables: EXP2[x] = alpha^(x mod 255) for x <= 508 and 0 above,
-// LOG2[a] = log_alpha(a) in [0, 254] with LOG2[0] = ZLOG (511). A product of logs
-// that involves a zero lands in the zero region of EXP2.
+// Device-side field tables: EXP2[x] = alpha^(x mod 255) for x <= 509 (the largest index
+// is log a + 255 - log b in a quotient) and 0 above; LOG2[a] = log_alpha(a) in [0, 254]
+// with LOG2[0] = ZLOG (511). A product of logs that involves a zero lands in the zero
+// region of EXP2.
 constexpr int ZLOG = 511;
 constexpr int EXP2_LEN = 1024;
 
@@ -246,7 +247,7 @@ struct GfLds {
 };
 
 // S_i = R(alpha^i) for i < E, where R(x) = sum_j r_j x^(E-1-j) is held in remainder
-// order (r_0 = highest). Horner over j per syndrome; EXP2 is zero above index 508, so a
+// order (r_0 = highest). Horner over j per syndrome; EXP2 is zero above index 509, so a
 // zero accumulator (LOG2 = ZLOG) contributes nothing.
 template <int EW>
 __device__ inline void syndromes_from_remainder(const uint32_t (&rem)[EW], uint32_t E, const GfLds &gf,
@@ -692,7 +693,7 @@ int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsig
     uint8_t exp2[EXP2_LEN];
     uint16_t log2[256];
     for (int x = 0; x < EXP2_LEN; ++x)
-        exp2[x] = x <= 508 ? rs->gf.exp_t[x % 255] : 0;
+        exp2[x] = x <= 509 ? rs->gf.exp_t[x % 255] : 0;
     log2[0] = ZLOG;
     for (int a = 1; a < 256; ++a)
         log2[a] = rs->gf.log_t[a] % 255;
