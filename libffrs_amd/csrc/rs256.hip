@@ -81,20 +81,24 @@ constexpr int EXP2_LEN = 1024;
 // ---------------------------------------------------------------- device code
 
 struct Rs256Tables {
-    const uint32_t *lut;     // S * 256 * EW dwords, slice-major
+    const uint32_t *lut;     // S * 256 rows of lut_row(EW) dwords, slice-major
     const uint8_t *exp2;     // EXP2_LEN
     const uint16_t *log2;    // 256
 };
+
+// LUT row stride in dwords: rows of 32 or 64 bytes are padded by 16 bytes so that random
+// row reads start at every 16-byte bank group of the LDS instead of every other one.
+__host__ __device__ constexpr int lut_row(int EW) { return (EW % 8 == 0) ? EW + 4 : EW; }
 
 template <int EW, int S>
 __device__ inline void lut_step(uint32_t (&rem)[EW], uint32_t w, const uint32_t *__restrict__ s_lut)
 {
     if constexpr (S == 4) {
         uint32_t x = rem[0] ^ w;
-        const uint32_t *t3 = s_lut + (3 * 256 + (x & 0xff)) * EW;
-        const uint32_t *t2 = s_lut + (2 * 256 + ((x >> 8) & 0xff)) * EW;
-        const uint32_t *t1 = s_lut + (1 * 256 + ((x >> 16) & 0xff)) * EW;
-        const uint32_t *t0 = s_lut + (0 * 256 + (x >> 24)) * EW;
+        const uint32_t *t3 = s_lut + (3 * 256 + (x & 0xff)) * lut_row(EW);
+        const uint32_t *t2 = s_lut + (2 * 256 + ((x >> 8) & 0xff)) * lut_row(EW);
+        const uint32_t *t1 = s_lut + (1 * 256 + ((x >> 16) & 0xff)) * lut_row(EW);
+        const uint32_t *t0 = s_lut + (0 * 256 + (x >> 24)) * lut_row(EW);
         if constexpr (EW % 4 == 0) {
 #pragma unroll
             for (int d = 0; d < EW; d += 4) {
@@ -122,8 +126,8 @@ __device__ inline void lut_step(uint32_t (&rem)[EW], uint32_t w, const uint32_t 
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             uint32_t x = rem[0] ^ (w >> (16 * h));
-            const uint32_t *t1 = s_lut + (256 + (x & 0xff)) * EW;
-            const uint32_t *t0 = s_lut + ((x >> 8) & 0xff) * EW;
+            const uint32_t *t1 = s_lut + (256 + (x & 0xff)) * lut_row(EW);
+            const uint32_t *t0 = s_lut + ((x >> 8) & 0xff) * lut_row(EW);
 #pragma unroll
             for (int d = 0; d < EW; ++d) {
                 uint32_t nx = (d + 1 < EW) ? rem[(d + 1) % EW] : 0u;
@@ -134,7 +138,7 @@ __device__ inline void lut_step(uint32_t (&rem)[EW], uint32_t w, const uint32_t 
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
             uint32_t f = (rem[0] ^ (w >> (8 * j))) & 0xff;
-            const uint32_t *t0 = s_lut + f * EW;
+            const uint32_t *t0 = s_lut + f * lut_row(EW);
 #pragma unroll
             for (int d = 0; d < EW; ++d) {
                 uint32_t nx = (d + 1 < EW) ? rem[(d + 1) % EW] : 0u;
@@ -204,7 +208,7 @@ __global__ __launch_bounds__(256) void rs256_encode_kernel(const uint8_t *__rest
                                                             uint32_t ecc, const uint32_t *__restrict__ lut)
 {
     extern __shared__ uint32_t s_lut[];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * EW); i += blockDim.x)
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * lut_row(EW)); i += blockDim.x)
         s_lut[i] = lut[i];
     __syncthreads();
 
@@ -312,10 +316,10 @@ __global__ __launch_bounds__(WG) void rs256_repair_kernel(uint8_t *__restrict__ 
 {
     extern __shared__ uint32_t s_mem[];
     uint32_t *s_lut = s_mem;
-    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem + S * 256 * EW);
+    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem + S * 256 * lut_row(EW));
     uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
     uint8_t *s_rows = reinterpret_cast<uint8_t *>(s_log2 + 256);
-    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * EW); i += blockDim.x)
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * lut_row(EW)); i += blockDim.x)
         s_lut[i] = tab.lut[i];
     for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
         s_exp2[i] = tab.exp2[i];
@@ -454,9 +458,9 @@ __global__ __launch_bounds__(256) void rs256_synds_kernel(const uint8_t *__restr
 {
     extern __shared__ uint32_t s_mem[];
     uint32_t *s_lut = s_mem;
-    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem + S * 256 * EW);
+    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem + S * 256 * lut_row(EW));
     uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
-    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * EW); i += blockDim.x)
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * lut_row(EW)); i += blockDim.x)
         s_lut[i] = tab.lut[i];
     for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
         s_exp2[i] = tab.exp2[i];
@@ -470,6 +474,443 @@ __global__ __launch_bounds__(256) void rs256_synds_kernel(const uint8_t *__restr
         message_remainder<EW, S>(rsrc, (int64_t)cw * k, k, s_lut, rem);
         xor_parity<EW>(rem, eccs + (size_t)cw * ecc, ecc);
         syndromes_from_remainder<EW>(rem, ecc, gf, synds + (size_t)cw * ecc);
+    }
+}
+
+// ======================================================================
+// Repair pipeline for ecc <= 32 (Lambda has at most 16 roots):
+//   rs256_synd_kernel   lane per codeword: remainder + XOR parity; zero -> status 1;
+//                       else syndromes from nibble tables ST[j][h][v] (E bytes each:
+//                       (v << 4h) alpha^(i (E-1-j)), i < E) and status PENDING
+//   rs256_bm_kernel     lane per codeword: Berlekamp-Massey; L > E/2 -> status 0; else
+//                       a record {L, Lambda_0..L, Omega_0..L-1} (Omega = S Lambda mod x^E
+//                       has degree < L once the LFSR of length L generates all E syndromes)
+//   rs256_chien_kernel  Chien search one codeword per wave (4 positions per lane, nibble
+//                       tables CT[i][h][v] of 256 position bytes addressed by wave-uniform
+//                       coefficients), then Forney one codeword per lane.
+// ======================================================================
+
+constexpr uint8_t RS_PENDING = 0xFE;
+constexpr int CHIEN_LMAX = 16;
+constexpr uint32_t CHIEN_TABLE_BYTES = CHIEN_LMAX * 2 * 16 * 256;
+
+// BM record of one codeword (ecc <= 32): byte 0 L, bytes 4..19 Omega_0..15,
+// bytes 20..36 Lambda_0..16, zero-filled, 48 bytes.
+__host__ __device__ inline uint32_t rec_lam_off() { return 20; }
+__host__ __device__ inline uint32_t rec_om_off(uint32_t) { return 4; }
+__host__ __device__ inline uint32_t rec_stride(uint32_t) { return 48; }
+
+template <int EW, int S>
+__global__ __launch_bounds__(256) void rs256_synd_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
+                                                          const uint8_t *__restrict__ eccs, uint32_t count,
+                                                          uint32_t k, uint32_t E, uint8_t *__restrict__ status,
+                                                          uint32_t *__restrict__ synd, const uint32_t *__restrict__ lut,
+                                                          const uint32_t *__restrict__ stab)
+{
+    extern __shared__ uint32_t s_mem[];
+    uint32_t *s_lut = s_mem;
+    uint32_t *s_st = s_mem + S * 256 * lut_row(EW);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * lut_row(EW)); i += blockDim.x)
+        s_lut[i] = lut[i];
+    for (uint32_t i = threadIdx.x; i < E * 32 * EW; i += blockDim.x)
+        s_st[i] = stab[i];
+    __syncthreads();
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += gridDim.x * blockDim.x) {
+        uint32_t rem[EW];
+        message_remainder<EW, S>(rsrc, (int64_t)cw * k, k, s_lut, rem);
+        if (xor_parity<EW>(rem, eccs + (size_t)cw * E, E) == 0) {
+            status[cw] = 1;
+            continue;
+        }
+        uint32_t acc[EW];
+#pragma unroll
+        for (int d = 0; d < EW; ++d)
+            acc[d] = 0;
+#pragma unroll
+        for (int j = 0; j < EW * 4; ++j) {
+            if ((uint32_t)j < E) {
+                uint32_t r = (rem[j >> 2] >> (8 * (j & 3))) & 0xffu;
+                const uint32_t *lo = s_st + ((j * 2 + 0) * 16 + (r & 15)) * EW;
+                const uint32_t *hi = s_st + ((j * 2 + 1) * 16 + (r >> 4)) * EW;
+                if constexpr (EW % 4 == 0) {
+#pragma unroll
+                    for (int d = 0; d < EW; d += 4) {
+                        uint4 a = *reinterpret_cast<const uint4 *>(lo + d);
+                        uint4 b = *reinterpret_cast<const uint4 *>(hi + d);
+                        acc[d] ^= a.x ^ b.x;
+                        acc[d + 1] ^= a.y ^ b.y;
+                        acc[d + 2] ^= a.z ^ b.z;
+                        acc[d + 3] ^= a.w ^ b.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int d = 0; d < EW; ++d)
+                        acc[d] ^= lo[d] ^ hi[d];
+                }
+            }
+        }
+        uint32_t *o = synd + (size_t)cw * EW;
+#pragma unroll
+        for (int d = 0; d < EW; ++d)
+            o[d] = acc[d];
+        status[cw] = RS_PENDING;
+    }
+}
+
+// Per-lane LDS row of the BM kernel: S[E], logS[E] (u16), Lambda[E], B[E].
+__host__ __device__ inline uint32_t bm_row_stride(uint32_t E)
+{
+    uint32_t dw = (E + 2 * E + E + E + 3) / 4;
+    return ((dw & 1) ? dw : dw + 1) * 4;
+}
+
+template <int WG>
+__global__ __launch_bounds__(WG) void rs256_bm_kernel(uint32_t count, uint32_t E, uint32_t sstride_words,
+                                                       uint8_t *__restrict__ status, const uint32_t *__restrict__ synd,
+                                                       uint8_t *__restrict__ rec, Rs256Tables tab)
+{
+    extern __shared__ uint32_t s_mem[];
+    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem);
+    uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
+    uint8_t *s_rows = reinterpret_cast<uint8_t *>(s_log2 + 256);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
+        s_exp2[i] = tab.exp2[i];
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
+        s_log2[i] = tab.log2[i];
+    __syncthreads();
+    GfLds gf{s_exp2, s_log2};
+    uint8_t *row = s_rows + threadIdx.x * bm_row_stride(E);
+    uint8_t *Sy = row;
+    uint16_t *lS = reinterpret_cast<uint16_t *>(row + ((E + 1) & ~1u));
+    uint8_t *lam = row + ((E + 1) & ~1u) + 2 * E;
+    uint8_t *Bx = lam + E;
+    const uint32_t RS = rec_stride(E), LO = rec_lam_off(), OO = rec_om_off(E);
+
+    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += gridDim.x * blockDim.x) {
+        if (status[cw] != RS_PENDING)
+            continue;
+        const uint32_t *sw = synd + (size_t)cw * sstride_words;
+        for (uint32_t w = 0; w < (E + 3) / 4; ++w) {
+            uint32_t v = sw[w];
+            for (uint32_t b = 0; b < 4 && w * 4 + b < E; ++b) {
+                uint32_t sb = (v >> (8 * b)) & 0xffu;
+                Sy[w * 4 + b] = (uint8_t)sb;
+                lS[w * 4 + b] = gf.log2[sb];
+            }
+        }
+        for (uint32_t i = 0; i < E; ++i) {
+            lam[i] = 0;
+            Bx[i] = 0;
+        }
+        lam[0] = 1;
+        Bx[0] = 1;
+        // reed_solomon.hpp:655-702, natural order; lenL / lenB bound the non-zero terms
+        uint32_t L = 0, m = 1, b = 1, lenL = 1, lenB = 1;
+        for (uint32_t nn = 0; nn < E; ++nn) {
+            uint32_t d = Sy[nn];
+            uint32_t top = L < lenL - 1 ? L : lenL - 1;
+            for (uint32_t i = 1; i <= top; ++i)
+                d ^= gf.exp2[gf.log2[lam[i]] + lS[nn - i]];
+            if (d == 0) {
+                m += 1;
+                continue;
+            }
+            uint32_t lc = gf.log2[gf.div(d, b)];
+            bool grow = 2 * L <= nn;
+            uint32_t hiB = m + lenB < E ? m + lenB : E;  // x^m B truncated to degrees < E
+            uint32_t hi = grow ? (hiB > lenL ? hiB : lenL) : hiB;
+            for (uint32_t ii = hi; ii-- > 0;) {
+                uint32_t old = lam[ii];
+                uint32_t sb = (ii >= m && ii < hiB) ? (uint32_t)Bx[ii - m] : 0u;
+                lam[ii] = (uint8_t)(old ^ gf.mul_log(lc, sb));
+                if (grow)
+                    Bx[ii] = (uint8_t)old;
+            }
+            if (grow) {
+                lenB = lenL;
+                L = nn + 1 - L;
+                b = d;
+                m = 1;
+            } else {
+                m += 1;
+            }
+            if (hiB > lenL)
+                lenL = hiB;
+        }
+        if (L > E / 2) {
+            status[cw] = 0;
+            continue;
+        }
+        uint8_t *r = rec + (size_t)cw * RS;
+        for (uint32_t i = 0; i < RS; ++i)
+            r[i] = 0;
+        r[0] = (uint8_t)L;
+        for (uint32_t i = 0; i <= E / 2; ++i)
+            r[LO + i] = i <= L ? lam[i] : 0;
+        // Omega_k = sum_{i <= k} Lambda_i S_{k-i}, k < L (reed_solomon.hpp:708-728)
+        for (uint32_t kk = 0; kk < E / 2; ++kk) {
+            uint32_t v = 0;
+            if (kk < L)
+                for (uint32_t i = 0; i <= kk; ++i)
+                    v ^= gf.exp2[gf.log2[lam[i]] + lS[kk - i]];
+            r[OO + kk] = (uint8_t)v;
+        }
+    }
+}
+
+// Berlekamp-Massey with every polynomial in registers, fully unrolled for a fixed E.
+// B is kept pre-shifted as x^m B (its logs only), so an update is Lambda_i ^= c B'_i and
+// every iteration shifts B' by one: B' <- x Lambda_old when L grows, else x B'. At
+// iteration n deg Lambda <= n + 1, which bounds the unrolled work. Same results as the
+// reference's reversed-storage version (reed_solomon.hpp:655-702), whose truncation of
+// degrees >= E never matters while L <= E / 2.
+template <int E, int WG>
+__global__ __launch_bounds__(WG) void rs256_bm_fixed_kernel(uint32_t count, uint8_t *__restrict__ status,
+                                                             const uint32_t *__restrict__ synd,
+                                                             uint8_t *__restrict__ rec, Rs256Tables tab)
+{
+    constexpr int EW = (E + 3) / 4;
+    extern __shared__ uint32_t s_mem[];
+    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem);
+    uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
+        s_exp2[i] = tab.exp2[i];
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
+        s_log2[i] = tab.log2[i];
+    __syncthreads();
+    const uint32_t RS = rec_stride(E), LO = rec_lam_off(), OO = rec_om_off(E);
+
+    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += gridDim.x * blockDim.x) {
+        if (status[cw] != RS_PENDING)
+            continue;
+        uint32_t S[E], lS[E];
+        const uint32_t *sw = synd + (size_t)cw * EW;
+#pragma unroll
+        for (int w = 0; w < EW; ++w) {
+            uint32_t v = sw[w];
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                if (w * 4 + b < E) {
+                    S[w * 4 + b] = (v >> (8 * b)) & 0xffu;
+                    lS[w * 4 + b] = s_log2[S[w * 4 + b]];
+                }
+        }
+        uint32_t lam[E], llam[E], lBp[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i) {
+            lam[i] = i == 0 ? 1u : 0u;
+            llam[i] = i == 0 ? 0u : (uint32_t)ZLOG;
+            lBp[i] = i == 1 ? 0u : (uint32_t)ZLOG;
+        }
+        uint32_t L = 0, lb = 0;
+#pragma unroll
+        for (int n = 0; n < E; ++n) {
+            uint32_t d = S[n];
+#pragma unroll
+            for (int i = 1; i <= n; ++i)
+                d ^= s_exp2[llam[i] + lS[n - i]];
+            bool nz = d != 0;
+            bool grow = nz && (2 * L <= (uint32_t)n);
+            uint32_t ld = s_log2[d];
+            uint32_t lc = ld + 255 - lb;
+            lc = lc >= 255 ? lc - 255 : lc;
+            const int hi = (n + 1 < E - 1) ? n + 1 : E - 1;
+            uint32_t old_llam[E];
+#pragma unroll
+            for (int i = 0; i < E; ++i)
+                old_llam[i] = llam[i];
+#pragma unroll
+            for (int i = 1; i <= hi; ++i) {
+                uint32_t t = nz ? (uint32_t)s_exp2[lc + lBp[i]] : 0u;
+                lam[i] ^= t;
+                llam[i] = s_log2[lam[i]];
+            }
+            const int bh = (n + 2 < E - 1) ? n + 2 : E - 1;
+#pragma unroll
+            for (int i = bh; i >= 1; --i)
+                lBp[i] = grow ? old_llam[i - 1] : lBp[i - 1];
+            lBp[0] = ZLOG;
+            if (grow) {
+                L = (uint32_t)n + 1 - L;
+                lb = ld;
+            }
+        }
+        if (L > (uint32_t)E / 2) {
+            status[cw] = 0;
+            continue;
+        }
+        uint8_t *r = rec + (size_t)cw * RS;
+        uint32_t words[64];
+#pragma unroll
+        for (int w = 0; w < 64; ++w)
+            words[w] = 0;
+        words[0] = L;
+#pragma unroll
+        for (int i = 0; i <= E / 2; ++i)
+            words[(LO + i) >> 2] |= (i <= (int)E ? lam[i < E ? i : 0] : 0u) << (8 * ((LO + i) & 3));
+#pragma unroll
+        for (int kk = 0; kk < E / 2; ++kk) {
+            uint32_t v = 0;
+#pragma unroll
+            for (int i = 0; i <= kk; ++i)
+                v ^= s_exp2[llam[i] + lS[kk - i]];
+            v = (uint32_t)kk < L ? v : 0u;
+            words[(OO + kk) >> 2] |= v << (8 * ((OO + kk) & 3));
+        }
+        uint32_t *rw = reinterpret_cast<uint32_t *>(r);
+#pragma unroll
+        for (int w = 0; w < 64; ++w)
+            if ((uint32_t)w * 4 < RS)
+                rw[w] = words[w];
+    }
+}
+
+// XOR a byte in global memory without reading it: an atomic XOR on the aligned dword
+// that holds it (the other three bytes get 0), issued without waiting for a result.
+__device__ inline void xor_byte(uint8_t *p, uint32_t v)
+{
+    uintptr_t a = reinterpret_cast<uintptr_t>(p);
+    atomicXor(reinterpret_cast<unsigned int *>(a & ~(uintptr_t)3), v << (8 * (a & 3)));
+}
+
+// Chien search and Forney, one codeword per wave (reed_solomon.hpp:516-538, 704-743).
+// Lane l holds positions 4l..4l+3 as the bytes of one dword; with the nibble tables
+// CT[i][h][v] (256 position bytes (v << 4h) alpha^(-i p)) a wave evaluates, for all
+// positions at once, Lambda_even(X^-1), Lambda_odd(X^-1) and Omega(X^-1), X = alpha^p.
+// Every table address is wave-uniform, so the LDS reads are conflict-free. At a root,
+// Lambda'(X^-1) = X Lambda_odd(X^-1) (characteristic 2), hence the reference's
+// e = X div(Omega, Lambda') equals div(Omega, Lambda_odd), or X Omega when Lambda_odd = 0
+// (its div(a, 0) = a); the lane holding the root applies the correction.
+template <int WG>
+__global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ msgs, uint8_t *__restrict__ eccs,
+                                                          uint32_t count, uint32_t k, uint32_t E,
+                                                          uint8_t *__restrict__ status, const uint8_t *__restrict__ rec,
+                                                          const uint32_t *__restrict__ ctab, Rs256Tables tab)
+{
+    extern __shared__ uint32_t s_mem[];
+    uint32_t *s_ct = s_mem;  // [CHIEN_LMAX][2][16][64] dwords
+    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem + CHIEN_TABLE_BYTES / 4);
+    uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
+    for (uint32_t i = threadIdx.x; i < CHIEN_TABLE_BYTES / 4; i += blockDim.x)
+        s_ct[i] = ctab[i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
+        s_exp2[i] = tab.exp2[i];
+    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
+        s_log2[i] = tab.log2[i];
+    __syncthreads();
+    GfLds gf{s_exp2, s_log2};
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loads below
+    const uint32_t n = k + E, RS = rec_stride(E), OO = rec_om_off(E);
+    const uint32_t *rec32 = reinterpret_cast<const uint32_t *>(rec);
+    const uint32_t RSW = RS / 4, OW = OO / 4;
+    for (uint32_t base = blockIdx.x * WG + wave * 64; base < count; base += gridDim.x * WG) {
+        uint32_t cw = base + lane;
+        bool pend = cw < count && status[cw] == RS_PENDING;
+        uint64_t pend_mask = __ballot(pend);
+        uint64_t ok_mask = 0;
+        // this tile's 64 records, one per lane (coalesced), read back per codeword by readlane
+        uint32_t recw[10];
+        {
+            const uint4 *r4 = reinterpret_cast<const uint4 *>(rec32 + (size_t)(cw < count ? cw : 0) * RSW);
+            uint4 a = r4[0], b = r4[1], c = r4[2];
+            recw[0] = a.x; recw[1] = a.y; recw[2] = a.z; recw[3] = a.w;
+            recw[4] = b.x; recw[5] = b.y; recw[6] = b.z; recw[7] = b.w;
+            recw[8] = c.x; recw[9] = c.y;
+        }
+        // Evaluate Lambda_even and Lambda_odd at the 4 positions of each lane for codeword o.
+        auto evaluate = [&](uint32_t o, uint32_t &Lo, uint32_t &v_even, uint32_t &v_odd) {
+            Lo = __builtin_amdgcn_readlane(recw[0], o) & 0xffu;
+            uint32_t lamw[5];
+#pragma unroll
+            for (int w = 0; w < 5; ++w)
+                lamw[w] = __builtin_amdgcn_readlane(recw[5 + w], o);  // Lambda_0..Lambda_19
+            v_even = 0x01010101u;
+            v_odd = 0;
+            for (uint32_t i0 = 1; i0 <= Lo; i0 += 4) {
+                uint32_t ae = 0, ao = 0;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    uint32_t i = i0 + j;
+                    uint32_t li = (lamw[i >> 2] >> (8 * (i & 3))) & 0xffu;
+                    li = i <= Lo ? li : 0u;
+                    const uint32_t *row = s_ct + ((i - 1) & 15) * 32 * 64;
+                    uint32_t t = row[(li & 15) * 64 + lane] ^ row[(16 + (li >> 4)) * 64 + lane];
+                    if (i & 1)
+                        ao ^= t;
+                    else
+                        ae ^= t;
+                }
+                v_even ^= ae;
+                v_odd ^= ao;
+            }
+        };
+        // Count the roots; when they are exactly L, the lanes holding roots evaluate
+        // Omega(X^-1) = sum_k Omega_k alpha^(-p k) (independent log-domain terms) and
+        // correct their positions.
+        auto settle = [&](uint32_t o, uint32_t Lo, uint32_t v_even, uint32_t v_odd) {
+            uint32_t val = v_even ^ v_odd;
+            uint32_t cnt = 0, zq = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                bool z = ((val >> (8 * q)) & 0xffu) == 0 && lane * 4 + q < n;
+                zq |= (uint32_t)z << q;
+                cnt += (uint32_t)__builtin_popcountll(__ballot(z));
+            }
+            if (cnt != Lo)
+                return;
+            ok_mask |= 1ull << o;
+            uint32_t cwo = base + o;
+            uint32_t lom[CHIEN_LMAX];
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                uint32_t ow = __builtin_amdgcn_readlane(recw[1 + w], o);  // Omega_{4w}..Omega_{4w+3}
+#pragma unroll
+                for (int b = 0; b < 4; ++b)
+                    lom[4 * w + b] = gf.log2[(ow >> (8 * b)) & 0xffu];
+            }
+            if (zq) {
+                for (uint32_t q = 0; q < 4; ++q) {
+                    if (!((zq >> q) & 1u))
+                        continue;
+                    uint32_t p = lane * 4 + q;
+                    uint32_t lxi = p ? 255 - p : 0;
+                    uint32_t num = 0, e = 0;
+#pragma unroll
+                    for (int kk = 0; kk < CHIEN_LMAX; ++kk) {
+                        num ^= gf.exp2[lom[kk] + e];
+                        e += lxi;
+                        e = e >= 255 ? e - 255 : e;
+                    }
+                    uint32_t dod = (v_odd >> (8 * q)) & 0xffu;
+                    uint32_t ln = gf.log2[num], ld = gf.log2[dod];
+                    uint32_t mag = num == 0 ? 0u : (dod ? (uint32_t)gf.exp2[ln + 255 - ld] : (uint32_t)gf.exp2[p + ln]);
+                    uint32_t at = n - 1 - p;
+                    uint8_t *dst = at < k ? msgs + (size_t)cwo * k + at : eccs + (size_t)cwo * E + (at - k);
+                    xor_byte(dst, mag);
+                }
+            }
+        };
+        while (pend_mask) {
+            uint32_t o1 = (uint32_t)__builtin_ctzll(pend_mask);
+            pend_mask &= pend_mask - 1;
+            uint32_t L1, e1, d1;
+            if (pend_mask) {
+                uint32_t o2 = (uint32_t)__builtin_ctzll(pend_mask);
+                pend_mask &= pend_mask - 1;
+                uint32_t L2, e2, d2;
+                evaluate(o1, L1, e1, d1);
+                evaluate(o2, L2, e2, d2);
+                settle(o1, L1, e1, d1);
+                settle(o2, L2, e2, d2);
+            } else {
+                evaluate(o1, L1, e1, d1);
+                settle(o1, L1, e1, d1);
+            }
+        }
+        if (pend)
+            status[cw] = ((ok_mask >> lane) & 1ull) ? 1 : 0;
     }
 }
 
@@ -488,6 +929,10 @@ struct ffrs_rs256 {
     uint32_t *d_lut = nullptr;
     uint8_t *d_exp2 = nullptr;
     uint16_t *d_log2 = nullptr;
+    uint32_t *d_stab = nullptr;  // syndrome nibble tables (ecc <= 32)
+    uint32_t *d_ctab = nullptr;  // Chien nibble tables (ecc <= 32)
+    void *d_work = nullptr;      // syndromes + BM records of the last repair
+    size_t work_size = 0;
 };
 
 namespace {
@@ -518,14 +963,14 @@ void lfsr_zero_step(const ffrs_rs256 &rs, uint8_t *r)
 std::vector<uint32_t> build_lut(const ffrs_rs256 &rs)
 {
     unsigned E = rs.ecc, EW = rs.EW, S = rs.S;
-    std::vector<uint32_t> lut((size_t)S * 256 * EW, 0);
+    std::vector<uint32_t> lut((size_t)S * 256 * lut_row((int)EW), 0);
     for (unsigned b = 0; b < 256; ++b) {
         uint8_t r[256] = {0};
         // b * x^E mod g: one LFSR step with feedback b
         r[0] = (uint8_t)b;
         lfsr_zero_step(rs, r);
         for (unsigned s = 0; s < S; ++s) {
-            uint32_t *row = &lut[((size_t)s * 256 + b) * EW];
+            uint32_t *row = &lut[((size_t)s * 256 + b) * lut_row((int)EW)];
             for (unsigned j = 0; j < E; ++j)
                 row[j / 4] |= (uint32_t)r[j] << (8 * (j % 4));
             lfsr_zero_step(rs, r);
@@ -538,9 +983,9 @@ template <int EW, int S>
 int launch_encode(ffrs_rs256 *rs, const uint8_t *d_msg, size_t count, uint32_t k, uint8_t *d_ecc)
 {
     ffrs_ctx *ctx = rs->ctx;
-    size_t lds = (size_t)S * 256 * EW * 4;
+    size_t lds = (size_t)S * 256 * lut_row(EW) * 4;
     size_t tiles = (count + 255) / 256;
-    int wgs_per_cu = lds <= 16384 ? 8 : (lds <= 32768 ? 4 : 2);
+    int wgs_per_cu = (int)std::min<size_t>(8, std::max<size_t>(1, (160 * 1024) / lds));
     int grid = persistent_grid(ctx, wgs_per_cu, tiles);
     if (int r = allow_lds(rs256_encode_kernel<EW, S>, lds))
         return r;
@@ -554,7 +999,7 @@ template <int EW, int S, int WG>
 int launch_repair(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_t count, uint32_t k, uint8_t *d_status)
 {
     ffrs_ctx *ctx = rs->ctx;
-    size_t lds = (size_t)S * 256 * EW * 4 + EXP2_LEN + 512 + (size_t)WG * repair_row_stride(rs->ecc);
+    size_t lds = (size_t)S * 256 * lut_row(EW) * 4 + EXP2_LEN + 512 + (size_t)WG * repair_row_stride(rs->ecc);
     if (lds > 160 * 1024)
         return fail(FFRS_ERR_INVALID, "repair workspace of %zu bytes exceeds LDS", lds);
     int per_cu = (int)std::max<size_t>(1, (160 * 1024) / lds);
@@ -573,13 +1018,115 @@ int launch_synds(ffrs_rs256 *rs, const uint8_t *d_msg, const uint8_t *d_ecc, siz
                  uint8_t *d_out)
 {
     ffrs_ctx *ctx = rs->ctx;
-    size_t lds = (size_t)S * 256 * EW * 4 + EXP2_LEN + 512;
+    size_t lds = (size_t)S * 256 * lut_row(EW) * 4 + EXP2_LEN + 512;
     int grid = persistent_grid(ctx, 2, (count + 255) / 256);
     Rs256Tables tab{rs->d_lut, rs->d_exp2, rs->d_log2};
     if (int r = allow_lds(rs256_synds_kernel<EW, S>, lds))
         return r;
     hipLaunchKernelGGL((rs256_synds_kernel<EW, S>), dim3(grid), dim3(256), lds, ctx->stream, d_msg,
                        (uint32_t)(count * k), d_ecc, (uint32_t)count, k, rs->ecc, d_out, tab);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+// ST[j][h][v]: E bytes (v << 4h) * alpha^(i (E-1-j)) for i < E, packed in EW dwords.
+std::vector<uint32_t> build_synd_tables(const ffrs_rs256 &rs)
+{
+    unsigned E = rs.ecc, EW = rs.EW;
+    std::vector<uint32_t> t((size_t)E * 32 * EW, 0);
+    for (unsigned j = 0; j < E; ++j)
+        for (unsigned h = 0; h < 2; ++h)
+            for (unsigned v = 0; v < 16; ++v) {
+                uint32_t *row = &t[(((size_t)j * 2 + h) * 16 + v) * EW];
+                uint8_t c = (uint8_t)(v << (4 * h));
+                for (unsigned i = 0; i < E; ++i) {
+                    uint8_t a = rs.gf.exp_t[(i * (E - 1 - j)) % 255];
+                    row[i / 4] |= (uint32_t)rs.gf.mul(c, a) << (8 * (i % 4));
+                }
+            }
+    return t;
+}
+
+// CT[i-1][h][v]: 256 position bytes (v << 4h) * alpha^(-i p) for p < 255 (byte 255 unused).
+std::vector<uint32_t> build_chien_tables(const ffrs_rs256 &rs)
+{
+    std::vector<uint32_t> t(CHIEN_TABLE_BYTES / 4, 0);
+    for (unsigned i = 1; i <= (unsigned)CHIEN_LMAX; ++i)
+        for (unsigned h = 0; h < 2; ++h)
+            for (unsigned v = 0; v < 16; ++v) {
+                uint32_t *row = &t[(((size_t)(i - 1) * 2 + h) * 16 + v) * 64];
+                uint8_t c = (uint8_t)(v << (4 * h));
+                for (unsigned p = 0; p < 255; ++p) {
+                    uint8_t a = rs.gf.exp_t[(255 - (i * p) % 255) % 255];
+                    row[p / 4] |= (uint32_t)rs.gf.mul(c, a) << (8 * (p % 4));
+                }
+            }
+    return t;
+}
+
+int reserve_work(ffrs_rs256 *rs, size_t bytes)
+{
+    if (rs->work_size >= bytes)
+        return FFRS_OK;
+    if (rs->d_work)
+        FFRS_HIP(hipFree(rs->d_work));
+    rs->d_work = nullptr;
+    rs->work_size = 0;
+    size_t sz = bytes + bytes / 4;
+    hipError_t e = hipMalloc(&rs->d_work, sz);
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_NOMEM, "repair workspace of %zu bytes: %s", sz, hipGetErrorString(e));
+    rs->work_size = sz;
+    return FFRS_OK;
+}
+
+template <int EW, int S>
+int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_t count, uint32_t k,
+                           uint8_t *d_status)
+{
+    ffrs_ctx *ctx = rs->ctx;
+    const uint32_t E = rs->ecc;
+    size_t synd_bytes = count * EW * 4;
+    size_t rec_bytes = count * rec_stride(E);
+    if (int r = reserve_work(rs, synd_bytes + rec_bytes + 256))
+        return r;
+    uint32_t *d_synd = (uint32_t *)rs->d_work;
+    uint8_t *d_rec = (uint8_t *)rs->d_work + ((synd_bytes + 255) & ~(size_t)255);
+    Rs256Tables tab{rs->d_lut, rs->d_exp2, rs->d_log2};
+
+    size_t lds1 = (size_t)S * 256 * lut_row(EW) * 4 + (size_t)E * 32 * EW * 4;
+    if (int r = allow_lds(rs256_synd_kernel<EW, S>, lds1))
+        return r;
+    int g1 = persistent_grid(ctx, (int)std::max<size_t>(1, (160 * 1024) / lds1), (count + 255) / 256);
+    hipLaunchKernelGGL((rs256_synd_kernel<EW, S>), dim3(g1), dim3(256), lds1, ctx->stream, d_msg,
+                       (uint32_t)(count * k), d_ecc, (uint32_t)count, k, E, d_status, d_synd, rs->d_lut, rs->d_stab);
+    FFRS_HIP(hipGetLastError());
+
+    if (E == 2 || E == 4 || E == 8 || E == 16 || E == 32) {
+        size_t lds2 = EXP2_LEN + 512;
+        int g2 = persistent_grid(ctx, 4, (count + 255) / 256);
+        auto k2 = E == 2 ? rs256_bm_fixed_kernel<2, 256> : E == 4 ? rs256_bm_fixed_kernel<4, 256>
+                  : E == 8 ? rs256_bm_fixed_kernel<8, 256> : E == 16 ? rs256_bm_fixed_kernel<16, 256>
+                  : rs256_bm_fixed_kernel<32, 256>;
+        hipLaunchKernelGGL(k2, dim3(g2), dim3(256), lds2, ctx->stream, (uint32_t)count, d_status,
+                           (const uint32_t *)d_synd, d_rec, tab);
+    } else {
+        size_t lds2 = EXP2_LEN + 512 + (size_t)256 * bm_row_stride(E);
+        if (int r = allow_lds(rs256_bm_kernel<256>, lds2))
+            return r;
+        int g2 = persistent_grid(ctx, (int)std::max<size_t>(1, (160 * 1024) / lds2), (count + 255) / 256);
+        hipLaunchKernelGGL((rs256_bm_kernel<256>), dim3(g2), dim3(256), lds2, ctx->stream, (uint32_t)count, E,
+                           (uint32_t)EW, d_status, (const uint32_t *)d_synd, d_rec, tab);
+    }
+    FFRS_HIP(hipGetLastError());
+
+    constexpr int WG3 = 1024;
+    size_t lds3 = CHIEN_TABLE_BYTES + EXP2_LEN + 512;
+    if (int r = allow_lds(rs256_chien_kernel<WG3>, lds3))
+        return r;
+    int g3 = persistent_grid(ctx, 1, (count + WG3 - 1) / WG3);
+    hipLaunchKernelGGL((rs256_chien_kernel<WG3>), dim3(g3), dim3(WG3), lds3, ctx->stream, d_msg, d_ecc,
+                       (uint32_t)count, k, E, d_status, (const uint8_t *)d_rec, rs->d_ctab, tab);
     FFRS_HIP(hipGetLastError());
     return FFRS_OK;
 }
@@ -627,10 +1174,10 @@ int repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d
         uint8_t *st = d_status + b;
         int r;
         switch (rs->EW) {
-        case 1: r = launch_repair<1, 4, 256>(rs, m, e, c, k, st); break;
-        case 2: r = launch_repair<2, 4, 256>(rs, m, e, c, k, st); break;
-        case 4: r = launch_repair<4, 4, 256>(rs, m, e, c, k, st); break;
-        case 8: r = launch_repair<8, 4, 256>(rs, m, e, c, k, st); break;
+        case 1: r = launch_repair_pipeline<1, 4>(rs, m, e, c, k, st); break;
+        case 2: r = launch_repair_pipeline<2, 4>(rs, m, e, c, k, st); break;
+        case 4: r = launch_repair_pipeline<4, 4>(rs, m, e, c, k, st); break;
+        case 8: r = launch_repair_pipeline<8, 4>(rs, m, e, c, k, st); break;
         case 16: r = launch_repair<16, 2, 128>(rs, m, e, c, k, st); break;
         case 32: r = launch_repair<32, 1, 64>(rs, m, e, c, k, st); break;
         default: r = launch_repair<64, 1, 64>(rs, m, e, c, k, st); break;
@@ -714,6 +1261,17 @@ int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsig
         e = hipMemcpy(rs->d_exp2, exp2, EXP2_LEN, hipMemcpyHostToDevice);
     if (e == hipSuccess)
         e = hipMemcpy(rs->d_log2, log2, 512, hipMemcpyHostToDevice);
+    if (e == hipSuccess && rs->EW <= 8) {
+        std::vector<uint32_t> st = build_synd_tables(*rs);
+        std::vector<uint32_t> ct = build_chien_tables(*rs);
+        e = hipMalloc(&rs->d_stab, st.size() * 4);
+        if (e == hipSuccess)
+            e = hipMalloc(&rs->d_ctab, ct.size() * 4);
+        if (e == hipSuccess)
+            e = hipMemcpy(rs->d_stab, st.data(), st.size() * 4, hipMemcpyHostToDevice);
+        if (e == hipSuccess)
+            e = hipMemcpy(rs->d_ctab, ct.data(), ct.size() * 4, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         ffrs_rs256_destroy(rs);
         return fail(FFRS_ERR_DEVICE, "table upload failed: %s", hipGetErrorString(e));
@@ -733,6 +1291,12 @@ void ffrs_rs256_destroy(ffrs_rs256 *rs)
         (void)hipFree(rs->d_exp2);
     if (rs->d_log2)
         (void)hipFree(rs->d_log2);
+    if (rs->d_stab)
+        (void)hipFree(rs->d_stab);
+    if (rs->d_ctab)
+        (void)hipFree(rs->d_ctab);
+    if (rs->d_work)
+        (void)hipFree(rs->d_work);
     delete rs;
 }
 
