@@ -99,13 +99,14 @@ def cpu_baseline(msgs, cws_corrupt, ecc, k, repair_sample):
     }
 
 
-def rocprof_traffic(kernel):
-    """HBM bytes per launch for `kernel` from the committed rocprofv3 --pmc summary, if any."""
+def rocprof_traffic():
+    """HBM bytes per dispatch of each kernel, from the committed rocprofv3 --pmc summary
+    (FETCH_SIZE and WRITE_SIZE scaled by what they read for a copy of known size)."""
     path = os.path.join(ROOT, "profiles", "traffic_rs256_plain_scale.json")
     try:
         with open(path) as f:
-            return json.load(f).get(kernel)
-    except (OSError, ValueError):
+            return json.load(f)["kernels"]
+    except (OSError, ValueError, KeyError):
         return None
 
 
@@ -140,43 +141,64 @@ def run_rs256_plain_scale(args):
     d_cm.upload(np.ascontiguousarray(cws[:, :k]))
     d_cp.upload(np.ascontiguousarray(cws[:, k:]))
 
-    def step(timed):
+    def step(instrumented):
         d_wm.copy_from(d_cm)
         d_wp.copy_from(d_cp)
         ctx.sync()
         t = time.perf_counter()
-        ctx.timer_begin()
         codec.encode_device(k, d_msg, N, d_par)
-        te = ctx.timer_end()
-        ctx.timer_begin()
+        ke = ctx.kernel_times() if instrumented else None
         codec.repair_device(k, d_wm, d_wp, N, d_st)
-        tr = ctx.timer_end()
-        wall = time.perf_counter() - t
-        return wall, te, tr
+        kr = ctx.kernel_times() if instrumented else None
+        ctx.sync()
+        return time.perf_counter() - t, ke, kr
 
     for _ in range(args.warmup):
         step(False)
-    walls, tes, trs = [], [], []
+    # timed steps: wall clock between two stream synchronisations
+    walls = [step(False)[0] for _ in range(args.steps)]
+    # the same steps again with a HIP event after every kernel, for the per-kernel split
+    ctx.kernel_timing(True)
+    kes, krs = [], []
     for _ in range(args.steps):
-        w, te, tr = step(True)
-        walls.append(w)
-        tes.append(te)
-        trs.append(tr)
+        _, ke, kr = step(True)
+        kes.append(ke)
+        krs.append(kr)
+    ctx.kernel_timing(False)
     st = np.empty(N, np.uint8)
     d_st.download(st)
     ms_step = 1e3 * float(np.mean(walls))
     msg_bytes = N * k
     value = msg_bytes / (ms_step * 1e-3) / 1e9
-    te_ms, tr_ms = float(np.mean(tes)), float(np.mean(trs))
+    # per-kernel durations from HIP events on the codec stream (launch order)
+    enc_names = ["rs256_encode_kernel"]
+    rep_names = ["rs256_synd_kernel", "rs256_bm_fixed_kernel", "rs256_chien_kernel"]
+    ke_ms = np.mean(np.array(kes), axis=0)
+    kr_ms = np.mean(np.array(krs), axis=0)
+    if len(ke_ms) != len(enc_names) or len(kr_ms) != len(rep_names):
+        raise RuntimeError(f"unexpected kernel count: encode {len(ke_ms)}, repair {len(kr_ms)}")
+    te_ms, tr_ms = float(ke_ms.sum()), float(kr_ms.sum())
     # algorithmic bytes per launch: encode reads k + writes ecc per codeword; repair reads
     # n per codeword and writes one status byte (corrections are sparse and left out)
     enc_bytes = N * (k + ecc)
     rep_bytes = N * (k + ecc) + N
-    if tr_ms >= te_ms:
-        dom, dom_ms, dom_bytes = "rs256_repair_kernel", tr_ms, rep_bytes
-    else:
-        dom, dom_ms, dom_bytes = "rs256_encode_kernel", te_ms, enc_bytes
-    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    traffic = rocprof_traffic()
+    launches = {
+        "rs256_encode": {"kernels": enc_names, "ms": te_ms, "bytes": enc_bytes},
+        "rs256_repair": {"kernels": rep_names, "ms": tr_ms, "bytes": rep_bytes},
+    }
+    dom = max(launches, key=lambda n: launches[n]["ms"])
+    d = launches[dom]
+    achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
+    dom_traffic = None
+    if traffic and all(n in traffic for n in d["kernels"]):
+        dom_traffic = int(sum(traffic[n]["bytes"] for n in d["kernels"]))
+    kernels = {}
+    for names, times in ((enc_names, ke_ms), (rep_names, kr_ms)):
+        for n, t in zip(names, times):
+            kernels[n] = {"ms": round(float(t), 4)}
+            if traffic and n in traffic:
+                kernels[n]["traffic"] = int(traffic[n]["bytes"])
     result = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -192,11 +214,14 @@ def run_rs256_plain_scale(args):
         "data": "synthetic (SplitMix64 stream, seed 2)",
         "config": {"workload": "rs256_plain_scale", "codewords": N, "ecc_len": ecc, "message_len": k,
                    "errors": "0-16 per codeword, 1% with 17-20", "device": dev_name},
-        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": rocprof_traffic(dom)},
-        "kernels": {"rs256_encode_kernel": {"ms": round(te_ms, 4), "gbps_msg": round(msg_bytes / te_ms / 1e6, 2)},
-                    "rs256_repair_kernel": {"ms": round(tr_ms, 4), "gbps_msg": round(msg_bytes / tr_ms / 1e6, 2),
-                                            "repaired_true": int(st.sum())}},
+        "roofline": {"bound": "hbm", "kernel": dom + " (" + " + ".join(d["kernels"]) + ")",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": dom_traffic},
+        "launches": {n: {"ms": round(v["ms"], 4), "alg_bytes": v["bytes"],
+                         "alg_gbps": round(v["bytes"] / v["ms"] / 1e6, 2),
+                         "msg_gbps": round(msg_bytes / v["ms"] / 1e6, 2)} for n, v in launches.items()},
+        "kernels": kernels,
+        "repaired_true": int(st.sum()),
     }
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")

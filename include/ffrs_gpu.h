@@ -69,6 +69,13 @@ int ffrs_memcpy_d2d(ffrs_ctx *ctx, void *d_dst, const void *d_src, size_t bytes)
 int ffrs_timer_begin(ffrs_ctx *ctx);
 int ffrs_timer_end(ffrs_ctx *ctx, float *ms);
 
+/* Per-kernel timing (no counterpart in the reference, which has no device): when enabled,
+ * every encode/repair call records a HIP event on the context stream after each kernel it
+ * launches; ffrs_ctx_kernel_times waits for the last one and writes the durations of the
+ * last call's kernels, in launch order, to ms[0..min(cap, *count)). Off by default. */
+int ffrs_ctx_kernel_timing(ffrs_ctx *ctx, int enable);
+int ffrs_ctx_kernel_times(ffrs_ctx *ctx, float *ms, int cap, int *count);
+
 /* ---- RS256 over GF(2^8) ------------------------------------------------------------ */
 /* Codec for one ecc_len (1..254) and field (primitive, polynomial); the parity and
  * syndrome tables are built on the host once and kept resident on the device.

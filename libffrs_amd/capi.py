@@ -38,6 +38,8 @@ SIGNATURES = [
     ("ffrs_memcpy_d2d", c_int, [c_vp, c_vp, c_vp, c_sz]),
     ("ffrs_timer_begin", c_int, [c_vp]),
     ("ffrs_timer_end", c_int, [c_vp, ctypes.POINTER(ctypes.c_float)]),
+    ("ffrs_ctx_kernel_timing", c_int, [c_vp, c_int]),
+    ("ffrs_ctx_kernel_times", c_int, [c_vp, ctypes.POINTER(ctypes.c_float), c_int, ctypes.POINTER(c_int)]),
     ("ffrs_rs256_create", c_int, [c_vp, c_uint, c_uint, c_uint, ctypes.POINTER(c_vp)]),
     ("ffrs_rs256_destroy", None, [c_vp]),
     ("ffrs_rs256_generator", c_int, [c_vp, c_u8p, c_u8p]),
@@ -117,6 +119,15 @@ class Context:
         ms = ctypes.c_float()
         check(lib().ffrs_timer_end(self.h, ctypes.byref(ms)), "timer_end")
         return ms.value
+
+    def kernel_timing(self, enable=True):
+        check(lib().ffrs_ctx_kernel_timing(self.h, 1 if enable else 0), "kernel_timing")
+
+    def kernel_times(self):
+        buf = (ctypes.c_float * 8)()
+        n = c_int()
+        check(lib().ffrs_ctx_kernel_times(self.h, buf, 8, ctypes.byref(n)), "kernel_times")
+        return [buf[i] for i in range(min(n.value, 8))]
 
 
 class DeviceBuffer:

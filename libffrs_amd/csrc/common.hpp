@@ -46,6 +46,11 @@ struct ffrs_ctx {
     hipEvent_t ev_begin = nullptr, ev_end = nullptr;
     ffrs::DevBuf stage[4];
     std::mutex mu;
+    // per-kernel timing of the last instrumented call (off unless enabled)
+    static constexpr int kMarks = 8;
+    bool marks_on = false;
+    int marks_n = 0;
+    hipEvent_t marks[kMarks] = {};
 };
 
 namespace ffrs {
@@ -53,6 +58,20 @@ namespace ffrs {
 // Ensures ctx->stage[i] holds at least `bytes` (+ 64 bytes of slack for edge loads).
 int stage_reserve(ffrs_ctx *ctx, int i, size_t bytes, void **out);
 int ctx_set_device(ffrs_ctx *ctx);
+
+// With kernel timing enabled, marks_begin records the first event of a call and
+// mark_kernel one event after each kernel; ffrs_ctx_kernel_times reads the gaps.
+inline void marks_begin(ffrs_ctx *ctx)
+{
+    ctx->marks_n = 0;
+    if (ctx->marks_on)
+        (void)hipEventRecord(ctx->marks[ctx->marks_n++], ctx->stream);
+}
+inline void mark_kernel(ffrs_ctx *ctx)
+{
+    if (ctx->marks_on && ctx->marks_n < ffrs_ctx::kMarks)
+        (void)hipEventRecord(ctx->marks[ctx->marks_n++], ctx->stream);
+}
 
 // Number of workgroups to keep resident for a persistent kernel.
 inline int persistent_grid(const ffrs_ctx *ctx, int wgs_per_cu, size_t work_tiles)

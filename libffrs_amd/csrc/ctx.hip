@@ -106,6 +106,9 @@ void ffrs_ctx_destroy(ffrs_ctx *ctx)
         (void)hipEventDestroy(ctx->ev_begin);
     if (ctx->ev_end)
         (void)hipEventDestroy(ctx->ev_end);
+    for (auto &e : ctx->marks)
+        if (e)
+            (void)hipEventDestroy(e);
     if (ctx->stream)
         (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -113,6 +116,8 @@ void ffrs_ctx_destroy(ffrs_ctx *ctx)
 
 int ffrs_ctx_synchronize(ffrs_ctx *ctx)
 {
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
     FFRS_HIP(hipSetDevice(ctx->device));
     FFRS_HIP(hipStreamSynchronize(ctx->stream));
     return FFRS_OK;
@@ -121,11 +126,16 @@ int ffrs_ctx_synchronize(ffrs_ctx *ctx)
 int ffrs_ctx_device_info(ffrs_ctx *ctx, char *name, size_t name_cap, int *cu_count, size_t *free_bytes,
                          size_t *total_bytes)
 {
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
     FFRS_HIP(hipSetDevice(ctx->device));
     hipDeviceProp_t prop;
     FFRS_HIP(hipGetDeviceProperties(&prop, ctx->device));
     if (name && name_cap) {
-        snprintf(name, name_cap, "%s (%s)", prop.name, prop.gcnArchName);
+        if (prop.name[0])
+            snprintf(name, name_cap, "%s (%s, %d CUs)", prop.name, prop.gcnArchName, prop.multiProcessorCount);
+        else
+            snprintf(name, name_cap, "%s (%d CUs)", prop.gcnArchName, prop.multiProcessorCount);
     }
     if (cu_count)
         *cu_count = ctx->cu_count;
@@ -140,6 +150,8 @@ int ffrs_ctx_device_info(ffrs_ctx *ctx, char *name, size_t name_cap, int *cu_cou
 
 int ffrs_device_alloc(ffrs_ctx *ctx, size_t bytes, void **d_ptr)
 {
+    if (!ctx || !d_ptr)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
     FFRS_HIP(hipSetDevice(ctx->device));
     hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 1);
     if (e != hipSuccess)
@@ -149,6 +161,8 @@ int ffrs_device_alloc(ffrs_ctx *ctx, size_t bytes, void **d_ptr)
 
 int ffrs_device_free(ffrs_ctx *ctx, void *d_ptr)
 {
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
     FFRS_HIP(hipSetDevice(ctx->device));
     FFRS_HIP(hipFree(d_ptr));
     return FFRS_OK;
@@ -156,6 +170,8 @@ int ffrs_device_free(ffrs_ctx *ctx, void *d_ptr)
 
 int ffrs_memcpy_h2d(ffrs_ctx *ctx, void *d_dst, const void *h_src, size_t bytes)
 {
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
     FFRS_HIP(hipSetDevice(ctx->device));
     FFRS_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
     FFRS_HIP(hipStreamSynchronize(ctx->stream));
@@ -164,6 +180,8 @@ int ffrs_memcpy_h2d(ffrs_ctx *ctx, void *d_dst, const void *h_src, size_t bytes)
 
 int ffrs_memcpy_d2h(ffrs_ctx *ctx, void *h_dst, const void *d_src, size_t bytes)
 {
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
     FFRS_HIP(hipSetDevice(ctx->device));
     FFRS_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
     FFRS_HIP(hipStreamSynchronize(ctx->stream));
@@ -172,6 +190,8 @@ int ffrs_memcpy_d2h(ffrs_ctx *ctx, void *h_dst, const void *d_src, size_t bytes)
 
 int ffrs_memcpy_d2d(ffrs_ctx *ctx, void *d_dst, const void *d_src, size_t bytes)
 {
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
     FFRS_HIP(hipSetDevice(ctx->device));
     FFRS_HIP(hipMemcpyAsync(d_dst, d_src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
     return FFRS_OK;
@@ -179,6 +199,8 @@ int ffrs_memcpy_d2d(ffrs_ctx *ctx, void *d_dst, const void *d_src, size_t bytes)
 
 int ffrs_timer_begin(ffrs_ctx *ctx)
 {
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
     FFRS_HIP(hipSetDevice(ctx->device));
     FFRS_HIP(hipEventRecord(ctx->ev_begin, ctx->stream));
     return FFRS_OK;
@@ -186,10 +208,41 @@ int ffrs_timer_begin(ffrs_ctx *ctx)
 
 int ffrs_timer_end(ffrs_ctx *ctx, float *ms)
 {
+    if (!ctx || !ms)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
     FFRS_HIP(hipSetDevice(ctx->device));
     FFRS_HIP(hipEventRecord(ctx->ev_end, ctx->stream));
     FFRS_HIP(hipEventSynchronize(ctx->ev_end));
     FFRS_HIP(hipEventElapsedTime(ms, ctx->ev_begin, ctx->ev_end));
+    return FFRS_OK;
+}
+
+int ffrs_ctx_kernel_timing(ffrs_ctx *ctx, int enable)
+{
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
+    FFRS_HIP(hipSetDevice(ctx->device));
+    if (enable)
+        for (auto &e : ctx->marks)
+            if (!e)
+                FFRS_HIP(hipEventCreate(&e));
+    ctx->marks_on = enable != 0;
+    ctx->marks_n = 0;
+    return FFRS_OK;
+}
+
+int ffrs_ctx_kernel_times(ffrs_ctx *ctx, float *ms, int cap, int *count)
+{
+    if (!ctx || !count || (cap > 0 && !ms))
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    FFRS_HIP(hipSetDevice(ctx->device));
+    int n = ctx->marks_n > 1 ? ctx->marks_n - 1 : 0;
+    *count = n;
+    if (n == 0)
+        return FFRS_OK;
+    FFRS_HIP(hipEventSynchronize(ctx->marks[ctx->marks_n - 1]));
+    for (int i = 0; i < n && i < cap; ++i)
+        FFRS_HIP(hipEventElapsedTime(&ms[i], ctx->marks[i], ctx->marks[i + 1]));
     return FFRS_OK;
 }
 

@@ -60,7 +60,9 @@ ffrs_ctx *default_ctx()
         int r = ffrs_ctx_create(dev, &g_ctx);
         if (r != FFRS_OK) {
             g_ctx = nullptr;
-            throw std::runtime_error(std::string("libffrs_amd: no usable HIP device: ") + ffrs_last_error());
+            throw std::runtime_error(std::string("libffrs_amd: no usable HIP device: ") + ffrs_last_error() +
+                                     " (a process that also uses PyTorch on the GPU must import torch before "
+                                     "libffrs_amd, so that both bind to one HIP runtime)");
         }
     }
     return g_ctx;

@@ -774,14 +774,18 @@ __device__ inline void xor_byte(uint8_t *p, uint32_t v)
     atomicXor(reinterpret_cast<unsigned int *>(a & ~(uintptr_t)3), v << (8 * (a & 3)));
 }
 
-// Chien search and Forney, one codeword per wave (reed_solomon.hpp:516-538, 704-743).
-// Lane l holds positions 4l..4l+3 as the bytes of one dword; with the nibble tables
-// CT[i][h][v] (256 position bytes (v << 4h) alpha^(-i p)) a wave evaluates, for all
-// positions at once, Lambda_even(X^-1), Lambda_odd(X^-1) and Omega(X^-1), X = alpha^p.
-// Every table address is wave-uniform, so the LDS reads are conflict-free. At a root,
-// Lambda'(X^-1) = X Lambda_odd(X^-1) (characteristic 2), hence the reference's
-// e = X div(Omega, Lambda') equals div(Omega, Lambda_odd), or X Omega when Lambda_odd = 0
-// (its div(a, 0) = a); the lane holding the root applies the correction.
+// Chien search one codeword per wave, then Forney one codeword per lane
+// (reed_solomon.hpp:516-538, 704-743).
+// Chien: lane l holds positions 4l..4l+3 as the bytes of one dword; with the nibble
+// tables CT[i][h][v] (256 position bytes (v << 4h) alpha^(-i p)) a wave evaluates
+// Lambda_even(X^-1) and Lambda_odd(X^-1), X = alpha^p, for all positions at once. Every
+// table address is wave-uniform, so the LDS reads are conflict-free. When the roots
+// number exactly L, each root is stored as (p, Lambda_odd(X^-1)) in the wave's LDS slots.
+// Forney: lane o takes codeword o of the tile. At a root, Lambda'(X^-1) = X Lambda_odd(X^-1)
+// (characteristic 2), so the reference's e = X div(Omega(X^-1), Lambda'(X^-1)) equals
+// div(Omega(X^-1), Lambda_odd(X^-1)), or X Omega(X^-1) when Lambda_odd = 0 (its
+// div(a, 0) = a). Omega(X^-1) = sum_k Omega_k alpha^(-p k) is summed from independent
+// log-domain terms.
 template <int WG>
 __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ msgs, uint8_t *__restrict__ eccs,
                                                           uint32_t count, uint32_t k, uint32_t E,
@@ -790,7 +794,8 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
 {
     extern __shared__ uint32_t s_mem[];
     uint32_t *s_ct = s_mem;  // [CHIEN_LMAX][2][16][64] dwords
-    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem + CHIEN_TABLE_BYTES / 4);
+    uint16_t *s_roots = reinterpret_cast<uint16_t *>(s_mem + CHIEN_TABLE_BYTES / 4);  // [WG/64][64][16]
+    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_roots + WG * CHIEN_LMAX);
     uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
     for (uint32_t i = threadIdx.x; i < CHIEN_TABLE_BYTES / 4; i += blockDim.x)
         s_ct[i] = ctab[i];
@@ -799,18 +804,23 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
     for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
         s_log2[i] = tab.log2[i];
     __syncthreads();
-    GfLds gf{s_exp2, s_log2};
     const uint32_t lane = threadIdx.x & 63;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: scalar loads below
-    const uint32_t n = k + E, RS = rec_stride(E), OO = rec_om_off(E);
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    uint16_t *my_roots = s_roots + wave * 64 * CHIEN_LMAX;
+    const uint32_t n = k + E;
     const uint32_t *rec32 = reinterpret_cast<const uint32_t *>(rec);
-    const uint32_t RSW = RS / 4, OW = OO / 4;
+    const uint32_t RSW = rec_stride(E) / 4;
+    uint64_t valid[4];  // lanes whose position 4 lane + q lies in the codeword
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        valid[q] = __builtin_amdgcn_ballot_w64(lane * 4 + q < n);
     for (uint32_t base = blockIdx.x * WG + wave * 64; base < count; base += gridDim.x * WG) {
         uint32_t cw = base + lane;
         bool pend = cw < count && status[cw] == RS_PENDING;
         uint64_t pend_mask = __ballot(pend);
-        uint64_t ok_mask = 0;
-        // this tile's 64 records, one per lane (coalesced), read back per codeword by readlane
+        if (!pend_mask)
+            continue;
+        // the tile's records, one per lane (coalesced); per codeword read back by readlane
         uint32_t recw[10];
         {
             const uint4 *r4 = reinterpret_cast<const uint4 *>(rec32 + (size_t)(cw < count ? cw : 0) * RSW);
@@ -819,98 +829,95 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
             recw[4] = b.x; recw[5] = b.y; recw[6] = b.z; recw[7] = b.w;
             recw[8] = c.x; recw[9] = c.y;
         }
-        // Evaluate Lambda_even and Lambda_odd at the 4 positions of each lane for codeword o.
-        auto evaluate = [&](uint32_t o, uint32_t &Lo, uint32_t &v_even, uint32_t &v_odd) {
-            Lo = __builtin_amdgcn_readlane(recw[0], o) & 0xffu;
-            uint32_t lamw[5];
-#pragma unroll
-            for (int w = 0; w < 5; ++w)
-                lamw[w] = __builtin_amdgcn_readlane(recw[5 + w], o);  // Lambda_0..Lambda_19
-            v_even = 0x01010101u;
-            v_odd = 0;
-            for (uint32_t i0 = 1; i0 <= Lo; i0 += 4) {
-                uint32_t ae = 0, ao = 0;
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    uint32_t i = i0 + j;
-                    uint32_t li = (lamw[i >> 2] >> (8 * (i & 3))) & 0xffu;
-                    li = i <= Lo ? li : 0u;
-                    const uint32_t *row = s_ct + ((i - 1) & 15) * 32 * 64;
-                    uint32_t t = row[(li & 15) * 64 + lane] ^ row[(16 + (li >> 4)) * 64 + lane];
-                    if (i & 1)
-                        ao ^= t;
-                    else
-                        ae ^= t;
-                }
-                v_even ^= ae;
-                v_odd ^= ao;
-            }
-        };
-        // Count the roots; when they are exactly L, the lanes holding roots evaluate
-        // Omega(X^-1) = sum_k Omega_k alpha^(-p k) (independent log-domain terms) and
-        // correct their positions.
+        uint64_t ok_mask = 0;
+        // Count the roots of codeword o; when they number exactly L, store them as
+        // (p, Lambda_odd(X^-1)) in the codeword's slots.
         auto settle = [&](uint32_t o, uint32_t Lo, uint32_t v_even, uint32_t v_odd) {
-            uint32_t val = v_even ^ v_odd;
-            uint32_t cnt = 0, zq = 0;
+            const uint32_t val = v_even ^ v_odd;
+            uint64_t zb[4];
+            uint32_t cnt = 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                bool z = ((val >> (8 * q)) & 0xffu) == 0 && lane * 4 + q < n;
-                zq |= (uint32_t)z << q;
-                cnt += (uint32_t)__builtin_popcountll(__ballot(z));
+                zb[q] = __builtin_amdgcn_ballot_w64((val & (0xffu << (8 * q))) == 0) & valid[q];
+                cnt += (uint32_t)__builtin_popcountll(zb[q]);
             }
             if (cnt != Lo)
                 return;
             ok_mask |= 1ull << o;
-            uint32_t cwo = base + o;
-            uint32_t lom[CHIEN_LMAX];
+            uint32_t slot = 0;
 #pragma unroll
-            for (int w = 0; w < 4; ++w) {
-                uint32_t ow = __builtin_amdgcn_readlane(recw[1 + w], o);  // Omega_{4w}..Omega_{4w+3}
-#pragma unroll
-                for (int b = 0; b < 4; ++b)
-                    lom[4 * w + b] = gf.log2[(ow >> (8 * b)) & 0xffu];
-            }
-            if (zq) {
-                for (uint32_t q = 0; q < 4; ++q) {
-                    if (!((zq >> q) & 1u))
-                        continue;
-                    uint32_t p = lane * 4 + q;
-                    uint32_t lxi = p ? 255 - p : 0;
-                    uint32_t num = 0, e = 0;
-#pragma unroll
-                    for (int kk = 0; kk < CHIEN_LMAX; ++kk) {
-                        num ^= gf.exp2[lom[kk] + e];
-                        e += lxi;
-                        e = e >= 255 ? e - 255 : e;
-                    }
-                    uint32_t dod = (v_odd >> (8 * q)) & 0xffu;
-                    uint32_t ln = gf.log2[num], ld = gf.log2[dod];
-                    uint32_t mag = num == 0 ? 0u : (dod ? (uint32_t)gf.exp2[ln + 255 - ld] : (uint32_t)gf.exp2[p + ln]);
-                    uint32_t at = n - 1 - p;
-                    uint8_t *dst = at < k ? msgs + (size_t)cwo * k + at : eccs + (size_t)cwo * E + (at - k);
-                    xor_byte(dst, mag);
-                }
+            for (int q = 0; q < 4; ++q) {
+                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(zb[q] >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)zb[q], 0u));
+                if ((zb[q] >> lane) & 1ull)
+                    my_roots[o * CHIEN_LMAX + slot + below] =
+                        (uint16_t)((lane * 4 + q) | (((v_odd >> (8 * q)) & 0xffu) << 8));
+                slot += (uint32_t)__builtin_popcountll(zb[q]);
             }
         };
         while (pend_mask) {
-            uint32_t o1 = (uint32_t)__builtin_ctzll(pend_mask);
+            const uint32_t o = (uint32_t)__builtin_ctzll(pend_mask);
             pend_mask &= pend_mask - 1;
-            uint32_t L1, e1, d1;
-            if (pend_mask) {
-                uint32_t o2 = (uint32_t)__builtin_ctzll(pend_mask);
-                pend_mask &= pend_mask - 1;
-                uint32_t L2, e2, d2;
-                evaluate(o1, L1, e1, d1);
-                evaluate(o2, L2, e2, d2);
-                settle(o1, L1, e1, d1);
-                settle(o2, L2, e2, d2);
-            } else {
-                evaluate(o1, L1, e1, d1);
-                settle(o1, L1, e1, d1);
+            const uint32_t Lo = __builtin_amdgcn_readlane(recw[0], o) & 0xffu;
+            uint32_t lamw[5];
+#pragma unroll
+            for (int w = 0; w < 5; ++w)
+                lamw[w] = __builtin_amdgcn_readlane(recw[5 + w], o);  // Lambda_0..Lambda_19
+            // Lambda_i = 0 for L < i <= 16 (records are zero-filled) and row v = 0 of every
+            // table is zero, so terms past L add nothing; whole groups of four past L are skipped.
+            uint32_t v_even = 0x01010101u, v_odd = 0;
+#pragma unroll
+            for (int g = 0; g < 4; ++g) {
+                if (4 * g + 1 > (int)Lo)
+                    break;
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int i = 4 * g + j + 1;
+                    const uint32_t li = (lamw[i >> 2] >> (8 * (i & 3))) & 0xffu;
+                    const uint32_t *row = s_ct + (i - 1) * 32 * 64 + lane;
+                    const uint32_t t = row[(li & 15) * 64] ^ row[(16 + (li >> 4)) * 64];
+                    if (i & 1)
+                        v_odd ^= t;
+                    else
+                        v_even ^= t;
+                }
             }
+            settle(o, Lo, v_even, v_odd);
         }
-        if (pend)
-            status[cw] = ((ok_mask >> lane) & 1ull) ? 1 : 0;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (!pend)
+            continue;
+        const bool ok = (ok_mask >> lane) & 1ull;
+        status[cw] = ok ? 1 : 0;
+        if (!ok)
+            continue;
+        const uint32_t L = recw[0] & 0xffu;
+        uint32_t lom[CHIEN_LMAX];
+#pragma unroll
+        for (int w = 0; w < 4; ++w)
+#pragma unroll
+            for (int b = 0; b < 4; ++b)
+                lom[4 * w + b] = s_log2[(recw[1 + w] >> (8 * b)) & 0xffu];
+        uint8_t *mrow = msgs + (size_t)cw * k;
+        uint8_t *erow = eccs + (size_t)cw * E;
+        for (uint32_t j = 0; j < L; ++j) {
+            const uint32_t r = my_roots[lane * CHIEN_LMAX + j];
+            const uint32_t p = r & 0xffu, dod = r >> 8;
+            const uint32_t lxi = p ? 255 - p : 0;
+            uint32_t num = 0, e = 0;
+#pragma unroll
+            for (int kk = 0; kk < CHIEN_LMAX; ++kk) {
+                num ^= s_exp2[lom[kk] + e];
+                e += lxi;
+                e = e >= 255 ? e - 255 : e;
+            }
+            const uint32_t ln = s_log2[num], ld = s_log2[dod];
+            const uint32_t mag = num == 0 ? 0u : (dod ? (uint32_t)s_exp2[ln + 255 - ld] : (uint32_t)s_exp2[p + ln]);
+            const uint32_t at = n - 1 - p;
+            xor_byte(at < k ? mrow + at : erow + (at - k), mag);
+        }
     }
 }
 
@@ -992,6 +999,7 @@ int launch_encode(ffrs_rs256 *rs, const uint8_t *d_msg, size_t count, uint32_t k
     hipLaunchKernelGGL((rs256_encode_kernel<EW, S>), dim3(grid), dim3(256), lds, ctx->stream, d_msg,
                        (uint32_t)(count * k), (uint32_t)count, k, d_ecc, rs->ecc, rs->d_lut);
     FFRS_HIP(hipGetLastError());
+    mark_kernel(ctx);
     return FFRS_OK;
 }
 
@@ -1010,6 +1018,7 @@ int launch_repair(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_t count, 
     hipLaunchKernelGGL((rs256_repair_kernel<EW, S, WG>), dim3(grid), dim3(WG), lds, ctx->stream, d_msg,
                        (uint32_t)(count * k), d_ecc, (uint32_t)count, k, rs->ecc, d_status, tab);
     FFRS_HIP(hipGetLastError());
+    mark_kernel(ctx);
     return FFRS_OK;
 }
 
@@ -1101,6 +1110,7 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     hipLaunchKernelGGL((rs256_synd_kernel<EW, S>), dim3(g1), dim3(256), lds1, ctx->stream, d_msg,
                        (uint32_t)(count * k), d_ecc, (uint32_t)count, k, E, d_status, d_synd, rs->d_lut, rs->d_stab);
     FFRS_HIP(hipGetLastError());
+    mark_kernel(ctx);
 
     if (E == 2 || E == 4 || E == 8 || E == 16 || E == 32) {
         size_t lds2 = EXP2_LEN + 512;
@@ -1119,15 +1129,17 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
                            (uint32_t)EW, d_status, (const uint32_t *)d_synd, d_rec, tab);
     }
     FFRS_HIP(hipGetLastError());
+    mark_kernel(ctx);
 
-    constexpr int WG3 = 1024;
-    size_t lds3 = CHIEN_TABLE_BYTES + EXP2_LEN + 512;
+    constexpr int WG3 = 768;
+    size_t lds3 = CHIEN_TABLE_BYTES + (size_t)WG3 * CHIEN_LMAX * 2 + EXP2_LEN + 512;
     if (int r = allow_lds(rs256_chien_kernel<WG3>, lds3))
         return r;
     int g3 = persistent_grid(ctx, 1, (count + WG3 - 1) / WG3);
     hipLaunchKernelGGL((rs256_chien_kernel<WG3>), dim3(g3), dim3(WG3), lds3, ctx->stream, d_msg, d_ecc,
                        (uint32_t)count, k, E, d_status, (const uint8_t *)d_rec, rs->d_ctab, tab);
     FFRS_HIP(hipGetLastError());
+    mark_kernel(ctx);
     return FFRS_OK;
 }
 
@@ -1136,6 +1148,7 @@ size_t max_batch(size_t k) { return std::max<size_t>(1, ((size_t)1 << 31) / std:
 
 int encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *d_msg, size_t count, uint8_t *d_ecc)
 {
+    marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
     uint32_t k = (uint32_t)message_len;
@@ -1163,6 +1176,7 @@ int encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *d_msg, size
 int repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d_ecc, size_t count,
                   uint8_t *d_status)
 {
+    marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
     uint32_t k = (uint32_t)message_len;
@@ -1315,6 +1329,8 @@ int ffrs_rs256_encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *
 {
     if (int r = check_codec(rs, message_len))
         return r;
+    if (count && (!d_msg || !d_ecc))
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     if (int r = ctx_set_device(rs->ctx))
         return r;
     return encode_device(rs, message_len, d_msg, count, d_ecc);
@@ -1325,6 +1341,8 @@ int ffrs_rs256_repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg,
 {
     if (int r = check_codec(rs, message_len))
         return r;
+    if (count && (!d_msg || !d_ecc || !d_status))
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     if (int r = ctx_set_device(rs->ctx))
         return r;
     return repair_device(rs, message_len, d_msg, d_ecc, count, d_status);
@@ -1341,6 +1359,8 @@ int ffrs_rs256_encode(ffrs_rs256 *rs, size_t message_len, const uint8_t *buf, si
         return fail(FFRS_ERR_SIZE, "output holds %zu bytes, %zu needed", out_size, need);
     if (size == 0)
         return FFRS_OK;
+    if (!buf || !out)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     ffrs_ctx *ctx = rs->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (int r = ctx_set_device(ctx))
@@ -1369,6 +1389,8 @@ int ffrs_rs256_repair(ffrs_rs256 *rs, size_t message_len, uint8_t *msg, uint8_t 
         return r;
     if (count == 0)
         return FFRS_OK;
+    if (!msg || !ecc || !status)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     ffrs_ctx *ctx = rs->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (int r = ctx_set_device(ctx))
@@ -1401,6 +1423,8 @@ int ffrs_rs256_synds(ffrs_rs256 *rs, size_t message_len, const uint8_t *msg, con
         return FFRS_OK;
     if (count > max_batch(message_len))
         return fail(FFRS_ERR_INVALID, "too many codewords in one syndrome call");
+    if (!msg || !ecc || !synds)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     ffrs_ctx *ctx = rs->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (int r = ctx_set_device(ctx))

@@ -783,6 +783,7 @@ int check_count(size_t count)
 
 int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vec)
 {
+    marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
     uint32_t g = group_size(rs->E), cpw = 64 / g;
@@ -799,11 +800,13 @@ int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t 
         hipLaunchKernelGGL(rsi16_encode_kernel<ScalarOps>, grid, dim3(64), lds, rs->ctx->stream, d_msg,
                            (uint32_t)count, d_ecc, rs->dev, g);
     FFRS_HIP(hipGetLastError());
+    mark_kernel(rs->ctx);
     return FFRS_OK;
 }
 
 int launch_repair(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count, uint8_t *d_status)
 {
+    marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
     uint32_t g = group_size(rs->E), cpw = 64 / g;
@@ -814,6 +817,7 @@ int launch_repair(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count
     hipLaunchKernelGGL(rsi16_repair_kernel, grid, dim3(64), lds, rs->ctx->stream, d_msg, d_ecc, (uint32_t)count,
                        d_status, rs->dev, g);
     FFRS_HIP(hipGetLastError());
+    mark_kernel(rs->ctx);
     return FFRS_OK;
 }
 
@@ -953,6 +957,8 @@ int ffrs_rsi16_encode_device(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count
         return fail(FFRS_ERR_INVALID, "codec is NULL");
     if (int r = check_count(count))
         return r;
+    if (count && (!d_msg || !d_ecc))
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     if (int r = ctx_set_device(rs->ctx))
         return r;
     return launch_encode(rs, d_msg, count, d_ecc, vector_lanes);
@@ -964,6 +970,8 @@ int ffrs_rsi16_repair_device(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, s
         return fail(FFRS_ERR_INVALID, "codec is NULL");
     if (int r = check_count(count))
         return r;
+    if (count && (!d_msg || !d_ecc || !d_status))
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     if (int r = ctx_set_device(rs->ctx))
         return r;
     return launch_repair(rs, d_msg, d_ecc, count, d_status);
@@ -977,6 +985,8 @@ int ffrs_rsi16_encode(ffrs_rsi16 *rs, const uint16_t *msg, size_t count, uint16_
         return r;
     if (count == 0)
         return FFRS_OK;
+    if (!msg || !ecc)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     ffrs_ctx *ctx = rs->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (int r = ctx_set_device(ctx))
@@ -1003,6 +1013,8 @@ int ffrs_rsi16_repair(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, size_t count
         return r;
     if (count == 0)
         return FFRS_OK;
+    if (!msg || !ecc || !status)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     ffrs_ctx *ctx = rs->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (int r = ctx_set_device(ctx))
@@ -1029,8 +1041,8 @@ int ffrs_rsi16_repair(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, size_t count
 int ffrs_rsi16_repair_known(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, const uint32_t *positions, size_t npos,
                             uint8_t *status)
 {
-    if (!rs)
-        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (!rs || !status || (npos && (!msg || !ecc || !positions)))
+        return fail(FFRS_ERR_INVALID, "NULL argument");
     if (npos == 0) {
         *status = FFRS_RSI16_NO_ERRORS;
         return FFRS_OK;
@@ -1078,6 +1090,8 @@ int ffrs_rsi16_synds(ffrs_rsi16 *rs, const uint16_t *msg, const uint16_t *ecc, s
         return r;
     if (count == 0)
         return FFRS_OK;
+    if (!msg || !ecc || !synds)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     ffrs_ctx *ctx = rs->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (int r = ctx_set_device(ctx))
@@ -1113,6 +1127,8 @@ int ffrs_rsi16_sugiyama(ffrs_rsi16 *rs, const uint32_t *synds, size_t count, uin
         return r;
     if (count == 0)
         return FFRS_OK;
+    if (!synds || !locator || !evaluator || !length)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     ffrs_ctx *ctx = rs->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (int r = ctx_set_device(ctx))
@@ -1145,6 +1161,8 @@ int ffrs_rsi16_roots(ffrs_rsi16 *rs, const uint32_t *poly, uint8_t *is_root)
 {
     if (!rs)
         return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (!poly || !is_root)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
     ffrs_ctx *ctx = rs->ctx;
     std::lock_guard<std::mutex> lk(ctx->mu);
     if (int r = ctx_set_device(ctx))

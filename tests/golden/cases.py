@@ -142,6 +142,12 @@ def rsi16_last_synd_zero(roots_ntt, n, ecc, count, seed=12):
     return out
 
 
+def rsi16_radius_case(n_cw=65536, seed=16):
+    """RSi16(256, 32) messages with 0-16 additive errors, searched for codewords the
+    reference fails to repair although they carry at most ecc/2 errors."""
+    return W.rsi16_plain(n_cw, 256, 32, 16, seed=seed)
+
+
 def rsi16_known_case(n_cw=500, seed=15):
     """RSi16(256, 32) codewords with 0-28 additive errors, and the error positions to
     hand to repair: the corrupted positions plus up to 4 clean ones (at most 32)."""

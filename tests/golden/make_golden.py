@@ -229,11 +229,40 @@ def main():
                               repaired_crc=C.row_crc(rep))
     print("known-position statuses", np.bincount(st, minlength=5))
 
+    out.update(radius_fixture(L))
+
     for key, d in out.items():
         np.savez_compressed(os.path.join(HERE, key + ".npz"), **d)
     total = sum(os.path.getsize(os.path.join(HERE, key + ".npz")) for key in out)
     print("wrote", len(out), "fixtures,", total, "bytes")
 
 
+def radius_fixture(L):
+    """(e) codewords with at most ecc/2 errors and exact parity on which the reference's
+    repair does not restore the codeword, plus as many ordinary ones; the oracle only
+    proposes candidates, the reference decides every expected output."""
+    from oracle import oracle as O
+    n, ecc, k = 256, 32, 224
+    ref = L.RSi16(n, ecc)
+    msgs, counts, s = C.rsi16_radius_case()
+    par = np.frombuffer(ref.encode(msgs.tobytes()), np.uint16).reshape(-1, ecc)
+    orc = O.RSi16(n, ecc)
+    cws = np.concatenate([msgs, par], 1)
+    W.rsi16_corrupt(cws, counts, s)
+    cm, cp = np.ascontiguousarray(cws[:, :k]), np.ascontiguousarray(cws[:, k:])
+    ost = orc.repair_batch(cm.copy(), cp.copy())
+    cand = [i for i in np.nonzero(~np.isin(ost, (0, 2)))[0] if not orc.synds(msgs[i], par[i]).any()]
+    rows = np.array(sorted(cand + list(range(0, 16 * 97, 97))), np.int64)
+    st, rep = ref_repair_rsi16(ref, cws[rows], k)
+    print("radius rows:", len(cand), "reference statuses", np.bincount(st, minlength=5))
+    return {"rsi16_edge_radius": dict(cws=cws[rows], counts=counts[rows], status=st, repaired=rep)}
+
+
 if __name__ == "__main__":
-    main()
+    if sys.argv[1:] == ["radius"]:
+        from libffrs_amd import workloads as W  # noqa: F811
+        d = radius_fixture(load_reference())
+        for key, v in d.items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    else:
+        main()

@@ -159,6 +159,15 @@ def check_rsi16_edges(be):
     assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
     assert (C.row_crc(np.concatenate([m, p], 1)) == fx["repaired_crc"]).all()
 
+    # codewords within the decoding radius that the reference's repair leaves unrepaired
+    fx = fixture("rsi16_edge_radius")
+    m = np.ascontiguousarray(fx["cws"][:, :k])
+    p = np.ascontiguousarray(fx["cws"][:, k:])
+    st = be.rsi16_repair_batch(n, ecc, m, p)
+    assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
+    assert (np.concatenate([m, p], 1) == fx["repaired"]).all()
+    assert (fx["counts"] <= ecc // 2).all() and (~np.isin(fx["status"], (0, 2))).sum() >= 3
+
     fx = fixture("rsi16_edge_const")
     assert (be.rsi16_encode_batch(n, ecc, fx["msgs"], True) == fx["parity"]).all()
     assert (be.rsi16_encode_batch(n, ecc, fx["msgs"], False) == fx["parity_scalar"]).all()
