@@ -1,16 +1,19 @@
-"""Benchmark: RS256 encode + repair GB/s on one MI355X (plain GPU path at scale).
+"""Benchmark: Reed-Solomon encode + repair GB/s on one MI355X (plain GPU path).
 
-Workload (default): RS256 ecc_len 32, n 255 (223 message bytes), 1,048,576 codewords
-(233,832,448 message bytes); every codeword is corrupted at 0-16 distinct bytes, 1% at
-17-20 (uncorrectable path). A step is one encode of all messages plus one repair of all
-corrupted codewords, with inputs resident in HBM. The corrupted input is restored from a
-pristine device copy before each step, outside the timed bracket.
+Default workload rs256_plain_scale: RS256 ecc_len 32, n 255 (223 message bytes), 1,048,576
+codewords (233,832,448 message bytes); every codeword is corrupted at 0-16 distinct bytes,
+1% at 17-20 (uncorrectable path). Other workloads (--workload): rs256_plain_small (ecc 8,
+4,096 codewords), rsi16_plain (RSi16(256, 32), 262,144 codewords) and rsi16_large
+(RSi16(4096, 256), 16,384 codewords). A step is one encode of all messages plus one repair
+of all corrupted codewords, with inputs resident in HBM. The corrupted input is restored
+from a pristine device copy before each step, outside the timed bracket.
 
-    python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
+    python bench.py [--workload W] [--steps K] [--warmup W] [--gpus 1] [--no-cpu-baseline]
 
 Prints one JSON line; `value` is message GB/s of the whole step, `roofline` reports the
-dominant kernel (HIP events on the codec's stream), `cpu_baseline` times the reference's
-own CPU build (or the oracle port where it is absent) on a bounded sample.
+dominant launch (its kernels timed with HIP events on the codec's stream), `cpu_baseline`
+times the reference's own CPU build (or the oracle port where it is absent) on a bounded
+sample.
 """
 import argparse
 import json
@@ -58,51 +61,98 @@ def load_reference_cpu():
         return None
 
 
-def cpu_baseline(msgs, cws_corrupt, ecc, k, repair_sample):
-    """Reference CPU time of encode (whole message buffer, one call) and repair (one call
-    per codeword, as its API works) on a bounded sample; returns message GB/s of a step."""
+WORKLOADS = {
+    # RS256 ecc 32 over 1,048,576 codewords of 223 bytes; 0-16 errors, 1% with 17-20
+    "rs256_plain_scale": dict(field="rs256", N=1048576, n=255, ecc=32, max_err=16, unc=0.01, seed=2,
+                              errors="0-16 per codeword, 1% with 17-20", cpu_repair=262144),
+    # RS256 ecc 8 over 4,096 codewords of 247 bytes; 0-4 errors
+    "rs256_plain_small": dict(field="rs256", N=4096, n=255, ecc=8, max_err=4, unc=0.0, seed=0,
+                              errors="0-4 per codeword", cpu_repair=4096),
+    # RSi16(256, 32) over 262,144 codewords; 0-16 additive errors
+    "rsi16_plain": dict(field="rsi16", N=262144, n=256, ecc=32, max_err=16, seed=3,
+                        errors="0-16 additive per codeword", cpu_repair=32768),
+    # RSi16(4096, 256) over 16,384 codewords; 0-128 additive errors
+    "rsi16_large": dict(field="rsi16", N=16384, n=4096, ecc=256, max_err=128, seed=3,
+                        errors="0-128 additive per codeword", cpu_repair=256),
+}
+
+
+def cpu_baseline(wl, msgs, cws, sample):
+    """The reference's CPU build (or the oracle port where it is absent) on the host cores:
+    encode of the whole message buffer in one call, and repair of `sample` corrupted
+    codewords one call each (the reference's API repairs one codeword per call), single
+    thread. Returns message GB/s of one encode + repair pass over the workload."""
+    n, ecc = wl["n"], wl["ecc"]
+    k = n - ecc
     ref = load_reference_cpu()
-    kind = "reference"
-    if ref is not None:
-        rs = ref.RS256(ecc_len=ecc)
-        t = time.perf_counter()
-        rs.encode(msgs.tobytes())
-        te = time.perf_counter() - t
-        mb = [bytearray(r[:k].tobytes()) for r in cws_corrupt[:repair_sample]]
-        pb = [bytearray(r[k:].tobytes()) for r in cws_corrupt[:repair_sample]]
-        t = time.perf_counter()
-        for m, p in zip(mb, pb):
-            rs.repair(m, p)
-        tr = time.perf_counter() - t
+    kind = "reference" if ref is not None else "port"
+    if wl["field"] == "rs256":
+        if ref is not None:
+            rs = ref.RS256(block_len=n, ecc_len=ecc)
+            t = time.perf_counter()
+            rs.encode(msgs.tobytes())
+            te = time.perf_counter() - t
+            mb = [bytearray(r[:k].tobytes()) for r in cws[:sample]]
+            pb = [bytearray(r[k:].tobytes()) for r in cws[:sample]]
+            t = time.perf_counter()
+            for m, p in zip(mb, pb):
+                rs.repair(m, p)
+            tr = time.perf_counter() - t
+        else:
+            from oracle import oracle as O
+            orc = O.RS256(ecc)
+            t = time.perf_counter()
+            orc.encode_batch(msgs, threads=1)
+            te = time.perf_counter() - t
+            m, p = np.ascontiguousarray(cws[:sample, :k]), np.ascontiguousarray(cws[:sample, k:])
+            t = time.perf_counter()
+            orc.decode_batch(m, p, threads=1)
+            tr = time.perf_counter() - t
+        elem = 1
     else:
-        kind = "port"
-        from oracle import oracle as O
-        orc = O.RS256(ecc)
-        t = time.perf_counter()
-        orc.encode_batch(msgs, threads=1)
-        te = time.perf_counter() - t
-        m = np.ascontiguousarray(cws_corrupt[:repair_sample, :k])
-        p = np.ascontiguousarray(cws_corrupt[:repair_sample, k:])
-        t = time.perf_counter()
-        orc.decode_batch(m, p, threads=1)
-        tr = time.perf_counter() - t
-    per_byte = te / msgs.size + tr / (repair_sample * k)
+        if ref is not None:
+            ri = ref.RSi16(n, ecc)  # the SIMD width the host supports (AVX-512: x16)
+            t = time.perf_counter()
+            ri.encode(msgs.tobytes())
+            te = time.perf_counter() - t
+            mb = [bytearray(r[:k].tobytes()) for r in cws[:sample]]
+            pb = [bytearray(r[k:].tobytes()) for r in cws[:sample]]
+            t = time.perf_counter()
+            for m, p in zip(mb, pb):
+                try:
+                    ri.repair(m, p)
+                except RuntimeError:
+                    pass
+            tr = time.perf_counter() - t
+        else:
+            from oracle import oracle as O
+            orc = O.RSi16(n, ecc)
+            t = time.perf_counter()
+            orc.encode(msgs, True, threads=1)
+            te = time.perf_counter() - t
+            m, p = np.ascontiguousarray(cws[:sample, :k]), np.ascontiguousarray(cws[:sample, k:])
+            t = time.perf_counter()
+            orc.repair_batch(m, p, threads=1)
+            tr = time.perf_counter() - t
+        elem = 2
+    N = msgs.shape[0]
+    per_byte = te / (N * k * elem) + tr / (sample * k * elem)
     return {
         "value": round(1.0 / per_byte / 1e9, 6),
         "unit": "GB/s",
         "cores": 1,
         "kind": kind,
-        "sample": f"encode of all {msgs.shape[0]} messages in one call ({te:.2f} s) + repair of the first "
-                  f"{repair_sample} corrupted codewords, one call each ({tr:.2f} s); single thread",
-        "encode_gbps": round(msgs.size / te / 1e9, 4),
-        "repair_gbps": round(repair_sample * k / tr / 1e9, 6),
+        "sample": f"encode of all {N} messages in one call ({te:.2f} s) + repair of the first {sample} "
+                  f"corrupted codewords, one call each ({tr:.2f} s); single thread, {os.cpu_count()} host CPUs",
+        "encode_gbps": round(N * k * elem / te / 1e9, 4),
+        "repair_gbps": round(sample * k * elem / tr / 1e9, 6),
     }
 
 
-def rocprof_traffic():
+def rocprof_traffic(workload):
     """HBM bytes per dispatch of each kernel, from the committed rocprofv3 --pmc summary
     (FETCH_SIZE and WRITE_SIZE scaled by what they read for a copy of known size)."""
-    path = os.path.join(ROOT, "profiles", "traffic_rs256_plain_scale.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}.json")
     try:
         with open(path) as f:
             return json.load(f)["kernels"]
@@ -110,33 +160,49 @@ def rocprof_traffic():
         return None
 
 
-def run_rs256_plain_scale(args):
+def run_plain(name, args):
+    """One step = encode of every message + repair of every corrupted codeword, with all
+    inputs resident in HBM; the corrupted input is restored by a device copy before each
+    step, outside the timed bracket."""
     from libffrs_amd import capi
     from libffrs_amd import workloads as W
 
-    N, ecc = args.codewords, 32
-    k = 255 - ecc
+    wl = dict(WORKLOADS[name])
+    if args.codewords:
+        wl["N"] = args.codewords
+    N, n, ecc = wl["N"], wl["n"], wl["ecc"]
+    k = n - ecc
+    rs256 = wl["field"] == "rs256"
+    elem = 1 if rs256 else 2
     t0 = time.perf_counter()
-    msgs, counts, s = W.rs256_plain(N, ecc, 16, seed=2, uncorrectable_frac=0.01)
+    if rs256:
+        msgs, counts, s = W.rs256_plain(N, ecc, wl["max_err"], seed=wl["seed"], uncorrectable_frac=wl["unc"])
+    else:
+        msgs, counts, s = W.rsi16_plain(N, n, ecc, wl["max_err"], seed=wl["seed"])
     log(f"generated {N} messages in {time.perf_counter() - t0:.1f} s")
 
     ctx = capi.Context(0)
     dev_name = ctx.info()[0]
-    codec = capi.RS256Codec(ctx, ecc)
-    d_msg = ctx.alloc(N * k)
-    d_par = ctx.alloc(N * ecc)
-    d_cm = ctx.alloc(N * k)     # pristine corrupted codewords
-    d_cp = ctx.alloc(N * ecc)
-    d_wm = ctx.alloc(N * k)     # working copy repaired in place
-    d_wp = ctx.alloc(N * ecc)
+    if rs256:
+        codec = capi.RS256Codec(ctx, ecc)
+        encode = lambda dm, dp: codec.encode_device(k, dm, N, dp)  # noqa: E731
+        repair = lambda dm, dp, ds: codec.repair_device(k, dm, dp, N, ds)  # noqa: E731
+    else:
+        codec = capi.RSi16Codec(ctx, n, ecc)
+        encode = lambda dm, dp: codec.encode_device(dm, N, dp, True)  # noqa: E731
+        repair = lambda dm, dp, ds: codec.repair_device(dm, dp, N, ds)  # noqa: E731
+    mb, pb = N * k * elem, N * ecc * elem
+    d_msg, d_par = ctx.alloc(mb), ctx.alloc(pb)
+    d_cm, d_cp = ctx.alloc(mb), ctx.alloc(pb)     # pristine corrupted codewords
+    d_wm, d_wp = ctx.alloc(mb), ctx.alloc(pb)     # working copy repaired in place
     d_st = ctx.alloc(N)
     d_msg.upload(msgs)
-    codec.encode_device(k, d_msg, N, d_par)
-    par = np.empty((N, ecc), np.uint8)
+    encode(d_msg, d_par)
+    par = np.empty((N, ecc), msgs.dtype)
     d_par.download(par)
     cws = np.concatenate([msgs, par], 1)
     t0 = time.perf_counter()
-    W.rs256_corrupt(cws, counts, s)
+    (W.rs256_corrupt if rs256 else W.rsi16_corrupt)(cws, counts, s)
     log(f"corrupted codewords in {time.perf_counter() - t0:.1f} s")
     d_cm.upload(np.ascontiguousarray(cws[:, :k]))
     d_cp.upload(np.ascontiguousarray(cws[:, k:]))
@@ -146,9 +212,9 @@ def run_rs256_plain_scale(args):
         d_wp.copy_from(d_cp)
         ctx.sync()
         t = time.perf_counter()
-        codec.encode_device(k, d_msg, N, d_par)
+        encode(d_msg, d_par)
         ke = ctx.kernel_times() if instrumented else None
-        codec.repair_device(k, d_wm, d_wp, N, d_st)
+        repair(d_wm, d_wp, d_st)
         kr = ctx.kernel_times() if instrumented else None
         ctx.sync()
         return time.perf_counter() - t, ke, kr
@@ -160,7 +226,7 @@ def run_rs256_plain_scale(args):
     # the same steps again with a HIP event after every kernel, for the per-kernel split
     ctx.kernel_timing(True)
     kes, krs = [], []
-    for _ in range(args.steps):
+    for _ in range(max(args.steps, 1)):
         _, ke, kr = step(True)
         kes.append(ke)
         krs.append(kr)
@@ -168,37 +234,44 @@ def run_rs256_plain_scale(args):
     st = np.empty(N, np.uint8)
     d_st.download(st)
     ms_step = 1e3 * float(np.mean(walls))
-    msg_bytes = N * k
+    msg_bytes = N * k * elem
     value = msg_bytes / (ms_step * 1e-3) / 1e9
-    # per-kernel durations from HIP events on the codec stream (launch order)
-    enc_names = ["rs256_encode_kernel"]
-    rep_names = ["rs256_synd_kernel", "rs256_bm_fixed_kernel", "rs256_chien_kernel"]
+    if rs256:
+        enc_names = ["rs256_encode_kernel"]
+        rep_names = (["rs256_synd_kernel", "rs256_bm_fixed_kernel", "rs256_chien_kernel"] if ecc <= 32
+                     else ["rs256_repair_kernel"])
+        if ecc <= 32 and ecc not in (2, 4, 8, 16, 32):
+            rep_names[1] = "rs256_bm_kernel"
+    else:
+        enc_names = ["rsi16_encode_kernel"]
+        rep_names = ["rsi16_repair_kernel"]
     ke_ms = np.mean(np.array(kes), axis=0)
     kr_ms = np.mean(np.array(krs), axis=0)
     if len(ke_ms) != len(enc_names) or len(kr_ms) != len(rep_names):
         raise RuntimeError(f"unexpected kernel count: encode {len(ke_ms)}, repair {len(kr_ms)}")
     te_ms, tr_ms = float(ke_ms.sum()), float(kr_ms.sum())
-    # algorithmic bytes per launch: encode reads k + writes ecc per codeword; repair reads
-    # n per codeword and writes one status byte (corrections are sparse and left out)
-    enc_bytes = N * (k + ecc)
-    rep_bytes = N * (k + ecc) + N
-    traffic = rocprof_traffic()
+    # algorithmic bytes per launch: encode reads the message and writes the parity; repair
+    # reads the codeword and writes one status byte (corrections are sparse and left out)
+    enc_bytes = N * (k + ecc) * elem
+    rep_bytes = N * n * elem + N
+    traffic = rocprof_traffic(name)
+    prefix = wl["field"]
     launches = {
-        "rs256_encode": {"kernels": enc_names, "ms": te_ms, "bytes": enc_bytes},
-        "rs256_repair": {"kernels": rep_names, "ms": tr_ms, "bytes": rep_bytes},
+        prefix + "_encode": {"kernels": enc_names, "ms": te_ms, "bytes": enc_bytes},
+        prefix + "_repair": {"kernels": rep_names, "ms": tr_ms, "bytes": rep_bytes},
     }
-    dom = max(launches, key=lambda n: launches[n]["ms"])
+    dom = max(launches, key=lambda x: launches[x]["ms"])
     d = launches[dom]
     achieved = d["bytes"] / (d["ms"] * 1e-3) / 1e9
     dom_traffic = None
-    if traffic and all(n in traffic for n in d["kernels"]):
-        dom_traffic = int(sum(traffic[n]["bytes"] for n in d["kernels"]))
+    if traffic and all(x in traffic for x in d["kernels"]):
+        dom_traffic = int(sum(traffic[x]["bytes"] for x in d["kernels"]))
     kernels = {}
     for names, times in ((enc_names, ke_ms), (rep_names, kr_ms)):
-        for n, t in zip(names, times):
-            kernels[n] = {"ms": round(float(t), 4)}
-            if traffic and n in traffic:
-                kernels[n]["traffic"] = int(traffic[n]["bytes"])
+        for x, t in zip(names, times):
+            kernels[x] = {"ms": round(float(t), 4)}
+            if traffic and x in traffic:
+                kernels[x]["traffic"] = int(traffic[x]["bytes"])
     result = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -210,22 +283,22 @@ def run_rs256_plain_scale(args):
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "u8",
-        "data": "synthetic (SplitMix64 stream, seed 2)",
-        "config": {"workload": "rs256_plain_scale", "codewords": N, "ecc_len": ecc, "message_len": k,
-                   "errors": "0-16 per codeword, 1% with 17-20", "device": dev_name},
+        "dtype": "u8" if rs256 else "u32 (GF(65537) residues of u16 symbols)",
+        "data": f"synthetic (SplitMix64 stream, seed {wl['seed']})",
+        "config": {"workload": name, "codewords": N, "block_len": n, "ecc_len": ecc, "message_len": k,
+                   "errors": wl["errors"], "device": dev_name},
         "roofline": {"bound": "hbm", "kernel": dom + " (" + " + ".join(d["kernels"]) + ")",
                      "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": dom_traffic},
-        "launches": {n: {"ms": round(v["ms"], 4), "alg_bytes": v["bytes"],
+        "launches": {x: {"ms": round(v["ms"], 4), "alg_bytes": v["bytes"],
                          "alg_gbps": round(v["bytes"] / v["ms"] / 1e6, 2),
-                         "msg_gbps": round(msg_bytes / v["ms"] / 1e6, 2)} for n, v in launches.items()},
+                         "msg_gbps": round(msg_bytes / v["ms"] / 1e6, 2)} for x, v in launches.items()},
         "kernels": kernels,
-        "repaired_true": int(st.sum()),
+        "status_counts": {int(v): int(c) for v, c in zip(*np.unique(st, return_counts=True))},
     }
     if not args.no_cpu_baseline:
         log("timing the CPU baseline ...")
-        result["cpu_baseline"] = cpu_baseline(msgs, cws, ecc, k, min(N, args.cpu_repair_sample))
+        result["cpu_baseline"] = cpu_baseline(wl, msgs, cws, min(N, args.cpu_repair_sample or wl["cpu_repair"]))
     for b in (d_msg, d_par, d_cm, d_cp, d_wm, d_wp, d_st):
         b.free()
     codec.close()
@@ -238,8 +311,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--codewords", type=int, default=1048576)
-    ap.add_argument("--cpu-repair-sample", type=int, default=262144)
+    ap.add_argument("--workload", default="rs256_plain_scale", choices=sorted(WORKLOADS))
+    ap.add_argument("--codewords", type=int, default=0, help="override the workload's codeword count")
+    ap.add_argument("--cpu-repair-sample", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -248,7 +322,7 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     ensure_built()
-    res = run_rs256_plain_scale(args)
+    res = run_plain(args.workload, args)
     print(json.dumps(res), flush=True)
 
 
