@@ -142,6 +142,38 @@ int ffrs_rsi16_encode_device(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count
 int ffrs_rsi16_repair_device(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count,
                              uint8_t *d_status);
 
+/* ---- mixed-size batches --------------------------------------------------------------
+ * No counterpart in the reference, whose codec objects serve one (block_len, ecc_len) each
+ * (pyrs256.hpp:58-97, pyrsi16.hpp:41-125): a plan lists `count` codewords, each with its own
+ * lengths and element offsets into one message buffer and one parity buffer. Running a plan
+ * is one persistent kernel that takes chunks of codewords from a device work queue (plus a
+ * one-thread reset of the queue); with use_graph set, the first run for a set of buffers is
+ * captured in a HIP graph and later runs replay it. Runs are asynchronous on the context's
+ * stream; status[i] belongs to the i-th codeword of the list. */
+typedef struct ffrs_rs256_plan ffrs_rs256_plan;
+typedef struct ffrs_rsi16_plan ffrs_rsi16_plan;
+
+/* RS256: offsets in bytes; message_len[i] + ecc_len[i] <= 255. */
+int ffrs_rs256_plan_create(ffrs_ctx *ctx, size_t count, const uint8_t *ecc_len, const uint8_t *message_len,
+                           const uint64_t *msg_off, const uint64_t *ecc_off, ffrs_rs256_plan **out);
+void ffrs_rs256_plan_destroy(ffrs_rs256_plan *plan);
+int ffrs_rs256_plan_use_graph(ffrs_rs256_plan *plan, int enable);
+int ffrs_rs256_plan_encode(ffrs_rs256_plan *plan, const uint8_t *d_msgs, size_t msg_bytes, uint8_t *d_eccs,
+                           size_t ecc_bytes);
+int ffrs_rs256_plan_repair(ffrs_rs256_plan *plan, uint8_t *d_msgs, size_t msg_bytes, uint8_t *d_eccs,
+                           size_t ecc_bytes, uint8_t *d_status);
+
+/* RSi16: offsets and sizes in u16 symbols; every (block_len, ecc_len) must be accepted by
+ * ffrs_rsi16_create (primitive 3). */
+int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_len, const uint16_t *ecc_len,
+                           const uint64_t *msg_off, const uint64_t *ecc_off, ffrs_rsi16_plan **out);
+void ffrs_rsi16_plan_destroy(ffrs_rsi16_plan *plan);
+int ffrs_rsi16_plan_use_graph(ffrs_rsi16_plan *plan, int enable);
+int ffrs_rsi16_plan_encode(ffrs_rsi16_plan *plan, const uint16_t *d_msgs, size_t msg_elems, uint16_t *d_eccs,
+                           size_t ecc_elems, int vector_lanes);
+int ffrs_rsi16_plan_repair(ffrs_rsi16_plan *plan, uint16_t *d_msgs, size_t msg_elems, uint16_t *d_eccs,
+                           size_t ecc_elems, uint8_t *d_status);
+
 #ifdef __cplusplus
 }
 #endif

@@ -60,6 +60,16 @@ SIGNATURES = [
     ("ffrs_rsi16_roots", c_int, [c_vp, c_vp, c_vp]),
     ("ffrs_rsi16_encode_device", c_int, [c_vp, c_vp, c_sz, c_vp, c_int]),
     ("ffrs_rsi16_repair_device", c_int, [c_vp, c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_rs256_plan_create", c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    ("ffrs_rs256_plan_destroy", None, [c_vp]),
+    ("ffrs_rs256_plan_use_graph", c_int, [c_vp, c_int]),
+    ("ffrs_rs256_plan_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz]),
+    ("ffrs_rs256_plan_repair", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_vp]),
+    ("ffrs_rsi16_plan_create", c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    ("ffrs_rsi16_plan_destroy", None, [c_vp]),
+    ("ffrs_rsi16_plan_use_graph", c_int, [c_vp, c_int]),
+    ("ffrs_rsi16_plan_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_int]),
+    ("ffrs_rsi16_plan_repair", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_vp]),
 ]
 
 
@@ -189,3 +199,64 @@ class RSi16Codec:
 
     def repair_device(self, d_msg, d_ecc, count, d_status):
         check(lib().ffrs_rsi16_repair_device(self.h, d_msg.ptr, d_ecc.ptr, count, d_status.ptr), "rsi16 repair")
+
+
+def _arr(a, dtype):
+    import numpy as np
+    return np.ascontiguousarray(a, dtype=dtype)
+
+
+class RS256Plan:
+    """A mixed-size RS256 work list (per-codeword ecc_len, message_len and byte offsets)."""
+
+    def __init__(self, ctx, ecc_len, message_len, msg_off, ecc_off, graph=False):
+        import numpy as np
+        self._keep = [_arr(ecc_len, np.uint8), _arr(message_len, np.uint8), _arr(msg_off, np.uint64),
+                      _arr(ecc_off, np.uint64)]
+        h = c_vp()
+        check(lib().ffrs_rs256_plan_create(ctx.h, len(self._keep[0]), *[a.ctypes.data for a in self._keep],
+                                           ctypes.byref(h)), "ffrs_rs256_plan_create")
+        self.h, self.ctx = h, ctx
+        if graph:
+            check(lib().ffrs_rs256_plan_use_graph(self.h, 1), "use_graph")
+
+    def close(self):
+        if self.h:
+            lib().ffrs_rs256_plan_destroy(self.h)
+            self.h = None
+
+    def encode(self, d_msgs, d_eccs):
+        check(lib().ffrs_rs256_plan_encode(self.h, d_msgs.ptr, d_msgs.nbytes, d_eccs.ptr, d_eccs.nbytes),
+              "rs256 plan encode")
+
+    def repair(self, d_msgs, d_eccs, d_status):
+        check(lib().ffrs_rs256_plan_repair(self.h, d_msgs.ptr, d_msgs.nbytes, d_eccs.ptr, d_eccs.nbytes,
+                                           d_status.ptr), "rs256 plan repair")
+
+
+class RSi16Plan:
+    """A mixed-size RSi16 work list (per-codeword block_len, ecc_len and u16 offsets)."""
+
+    def __init__(self, ctx, block_len, ecc_len, msg_off, ecc_off, graph=False):
+        import numpy as np
+        self._keep = [_arr(block_len, np.uint16), _arr(ecc_len, np.uint16), _arr(msg_off, np.uint64),
+                      _arr(ecc_off, np.uint64)]
+        h = c_vp()
+        check(lib().ffrs_rsi16_plan_create(ctx.h, len(self._keep[0]), *[a.ctypes.data for a in self._keep],
+                                           ctypes.byref(h)), "ffrs_rsi16_plan_create")
+        self.h, self.ctx = h, ctx
+        if graph:
+            check(lib().ffrs_rsi16_plan_use_graph(self.h, 1), "use_graph")
+
+    def close(self):
+        if self.h:
+            lib().ffrs_rsi16_plan_destroy(self.h)
+            self.h = None
+
+    def encode(self, d_msgs, d_eccs, vector_lanes=True):
+        check(lib().ffrs_rsi16_plan_encode(self.h, d_msgs.ptr, d_msgs.nbytes // 2, d_eccs.ptr, d_eccs.nbytes // 2,
+                                           1 if vector_lanes else 0), "rsi16 plan encode")
+
+    def repair(self, d_msgs, d_eccs, d_status):
+        check(lib().ffrs_rsi16_plan_repair(self.h, d_msgs.ptr, d_msgs.nbytes // 2, d_eccs.ptr, d_eccs.nbytes // 2,
+                                           d_status.ptr), "rsi16 plan repair")

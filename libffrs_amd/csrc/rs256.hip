@@ -308,6 +308,123 @@ __host__ __device__ inline uint32_t repair_row_stride(uint32_t ecc)
     return dw * 4;
 }
 
+// Lane-mode repair of one codeword (reed_solomon.hpp:559-595): message of k bytes at
+// byte offset msg_off of `rsrc` (the same bytes as pm[0..k)), parity pe[0..E). Returns the
+// status (1: no errors or repaired, 0: not repairable, buffers untouched). `row` is this
+// lane's LDS scratch of repair_row_stride(E) bytes.
+template <int EW, int S>
+__device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, int64_t msg_off, uint8_t *pm, uint8_t *pe,
+                                     uint32_t k, uint32_t E, const uint32_t *__restrict__ s_lut, const GfLds &gf,
+                                     uint8_t *row)
+{
+    const uint32_t n = k + E;
+    uint8_t *Sy = row;
+    uint8_t *lam = row + E;
+    uint8_t *Bx = row + 2 * E;
+    uint16_t *llam = reinterpret_cast<uint16_t *>(row + ((3 * E + 1) & ~1u));
+    // ---- R(x) = C(x) mod g(x): remainder of the message XOR the received parity
+    uint32_t rem[EW];
+    message_remainder<EW, S>(rsrc, msg_off, k, s_lut, rem);
+    if (xor_parity<EW>(rem, pe, E) == 0)  // all syndromes zero (reed_solomon.hpp:565-566)
+        return 1;
+    syndromes_from_remainder<EW>(rem, E, gf, Sy);
+
+    // ---- Berlekamp-Massey (reed_solomon.hpp:655-702): locator capacity E, degree
+    //      terms >= E dropped, L returned. Natural order lam[i] = Lambda_i.
+    for (uint32_t i = 0; i < E; ++i) {
+        lam[i] = 0;
+        Bx[i] = 0;
+    }
+    lam[0] = 1;
+    Bx[0] = 1;
+    uint32_t L = 0, m = 1, b = 1;
+    for (uint32_t nn = 0; nn < E; ++nn) {
+        uint32_t d = Sy[nn];
+        for (uint32_t i = 1; i <= L; ++i)
+            d ^= gf.mul(lam[i], Sy[nn - i]);
+        if (d == 0) {
+            m += 1;
+        } else {
+            uint32_t c = gf.div(d, b);
+            uint32_t lc = gf.log2[c];
+            bool grow = 2 * L <= nn;
+            // lam <- lam - c x^m B ; if grow, B <- old lam (downwards keeps B[i-m] unread-after-write)
+            for (uint32_t ii = E; ii-- > 0;) {
+                uint32_t old = lam[ii];
+                uint32_t sb = ii >= m ? (uint32_t)Bx[ii - m] : 0u;
+                lam[ii] = (uint8_t)(old ^ gf.mul_log(lc, sb));
+                if (grow)
+                    Bx[ii] = (uint8_t)old;
+            }
+            if (grow) {
+                L = nn + 1 - L;
+                b = d;
+                m = 1;
+            } else {
+                m += 1;
+            }
+        }
+    }
+    if (L > E / 2)
+        return 0;
+
+    // ---- Omega = S(x) Lambda(x) mod x^E (reed_solomon.hpp:708-728); B(x) is dead now
+    for (uint32_t i = 0; i <= L; ++i)
+        llam[i] = gf.log2[lam[i]];
+    uint8_t *om = Bx;
+    for (uint32_t kk = 0; kk < E; ++kk) {
+        uint32_t v = 0;
+        uint32_t top = kk < L ? kk : L;
+        for (uint32_t i = 0; i <= top; ++i)
+            v ^= gf.exp2[llam[i] + gf.log2[Sy[kk - i]]];
+        om[kk] = (uint8_t)v;
+    }
+
+    // ---- Chien search over [0, n): Lambda(alpha^-p) == 0, ascending p
+    //      (reed_solomon.hpp:516-538); positions go to the dead syndrome row
+    uint8_t *pos = Sy;
+    uint32_t roots = 0;
+    for (uint32_t p = 0; p < n; ++p) {
+        uint32_t pinv = p ? 255 - p : 0;  // alpha^-p = alpha^(255 - p)
+        uint32_t e = 0, acc = 0;
+        for (uint32_t i = 0; i <= L; ++i) {
+            acc ^= gf.exp2[llam[i] + e];
+            e += pinv;
+            e = e >= 255 ? e - 255 : e;
+        }
+        if (acc == 0) {
+            if (roots < E)
+                pos[roots] = (uint8_t)p;
+            roots++;
+        }
+    }
+    if (roots != L)
+        return 0;
+
+    // ---- Forney (reed_solomon.hpp:730-743): e_p = X Omega(X^-1) / Lambda'(X^-1),
+    //      X = alpha^p, Lambda' = odd terms of Lambda (characteristic 2)
+    for (uint32_t r = 0; r < L; ++r) {
+        uint32_t p = pos[r];
+        uint32_t pinv = p ? 255 - p : 0;
+        uint32_t num = 0, den = 0, e = 0;
+        for (uint32_t kk = 0; kk < E; ++kk) {
+            num ^= gf.exp2[gf.log2[om[kk]] + e];
+            if ((kk & 1) == 0 && kk + 1 <= L)
+                den ^= gf.exp2[llam[kk + 1] + e];
+            e += pinv;
+            e = e >= 255 ? e - 255 : e;
+        }
+        uint32_t X = gf.exp2[p];
+        uint32_t mag = gf.mul(X, gf.div(num, den));
+        uint32_t at = n - 1 - p;
+        if (at < k)
+            pm[at] ^= (uint8_t)mag;
+        else
+            pe[at - k] ^= (uint8_t)mag;
+    }
+    return 1;
+}
+
 template <int EW, int S, int WG>
 __global__ __launch_bounds__(WG) void rs256_repair_kernel(uint8_t *__restrict__ msgs, uint32_t msg_bytes,
                                                            uint8_t *__restrict__ eccs, uint32_t count, uint32_t k,
@@ -327,127 +444,12 @@ __global__ __launch_bounds__(WG) void rs256_repair_kernel(uint8_t *__restrict__ 
         s_log2[i] = tab.log2[i];
     __syncthreads();
     GfLds gf{s_exp2, s_log2};
-
-    const uint32_t E = ecc;
-    const uint32_t n = k + E;
-    uint8_t *row = s_rows + threadIdx.x * repair_row_stride(E);
-    uint8_t *Sy = row;
-    uint8_t *lam = row + E;
-    uint8_t *Bx = row + 2 * E;
-    uint16_t *llam = reinterpret_cast<uint16_t *>(row + ((3 * E + 1) & ~1u));
-
+    uint8_t *row = s_rows + threadIdx.x * repair_row_stride(ecc);
     __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
     uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += stride) {
-        // ---- R(x) = C(x) mod g(x): remainder of the message XOR the received parity
-        uint32_t rem[EW];
-        message_remainder<EW, S>(rsrc, (int64_t)cw * k, k, s_lut, rem);
-        uint8_t *pe = eccs + (size_t)cw * E;
-        if (xor_parity<EW>(rem, pe, E) == 0) {  // all syndromes zero (reed_solomon.hpp:565-566)
-            status[cw] = 1;
-            continue;
-        }
-        syndromes_from_remainder<EW>(rem, E, gf, Sy);
-
-        // ---- Berlekamp-Massey (reed_solomon.hpp:655-702): locator capacity E, degree
-        //      terms >= E dropped, L returned. Natural order lam[i] = Lambda_i.
-        for (uint32_t i = 0; i < E; ++i) {
-            lam[i] = 0;
-            Bx[i] = 0;
-        }
-        lam[0] = 1;
-        Bx[0] = 1;
-        uint32_t L = 0, m = 1, b = 1;
-        for (uint32_t nn = 0; nn < E; ++nn) {
-            uint32_t d = Sy[nn];
-            for (uint32_t i = 1; i <= L; ++i)
-                d ^= gf.mul(lam[i], Sy[nn - i]);
-            if (d == 0) {
-                m += 1;
-            } else {
-                uint32_t c = gf.div(d, b);
-                uint32_t lc = gf.log2[c];
-                bool grow = 2 * L <= nn;
-                // lam <- lam - c x^m B ; if grow, B <- old lam (downwards keeps B[i-m] unread-after-write)
-                for (uint32_t ii = E; ii-- > 0;) {
-                    uint32_t old = lam[ii];
-                    uint32_t sb = ii >= m ? (uint32_t)Bx[ii - m] : 0u;
-                    lam[ii] = (uint8_t)(old ^ gf.mul_log(lc, sb));
-                    if (grow)
-                        Bx[ii] = (uint8_t)old;
-                }
-                if (grow) {
-                    L = nn + 1 - L;
-                    b = d;
-                    m = 1;
-                } else {
-                    m += 1;
-                }
-            }
-        }
-        if (L > E / 2) {
-            status[cw] = 0;
-            continue;
-        }
-
-        // ---- Omega = S(x) Lambda(x) mod x^E (reed_solomon.hpp:708-728); B(x) is dead now
-        for (uint32_t i = 0; i <= L; ++i)
-            llam[i] = gf.log2[lam[i]];
-        uint8_t *om = Bx;
-        for (uint32_t kk = 0; kk < E; ++kk) {
-            uint32_t v = 0;
-            uint32_t top = kk < L ? kk : L;
-            for (uint32_t i = 0; i <= top; ++i)
-                v ^= gf.exp2[llam[i] + gf.log2[Sy[kk - i]]];
-            om[kk] = (uint8_t)v;
-        }
-
-        // ---- Chien search over [0, n): Lambda(alpha^-p) == 0, ascending p
-        //      (reed_solomon.hpp:516-538); positions go to the dead syndrome row
-        uint8_t *pos = Sy;
-        uint32_t roots = 0;
-        for (uint32_t p = 0; p < n; ++p) {
-            uint32_t pinv = p ? 255 - p : 0;  // alpha^-p = alpha^(255 - p)
-            uint32_t e = 0, acc = 0;
-            for (uint32_t i = 0; i <= L; ++i) {
-                acc ^= gf.exp2[llam[i] + e];
-                e += pinv;
-                e = e >= 255 ? e - 255 : e;
-            }
-            if (acc == 0) {
-                if (roots < E)
-                    pos[roots] = (uint8_t)p;
-                roots++;
-            }
-        }
-        if (roots != L) {
-            status[cw] = 0;
-            continue;
-        }
-
-        // ---- Forney (reed_solomon.hpp:730-743): e_p = X Omega(X^-1) / Lambda'(X^-1),
-        //      X = alpha^p, Lambda' = odd terms of Lambda (characteristic 2)
-        for (uint32_t r = 0; r < L; ++r) {
-            uint32_t p = pos[r];
-            uint32_t pinv = p ? 255 - p : 0;
-            uint32_t num = 0, den = 0, e = 0;
-            for (uint32_t kk = 0; kk < E; ++kk) {
-                num ^= gf.exp2[gf.log2[om[kk]] + e];
-                if ((kk & 1) == 0 && kk + 1 <= L)
-                    den ^= gf.exp2[llam[kk + 1] + e];
-                e += pinv;
-                e = e >= 255 ? e - 255 : e;
-            }
-            uint32_t X = gf.exp2[p];
-            uint32_t mag = gf.mul(X, gf.div(num, den));
-            uint32_t at = n - 1 - p;
-            if (at < k)
-                msgs[(size_t)cw * k + at] ^= (uint8_t)mag;
-            else
-                pe[at - k] ^= (uint8_t)mag;
-        }
-        status[cw] = 1;
-    }
+    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += stride)
+        status[cw] = repair_one<EW, S>(rsrc, (int64_t)cw * k, msgs + (size_t)cw * k, eccs + (size_t)cw * ecc, k, ecc,
+                                       s_lut, gf, row);
 }
 
 template <int EW, int S>
@@ -918,6 +920,123 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
             const uint32_t at = n - 1 - p;
             xor_byte(at < k ? mrow + at : erow + (at - k), mag);
         }
+    }
+}
+
+
+// ======================================================================
+// Mixed-size batches: a plan holds a work list of codewords with their own message and
+// ecc lengths, sorted by ecc then by descending length and cut into chunks of at most
+// WG codewords of one ecc. One persistent launch serves the whole list: each workgroup
+// takes the next chunk from a device counter, loads that ecc's LFSR table into LDS when
+// it differs from the one it holds, and runs one codeword per lane.
+// ======================================================================
+
+struct PlanItem {
+    uint64_t msg_off;  // byte offset of the message in the message buffer
+    uint64_t ecc_off;  // byte offset of the parity in the parity buffer
+    uint32_t idx;      // position of the codeword in the caller's list (status index)
+    uint16_t k;
+    uint8_t E;
+    uint8_t cls;
+};
+
+struct PlanChunk {
+    uint32_t first;
+    uint16_t count;
+    uint16_t cls;
+};
+
+struct PlanClass {
+    const uint32_t *lut;
+    uint32_t ecc, EW, S;
+};
+
+__global__ void plan_reset_kernel(uint32_t *queue) { *queue = 0; }
+
+template <int EW, int S, bool REPAIR>
+__device__ inline void plan_item(__amdgpu_buffer_rsrc_t rsrc, uint8_t *msgs, uint8_t *eccs, uint8_t *status,
+                                 const PlanItem &it, const uint32_t *s_lut, const GfLds &gf, uint8_t *row)
+{
+    if constexpr (REPAIR) {
+        status[it.idx] = repair_one<EW, S>(rsrc, (int64_t)it.msg_off, msgs + it.msg_off, eccs + it.ecc_off, it.k,
+                                           it.E, s_lut, gf, row);
+    } else {
+        uint32_t rem[EW];
+        message_remainder<EW, S>(rsrc, (int64_t)it.msg_off, it.k, s_lut, rem);
+        uint8_t *o = eccs + it.ecc_off;
+        if ((it.E & 3) == 0 && (it.ecc_off & 3) == 0) {
+            uint32_t *o4 = reinterpret_cast<uint32_t *>(o);
+#pragma unroll
+            for (int d = 0; d < EW; ++d)
+                if ((uint32_t)d * 4 < it.E)
+                    o4[d] = rem[d];
+        } else {
+            for (uint32_t j = 0; j < it.E; ++j)
+                o[j] = rem_byte<EW>(rem, j);
+        }
+    }
+}
+
+template <int MAXEW, int WG, bool REPAIR>
+__global__ __launch_bounds__(WG) void rs256_plan_kernel(uint8_t *__restrict__ msgs, uint32_t msg_bytes,
+                                                         uint8_t *__restrict__ eccs, uint8_t *__restrict__ status,
+                                                         const PlanItem *__restrict__ items,
+                                                         const PlanChunk *__restrict__ chunks, uint32_t nchunks,
+                                                         const PlanClass *__restrict__ classes, uint32_t *queue,
+                                                         uint32_t lut_words, uint32_t row_stride, Rs256Tables tab)
+{
+    extern __shared__ uint32_t s_mem[];
+    __shared__ uint32_t s_chunk;
+    uint32_t *s_lut = s_mem;
+    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem + lut_words);
+    uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
+    uint8_t *s_rows = reinterpret_cast<uint8_t *>(s_log2 + 256);
+    if constexpr (REPAIR) {
+        for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += WG)
+            s_exp2[i] = tab.exp2[i];
+        for (uint32_t i = threadIdx.x; i < 256u; i += WG)
+            s_log2[i] = tab.log2[i];
+    }
+    GfLds gf{s_exp2, s_log2};
+    uint8_t *row = s_rows + threadIdx.x * row_stride;
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    uint32_t loaded = 0xffffffffu;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_chunk = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t c = s_chunk;
+        if (c >= nchunks)
+            break;
+        const PlanChunk ch = chunks[c];
+        const PlanClass cl = classes[ch.cls];
+        if (ch.cls != loaded) {
+            const uint32_t words = cl.S * 256u * (uint32_t)lut_row((int)cl.EW);
+            for (uint32_t i = threadIdx.x; i < words; i += WG)
+                s_lut[i] = cl.lut[i];
+            loaded = ch.cls;
+        }
+        __syncthreads();
+        if (threadIdx.x < ch.count) {
+            const PlanItem it = items[ch.first + threadIdx.x];
+            switch (cl.EW) {
+            case 1: plan_item<1, 4, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
+            case 2: plan_item<2, 4, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
+            case 4: plan_item<4, 4, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
+            case 8: plan_item<8, 4, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
+            case 16: plan_item<16, 2, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
+            default:
+                if constexpr (MAXEW >= 32) {
+                    if (cl.EW == 32)
+                        plan_item<32, 1, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row);
+                    else
+                        plan_item<64, 1, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row);
+                }
+                break;
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -1457,6 +1576,237 @@ int ffrs_rs256_synds(ffrs_rs256 *rs, size_t message_len, const uint8_t *msg, con
     FFRS_HIP(hipMemcpyAsync(synds, d_o, eb, hipMemcpyDeviceToHost, ctx->stream));
     FFRS_HIP(hipStreamSynchronize(ctx->stream));
     return FFRS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- mixed-size plans
+
+struct ffrs_rs256_plan {
+    ffrs_ctx *ctx = nullptr;
+    std::vector<ffrs_rs256 *> codecs;  // one per distinct ecc_len (class)
+    PlanItem *d_items = nullptr;
+    PlanChunk *d_chunks = nullptr;
+    PlanClass *d_classes = nullptr;
+    uint32_t *d_queue = nullptr;
+    uint32_t count = 0, nchunks = 0, max_ew = 1, max_e = 1, wg = 256, lut_words = 0;
+    uint64_t msg_end = 0, ecc_end = 0;
+    bool use_graph = false;
+    struct Graph {
+        int mode;
+        const void *m, *e, *st;
+        size_t mb;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+};
+
+namespace {
+
+void plan_free(ffrs_rs256_plan *p)
+{
+    (void)hipSetDevice(p->ctx->device);
+    for (auto &g : p->graphs)
+        (void)hipGraphExecDestroy(g.exec);
+    for (void *d : {(void *)p->d_items, (void *)p->d_chunks, (void *)p->d_classes, (void *)p->d_queue})
+        if (d)
+            (void)hipFree(d);
+    for (auto *c : p->codecs)
+        ffrs_rs256_destroy(c);
+    delete p;
+}
+
+template <int MAXEW, int WG, bool REPAIR>
+int plan_launch(ffrs_rs256_plan *p, uint8_t *msgs, size_t msg_bytes, uint8_t *eccs, uint8_t *status)
+{
+    ffrs_ctx *ctx = p->ctx;
+    const ffrs_rs256 *c0 = p->codecs[0];
+    Rs256Tables tab{nullptr, c0->d_exp2, c0->d_log2};
+    uint32_t row_stride = REPAIR ? repair_row_stride(p->max_e) : 0;
+    size_t lds = (size_t)p->lut_words * 4 + (REPAIR ? EXP2_LEN + 512 + (size_t)WG * row_stride : 0);
+    if (lds > 160 * 1024)
+        return fail(FFRS_ERR_INVALID, "plan workspace of %zu bytes exceeds LDS", lds);
+    auto kern = rs256_plan_kernel<MAXEW, WG, REPAIR>;
+    if (int r = allow_lds(kern, lds))
+        return r;
+    int per_cu = (int)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(lds, 1));
+    int grid = persistent_grid(ctx, std::min(per_cu, 2048 / WG), p->nchunks);
+    hipLaunchKernelGGL(plan_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, p->d_queue);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), lds, ctx->stream, msgs, (uint32_t)msg_bytes, eccs, status,
+                       p->d_items, p->d_chunks, p->nchunks, p->d_classes, p->d_queue, p->lut_words, row_stride, tab);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+int plan_enqueue(ffrs_rs256_plan *p, int mode, uint8_t *msgs, size_t msg_bytes, uint8_t *eccs, uint8_t *status)
+{
+    bool small = p->max_ew <= 16;
+    if (mode == 0)
+        return small ? plan_launch<16, 256, false>(p, msgs, msg_bytes, eccs, status)
+                     : plan_launch<64, 64, false>(p, msgs, msg_bytes, eccs, status);
+    return small ? plan_launch<16, 256, true>(p, msgs, msg_bytes, eccs, status)
+                 : plan_launch<64, 64, true>(p, msgs, msg_bytes, eccs, status);
+}
+
+int plan_run(ffrs_rs256_plan *p, int mode, uint8_t *msgs, size_t msg_bytes, uint8_t *eccs, uint8_t *status)
+{
+    ffrs_ctx *ctx = p->ctx;
+    if (int r = ctx_set_device(ctx))
+        return r;
+    if (p->count == 0)
+        return FFRS_OK;
+    if (msg_bytes < p->msg_end || msg_bytes >= ((size_t)1 << 31))
+        return fail(FFRS_ERR_SIZE, "message buffer of %zu bytes: the plan needs %llu (and < 2 GiB)", msg_bytes,
+                    (unsigned long long)p->msg_end);
+    marks_begin(ctx);
+    if (!p->use_graph) {
+        if (int r = plan_enqueue(p, mode, msgs, msg_bytes, eccs, status))
+            return r;
+        mark_kernel(ctx);
+        return FFRS_OK;
+    }
+    for (auto &g : p->graphs)
+        if (g.mode == mode && g.m == msgs && g.e == eccs && g.st == status && g.mb == msg_bytes) {
+            FFRS_HIP(hipGraphLaunch(g.exec, ctx->stream));
+            mark_kernel(ctx);
+            return FFRS_OK;
+        }
+    // first run with these buffers: capture the reset and the persistent launch
+    hipGraph_t graph = nullptr;
+    FFRS_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int r = plan_enqueue(p, mode, msgs, msg_bytes, eccs, status);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (r)
+        return r;
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+    p->graphs.push_back({mode, msgs, eccs, status, msg_bytes, exec});
+    FFRS_HIP(hipGraphLaunch(exec, ctx->stream));
+    mark_kernel(ctx);
+    return FFRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffrs_rs256_plan_create(ffrs_ctx *ctx, size_t count, const uint8_t *ecc_len, const uint8_t *message_len,
+                           const uint64_t *msg_off, const uint64_t *ecc_off, ffrs_rs256_plan **out)
+{
+    if (!ctx || !out || (count && (!ecc_len || !message_len || !msg_off || !ecc_off)))
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    if (count > 0xffffffffu)
+        return fail(FFRS_ERR_INVALID, "too many codewords in one plan");
+    for (size_t i = 0; i < count; ++i)
+        if (ecc_len[i] < 1 || message_len[i] < 1 || (unsigned)message_len[i] + ecc_len[i] > 255)
+            return fail(FFRS_ERR_INVALID, "codeword %zu: message_len %u, ecc_len %u (block_len must be <= 255)", i,
+                        message_len[i], ecc_len[i]);
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    auto *p = new ffrs_rs256_plan();
+    p->ctx = ctx;
+    p->count = (uint32_t)count;
+    int cls_of[256];
+    std::fill(cls_of, cls_of + 256, -1);
+    std::vector<PlanClass> classes;
+    for (size_t i = 0; i < count; ++i) {
+        unsigned e = ecc_len[i];
+        if (cls_of[e] >= 0)
+            continue;
+        ffrs_rs256 *c = nullptr;
+        if (int r = ffrs_rs256_create(ctx, e, 2, 0x11D, &c)) {
+            plan_free(p);
+            return r;
+        }
+        cls_of[e] = (int)p->codecs.size();
+        p->codecs.push_back(c);
+        classes.push_back({c->d_lut, e, c->EW, c->S});
+        p->max_ew = std::max(p->max_ew, c->EW);
+        p->max_e = std::max(p->max_e, e);
+        p->lut_words = std::max(p->lut_words, c->S * 256u * (uint32_t)lut_row((int)c->EW));
+    }
+    p->wg = p->max_ew <= 16 ? 256 : 64;
+    std::vector<uint32_t> order(count);
+    for (size_t i = 0; i < count; ++i)
+        order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        if (ecc_len[a] != ecc_len[b])
+            return ecc_len[a] < ecc_len[b];
+        return message_len[a] > message_len[b];
+    });
+    std::vector<PlanItem> items(count);
+    std::vector<PlanChunk> chunks;
+    for (size_t j = 0; j < count; ++j) {
+        uint32_t i = order[j];
+        items[j] = {msg_off[i], ecc_off[i], i, message_len[i], ecc_len[i], (uint8_t)cls_of[ecc_len[i]]};
+        p->msg_end = std::max<uint64_t>(p->msg_end, msg_off[i] + message_len[i]);
+        p->ecc_end = std::max<uint64_t>(p->ecc_end, ecc_off[i] + ecc_len[i]);
+        if (chunks.empty() || chunks.back().cls != items[j].cls || chunks.back().count == p->wg)
+            chunks.push_back({(uint32_t)j, 0, (uint16_t)items[j].cls});
+        chunks.back().count++;
+    }
+    p->nchunks = (uint32_t)chunks.size();
+    hipError_t e = hipSuccess;
+    auto up = [&](auto **dst, const auto &v) {
+        size_t bytes = std::max<size_t>(1, v.size() * sizeof(v[0]));
+        if (e == hipSuccess)
+            e = hipMalloc((void **)dst, bytes);
+        if (e == hipSuccess && !v.empty())
+            e = hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
+    };
+    up(&p->d_items, items);
+    up(&p->d_chunks, chunks);
+    up(&p->d_classes, classes);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&p->d_queue, 4);
+    if (e != hipSuccess) {
+        plan_free(p);
+        return fail(FFRS_ERR_NOMEM, "plan upload failed: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return FFRS_OK;
+}
+
+void ffrs_rs256_plan_destroy(ffrs_rs256_plan *plan)
+{
+    if (plan)
+        plan_free(plan);
+}
+
+int ffrs_rs256_plan_use_graph(ffrs_rs256_plan *plan, int enable)
+{
+    if (!plan)
+        return fail(FFRS_ERR_INVALID, "plan is NULL");
+    plan->use_graph = enable != 0;
+    return FFRS_OK;
+}
+
+int ffrs_rs256_plan_encode(ffrs_rs256_plan *plan, const uint8_t *d_msgs, size_t msg_bytes, uint8_t *d_eccs,
+                           size_t ecc_bytes)
+{
+    if (!plan || (plan->count && (!d_msgs || !d_eccs)))
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    if (ecc_bytes < plan->ecc_end)
+        return fail(FFRS_ERR_SIZE, "parity buffer of %zu bytes: the plan needs %llu", ecc_bytes,
+                    (unsigned long long)plan->ecc_end);
+    return plan_run(plan, 0, const_cast<uint8_t *>(d_msgs), msg_bytes, d_eccs, nullptr);
+}
+
+int ffrs_rs256_plan_repair(ffrs_rs256_plan *plan, uint8_t *d_msgs, size_t msg_bytes, uint8_t *d_eccs,
+                           size_t ecc_bytes, uint8_t *d_status)
+{
+    if (!plan || (plan->count && (!d_msgs || !d_eccs || !d_status)))
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    if (ecc_bytes < plan->ecc_end)
+        return fail(FFRS_ERR_SIZE, "parity buffer of %zu bytes: the plan needs %llu", ecc_bytes,
+                    (unsigned long long)plan->ecc_end);
+    return plan_run(plan, 1, d_msgs, msg_bytes, d_eccs, d_status);
 }
 
 }  // extern "C"

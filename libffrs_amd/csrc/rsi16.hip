@@ -231,22 +231,12 @@ __device__ inline uint32_t poly_len(const uint32_t *r, uint32_t max)
 // ---------------------------------------------------------------- encode
 
 // One codeword per lane group: pntt_message_residue + _mix_ecc_residue
-// (rsi16v_impl.hpp:93-119), values written as & 0xFFFF.
+// (rsi16v_impl.hpp:93-119), values written as & 0xFFFF. buf holds 2 E words.
 template <class O>
-__global__ __launch_bounds__(64) void rsi16_encode_kernel(const uint16_t *__restrict__ msgs, uint32_t count,
-                                                           uint16_t *__restrict__ out, Rsi16Dev T, uint32_t g)
+__device__ inline void encode_group(const Rsi16Dev &T, const uint16_t *m, uint16_t *o, uint32_t *buf, Grp G)
 {
-    extern __shared__ uint32_t s_mem[];
     const uint32_t E = T.E;
-    Grp G{threadIdx.x % g, g};
-    uint32_t grp = threadIdx.x / g;
-    uint32_t cpw = 64 / g;
-    uint32_t cw = blockIdx.x * cpw + grp;
-    if (cw >= count)
-        return;
-    uint32_t *buf = s_mem + grp * 2 * E;
     uint32_t *acc = buf + E;
-    const uint16_t *m = msgs + (size_t)cw * T.k;
     uint32_t nb = T.n / E;
     for (uint32_t c = 0; c + 1 < nb; ++c) {
         for (uint32_t j = G.gl; j < E; j += G.g)
@@ -263,9 +253,23 @@ __global__ __launch_bounds__(64) void rsi16_encode_kernel(const uint16_t *__rest
         acc[j] = O::mul(O::mod_p(acc[j]), T.mix[j]);
     wave_sync();
     gs_full<O>(acc, E, T.roots_i_ecc, G);
-    uint16_t *o = out + (size_t)cw * E;
     for (uint32_t j = G.gl; j < E; j += G.g)
         o[j] = (uint16_t)(acc[j] & 0xffff);
+}
+
+template <class O>
+__global__ __launch_bounds__(64) void rsi16_encode_kernel(const uint16_t *__restrict__ msgs, uint32_t count,
+                                                           uint16_t *__restrict__ out, Rsi16Dev T, uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E;
+    Grp G{threadIdx.x % g, g};
+    uint32_t grp = threadIdx.x / g;
+    uint32_t cpw = 64 / g;
+    uint32_t cw = blockIdx.x * cpw + grp;
+    if (cw >= count)
+        return;
+    encode_group<O>(T, msgs + (size_t)cw * T.k, out + (size_t)cw * E, s_mem + grp * 2 * E, G);
 }
 
 // ---------------------------------------------------------------- syndromes
@@ -517,22 +521,12 @@ __device__ inline uint32_t eval_poly(const uint32_t *r, uint32_t len, uint32_t x
 // Per group LDS: syn/evaluator E, locator E, derivative E, roots chunk E, Sugiyama 5E.
 constexpr uint32_t REPAIR_WORDS = 9;
 
-__global__ __launch_bounds__(64) void rsi16_repair_kernel(uint16_t *__restrict__ msgs, uint16_t *__restrict__ eccs,
-                                                           uint32_t count, uint8_t *__restrict__ status, Rsi16Dev T,
-                                                           uint32_t g)
+// Repair of one codeword by one lane group (rsi16v_impl.hpp:122-168); lane 0 of the
+// group writes the status to *st. base holds REPAIR_WORDS * E words.
+__device__ inline void repair_group(const Rsi16Dev &T, uint16_t *m, uint16_t *e, uint32_t *base, Grp G, uint8_t *st)
 {
-    extern __shared__ uint32_t s_mem[];
     const uint32_t E = T.E;
-    Grp G{threadIdx.x % g, g};
-    uint32_t grp = threadIdx.x / g;
-    uint32_t cpw = 64 / g;
-    uint32_t cw = blockIdx.x * cpw + grp;
-    if (cw >= count)
-        return;
-    uint32_t *base = s_mem + grp * REPAIR_WORDS * E;
     uint32_t *syn = base, *loc = base + E, *der = base + 2 * E, *chunk = base + 3 * E, *w = base + 4 * E;
-    uint16_t *m = msgs + (size_t)cw * T.k;
-    uint16_t *e = eccs + (size_t)cw * E;
 
     pntt_codeword(T, m, e, chunk, syn, G);
     bool any = false;
@@ -540,13 +534,13 @@ __global__ __launch_bounds__(64) void rsi16_repair_kernel(uint16_t *__restrict__
         any |= syn[j] != 0;
     if (!any) {
         if (G.gl == 0)
-            status[cw] = FFRS_RSI16_NO_ERRORS;
+            *st = FFRS_RSI16_NO_ERRORS;
         return;
     }
     uint32_t L = sugiyama(T, loc, syn, w, G);
     if (L == 0xffffffffu) {
         if (G.gl == 0)
-            status[cw] = FFRS_RSI16_RAISES;
+            *st = FFRS_RSI16_RAISES;
         return;
     }
     uint32_t eval_len = E - L + 1;
@@ -557,7 +551,7 @@ __global__ __launch_bounds__(64) void rsi16_repair_kernel(uint16_t *__restrict__
     uint32_t der_len = poly_len(der, E);
     if (L > E) {  // py_assert at rsi16v_impl.hpp:454
         if (G.gl == 0)
-            status[cw] = FFRS_RSI16_RAISES;
+            *st = FFRS_RSI16_RAISES;
         return;
     }
     uint32_t lanes = L != 0 ? 0xffffffffu : 0u;
@@ -593,7 +587,22 @@ __global__ __launch_bounds__(64) void rsi16_repair_kernel(uint16_t *__restrict__
     for (uint32_t j = 0; j < G.g; ++j)
         total += chunk[j];
     if (G.gl == 0)
-        status[cw] = total > 0 ? FFRS_RSI16_REPAIR_OK : (lanes ? FFRS_RSI16_ERROR_LOCATION_FAIL : FFRS_RSI16_NO_ERRORS);
+        *st = total > 0 ? FFRS_RSI16_REPAIR_OK : (lanes ? FFRS_RSI16_ERROR_LOCATION_FAIL : FFRS_RSI16_NO_ERRORS);
+}
+
+__global__ __launch_bounds__(64) void rsi16_repair_kernel(uint16_t *__restrict__ msgs, uint16_t *__restrict__ eccs,
+                                                           uint32_t count, uint8_t *__restrict__ status, Rsi16Dev T,
+                                                           uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E;
+    Grp G{threadIdx.x % g, g};
+    uint32_t grp = threadIdx.x / g;
+    uint32_t cpw = 64 / g;
+    uint32_t cw = blockIdx.x * cpw + grp;
+    if (cw >= count)
+        return;
+    repair_group(T, msgs + (size_t)cw * T.k, eccs + (size_t)cw * E, s_mem + grp * REPAIR_WORDS * E, G, status + cw);
 }
 
 // Syndromes only (py_synd).
@@ -728,7 +737,68 @@ __global__ __launch_bounds__(64) void rsi16_repair_known_kernel(uint16_t *__rest
         *status = FFRS_RSI16_REPAIR_OK;
 }
 
+// ---------------------------------------------------------------- mixed-size plans
+
+// A plan's work list is sorted by (block_len, ecc_len) class and cut into chunks of the
+// class's codewords-per-wave; single-wave workgroups take chunks from a device counter.
+struct Rsi16PlanItem {
+    uint64_t msg_off;  // element (u16) offsets
+    uint64_t ecc_off;
+    uint32_t idx;
+    uint32_t cls;
+};
+
+struct Rsi16PlanChunk {
+    uint32_t first;
+    uint16_t count;
+    uint16_t cls;
+};
+
+struct Rsi16PlanClass {
+    Rsi16Dev T;
+    uint32_t g;
+};
+
+__global__ void rsi16_plan_reset_kernel(uint32_t *queue) { *queue = 0; }
+
+template <int MODE>  // 0: encode (vector words), 1: encode (scalar words), 2: repair
+__global__ __launch_bounds__(64) void rsi16_plan_kernel(uint16_t *__restrict__ msgs, uint16_t *__restrict__ eccs,
+                                                         uint8_t *__restrict__ status,
+                                                         const Rsi16PlanItem *__restrict__ items,
+                                                         const Rsi16PlanChunk *__restrict__ chunks, uint32_t nchunks,
+                                                         const Rsi16PlanClass *__restrict__ classes, uint32_t *queue)
+{
+    extern __shared__ uint32_t s_mem[];
+    __shared__ uint32_t s_chunk;
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_chunk = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t c = s_chunk;
+        if (c >= nchunks)
+            break;
+        const Rsi16PlanChunk ch = chunks[c];
+        const Rsi16PlanClass &cl = classes[ch.cls];
+        const uint32_t g = cl.g, E = cl.T.E;
+        Grp G{threadIdx.x % g, g};
+        const uint32_t grp = threadIdx.x / g;
+        if (grp < ch.count) {
+            const Rsi16PlanItem it = items[ch.first + grp];
+            if constexpr (MODE == 2)
+                repair_group(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * REPAIR_WORDS * E, G,
+                             status + it.idx);
+            else if constexpr (MODE == 0)
+                encode_group<VectorOps>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * 2 * E, G);
+            else
+                encode_group<ScalarOps>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * 2 * E, G);
+        }
+        __syncthreads();
+    }
+}
+
+
 }  // namespace ffrs
+
 
 // ---------------------------------------------------------------- host side + C ABI
 
@@ -1180,6 +1250,208 @@ int ffrs_rsi16_roots(ffrs_rsi16 *rs, const uint32_t *poly, uint8_t *is_root)
     FFRS_HIP(hipMemcpyAsync(is_root, d_h, rs->n, hipMemcpyDeviceToHost, ctx->stream));
     FFRS_HIP(hipStreamSynchronize(ctx->stream));
     return FFRS_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- mixed-size plans (host)
+
+struct ffrs_rsi16_plan {
+    ffrs_ctx *ctx = nullptr;
+    std::vector<ffrs_rsi16 *> codecs;
+    Rsi16PlanItem *d_items = nullptr;
+    Rsi16PlanChunk *d_chunks = nullptr;
+    Rsi16PlanClass *d_classes = nullptr;
+    uint32_t *d_queue = nullptr;
+    uint32_t count = 0, nchunks = 0, lds_enc = 0, lds_rep = 0;
+    uint64_t msg_end = 0, ecc_end = 0;
+    bool use_graph = false;
+    struct Graph {
+        int mode;
+        const void *m, *e, *st;
+        hipGraphExec_t exec;
+    };
+    std::vector<Graph> graphs;
+};
+
+namespace {
+
+void rsi16_plan_free(ffrs_rsi16_plan *p)
+{
+    (void)hipSetDevice(p->ctx->device);
+    for (auto &g : p->graphs)
+        (void)hipGraphExecDestroy(g.exec);
+    for (void *d : {(void *)p->d_items, (void *)p->d_chunks, (void *)p->d_classes, (void *)p->d_queue})
+        if (d)
+            (void)hipFree(d);
+    for (auto *c : p->codecs)
+        ffrs_rsi16_destroy(c);
+    delete p;
+}
+
+int rsi16_plan_enqueue(ffrs_rsi16_plan *p, int mode, uint16_t *msgs, uint16_t *eccs, uint8_t *status)
+{
+    ffrs_ctx *ctx = p->ctx;
+    size_t lds = mode == 2 ? p->lds_rep : p->lds_enc;
+    auto kern = mode == 2 ? rsi16_plan_kernel<2> : mode == 0 ? rsi16_plan_kernel<0> : rsi16_plan_kernel<1>;
+    if (int r = allow_lds(kern, lds))
+        return r;
+    int per_cu = (int)std::max<size_t>(1, std::min<size_t>(32, (160 * 1024) / std::max<size_t>(lds, 1)));
+    int grid = persistent_grid(ctx, per_cu, p->nchunks);
+    hipLaunchKernelGGL(rsi16_plan_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, p->d_queue);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, msgs, eccs, status, p->d_items, p->d_chunks,
+                       p->nchunks, p->d_classes, p->d_queue);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+int rsi16_plan_run(ffrs_rsi16_plan *p, int mode, uint16_t *msgs, size_t msg_elems, uint16_t *eccs, size_t ecc_elems,
+                   uint8_t *status)
+{
+    ffrs_ctx *ctx = p->ctx;
+    if (int r = ctx_set_device(ctx))
+        return r;
+    if (p->count == 0)
+        return FFRS_OK;
+    if (msg_elems < p->msg_end || ecc_elems < p->ecc_end)
+        return fail(FFRS_ERR_SIZE, "buffers of %zu / %zu symbols: the plan needs %llu / %llu", msg_elems, ecc_elems,
+                    (unsigned long long)p->msg_end, (unsigned long long)p->ecc_end);
+    marks_begin(ctx);
+    if (!p->use_graph) {
+        if (int r = rsi16_plan_enqueue(p, mode, msgs, eccs, status))
+            return r;
+        mark_kernel(ctx);
+        return FFRS_OK;
+    }
+    for (auto &g : p->graphs)
+        if (g.mode == mode && g.m == msgs && g.e == eccs && g.st == status) {
+            FFRS_HIP(hipGraphLaunch(g.exec, ctx->stream));
+            mark_kernel(ctx);
+            return FFRS_OK;
+        }
+    hipGraph_t graph = nullptr;
+    FFRS_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    int r = rsi16_plan_enqueue(p, mode, msgs, eccs, status);
+    hipError_t e = hipStreamEndCapture(ctx->stream, &graph);
+    if (r)
+        return r;
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_DEVICE, "graph capture failed: %s", hipGetErrorString(e));
+    hipGraphExec_t exec = nullptr;
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+    p->graphs.push_back({mode, msgs, eccs, status, exec});
+    FFRS_HIP(hipGraphLaunch(exec, ctx->stream));
+    mark_kernel(ctx);
+    return FFRS_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_len, const uint16_t *ecc_len,
+                           const uint64_t *msg_off, const uint64_t *ecc_off, ffrs_rsi16_plan **out)
+{
+    if (!ctx || !out || (count && (!block_len || !ecc_len || !msg_off || !ecc_off)))
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    if (count > 0xffffffffu)
+        return fail(FFRS_ERR_INVALID, "too many codewords in one plan");
+    auto *p = new ffrs_rsi16_plan();
+    p->ctx = ctx;
+    p->count = (uint32_t)count;
+    std::vector<std::pair<uint32_t, uint32_t>> keys;  // (block_len, ecc_len) per class
+    std::vector<uint32_t> cls(count);
+    std::vector<Rsi16PlanClass> classes;
+    for (size_t i = 0; i < count; ++i) {
+        std::pair<uint32_t, uint32_t> key{block_len[i], ecc_len[i]};
+        size_t c = std::find(keys.begin(), keys.end(), key) - keys.begin();
+        if (c == keys.size()) {
+            ffrs_rsi16 *codec = nullptr;
+            if (int r = ffrs_rsi16_create(ctx, block_len[i], ecc_len[i], 3, &codec)) {
+                rsi16_plan_free(p);
+                return r;
+            }
+            keys.push_back(key);
+            p->codecs.push_back(codec);
+            uint32_t g = group_size(codec->E);
+            classes.push_back({codec->dev, g});
+            p->lds_enc = std::max<uint32_t>(p->lds_enc, (64 / g) * 2 * codec->E * 4);
+            p->lds_rep = std::max<uint32_t>(p->lds_rep, (64 / g) * REPAIR_WORDS * codec->E * 4);
+        }
+        cls[i] = (uint32_t)c;
+    }
+    std::vector<uint32_t> order(count);
+    for (size_t i = 0; i < count; ++i)
+        order[i] = (uint32_t)i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cls[a] < cls[b]; });
+    std::vector<Rsi16PlanItem> items(count);
+    std::vector<Rsi16PlanChunk> chunks;
+    for (size_t j = 0; j < count; ++j) {
+        uint32_t i = order[j], c = cls[i];
+        const ffrs_rsi16 *codec = p->codecs[c];
+        items[j] = {msg_off[i], ecc_off[i], i, c};
+        p->msg_end = std::max<uint64_t>(p->msg_end, msg_off[i] + codec->k);
+        p->ecc_end = std::max<uint64_t>(p->ecc_end, ecc_off[i] + codec->E);
+        uint32_t cpw = 64 / classes[c].g;
+        if (chunks.empty() || chunks.back().cls != c || chunks.back().count == cpw)
+            chunks.push_back({(uint32_t)j, 0, (uint16_t)c});
+        chunks.back().count++;
+    }
+    p->nchunks = (uint32_t)chunks.size();
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    hipError_t e = hipSetDevice(ctx->device);
+    auto up = [&](auto **dst, const auto &v) {
+        size_t bytes = std::max<size_t>(1, v.size() * sizeof(v[0]));
+        if (e == hipSuccess)
+            e = hipMalloc((void **)dst, bytes);
+        if (e == hipSuccess && !v.empty())
+            e = hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
+    };
+    up(&p->d_items, items);
+    up(&p->d_chunks, chunks);
+    up(&p->d_classes, classes);
+    if (e == hipSuccess)
+        e = hipMalloc((void **)&p->d_queue, 4);
+    if (e != hipSuccess) {
+        rsi16_plan_free(p);
+        return fail(FFRS_ERR_NOMEM, "plan upload failed: %s", hipGetErrorString(e));
+    }
+    *out = p;
+    return FFRS_OK;
+}
+
+void ffrs_rsi16_plan_destroy(ffrs_rsi16_plan *plan)
+{
+    if (plan)
+        rsi16_plan_free(plan);
+}
+
+int ffrs_rsi16_plan_use_graph(ffrs_rsi16_plan *plan, int enable)
+{
+    if (!plan)
+        return fail(FFRS_ERR_INVALID, "plan is NULL");
+    plan->use_graph = enable != 0;
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_plan_encode(ffrs_rsi16_plan *plan, const uint16_t *d_msgs, size_t msg_elems, uint16_t *d_eccs,
+                           size_t ecc_elems, int vector_lanes)
+{
+    if (!plan || (plan->count && (!d_msgs || !d_eccs)))
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    return rsi16_plan_run(plan, vector_lanes ? 0 : 1, const_cast<uint16_t *>(d_msgs), msg_elems, d_eccs, ecc_elems,
+                          nullptr);
+}
+
+int ffrs_rsi16_plan_repair(ffrs_rsi16_plan *plan, uint16_t *d_msgs, size_t msg_elems, uint16_t *d_eccs,
+                           size_t ecc_elems, uint8_t *d_status)
+{
+    if (!plan || (plan->count && (!d_msgs || !d_eccs || !d_status)))
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    return rsi16_plan_run(plan, 2, d_msgs, msg_elems, d_eccs, ecc_elems, d_status);
 }
 
 }  // extern "C"

@@ -199,3 +199,135 @@ class GpuBackend:
                 cws[i][:k] = m[j]
                 cws[i][k:] = p[j]
         return st
+
+
+class PlanBackend(GpuBackend):
+    """Mixed batches through the plan entry points: one persistent launch per field over a
+    work list, codewords stored in a shuffled order with gaps between them. Repair runs
+    eagerly and then three times from a captured HIP graph (inputs restored in between);
+    every replay must equal the eager result."""
+
+    def __init__(self, graph_replays=3):
+        super().__init__()
+        from libffrs_amd import capi
+        self.capi = capi
+        self.ctx = capi.Context(0)
+        self.replays = graph_replays
+
+    def _layout(self, sizes, seed):
+        rng = np.random.default_rng(seed)
+        order = rng.permutation(len(sizes))
+        off = np.zeros(len(sizes), np.uint64)
+        pos = 0
+        for i in order:
+            pos += int(rng.integers(0, 3))  # small gaps, odd alignments
+            off[i] = pos
+            pos += int(sizes[i])
+        return off, pos + 8
+
+    def _dev(self, arr):
+        b = self.ctx.alloc(max(arr.nbytes, 1))
+        if arr.nbytes:
+            b.upload(np.ascontiguousarray(arr))
+        return b
+
+    def _run_mixed(self, make_plan, elem, dtype, k_of, e_of, msgs_list, pars_list=None, repair=False):
+        n_items = len(msgs_list)
+        moff, mtot = self._layout([k_of(i) for i in range(n_items)], 1)
+        eoff, etot = self._layout([e_of(i) for i in range(n_items)], 2)
+        hm = np.zeros(mtot, dtype)
+        he = np.zeros(etot, dtype)
+        for i in range(n_items):
+            hm[int(moff[i]):int(moff[i]) + k_of(i)] = msgs_list[i]
+            if pars_list is not None:
+                he[int(eoff[i]):int(eoff[i]) + e_of(i)] = pars_list[i]
+        results = []
+        for graph in (False, True):
+            plan = make_plan(moff, eoff, graph)
+            for rep in range(1 if not graph else self.replays):
+                dm, de, ds = self._dev(hm), self._dev(he), self.ctx.alloc(n_items)
+                if repair:
+                    plan.repair(dm, de, ds)
+                else:
+                    plan.encode(dm, de)
+                self.ctx.sync()
+                om, oe, st = np.empty_like(hm), np.empty_like(he), np.zeros(n_items, np.uint8)
+                dm.download(om)
+                de.download(oe)
+                ds.download(st)
+                for b in (dm, de, ds):
+                    b.free()
+                results.append((om, oe, st))
+            plan.close()
+        for r in results[1:]:
+            assert (r[0] == results[0][0]).all() and (r[1] == results[0][1]).all()
+            if repair:
+                assert (r[2] == results[0][2]).all()
+        om, oe, st = results[0]
+        outs_m = [om[int(moff[i]):int(moff[i]) + k_of(i)] for i in range(n_items)]
+        outs_e = [oe[int(eoff[i]):int(eoff[i]) + e_of(i)] for i in range(n_items)]
+        return outs_m, outs_e, st
+
+    def rs256_mixed_encode(self, eccs, lens, msgs):
+        mk = lambda mo, eo, g: self.capi.RS256Plan(self.ctx, eccs, lens, mo, eo, graph=g)  # noqa: E731
+        _, pars, _ = self._run_mixed(mk, 1, np.uint8, lambda i: int(lens[i]), lambda i: int(eccs[i]), msgs)
+        return pars
+
+    def rs256_mixed_repair(self, eccs, lens, cws):
+        mk = lambda mo, eo, g: self.capi.RS256Plan(self.ctx, eccs, lens, mo, eo, graph=g)  # noqa: E731
+        ms = [c[:int(l)] for c, l in zip(cws, lens)]
+        ps = [c[int(l):] for c, l in zip(cws, lens)]
+        om, oe, st = self._run_mixed(mk, 1, np.uint8, lambda i: int(lens[i]), lambda i: int(eccs[i]), ms, ps,
+                                     repair=True)
+        for i, c in enumerate(cws):
+            c[:int(lens[i])] = om[i]
+            c[int(lens[i]):] = oe[i]
+        return st
+
+    def rsi16_mixed_encode(self, ns, eccs, msgs):
+        mk = lambda mo, eo, g: self.capi.RSi16Plan(self.ctx, ns, eccs, mo, eo, graph=g)  # noqa: E731
+        _, pars, _ = self._run_mixed(mk, 2, np.uint16, lambda i: int(ns[i] - eccs[i]), lambda i: int(eccs[i]), msgs)
+        return pars
+
+    def rsi16_mixed_repair(self, ns, eccs, cws):
+        mk = lambda mo, eo, g: self.capi.RSi16Plan(self.ctx, ns, eccs, mo, eo, graph=g)  # noqa: E731
+        ks = [int(n - e) for n, e in zip(ns, eccs)]
+        ms = [c[:k] for c, k in zip(cws, ks)]
+        ps = [c[k:] for c, k in zip(cws, ks)]
+        om, oe, st = self._run_mixed(mk, 2, np.uint16, lambda i: ks[i], lambda i: int(eccs[i]), ms, ps, repair=True)
+        for i, c in enumerate(cws):
+            c[:ks[i]] = om[i]
+            c[ks[i]:] = oe[i]
+        return st
+
+    # uniform batches through the same plans (one class), so that every fixture set
+    # also exercises the work-queue kernels, including the variant for ecc_len > 64
+    def rs256_encode_batch(self, ecc, msgs):
+        count, k = msgs.shape
+        par = self.rs256_mixed_encode(np.full(count, ecc), np.full(count, k), list(msgs))
+        return np.array(par, np.uint8).reshape(count, ecc)
+
+    def rs256_repair_batch(self, ecc, msgs, parity):
+        count, k = msgs.shape
+        cws = [np.concatenate([msgs[i], parity[i]]) for i in range(count)]
+        st = self.rs256_mixed_repair(np.full(count, ecc), np.full(count, k), cws)
+        for i in range(count):
+            msgs[i] = cws[i][:k]
+            parity[i] = cws[i][k:]
+        return st
+
+    def rsi16_encode_batch(self, n, ecc, msgs, vector=True):
+        if not vector:
+            return super().rsi16_encode_batch(n, ecc, msgs, vector)
+        count = msgs.shape[0]
+        par = self.rsi16_mixed_encode(np.full(count, n), np.full(count, ecc), list(msgs))
+        return np.array(par, np.uint16).reshape(count, ecc)
+
+    def rsi16_repair_batch(self, n, ecc, msgs, parity):
+        count, k = msgs.shape
+        cws = [np.concatenate([msgs[i], parity[i]]) for i in range(count)]
+        st = self.rsi16_mixed_repair(np.full(count, n), np.full(count, ecc), cws)
+        for i in range(count):
+            msgs[i] = cws[i][:k]
+            parity[i] = cws[i][k:]
+        return st
