@@ -306,6 +306,219 @@ def run_plain(name, args):
     return result
 
 
+def run_mixed(args):
+    """Mixed-size batch: 200,000 RS256 codewords (ecc_len uniform in {2,4,8,16,32,64},
+    message length uniform in [1, 255 - ecc_len], errors uniform in [0, ecc_len/2]) and
+    50,000 RSi16 codewords (block_len in {64,256,1024}, ecc_len in {8,16,32}, errors in
+    [0, ecc_len/2]). The fused path runs one persistent work-queue launch per field and
+    stage, replayed from HIP graphs; the plain path runs the per-size entry points once per
+    distinct (field, block_len, ecc_len), issued from the host and also replayed from one
+    captured graph (its best case). Both paths are checked to give identical results."""
+    from libffrs_amd import capi
+    from libffrs_amd import workloads as W
+
+    n_rs, n_ri = args.mixed_rs256, args.mixed_rsi16
+    t0 = time.perf_counter()
+    eccs, lens, counts, s = W.rs256_mixed(n_rs, seed=4)
+    msg_all = s.bytes(int(lens.sum()))
+    starts = np.concatenate([[0], np.cumsum(lens)])  # item starts in generation order
+    # group-contiguous layout: groups of equal (ecc_len, message_len) stored back to back
+    order = np.lexsort((lens, eccs))
+    eccs, lens, counts = eccs[order], lens[order], counts[order]
+    msgs = [msg_all[starts[o]:starts[o + 1]] for o in order]
+    moff = np.concatenate([[0], np.cumsum(lens)[:-1]]).astype(np.uint64)
+    eoff = np.concatenate([[0], np.cumsum(eccs)[:-1]]).astype(np.uint64)
+    hm = np.concatenate(msgs).astype(np.uint8)
+    keys = np.stack([eccs, lens], 1)
+    gstart = np.concatenate([[0], np.nonzero((keys[1:] != keys[:-1]).any(1))[0] + 1, [n_rs]])
+    ns_, eccs_i, counts_i, s2 = W.rsi16_mixed(n_ri, seed=5)
+    order_i = np.lexsort((eccs_i, ns_))
+    ns_, eccs_i, counts_i = ns_[order_i], eccs_i[order_i], counts_i[order_i]
+    ks_i = ns_ - eccs_i
+    hm_i = s2.u16(int(ks_i.sum()))
+    moff_i = np.concatenate([[0], np.cumsum(ks_i)[:-1]]).astype(np.uint64)
+    eoff_i = np.concatenate([[0], np.cumsum(eccs_i)[:-1]]).astype(np.uint64)
+    keys_i = np.stack([ns_, eccs_i], 1)
+    gstart_i = np.concatenate([[0], np.nonzero((keys_i[1:] != keys_i[:-1]).any(1))[0] + 1, [n_ri]])
+    log(f"mixed batch generated in {time.perf_counter() - t0:.1f} s: {len(gstart) - 1} RS256 and "
+        f"{len(gstart_i) - 1} RSi16 size classes")
+
+    ctx = capi.Context(0)
+    plan = capi.RS256Plan(ctx, eccs, lens, moff, eoff, graph=not args.no_graph)
+    plan_i = capi.RSi16Plan(ctx, ns_, eccs_i, moff_i, eoff_i, graph=not args.no_graph)
+    d_m, d_p = ctx.alloc(hm.nbytes), ctx.alloc(int(eccs.sum()))
+    d_mi, d_pi = ctx.alloc(hm_i.nbytes), ctx.alloc(int(eccs_i.sum()) * 2)
+    d_m.upload(hm)
+    d_mi.upload(hm_i)
+    plan.encode(d_m, d_p)
+    plan_i.encode(d_mi, d_pi)
+    hp = np.empty(int(eccs.sum()), np.uint8)
+    hp_i = np.empty(int(eccs_i.sum()), np.uint16)
+    d_p.download(hp)
+    d_pi.download(hp_i)
+    # corrupt group by group (rows of one group share their length)
+    cm, cp = hm.copy(), hp.copy()
+    for a, b in zip(gstart[:-1], gstart[1:]):
+        k, e = int(lens[a]), int(eccs[a])
+        m0, p0 = int(moff[a]), int(eoff[a])
+        cw = np.concatenate([cm[m0:m0 + (b - a) * k].reshape(-1, k), cp[p0:p0 + (b - a) * e].reshape(-1, e)], 1)
+        W.rs256_corrupt(cw, counts[a:b], s)
+        cm[m0:m0 + (b - a) * k] = cw[:, :k].ravel()
+        cp[p0:p0 + (b - a) * e] = cw[:, k:].ravel()
+    cm_i, cp_i = hm_i.copy(), hp_i.copy()
+    for a, b in zip(gstart_i[:-1], gstart_i[1:]):
+        k, e = int(ks_i[a]), int(eccs_i[a])
+        m0, p0 = int(moff_i[a]), int(eoff_i[a])
+        cw = np.concatenate([cm_i[m0:m0 + (b - a) * k].reshape(-1, k),
+                             cp_i[p0:p0 + (b - a) * e].reshape(-1, e)], 1)
+        W.rsi16_corrupt(cw, counts_i[a:b], s2)
+        cm_i[m0:m0 + (b - a) * k] = cw[:, :k].ravel()
+        cp_i[p0:p0 + (b - a) * e] = cw[:, k:].ravel()
+    d_cm, d_cp, d_cmi, d_cpi = ctx.alloc(cm.nbytes), ctx.alloc(cp.nbytes), ctx.alloc(cm_i.nbytes), ctx.alloc(cp_i.nbytes)
+    d_cm.upload(cm)
+    d_cp.upload(cp)
+    d_cmi.upload(cm_i)
+    d_cpi.upload(cp_i)
+    d_wm, d_wp, d_wmi, d_wpi = ctx.alloc(cm.nbytes), ctx.alloc(cp.nbytes), ctx.alloc(cm_i.nbytes), ctx.alloc(cp_i.nbytes)
+    d_st, d_sti = ctx.alloc(n_rs), ctx.alloc(n_ri)
+
+    codecs = {int(e): capi.RS256Codec(ctx, int(e)) for e in np.unique(eccs)}
+    codecs_i = {(int(n), int(e)): capi.RSi16Codec(ctx, int(n), int(e)) for n, e in np.unique(keys_i, axis=0)}
+
+    def restore():
+        d_wm.copy_from(d_cm)
+        d_wp.copy_from(d_cp)
+        d_wmi.copy_from(d_cmi)
+        d_wpi.copy_from(d_cpi)
+        ctx.sync()
+
+    def fused_encode():
+        plan.encode(d_m, d_p)
+        plan_i.encode(d_mi, d_pi)
+
+    def fused_repair():
+        plan.repair(d_wm, d_wp, d_st)
+        plan_i.repair(d_wmi, d_wpi, d_sti)
+
+    def plain_encode():
+        for a, b in zip(gstart[:-1], gstart[1:]):
+            k, e = int(lens[a]), int(eccs[a])
+            codecs[e].encode_device(k, d_m.view(int(moff[a]), (b - a) * k), int(b - a),
+                                    d_p.view(int(eoff[a]), (b - a) * e))
+        for a, b in zip(gstart_i[:-1], gstart_i[1:]):
+            k, e = int(ks_i[a]), int(eccs_i[a])
+            codecs_i[(int(ns_[a]), e)].encode_device(d_mi.view(2 * int(moff_i[a]), 2 * (b - a) * k), int(b - a),
+                                                     d_pi.view(2 * int(eoff_i[a]), 2 * (b - a) * e))
+
+    def plain_repair():
+        for a, b in zip(gstart[:-1], gstart[1:]):
+            k, e = int(lens[a]), int(eccs[a])
+            codecs[e].repair_device(k, d_wm.view(int(moff[a]), (b - a) * k), d_wp.view(int(eoff[a]), (b - a) * e),
+                                    int(b - a), d_st.view(int(a), int(b - a)))
+        for a, b in zip(gstart_i[:-1], gstart_i[1:]):
+            k, e = int(ks_i[a]), int(eccs_i[a])
+            codecs_i[(int(ns_[a]), e)].repair_device(d_wmi.view(2 * int(moff_i[a]), 2 * (b - a) * k),
+                                                     d_wpi.view(2 * int(eoff_i[a]), 2 * (b - a) * e), int(b - a),
+                                                     d_sti.view(int(a), int(b - a)))
+
+    def results():
+        out = [np.empty_like(cm), np.empty_like(cp), np.empty(n_rs, np.uint8), np.empty_like(cm_i),
+               np.empty_like(cp_i), np.empty(n_ri, np.uint8)]
+        for b, o in zip((d_wm, d_wp, d_st, d_wmi, d_wpi, d_sti), out):
+            b.download(o)
+        return out
+
+    def timed(fn, reps, before=None):
+        ms = []
+        for _ in range(reps):
+            if before:
+                before()
+            ctx.sync()
+            t = time.perf_counter()
+            ctx.timer_begin()
+            fn()
+            gpu = ctx.timer_end()
+            ms.append((1e3 * (time.perf_counter() - t), gpu))
+        return float(np.median([m[0] for m in ms])), float(np.median([m[1] for m in ms]))
+
+    # correctness: fused and plain give the same parity, bytes and statuses
+    fused_encode()
+    ctx.sync()
+    restore()
+    fused_repair()
+    ctx.sync()
+    r_fused = results()
+    par_fused = np.empty_like(hp)
+    d_p.download(par_fused)
+    plain_encode()
+    restore()
+    plain_repair()
+    ctx.sync()
+    r_plain = results()
+    par_plain = np.empty_like(hp)
+    d_p.download(par_plain)
+    same = bool((par_fused == par_plain).all() and all((x == y).all() for x, y in zip(r_fused, r_plain)))
+    if not same:
+        raise RuntimeError("fused and plain mixed-batch results differ")
+
+    reps = max(args.steps, 3)
+    for _ in range(args.warmup):
+        fused_encode()
+        restore()
+        fused_repair()
+    fe = timed(fused_encode, reps)
+    fr = timed(fused_repair, reps, restore)
+    pe = timed(plain_encode, reps)
+    pr = timed(plain_repair, reps, restore)
+    pge = pgr = (float("nan"), float("nan"))
+    if not args.no_graph:
+        # the plain launches replayed from one captured graph: no host issue cost
+        ctx.capture_begin()
+        plain_encode()
+        g_pe = ctx.capture_end()
+        ctx.capture_begin()
+        plain_repair()
+        g_pr = ctx.capture_end()
+        pge = timed(g_pe.launch, reps)
+        pgr = timed(g_pr.launch, reps, restore)
+        g_pe.close()
+        g_pr.close()
+
+    msg_bytes = hm.nbytes + hm_i.nbytes
+    st_f, sti_f = r_fused[2], r_fused[5]
+
+    def gbps(ms):
+        return round(msg_bytes / (ms * 1e-3) / 1e9, 3)
+
+    res = {
+        "workload": "mixed (200,000 RS256 + 50,000 RSi16 codewords)",
+        "codewords": {"rs256": n_rs, "rsi16": n_ri},
+        "size_classes": {"rs256": len(gstart) - 1, "rsi16": len(gstart_i) - 1},
+        "message_bytes": msg_bytes,
+        "fused": {"encode_ms": round(fe[1], 4), "repair_ms": round(fr[1], 4),
+                  "encode_gbps": gbps(fe[1]), "repair_gbps": gbps(fr[1])},
+        "plain_host_issued": {"encode_ms": round(pe[0], 4), "repair_ms": round(pr[0], 4),
+                              "encode_gpu_ms": round(pe[1], 4), "repair_gpu_ms": round(pr[1], 4)},
+        "plain_graph": ({"encode_ms": round(pge[1], 4), "repair_ms": round(pgr[1], 4)}
+                        if not args.no_graph else None),
+        "repair_speedup_vs_plain": round(pr[0] / fr[1], 2),
+        "repair_speedup_vs_plain_graph": round(pgr[1] / fr[1], 2) if not args.no_graph else None,
+        "encode_speedup_vs_plain_graph": round(pge[1] / fe[1], 2) if not args.no_graph else None,
+        "graphs": not args.no_graph,
+        "identical_results": same,
+        "status_counts": {"rs256": {int(v): int(c) for v, c in zip(*np.unique(st_f, return_counts=True))},
+                          "rsi16": {int(v): int(c) for v, c in zip(*np.unique(sti_f, return_counts=True))}},
+    }
+    for c in list(codecs.values()) + list(codecs_i.values()):
+        c.close()
+    plan.close()
+    plan_i.close()
+    for b in (d_m, d_p, d_mi, d_pi, d_cm, d_cp, d_cmi, d_cpi, d_wm, d_wp, d_wmi, d_wpi, d_st, d_sti):
+        b.free()
+    ctx.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -315,6 +528,11 @@ def main():
     ap.add_argument("--codewords", type=int, default=0, help="override the workload's codeword count")
     ap.add_argument("--cpu-repair-sample", type=int, default=0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mixed", action="store_true", help="run only the mixed-size batch and print its result")
+    ap.add_argument("--no-mixed", action="store_true", help="leave the mixed-size batch out of the bench line")
+    ap.add_argument("--no-graph", action="store_true", help="mixed batch without HIP graphs (for tracing tools)")
+    ap.add_argument("--mixed-rs256", type=int, default=200000)
+    ap.add_argument("--mixed-rsi16", type=int, default=50000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU only; --gpus {args.gpus} is not supported by this build "
@@ -322,7 +540,13 @@ def main():
               file=sys.stderr)
         sys.exit(2)
     ensure_built()
+    if args.mixed:
+        print(json.dumps(run_mixed(args)), flush=True)
+        return
     res = run_plain(args.workload, args)
+    if not args.no_mixed:
+        log("running the mixed-size batch ...")
+        res["mixed"] = run_mixed(args)
     print(json.dumps(res), flush=True)
 
 

@@ -74,6 +74,16 @@ int ffrs_timer_end(ffrs_ctx *ctx, float *ms);
  * launches; ffrs_ctx_kernel_times waits for the last one and writes the durations of the
  * last call's kernels, in launch order, to ms[0..min(cap, *count)). Off by default. */
 int ffrs_ctx_kernel_timing(ffrs_ctx *ctx, int enable);
+
+/* Stream capture (no counterpart in the reference): the *_device and plan entry points
+ * called between capture_begin and capture_end on this context are recorded, not run, and
+ * replayed by ffrs_graph_launch on the context's stream. Codec workspaces must already be
+ * large enough (run the same calls once before capturing). */
+typedef struct ffrs_graph ffrs_graph;
+int ffrs_ctx_capture_begin(ffrs_ctx *ctx);
+int ffrs_ctx_capture_end(ffrs_ctx *ctx, ffrs_graph **out);
+int ffrs_graph_launch(ffrs_graph *graph);
+void ffrs_graph_destroy(ffrs_graph *graph);
 int ffrs_ctx_kernel_times(ffrs_ctx *ctx, float *ms, int cap, int *count);
 
 /* ---- RS256 over GF(2^8) ------------------------------------------------------------ */
@@ -83,6 +93,11 @@ int ffrs_ctx_kernel_times(ffrs_ctx *ctx, float *ms, int cap, int *count);
 int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsigned polynomial,
                       ffrs_rs256 **out);
 void ffrs_rs256_destroy(ffrs_rs256 *rs);
+/* Repair kernels (no counterpart in the reference; results do not depend on it): 0 and 1
+ * use the syndrome / Berlekamp-Massey / Chien pipeline where ecc_len <= 32 (default), 2
+ * one kernel with one codeword per lane; ecc_len > 32 always uses the latter. */
+int ffrs_rs256_set_repair_mode(ffrs_rs256 *rs, int mode);
+
 /* generator[0..ecc_len], highest degree first (libffrs/reed_solomon.hpp:55-88) */
 int ffrs_rs256_generator(const ffrs_rs256 *rs, uint8_t *generator, uint8_t *roots);
 

@@ -40,9 +40,14 @@ SIGNATURES = [
     ("ffrs_timer_end", c_int, [c_vp, ctypes.POINTER(ctypes.c_float)]),
     ("ffrs_ctx_kernel_timing", c_int, [c_vp, c_int]),
     ("ffrs_ctx_kernel_times", c_int, [c_vp, ctypes.POINTER(ctypes.c_float), c_int, ctypes.POINTER(c_int)]),
+    ("ffrs_ctx_capture_begin", c_int, [c_vp]),
+    ("ffrs_ctx_capture_end", c_int, [c_vp, ctypes.POINTER(c_vp)]),
+    ("ffrs_graph_launch", c_int, [c_vp]),
+    ("ffrs_graph_destroy", None, [c_vp]),
     ("ffrs_rs256_create", c_int, [c_vp, c_uint, c_uint, c_uint, ctypes.POINTER(c_vp)]),
     ("ffrs_rs256_destroy", None, [c_vp]),
     ("ffrs_rs256_generator", c_int, [c_vp, c_u8p, c_u8p]),
+    ("ffrs_rs256_set_repair_mode", c_int, [c_vp, c_int]),
     ("ffrs_rs256_encode", c_int, [c_vp, c_sz, c_vp, c_sz, c_vp, c_sz]),
     ("ffrs_rs256_repair", c_int, [c_vp, c_sz, c_vp, c_vp, c_sz, c_vp]),
     ("ffrs_rs256_synds", c_int, [c_vp, c_sz, c_vp, c_vp, c_sz, c_vp]),
@@ -130,6 +135,14 @@ class Context:
         check(lib().ffrs_timer_end(self.h, ctypes.byref(ms)), "timer_end")
         return ms.value
 
+    def capture_begin(self):
+        check(lib().ffrs_ctx_capture_begin(self.h), "capture_begin")
+
+    def capture_end(self):
+        g = c_vp()
+        check(lib().ffrs_ctx_capture_end(self.h, ctypes.byref(g)), "capture_end")
+        return Graph(g)
+
     def kernel_timing(self, enable=True):
         check(lib().ffrs_ctx_kernel_timing(self.h, 1 if enable else 0), "kernel_timing")
 
@@ -138,6 +151,21 @@ class Context:
         n = c_int()
         check(lib().ffrs_ctx_kernel_times(self.h, buf, 8, ctypes.byref(n)), "kernel_times")
         return [buf[i] for i in range(min(n.value, 8))]
+
+
+class Graph:
+    """A captured sequence of device work, replayed on its context's stream."""
+
+    def __init__(self, h):
+        self.h = h
+
+    def launch(self):
+        check(lib().ffrs_graph_launch(self.h), "graph launch")
+
+    def close(self):
+        if self.h:
+            lib().ffrs_graph_destroy(self.h)
+            self.h = None
 
 
 class DeviceBuffer:
@@ -161,6 +189,16 @@ class DeviceBuffer:
         n = other.nbytes if nbytes is None else nbytes
         check(lib().ffrs_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, n), "d2d")
 
+    def view(self, offset, nbytes):
+        """A non-owning window [offset, offset + nbytes) of this buffer."""
+        assert 0 <= offset and offset + nbytes <= self.nbytes
+        return BufferView(self.ptr + offset, nbytes)
+
+
+class BufferView:
+    def __init__(self, ptr, nbytes):
+        self.ptr, self.nbytes = ptr, nbytes
+
 
 class RS256Codec:
     def __init__(self, ctx, ecc_len, primitive=2, polynomial=0x11D):
@@ -172,6 +210,9 @@ class RS256Codec:
         if self.h:
             lib().ffrs_rs256_destroy(self.h)
             self.h = None
+
+    def set_repair_mode(self, mode):
+        check(lib().ffrs_rs256_set_repair_mode(self.h, int(mode)), "set_repair_mode")
 
     def encode_device(self, message_len, d_msg, count, d_ecc):
         check(lib().ffrs_rs256_encode_device(self.h, message_len, d_msg.ptr, count, d_ecc.ptr), "rs256 encode")

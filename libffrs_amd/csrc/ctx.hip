@@ -217,6 +217,54 @@ int ffrs_timer_end(ffrs_ctx *ctx, float *ms)
     return FFRS_OK;
 }
 
+struct ffrs_graph {
+    ffrs_ctx *ctx;
+    hipGraphExec_t exec;
+};
+
+int ffrs_ctx_capture_begin(ffrs_ctx *ctx)
+{
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
+    FFRS_HIP(hipSetDevice(ctx->device));
+    FFRS_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    return FFRS_OK;
+}
+
+int ffrs_ctx_capture_end(ffrs_ctx *ctx, ffrs_graph **out)
+{
+    if (!ctx || !out)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    FFRS_HIP(hipSetDevice(ctx->device));
+    hipGraph_t graph = nullptr;
+    FFRS_HIP(hipStreamEndCapture(ctx->stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_DEVICE, "graph instantiation failed: %s", hipGetErrorString(e));
+    *out = new ffrs_graph{ctx, exec};
+    return FFRS_OK;
+}
+
+int ffrs_graph_launch(ffrs_graph *g)
+{
+    if (!g)
+        return fail(FFRS_ERR_INVALID, "NULL graph");
+    FFRS_HIP(hipSetDevice(g->ctx->device));
+    FFRS_HIP(hipGraphLaunch(g->exec, g->ctx->stream));
+    return FFRS_OK;
+}
+
+void ffrs_graph_destroy(ffrs_graph *g)
+{
+    if (!g)
+        return;
+    (void)hipSetDevice(g->ctx->device);
+    (void)hipGraphExecDestroy(g->exec);
+    delete g;
+}
+
 int ffrs_ctx_kernel_timing(ffrs_ctx *ctx, int enable)
 {
     if (!ctx)

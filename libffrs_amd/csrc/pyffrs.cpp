@@ -585,7 +585,9 @@ PYBIND11_MODULE(libffrs, m)
         .def("repair_batch", &PyRS256::repair_batch,
              "Repair count back-to-back codewords (message and ecc buffers); returns one status byte each",
              "buffer"_a, "ecc"_a)
-        .def("_synds", &PyRS256::synds, "Compute syndromes", "buffer"_a);
+        .def("_synds", &PyRS256::synds, "Compute syndromes", "buffer"_a)
+        .def("_set_repair_mode", [](PyRS256 &s, int mode) { check(ffrs_rs256_set_repair_mode(s.codec, mode), "RS256"); },
+             "0: automatic, 1: pipeline (ecc_len <= 32), 2: one codeword per lane", "mode"_a);
 
     py::enum_<RepairStatus>(m, "RepairStatus", "Repair operation status")
         .value("NoErrors", RepairStatus::NoErrors)

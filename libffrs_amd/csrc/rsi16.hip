@@ -1386,7 +1386,15 @@ int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_le
     std::vector<uint32_t> order(count);
     for (size_t i = 0; i < count; ++i)
         order[i] = (uint32_t)i;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return cls[a] < cls[b]; });
+    // costliest classes first (largest ecc_len, then largest block_len)
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const auto &ka = keys[cls[a]], &kb = keys[cls[b]];
+        if (ka.second != kb.second)
+            return ka.second > kb.second;
+        if (ka.first != kb.first)
+            return ka.first > kb.first;
+        return cls[a] < cls[b];
+    });
     std::vector<Rsi16PlanItem> items(count);
     std::vector<Rsi16PlanChunk> chunks;
     for (size_t j = 0; j < count; ++j) {

@@ -83,16 +83,18 @@ class OracleBackend:
 class GpuBackend:
     """The HIP path through libffrs_amd's public API."""
 
-    def __init__(self):
+    def __init__(self, repair_mode=0):
         import libffrs_amd
         self.L = libffrs_amd
         self._rs = {}
         self._ri = {}
+        self.repair_mode = repair_mode
 
     def rs(self, ecc, block_len=255):
         key = ecc
         if key not in self._rs:
             self._rs[key] = self.L.RS256(block_len=255 if ecc < 255 else None, ecc_len=ecc)
+            self._rs[key]._set_repair_mode(self.repair_mode)
         r = self._rs[key]
         r.block_len = block_len
         return r

@@ -13,18 +13,25 @@ def gpu():
     return GpuBackend()
 
 
+# RS256 repair: automatic kernel choice, the three-kernel pipeline, one kernel per lane
+@pytest.fixture(scope="module", params=[0, 1, 2], ids=["auto", "pipeline", "lane"])
+def gpu_rs(request):
+    from backends import GpuBackend
+    return GpuBackend(repair_mode=request.param)
+
+
 @pytest.mark.parametrize("name", C.RS256_PLAIN_CASES)
-def test_rs256_plain(gpu, name):
-    R.check_rs256_plain(gpu, name)
+def test_rs256_plain(gpu_rs, name):
+    R.check_rs256_plain(gpu_rs, name)
 
 
 @pytest.mark.parametrize("shape", C.RS256_SHAPES, ids=lambda s: f"{s[0]}_{s[1]}")
-def test_rs256_shapes(gpu, shape):
-    R.check_rs256_shape(gpu, *shape)
+def test_rs256_shapes(gpu_rs, shape):
+    R.check_rs256_shape(gpu_rs, *shape)
 
 
-def test_rs256_mixed(gpu):
-    R.check_rs256_mixed(gpu)
+def test_rs256_mixed(gpu_rs):
+    R.check_rs256_mixed(gpu_rs)
 
 
 @pytest.mark.parametrize("name", C.RSI16_PLAIN_CASES)
