@@ -519,6 +519,76 @@ def run_mixed(args):
     return res
 
 
+def run_serving(args):
+    """Serving loop: Poisson arrivals (args.serve_rate per second) of par-style requests, 70%
+    encode and 30% verify+repair, files log-uniform in 4 KiB-64 MiB split into RSi16(1024, 128)
+    blocks (ecc 128: 12.5% parity, the legal neighbour of 10%), repair requests corrupted at
+    0-5% of their symbols (half single errors, half 8-64 symbol bursts). Requests are batched
+    continuously by libffrs_amd.serving.BlockServer; latency runs from submission to the
+    moment the last block's results are back in host memory."""
+    from libffrs_amd import workloads as W
+    from libffrs_amd.serving import BlockServer
+
+    n_req, rate = args.serve_requests, args.serve_rate
+    t_arr, kinds, sizes, s = W.serving_requests(n_req, rate, seed=6)
+    t0 = time.perf_counter()
+    datas = [s.bytes(int(sz)) for sz in sizes]
+    srv = BlockServer(block_len=1024, ecc_len=128, slots=args.serve_slots, slot_blocks=args.serve_slot_blocks,
+                      piece_blocks=2048)
+    # repair requests: encode their files first (not timed), then corrupt codewords
+    payload = []
+    originals = {}
+    for i, (kind, d) in enumerate(zip(kinds, datas)):
+        if kind == "encode":
+            payload.append((kind, d, None))
+            continue
+        r = srv.submit("encode", d)
+        r.done.wait()
+        msgs, par = r.msgs.copy(), r.parity.copy()
+        originals[i] = r.msgs.copy()
+        W.serving_corrupt(msgs, par, s)
+        payload.append((kind, msgs.reshape(-1).view(np.uint8), par))
+    log(f"serving: {n_req} requests prepared in {time.perf_counter() - t0:.1f} s "
+        f"({sizes.sum() / 1e9:.2f} GB, rate {rate}/s)")
+    srv.batches.clear()
+    reqs = []
+    start = time.perf_counter()
+    for i, (kind, d, par) in enumerate(payload):
+        delay = start + t_arr[i] - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        reqs.append(srv.submit(kind, d, par))
+    for r in reqs:
+        r.done.wait()
+    end = max(r.t_done for r in reqs)
+    lat = np.array([r.latency for r in reqs]) * 1e3
+    total = sum(r.nbytes for r in reqs)
+    restored = sum(int((reqs[i].msgs == originals[i]).all(axis=1).sum()) for i in originals)
+    rep_blocks = sum(reqs[i].blocks for i in originals)
+    ok_blocks = sum(int(np.isin(reqs[i].status, (0, 2)).sum()) for i in originals)
+    mixed = sum(1 for b in srv.batches if b[1] > 1)
+    res = {
+        "workload": "serving (Poisson arrivals, 70% encode / 30% verify+repair, RSi16(1024, 128) blocks)",
+        "requests": n_req,
+        "rate_per_s": rate,
+        "offered_gbps": round(float(sizes.mean()) * rate / 1e9, 3),
+        "bytes": int(total),
+        "seconds": round(end - start, 3),
+        "gbps": round(total / (end - start) / 1e9, 3),
+        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
+                       "max": round(float(lat.max()), 3), "mean": round(float(lat.mean()), 3)},
+        "batches": len(srv.batches),
+        "batches_with_several_requests": mixed,
+        "slots": args.serve_slots,
+        "hbm_budget_bytes": srv.hbm_budget,
+        "repair_blocks": rep_blocks,
+        "repair_blocks_ok_status": ok_blocks,
+        "repair_blocks_restored": restored,
+    }
+    srv.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -531,6 +601,12 @@ def main():
     ap.add_argument("--mixed", action="store_true", help="run only the mixed-size batch and print its result")
     ap.add_argument("--no-mixed", action="store_true", help="leave the mixed-size batch out of the bench line")
     ap.add_argument("--no-graph", action="store_true", help="mixed batch without HIP graphs (for tracing tools)")
+    ap.add_argument("--serving", action="store_true", help="run only the serving loop and print its result")
+    ap.add_argument("--no-serving", action="store_true", help="leave the serving loop out of the bench line")
+    ap.add_argument("--serve-requests", type=int, default=300)
+    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--serve-slots", type=int, default=4)
+    ap.add_argument("--serve-slot-blocks", type=int, default=16384)
     ap.add_argument("--mixed-rs256", type=int, default=200000)
     ap.add_argument("--mixed-rsi16", type=int, default=50000)
     args = ap.parse_args()
@@ -543,10 +619,16 @@ def main():
     if args.mixed:
         print(json.dumps(run_mixed(args)), flush=True)
         return
+    if args.serving:
+        print(json.dumps(run_serving(args)), flush=True)
+        return
     res = run_plain(args.workload, args)
     if not args.no_mixed:
         log("running the mixed-size batch ...")
         res["mixed"] = run_mixed(args)
+    if not args.no_serving:
+        log("running the serving loop ...")
+        res["serving"] = run_serving(args)
     print(json.dumps(res), flush=True)
 
 

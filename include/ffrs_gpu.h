@@ -62,6 +62,12 @@ int ffrs_ctx_device_info(ffrs_ctx *ctx, char *name, size_t name_cap, int *cu_cou
                          size_t *total_bytes);
 int ffrs_device_alloc(ffrs_ctx *ctx, size_t bytes, void **d_ptr);
 int ffrs_device_free(ffrs_ctx *ctx, void *d_ptr);
+/* Pinned host memory (hipHostMalloc) for staging, and copies that only enqueue on the
+ * context's stream (pageable memory makes them synchronous). */
+int ffrs_host_alloc(ffrs_ctx *ctx, size_t bytes, void **h_ptr);
+int ffrs_host_free(ffrs_ctx *ctx, void *h_ptr);
+int ffrs_memcpy_h2d_async(ffrs_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
+int ffrs_memcpy_d2h_async(ffrs_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 int ffrs_memcpy_h2d(ffrs_ctx *ctx, void *d_dst, const void *h_src, size_t bytes);
 int ffrs_memcpy_d2h(ffrs_ctx *ctx, void *h_dst, const void *d_src, size_t bytes);
 int ffrs_memcpy_d2d(ffrs_ctx *ctx, void *d_dst, const void *d_src, size_t bytes);

@@ -33,6 +33,10 @@ SIGNATURES = [
                                      ctypes.POINTER(c_sz)]),
     ("ffrs_device_alloc", c_int, [c_vp, c_sz, ctypes.POINTER(c_vp)]),
     ("ffrs_device_free", c_int, [c_vp, c_vp]),
+    ("ffrs_host_alloc", c_int, [c_vp, c_sz, ctypes.POINTER(c_vp)]),
+    ("ffrs_host_free", c_int, [c_vp, c_vp]),
+    ("ffrs_memcpy_h2d_async", c_int, [c_vp, c_vp, c_vp, c_sz]),
+    ("ffrs_memcpy_d2h_async", c_int, [c_vp, c_vp, c_vp, c_sz]),
     ("ffrs_memcpy_h2d", c_int, [c_vp, c_vp, c_vp, c_sz]),
     ("ffrs_memcpy_d2h", c_int, [c_vp, c_vp, c_vp, c_sz]),
     ("ffrs_memcpy_d2d", c_int, [c_vp, c_vp, c_vp, c_sz]),
@@ -127,6 +131,17 @@ class Context:
         check(lib().ffrs_device_alloc(self.h, nbytes, ctypes.byref(p)), "alloc")
         return DeviceBuffer(self, p.value, nbytes)
 
+    def host_alloc(self, nbytes):
+        """Pinned host memory as a uint8 numpy array (freed with host_free)."""
+        import numpy as np
+        p = c_vp()
+        check(lib().ffrs_host_alloc(self.h, nbytes, ctypes.byref(p)), "host_alloc")
+        arr = np.ctypeslib.as_array((ctypes.c_uint8 * nbytes).from_address(p.value))
+        return arr
+
+    def host_free(self, arr):
+        check(lib().ffrs_host_free(self.h, arr.ctypes.data), "host_free")
+
     def timer_begin(self):
         check(lib().ffrs_timer_begin(self.h), "timer_begin")
 
@@ -188,6 +203,15 @@ class DeviceBuffer:
     def copy_from(self, other, nbytes=None):
         n = other.nbytes if nbytes is None else nbytes
         check(lib().ffrs_memcpy_d2d(self.ctx.h, self.ptr, other.ptr, n), "d2d")
+
+    def upload_async(self, arr, offset=0):
+        """Enqueue a copy from (pinned) host memory into [offset, offset + arr.nbytes)."""
+        assert offset + arr.nbytes <= self.nbytes and arr.flags.c_contiguous
+        check(lib().ffrs_memcpy_h2d_async(self.ctx.h, self.ptr + offset, arr.ctypes.data, arr.nbytes), "h2d async")
+
+    def download_async(self, arr, offset=0):
+        assert offset + arr.nbytes <= self.nbytes and arr.flags.c_contiguous
+        check(lib().ffrs_memcpy_d2h_async(self.ctx.h, arr.ctypes.data, self.ptr + offset, arr.nbytes), "d2h async")
 
     def view(self, offset, nbytes):
         """A non-owning window [offset, offset + nbytes) of this buffer."""

@@ -159,6 +159,44 @@ int ffrs_device_alloc(ffrs_ctx *ctx, size_t bytes, void **d_ptr)
     return FFRS_OK;
 }
 
+int ffrs_host_alloc(ffrs_ctx *ctx, size_t bytes, void **h_ptr)
+{
+    if (!ctx || !h_ptr)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    FFRS_HIP(hipSetDevice(ctx->device));
+    hipError_t e = hipHostMalloc(h_ptr, bytes ? bytes : 1, hipHostMallocDefault);
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_NOMEM, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    return FFRS_OK;
+}
+
+int ffrs_host_free(ffrs_ctx *ctx, void *h_ptr)
+{
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
+    FFRS_HIP(hipSetDevice(ctx->device));
+    FFRS_HIP(hipHostFree(h_ptr));
+    return FFRS_OK;
+}
+
+int ffrs_memcpy_h2d_async(ffrs_ctx *ctx, void *d_dst, const void *h_src, size_t bytes)
+{
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
+    FFRS_HIP(hipSetDevice(ctx->device));
+    FFRS_HIP(hipMemcpyAsync(d_dst, h_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_memcpy_d2h_async(ffrs_ctx *ctx, void *h_dst, const void *d_src, size_t bytes)
+{
+    if (!ctx)
+        return fail(FFRS_ERR_INVALID, "NULL context");
+    FFRS_HIP(hipSetDevice(ctx->device));
+    FFRS_HIP(hipMemcpyAsync(h_dst, d_src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    return FFRS_OK;
+}
+
 int ffrs_device_free(ffrs_ctx *ctx, void *d_ptr)
 {
     if (!ctx)

@@ -166,3 +166,61 @@ def rsi16_mixed(n_items, seed):
     u = s.u64(n_items) >> np.uint64(32)
     counts = ((u * (eccs // 2 + 1).astype(np.uint64)) >> np.uint64(32)).astype(np.int64)
     return ns.astype(np.int64), eccs.astype(np.int64), counts, s
+
+
+SERVE_MIN_BYTES, SERVE_MAX_BYTES = 4096, 64 << 20
+
+
+def serving_requests(n_requests, rate, seed, min_bytes=SERVE_MIN_BYTES, max_bytes=SERVE_MAX_BYTES,
+                     encode_frac=0.7):
+    """Poisson arrivals at `rate` per second: (arrival s, kind, size bytes) per request, sizes
+    log-uniform in [min_bytes, max_bytes], `encode_frac` of them encode and the rest
+    verify+repair."""
+    s = Stream(seed)
+    u = (s.u64(n_requests) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    gaps = -np.log1p(-u) / rate
+    t = np.cumsum(gaps) - gaps[0]
+    v = (s.u64(n_requests) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    sizes = np.floor(min_bytes * (max_bytes / min_bytes) ** v).astype(np.int64)
+    w = (s.u64(n_requests) >> np.uint64(11)).astype(np.float64) / float(1 << 53)
+    kinds = np.where(w < encode_frac, "encode", "repair")
+    return t, kinds, sizes, s
+
+
+def serving_corrupt(msgs, parity, stream, max_frac=0.05):
+    """Corrupts a request's codewords in place: a fraction uniform in [0, max_frac] of all
+    symbols (message and parity, codeword-major positions), half as single-symbol errors
+    and half as bursts of 8-64 contiguous symbols, each an additive non-zero offset mod
+    65537 (resampled while the result would be 65536). Returns the number of corrupted
+    symbol writes."""
+    blocks, k = msgs.shape
+    ecc = parity.shape[1]
+    n = k + ecc
+    total = blocks * n
+    frac = float((stream.u64(1)[0] >> np.uint64(11))) / float(1 << 53) * max_frac
+    count = int(round(frac * total))
+    singles = count // 2
+    pos = [stream.ints(0, total - 1, singles)] if singles else []
+    burst_budget = count - singles
+    while burst_budget > 0:
+        ln = int(stream.ints(8, 64, 1)[0])
+        ln = min(ln, burst_budget, total)
+        start = int(stream.ints(0, total - ln, 1)[0])
+        pos.append(np.arange(start, start + ln, dtype=np.int64))
+        burst_budget -= ln
+    if not pos:
+        return 0
+    pos = np.concatenate(pos)
+    b, j = pos // n, pos % n
+    flat = np.concatenate([msgs, parity], 1)
+    v = flat[b, j].astype(np.int64)
+    o = stream.ints(1, 65536, v.size)
+    nv = (v + o) % 65537
+    bad = np.nonzero(nv == 65536)[0]
+    while bad.size:
+        nv[bad] = (v[bad] + stream.ints(1, 65536, bad.size)) % 65537
+        bad = bad[nv[bad] == 65536]
+    flat[b, j] = nv.astype(np.uint16)
+    msgs[:] = flat[:, :k]
+    parity[:] = flat[:, k:]
+    return int(pos.size)
