@@ -604,7 +604,7 @@ def main():
     ap.add_argument("--serving", action="store_true", help="run only the serving loop and print its result")
     ap.add_argument("--no-serving", action="store_true", help="leave the serving loop out of the bench line")
     ap.add_argument("--serve-requests", type=int, default=300)
-    ap.add_argument("--serve-rate", type=float, default=100.0)
+    ap.add_argument("--serve-rate", type=float, default=800.0)
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-slot-blocks", type=int, default=16384)
     ap.add_argument("--mixed-rs256", type=int, default=200000)

@@ -59,6 +59,8 @@ SIGNATURES = [
     ("ffrs_rs256_repair_device", c_int, [c_vp, c_sz, c_vp, c_vp, c_sz, c_vp]),
     ("ffrs_rsi16_create", c_int, [c_vp, c_sz, c_sz, c_u32, ctypes.POINTER(c_vp)]),
     ("ffrs_rsi16_destroy", None, [c_vp]),
+    ("ffrs_rsi16_set_groups", c_int, [c_vp, c_uint, c_uint]),
+    ("ffrs_rsi16_get_groups", c_int, [c_vp, ctypes.POINTER(c_uint), ctypes.POINTER(c_uint)]),
     ("ffrs_rsi16_root", c_u32, [c_vp]),
     ("ffrs_rsi16_ntt_len", c_sz, [c_vp]),
     ("ffrs_rsi16_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_int]),
@@ -257,6 +259,14 @@ class RSi16Codec:
         if self.h:
             lib().ffrs_rsi16_destroy(self.h)
             self.h = None
+
+    def set_groups(self, g_encode=0, g_repair=0):
+        check(lib().ffrs_rsi16_set_groups(self.h, int(g_encode), int(g_repair)), "set_groups")
+
+    def groups(self):
+        a, b = c_uint(), c_uint()
+        check(lib().ffrs_rsi16_get_groups(self.h, ctypes.byref(a), ctypes.byref(b)), "get_groups")
+        return a.value, b.value
 
     def encode_device(self, d_msg, count, d_ecc, vector_lanes=True):
         check(lib().ffrs_rsi16_encode_device(self.h, d_msg.ptr, count, d_ecc.ptr, 1 if vector_lanes else 0),

@@ -633,5 +633,7 @@ PYBIND11_MODULE(libffrs, m)
         .def("_sugiyama", &PyRSi16::sugiyama, "Compute error locator and evaluator polynomials", "synd"_a)
         .def("message_offset", &PyRSi16::message_offset, "row"_a, "col"_a)
         .def("ecc_offset", &PyRSi16::ecc_offset, "row"_a, "col"_a)
-        .def("_roots", &PyRSi16::roots, "Find locator polynomial roots", "synd"_a);
+        .def("_roots", &PyRSi16::roots, "Find locator polynomial roots", "synd"_a)
+        .def("_set_groups", [](PyRSi16 &s, unsigned ge, unsigned gr) { check(ffrs_rsi16_set_groups(s.codec, ge, gr), "RSi16"); },
+             "Lanes per codeword of the encode and repair kernels (0 keeps)", "g_encode"_a, "g_repair"_a);
 }

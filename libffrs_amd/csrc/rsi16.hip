@@ -557,35 +557,43 @@ __device__ inline void repair_group(const Rsi16Dev &T, uint16_t *m, uint16_t *e,
     uint32_t lanes = L != 0 ? 0xffffffffu : 0u;
     uint32_t dmax = der_len & lanes, emax = eval_len & lanes;
     uint32_t nb = T.n / E;
-    uint32_t found = 0;
-    // NTT::pintt_ecc (ntt.hpp:206-220), one chunk of E positions at a time
+    // NTT::pintt_ecc (ntt.hpp:206-220), one chunk of E positions at a time. The zeros are
+    // listed first (the Sugiyama scratch is free now) and corrected afterwards with one
+    // root per lane: corrections touch only the block, never the values the scan reads,
+    // so the order does not change the result. A non-zero locator of degree < E has
+    // fewer than E roots.
+    uint32_t *list = w, *nroots = w + E;
+    if (G.gl == 0)
+        *nroots = 0;
+    wave_sync();
     for (uint32_t c = 0; c < nb; ++c) {
         for (uint32_t j = G.gl; j < E; j += G.g)
             chunk[j] = S::mul(loc[j], T.pintt_shift[c * E + j]);
         wave_sync();
         gs_full<S>(chunk, E, T.roots_i_ecc, G);
-        for (uint32_t j = G.gl; j < E; j += G.g) {
-            uint32_t pos = c * E + j;
-            if (chunk[j] == 0 && lanes) {
-                found++;
-                uint32_t ri = rbo_bits(pos, T.nbits);
-                uint32_t x = T.roots_ntt[ri];
-                uint32_t xi = T.roots_ntt[(T.N - ri) & (T.N - 1)];
-                uint32_t num = eval_poly(syn, emax, xi);
-                uint32_t den = eval_poly(der, dmax, xi);
-                uint32_t err = S::mul(g_div(num, den), x);
-                uint16_t *at = pos < T.k ? &m[pos] : &e[pos - T.k];
-                *at = (uint16_t)(S::add((uint32_t)*at, err) & 0xffff);
-            }
+        if (lanes) {
+            for (uint32_t j = G.gl; j < E; j += G.g)
+                if (chunk[j] == 0) {
+                    uint32_t slot = atomicAdd(nroots, 1u);
+                    if (slot < E)
+                        list[slot] = c * E + j;
+                }
         }
         wave_sync();
     }
-    // group-wide root count: lanes publish their counts through the chunk row
-    chunk[G.gl] = found;
-    wave_sync();
-    uint32_t total = 0;
-    for (uint32_t j = 0; j < G.g; ++j)
-        total += chunk[j];
+    const uint32_t total = *nroots;
+    const uint32_t listed = total < E ? total : E;
+    for (uint32_t r = G.gl; r < listed; r += G.g) {
+        uint32_t pos = list[r];
+        uint32_t ri = rbo_bits(pos, T.nbits);
+        uint32_t x = T.roots_ntt[ri];
+        uint32_t xi = T.roots_ntt[(T.N - ri) & (T.N - 1)];
+        uint32_t num = eval_poly(syn, emax, xi);
+        uint32_t den = eval_poly(der, dmax, xi);
+        uint32_t err = S::mul(g_div(num, den), x);  // _forney (rsi16v_impl.hpp:500-512)
+        uint16_t *at = pos < T.k ? &m[pos] : &e[pos - T.k];
+        *at = (uint16_t)(S::add((uint32_t)*at, err) & 0xffff);
+    }
     if (G.gl == 0)
         *st = total > 0 ? FFRS_RSI16_REPAIR_OK : (lanes ? FFRS_RSI16_ERROR_LOCATION_FAIL : FFRS_RSI16_NO_ERRORS);
 }
@@ -809,6 +817,7 @@ struct ffrs_rsi16 {
     uint32_t n = 0, k = 0, E = 0, N = 0, nbits = 0, primitive = 0, root = 0;
     uint32_t *d_tables = nullptr;
     Rsi16Dev dev{};
+    uint32_t g_enc = 0, g_rep = 0;  // lanes per codeword of the encode and repair kernels
 };
 
 namespace {
@@ -856,7 +865,7 @@ int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t 
     marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
-    uint32_t g = group_size(rs->E), cpw = 64 / g;
+    uint32_t g = rs->g_enc, cpw = 64 / g;
     size_t lds = (size_t)cpw * 2 * rs->E * 4;
     dim3 grid((unsigned)((count + cpw - 1) / cpw));
     if (int r = allow_lds(rsi16_encode_kernel<VectorOps>, lds))
@@ -879,7 +888,7 @@ int launch_repair(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count
     marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
-    uint32_t g = group_size(rs->E), cpw = 64 / g;
+    uint32_t g = rs->g_rep, cpw = 64 / g;
     size_t lds = (size_t)cpw * REPAIR_WORDS * rs->E * 4;
     dim3 grid((unsigned)((count + cpw - 1) / cpw));
     if (int r = allow_lds(rsi16_repair_kernel, lds))
@@ -1004,6 +1013,7 @@ int ffrs_rsi16_create(ffrs_ctx *ctx, size_t block_len, size_t ecc_len, uint32_t 
         ffrs_rsi16_destroy(rs);
         return fail(FFRS_ERR_DEVICE, "table upload failed: %s", hipGetErrorString(herr));
     }
+    rs->g_enc = rs->g_rep = group_size(rs->E);
     *out = rs;
     return FFRS_OK;
 }
@@ -1019,6 +1029,32 @@ void ffrs_rsi16_destroy(ffrs_rsi16 *rs)
 }
 
 uint32_t ffrs_rsi16_root(const ffrs_rsi16 *rs) { return rs ? rs->root : 0; }
+
+int ffrs_rsi16_set_groups(ffrs_rsi16 *rs, unsigned g_encode, unsigned g_repair)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    unsigned cap = std::min<unsigned>(64, rs->E);
+    for (unsigned g : {g_encode, g_repair})
+        if (g && ((g & (g - 1)) || g > cap))
+            return fail(FFRS_ERR_INVALID, "lanes per codeword must be a power of two <= %u, got %u", cap, g);
+    if (g_repair && (size_t)(64 / g_repair) * REPAIR_WORDS * rs->E * 4 > 160 * 1024)
+        return fail(FFRS_ERR_INVALID, "%u lanes per codeword need more LDS than a CU has", g_repair);
+    if (g_encode)
+        rs->g_enc = g_encode;
+    if (g_repair)
+        rs->g_rep = g_repair;
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_get_groups(const ffrs_rsi16 *rs, unsigned *g_encode, unsigned *g_repair)
+{
+    if (!rs || !g_encode || !g_repair)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    *g_encode = rs->g_enc;
+    *g_repair = rs->g_rep;
+    return FFRS_OK;
+}
 size_t ffrs_rsi16_ntt_len(const ffrs_rsi16 *rs) { return rs ? rs->N : 0; }
 
 int ffrs_rsi16_encode_device(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vector_lanes)
@@ -1376,7 +1412,7 @@ int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_le
             }
             keys.push_back(key);
             p->codecs.push_back(codec);
-            uint32_t g = group_size(codec->E);
+            uint32_t g = codec->g_rep;
             classes.push_back({codec->dev, g});
             p->lds_enc = std::max<uint32_t>(p->lds_enc, (64 / g) * 2 * codec->E * 4);
             p->lds_rep = std::max<uint32_t>(p->lds_rep, (64 / g) * REPAIR_WORDS * codec->E * 4);
