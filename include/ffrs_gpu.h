@@ -147,6 +147,16 @@ size_t ffrs_rsi16_ntt_len(const ffrs_rsi16 *rs);
  * SIMD instantiations' lane arithmetic (W = 4/8/16, the reference's default on x86-64),
  * 0 the scalar W = 1 one. Parity symbols are written as value & 0xFFFF. */
 int ffrs_rsi16_encode(ffrs_rsi16 *rs, const uint16_t *msg, size_t count, uint16_t *ecc, int vector_lanes);
+/* Interleaved encode, RSi16(block_len, ecc_len, interleave) (libffrs/pyrsi16.hpp:143-150,
+ * 324-361): `blocks` interleaved blocks of message_len * interleave symbols, symbol i of
+ * column c at i * interleave + c; parity symbol j of column c at j * interleave + c of the
+ * block's ecc_len * interleave outputs. Every block is encoded (the reference fills only the
+ * first block of a multi-block call and leaves the others uninitialised). */
+int ffrs_rsi16_encode_interleaved(ffrs_rsi16 *rs, const uint16_t *msg, size_t blocks, size_t interleave, uint16_t *ecc,
+                                  int vector_lanes);
+int ffrs_rsi16_encode_interleaved_device(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t blocks, size_t interleave,
+                                         uint16_t *d_ecc, int vector_lanes);
+
 /* `count` calls of PyRSi16::py_repair without error positions (libffrs/pyrsi16.hpp:588-631
  * -> rsi16v_impl.hpp:122-168); repaired in place, status[i] as above. */
 int ffrs_rsi16_repair(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, size_t count, uint8_t *status);

@@ -64,6 +64,8 @@ SIGNATURES = [
     ("ffrs_rsi16_root", c_u32, [c_vp]),
     ("ffrs_rsi16_ntt_len", c_sz, [c_vp]),
     ("ffrs_rsi16_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_int]),
+    ("ffrs_rsi16_encode_interleaved", c_int, [c_vp, c_vp, c_sz, c_sz, c_vp, c_int]),
+    ("ffrs_rsi16_encode_interleaved_device", c_int, [c_vp, c_vp, c_sz, c_sz, c_vp, c_int]),
     ("ffrs_rsi16_repair", c_int, [c_vp, c_vp, c_vp, c_sz, c_vp]),
     ("ffrs_rsi16_repair_known", c_int, [c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
     ("ffrs_rsi16_synds", c_int, [c_vp, c_vp, c_vp, c_sz, c_vp]),

@@ -340,7 +340,7 @@ public:
         PY_ASSERT(block_len % ecc_len == 0, "block_len must be a multiple of ecc_len. block_len: " +
                                                 std::to_string(block_len) + " ecc_len: " + std::to_string(ecc_len));
         PY_ASSERT(block_len > ecc_len, "block_len must exceed ecc_len");
-        PY_ASSERT(interleave == 1, "interleave > 1 is not supported by this build");
+        PY_ASSERT(interleave >= 1 && interleave <= 0xffff, "interleave must be in [1, 65535]");
         int r = ffrs_rsi16_create(default_ctx(), block_len, ecc_len, primitive, &codec);
         check(r, "RSi16");
     }
@@ -361,13 +361,14 @@ public:
         PY_ASSERT(buf.size % im == 0, std::to_string(buf.size));
         size_t count = buf.size / im;
         py::bytearray out;
-        PyByteArray_Resize(out.ptr(), (Py_ssize_t)(count * ecc_len * 2));
+        PyByteArray_Resize(out.ptr(), (Py_ssize_t)(count * ecc_len * interleave * 2));
         uint16_t *o = reinterpret_cast<uint16_t *>(PyByteArray_AsString(out.ptr()));
         int r;
         bool vec = vector_lanes();
         {
             py::gil_scoped_release nogil;
-            r = ffrs_rsi16_encode(codec, buf.data, count, o, vec ? 1 : 0);
+            r = interleave == 1 ? ffrs_rsi16_encode(codec, buf.data, count, o, vec ? 1 : 0)
+                                : ffrs_rsi16_encode_interleaved(codec, buf.data, count, interleave, o, vec ? 1 : 0);
         }
         check(r, "RSi16.encode");
         return out;
@@ -385,8 +386,11 @@ public:
         Buf<uint16_t> m(message), e(ecc);
         if (error_pos && error_pos->empty())
             return RepairStatus::NoErrors;
-        PY_ASSERT(m.size == message_len, std::to_string(m.size));
-        PY_ASSERT(e.size == ecc_len, std::to_string(e.size));
+        PY_ASSERT(m.size == message_len * interleave, std::to_string(m.size));
+        PY_ASSERT(e.size == ecc_len * interleave, std::to_string(e.size));
+        if (interleave != 1)
+            throw std::runtime_error("repair with interleave > 1 is not supported by this build (the reference repairs "
+                                     "columns in coupled SIMD groups, pyrsi16.hpp:391-522)");
         uint8_t st = 0;
         int r;
         if (error_pos) {
@@ -413,6 +417,7 @@ public:
     py::bytearray repair_batch(py::buffer message, py::buffer ecc)
     {
         Buf<uint16_t> m(message), e(ecc);
+        PY_ASSERT(interleave == 1, "repair_batch needs interleave 1");
         size_t count = m.size / message_len;
         PY_ASSERT(m.size == count * message_len && e.size == count * ecc_len, "batch sizes");
         py::bytearray out;

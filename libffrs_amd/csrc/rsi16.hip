@@ -233,14 +233,15 @@ __device__ inline uint32_t poly_len(const uint32_t *r, uint32_t max)
 // One codeword per lane group: pntt_message_residue + _mix_ecc_residue
 // (rsi16v_impl.hpp:93-119), values written as & 0xFFFF. buf holds 2 E words.
 template <class O>
-__device__ inline void encode_group(const Rsi16Dev &T, const uint16_t *m, uint16_t *o, uint32_t *buf, Grp G)
+__device__ inline void encode_group(const Rsi16Dev &T, const uint16_t *m, uint16_t *o, uint32_t *buf, Grp G,
+                                    uint32_t stride = 1)
 {
     const uint32_t E = T.E;
     uint32_t *acc = buf + E;
     uint32_t nb = T.n / E;
     for (uint32_t c = 0; c + 1 < nb; ++c) {
         for (uint32_t j = G.gl; j < E; j += G.g)
-            buf[j] = m[c * E + j];
+            buf[j] = m[(size_t)(c * E + j) * stride];
         wave_sync();
         ct_residue<O>(buf, E, T.roots_ecc, G);
         for (uint32_t j = G.gl; j < E; j += G.g) {
@@ -254,12 +255,15 @@ __device__ inline void encode_group(const Rsi16Dev &T, const uint16_t *m, uint16
     wave_sync();
     gs_full<O>(acc, E, T.roots_i_ecc, G);
     for (uint32_t j = G.gl; j < E; j += G.g)
-        o[j] = (uint16_t)(acc[j] & 0xffff);
+        o[(size_t)j * stride] = (uint16_t)(acc[j] & 0xffff);
 }
 
+// Codeword cw = b * I + c is column c of interleaved block b: symbol i of it sits at
+// i * I + c of the block (pyrsi16.hpp:222-232, 324-361); I = 1 is the plain layout.
 template <class O>
 __global__ __launch_bounds__(64) void rsi16_encode_kernel(const uint16_t *__restrict__ msgs, uint32_t count,
-                                                           uint16_t *__restrict__ out, Rsi16Dev T, uint32_t g)
+                                                           uint16_t *__restrict__ out, Rsi16Dev T, uint32_t g,
+                                                           uint32_t I)
 {
     extern __shared__ uint32_t s_mem[];
     const uint32_t E = T.E;
@@ -269,7 +273,8 @@ __global__ __launch_bounds__(64) void rsi16_encode_kernel(const uint16_t *__rest
     uint32_t cw = blockIdx.x * cpw + grp;
     if (cw >= count)
         return;
-    encode_group<O>(T, msgs + (size_t)cw * T.k, out + (size_t)cw * E, s_mem + grp * 2 * E, G);
+    const size_t b = cw / I, c = cw % I;
+    encode_group<O>(T, msgs + b * T.k * I + c, out + b * E * I + c, s_mem + grp * 2 * E, G, I);
 }
 
 // ---------------------------------------------------------------- syndromes
@@ -860,7 +865,7 @@ int check_count(size_t count)
     return FFRS_OK;
 }
 
-int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vec)
+int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vec, uint32_t I = 1)
 {
     marks_begin(rs->ctx);
     if (count == 0)
@@ -874,10 +879,10 @@ int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t 
         return r;
     if (vec)
         hipLaunchKernelGGL(rsi16_encode_kernel<VectorOps>, grid, dim3(64), lds, rs->ctx->stream, d_msg,
-                           (uint32_t)count, d_ecc, rs->dev, g);
+                           (uint32_t)count, d_ecc, rs->dev, g, I);
     else
         hipLaunchKernelGGL(rsi16_encode_kernel<ScalarOps>, grid, dim3(64), lds, rs->ctx->stream, d_msg,
-                           (uint32_t)count, d_ecc, rs->dev, g);
+                           (uint32_t)count, d_ecc, rs->dev, g, I);
     FFRS_HIP(hipGetLastError());
     mark_kernel(rs->ctx);
     return FFRS_OK;
@@ -1105,6 +1110,54 @@ int ffrs_rsi16_encode(ffrs_rsi16 *rs, const uint16_t *msg, size_t count, uint16_
         return r;
     FFRS_HIP(hipMemcpyAsync(d_m, msg, mb, hipMemcpyHostToDevice, ctx->stream));
     if (int r = launch_encode(rs, (const uint16_t *)d_m, count, (uint16_t *)d_e, vector_lanes))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(ecc, d_e, eb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_encode_interleaved_device(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t blocks, size_t interleave,
+                                         uint16_t *d_ecc, int vector_lanes)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (interleave == 0 || interleave > 0xffffu)
+        return fail(FFRS_ERR_INVALID, "interleave must be in [1, 65535], got %zu", interleave);
+    if (int r = check_count(blocks * interleave))
+        return r;
+    if (blocks && (!d_msg || !d_ecc))
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
+    if (int r = ctx_set_device(rs->ctx))
+        return r;
+    return launch_encode(rs, d_msg, blocks * interleave, d_ecc, vector_lanes, (uint32_t)interleave);
+}
+
+int ffrs_rsi16_encode_interleaved(ffrs_rsi16 *rs, const uint16_t *msg, size_t blocks, size_t interleave, uint16_t *ecc,
+                                  int vector_lanes)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (interleave == 0 || interleave > 0xffffu)
+        return fail(FFRS_ERR_INVALID, "interleave must be in [1, 65535], got %zu", interleave);
+    if (int r = check_count(blocks * interleave))
+        return r;
+    if (blocks == 0)
+        return FFRS_OK;
+    if (!msg || !ecc)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    size_t mb = blocks * interleave * rs->k * 2, eb = blocks * interleave * rs->E * 2;
+    void *d_m, *d_e;
+    if (int r = stage_reserve(ctx, 0, mb, &d_m))
+        return r;
+    if (int r = stage_reserve(ctx, 1, eb, &d_e))
+        return r;
+    FFRS_HIP(hipMemcpyAsync(d_m, msg, mb, hipMemcpyHostToDevice, ctx->stream));
+    if (int r = launch_encode(rs, (const uint16_t *)d_m, blocks * interleave, (uint16_t *)d_e, vector_lanes,
+                              (uint32_t)interleave))
         return r;
     FFRS_HIP(hipMemcpyAsync(ecc, d_e, eb, hipMemcpyDeviceToHost, ctx->stream));
     FFRS_HIP(hipStreamSynchronize(ctx->stream));

@@ -58,6 +58,9 @@ class OracleBackend:
     def rsi16_synds(self, n, ecc, msg, parity):
         return self.ri(n, ecc).synds(msg, parity)
 
+    def rsi16_encode_interleaved(self, n, ecc, il, blocks, vector=True):
+        return _interleaved_by_columns(lambda c: self.ri(n, ecc).encode(c, vector), n, ecc, il, blocks)
+
     def rsi16_sugiyama(self, n, ecc, synds):
         L, loc, ev = self.ri(n, ecc).sugiyama(synds)
         return None if L == 0xFFFFFFFF else (loc, ev)
@@ -78,6 +81,16 @@ class OracleBackend:
             cws[i][:k] = m
             cws[i][k:] = p
         return st
+
+
+def _interleaved_by_columns(encode, n, ecc, il, blocks):
+    """Interleaved encode through a per-codeword encoder: column c of a block is symbols
+    c, c + il, c + 2 il, ... of it; parity symbol j of column c goes to j * il + c."""
+    k = n - ecc
+    blocks = np.asarray(blocks, np.uint16).reshape(-1, k, il)
+    cols = np.ascontiguousarray(blocks.transpose(0, 2, 1).reshape(-1, k))
+    par = encode(cols).reshape(-1, il, ecc)
+    return np.ascontiguousarray(par.transpose(0, 2, 1)).reshape(-1, ecc * il)
 
 
 class GpuBackend:
@@ -157,6 +170,14 @@ class GpuBackend:
         # the reference sizes buffers in bytes (util.hpp:30-55), so u16 arrays go in as byte views
         st = self.ri(n, ecc).repair_batch(msgs.view(np.uint8), parity.view(np.uint8))
         return np.frombuffer(bytes(st), np.uint8).copy()
+
+    def rsi16_encode_interleaved(self, n, ecc, il, blocks, vector=True):
+        kw = {} if vector else dict(simd_x4=False, simd_x8=False, simd_x16=False)
+        key = (n, ecc, il, vector)
+        if key not in self._ri:
+            self._ri[key] = self.L.RSi16(n, ecc, il, **kw)
+        out = self._ri[key].encode(np.ascontiguousarray(blocks, np.uint16).tobytes())
+        return np.frombuffer(bytes(out), np.uint16).reshape(-1, ecc * il).copy()
 
     def rsi16_synds(self, n, ecc, msg, parity):
         return np.array(self.ri(n, ecc)._synd(np.ascontiguousarray(msg, np.uint16).tobytes(),

@@ -230,6 +230,7 @@ def main():
     print("known-position statuses", np.bincount(st, minlength=5))
 
     out.update(radius_fixture(L))
+    out.update(interleave_fixture(L))
 
     for key, d in out.items():
         np.savez_compressed(os.path.join(HERE, key + ".npz"), **d)
@@ -258,8 +259,31 @@ def radius_fixture(L):
     return {"rsi16_edge_radius": dict(cws=cws[rows], counts=counts[rows], status=st, repaired=rep)}
 
 
+INTERLEAVE_CASES = ((64, 8, 3), (256, 32, 16), (256, 32, 5), (1024, 128, 2), (64, 16, 33))
+
+
+def interleave_fixture(L):
+    """(f) interleaved encode, RSi16(n, ecc, interleave): one interleaved block per call (the
+    reference leaves the later blocks of a multi-block call uninitialised), with the
+    host's SIMD words and with the scalar words."""
+    out = {}
+    s = W.Stream(17)
+    for n, ecc, il in INTERLEAVE_CASES:
+        k = n - ecc
+        msgs = s.u16(6 * k * il).reshape(6, k * il)
+        vec = L.RSi16(n, ecc, il)
+        sca = L.RSi16(n, ecc, il, simd_x4=False, simd_x8=False, simd_x16=False)
+        pv = np.stack([np.frombuffer(vec.encode(m.tobytes()), np.uint16) for m in msgs])
+        ps = np.stack([np.frombuffer(sca.encode(m.tobytes()), np.uint16) for m in msgs])
+        out[f"rsi16_interleave_{n}_{ecc}_{il}"] = dict(msgs=msgs, parity=pv, parity_scalar=ps)
+    return out
+
+
 if __name__ == "__main__":
-    if sys.argv[1:] == ["radius"]:
+    if sys.argv[1:] == ["interleave"]:
+        for key, v in interleave_fixture(load_reference()).items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    elif sys.argv[1:] == ["radius"]:
         from libffrs_amd import workloads as W  # noqa: F811
         d = radius_fixture(load_reference())
         for key, v in d.items():

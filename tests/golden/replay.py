@@ -192,3 +192,17 @@ def check_rsi16_known(be):
         cws[i, k:] = p
     assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
     assert (C.row_crc(cws) == fx["repaired_crc"]).all()
+
+
+def check_rsi16_interleaved(be):
+    """Interleaved encode against the reference, one block per call and all blocks of a
+    case in one call (every block encoded)."""
+    from make_golden import INTERLEAVE_CASES
+    for n, ecc, il in INTERLEAVE_CASES:
+        fx = fixture(f"rsi16_interleave_{n}_{ecc}_{il}")
+        for vector, key in ((True, "parity"), (False, "parity_scalar")):
+            for m, p in zip(fx["msgs"], fx[key]):
+                got = be.rsi16_encode_interleaved(n, ecc, il, m[None], vector)
+                assert (got.reshape(-1) == p).all(), (n, ecc, il, vector)
+            got = be.rsi16_encode_interleaved(n, ecc, il, fx["msgs"], vector)
+            assert (got.reshape(fx[key].shape) == fx[key]).all(), (n, ecc, il, vector, "multi-block")

@@ -49,3 +49,17 @@ def test_rsi16_edges(gpu):
 
 def test_rsi16_known(gpu):
     R.check_rsi16_known(gpu)
+
+
+def test_rsi16_interleaved(gpu):
+    R.check_rsi16_interleaved(gpu)
+
+
+def test_rsi16_interleaved_repair_raises(gpu):
+    import numpy as np
+    ri = gpu.L.RSi16(64, 8, 3)
+    assert ri.message_len == 56 * 3 and ri.ecc_len == 8 * 3 and ri.interleave == 3
+    m = np.zeros(56 * 3, np.uint16)
+    e = np.zeros(8 * 3, np.uint16)
+    with pytest.raises(RuntimeError, match="interleave"):
+        ri.repair(m.view(np.uint8), e.view(np.uint8))

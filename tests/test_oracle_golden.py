@@ -34,3 +34,7 @@ def test_rsi16_edges(oracle_backend):
 
 def test_rsi16_known(oracle_backend):
     R.check_rsi16_known(oracle_backend)
+
+
+def test_rsi16_interleaved(oracle_backend):
+    R.check_rsi16_interleaved(oracle_backend)
