@@ -135,8 +135,9 @@ int ffrs_rsi16_create(ffrs_ctx *ctx, size_t block_len, size_t ecc_len, uint32_t 
 void ffrs_rsi16_destroy(ffrs_rsi16 *rs);
 /* Lanes per codeword of the encode and repair kernels (no counterpart in the reference;
  * results do not depend on them): powers of two up to min(64, ecc_len); 0 keeps the
- * current value. Defaults min(64, ecc_len / 2); libffrs_amd.autotune picks them per
- * (block_len, ecc_len) by timing. */
+ * current value. Encode with 1 lane and ecc_len <= 32 runs the transforms in registers.
+ * Defaults: encode 1 for ecc_len <= 32, else min(64, ecc_len / 2); repair min(64, ecc_len / 2).
+ * libffrs_amd.autotune picks them per (block_len, ecc_len) by timing. */
 int ffrs_rsi16_set_groups(ffrs_rsi16 *rs, unsigned g_encode, unsigned g_repair);
 int ffrs_rsi16_get_groups(const ffrs_rsi16 *rs, unsigned *g_encode, unsigned *g_repair);
 uint32_t ffrs_rsi16_root(const ffrs_rsi16 *rs);

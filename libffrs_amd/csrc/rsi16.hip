@@ -277,6 +277,114 @@ __global__ __launch_bounds__(64) void rsi16_encode_kernel(const uint16_t *__rest
     encode_group<O>(T, msgs + b * T.k * I + c, out + b * E * I + c, s_mem + grp * 2 * E, G, I);
 }
 
+// ---- one codeword per lane, transforms in registers (ecc_len <= 32)
+
+// ct_residue with the E values in registers; the butterflies of a stage are independent,
+// so running them in sequence gives the same words as the cooperative version.
+template <class O, int E>
+__device__ inline void ct_residue_reg(uint32_t (&x)[E], const uint32_t *__restrict__ roots)
+{
+#pragma unroll
+    for (int s = 1; s < E; s <<= 1) {
+        const int ef = E / (2 * s);
+#pragma unroll
+        for (int b = 0; b < E / 2; ++b) {
+            const int j = b & (s - 1);
+            const int i = ((b - j) << 1) + j;
+            const uint32_t a = x[i], c = x[i + s];
+            if (j) {
+                const uint32_t m = O::mul_residue(O::mod_p(c), roots[ef * j]);
+                x[i] = a + m;
+                x[i + s] = a - m;
+            } else {
+                x[i] = a + c;
+                x[i + s] = a - c;
+            }
+        }
+    }
+}
+
+// gs_full with the E values in registers.
+template <class O, int E>
+__device__ inline void gs_reg(uint32_t (&x)[E], const uint32_t *__restrict__ roots)
+{
+    int ef = 0;
+#pragma unroll
+    for (int s = E / 2; s > 0; s >>= 1, ++ef) {
+#pragma unroll
+        for (int b = 0; b < E / 2; ++b) {
+            const int j = b & (s - 1);
+            const int i = ((b - j) << 1) + j;
+            const uint32_t a = x[i], c = x[i + s];
+            x[i] = O::add(a, c);
+            const uint32_t d = O::sub(a, c);
+            x[i + s] = j ? O::mul(d, roots[j << ef]) : d;
+        }
+    }
+}
+
+// encode_group with one codeword per lane: the message is read in 16-byte pieces (for
+// ecc_len >= 8 a codeword row and every chunk of it start 16-byte aligned), the twiddles
+// and shifts are wave-uniform scalar loads.
+template <class O, int E>
+__global__ __launch_bounds__(256) void rsi16_encode_lane_kernel(const uint16_t *__restrict__ msgs, uint32_t count,
+                                                                uint16_t *__restrict__ out, Rsi16Dev T)
+{
+    const uint32_t cw = blockIdx.x * 256 + threadIdx.x;
+    if (cw >= count)
+        return;
+    const uint32_t nb = T.n / E;
+    const uint16_t *m = msgs + (size_t)cw * T.k;
+    uint32_t acc[E];
+    for (uint32_t c = 0; c + 1 < nb; ++c) {
+        uint32_t x[E];
+        const uint16_t *mc = m + c * E;
+        if constexpr (E >= 8) {
+#pragma unroll
+            for (int q = 0; q < E / 8; ++q) {
+                const uint4 v = reinterpret_cast<const uint4 *>(mc)[q];
+                const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                for (int h = 0; h < 4; ++h) {
+                    x[8 * q + 2 * h] = w[h] & 0xffffu;
+                    x[8 * q + 2 * h + 1] = w[h] >> 16;
+                }
+            }
+        } else {
+#pragma unroll
+            for (int j = 0; j < E; ++j)
+                x[j] = mc[j];
+        }
+        ct_residue_reg<O, E>(x, T.roots_ecc);
+        const uint32_t *sh = T.pntt_shift + c * E;
+#pragma unroll
+        for (int j = 0; j < E; ++j) {
+            const uint32_t v = O::mul_residue(O::mod_p(x[j]), sh[j]);
+            acc[j] = c ? acc[j] + v : v;
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+        acc[j] = O::mul(O::mod_p(acc[j]), T.mix[j]);
+    gs_reg<O, E>(acc, T.roots_i_ecc);
+    uint16_t *o = out + (size_t)cw * E;
+    if constexpr (E >= 8) {
+#pragma unroll
+        for (int q = 0; q < E / 8; ++q) {
+            uint4 v;
+            v.x = (acc[8 * q + 0] & 0xffffu) | (acc[8 * q + 1] << 16);
+            v.y = (acc[8 * q + 2] & 0xffffu) | (acc[8 * q + 3] << 16);
+            v.z = (acc[8 * q + 4] & 0xffffu) | (acc[8 * q + 5] << 16);
+            v.w = (acc[8 * q + 6] & 0xffffu) | (acc[8 * q + 7] << 16);
+            reinterpret_cast<uint4 *>(o)[q] = v;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            o[j] = (uint16_t)(acc[j] & 0xffff);
+    }
+}
+
 // ---------------------------------------------------------------- syndromes
 
 // NTT::pntt (ntt.hpp:133-152) of the codeword msg || ecc into syn[0..E); buf is E scratch.
@@ -865,12 +973,30 @@ int check_count(size_t count)
     return FFRS_OK;
 }
 
+template <class O>
+int launch_encode_lane(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc)
+{
+    dim3 grid((unsigned)((count + 255) / 256));
+    auto *k = rs->E == 2 ? rsi16_encode_lane_kernel<O, 2> : rs->E == 4 ? rsi16_encode_lane_kernel<O, 4>
+              : rs->E == 8 ? rsi16_encode_lane_kernel<O, 8> : rs->E == 16 ? rsi16_encode_lane_kernel<O, 16>
+              : rsi16_encode_lane_kernel<O, 32>;
+    hipLaunchKernelGGL(k, grid, dim3(256), 0, rs->ctx->stream, d_msg, (uint32_t)count, d_ecc, rs->dev);
+    FFRS_HIP(hipGetLastError());
+    mark_kernel(rs->ctx);
+    return FFRS_OK;
+}
+
 int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vec, uint32_t I = 1)
 {
     marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
-    uint32_t g = rs->g_enc, cpw = 64 / g;
+    // one lane per codeword (g = 1): the register kernel for ecc_len <= 32, plain layout
+    const bool aligned = rs->E < 8 || (((uintptr_t)d_msg | (uintptr_t)d_ecc) & 15) == 0;
+    if (rs->g_enc == 1 && rs->E <= 32 && I == 1 && aligned)
+        return vec ? launch_encode_lane<VectorOps>(rs, d_msg, count, d_ecc)
+                   : launch_encode_lane<ScalarOps>(rs, d_msg, count, d_ecc);
+    uint32_t g = rs->g_enc == 1 && rs->E <= 32 ? group_size(rs->E) : rs->g_enc, cpw = 64 / g;
     size_t lds = (size_t)cpw * 2 * rs->E * 4;
     dim3 grid((unsigned)((count + cpw - 1) / cpw));
     if (int r = allow_lds(rsi16_encode_kernel<VectorOps>, lds))
@@ -1018,7 +1144,8 @@ int ffrs_rsi16_create(ffrs_ctx *ctx, size_t block_len, size_t ecc_len, uint32_t 
         ffrs_rsi16_destroy(rs);
         return fail(FFRS_ERR_DEVICE, "table upload failed: %s", hipGetErrorString(herr));
     }
-    rs->g_enc = rs->g_rep = group_size(rs->E);
+    rs->g_rep = group_size(rs->E);
+    rs->g_enc = rs->E <= 32 ? 1u : group_size(rs->E);  // register kernel, one codeword per lane
     *out = rs;
     return FFRS_OK;
 }
