@@ -117,6 +117,13 @@ __device__ inline uint32_t buf_load_u32(__amdgpu_buffer_rsrc_t r, uint32_t byte_
     return (uint32_t)__builtin_amdgcn_raw_buffer_load_b32(r, (int)byte_off, 0, 0);
 }
 
+// 16 bytes at a 4-byte aligned offset; the caller keeps all of them in range.
+__device__ inline uint4 buf_load_u32x4(__amdgpu_buffer_rsrc_t r, uint32_t byte_off)
+{
+    auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)byte_off, 0, 0);
+    return make_uint4(v[0], v[1], v[2], v[3]);
+}
+
 __device__ inline uint32_t alignbyte(uint32_t hi, uint32_t lo, uint32_t shift)
 {
     return __builtin_amdgcn_alignbyte(hi, lo, shift);

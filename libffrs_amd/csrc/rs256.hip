@@ -149,10 +149,10 @@ __device__ inline void lut_step(uint32_t (&rem)[EW], uint32_t w, const uint32_t 
 }
 
 // Remainder of x^ecc * M(x) by g(x) for the message of `k` bytes at byte offset
-// `msg_off` of the buffer behind `rsrc`. The message is fed in 4-byte words after
+// `msg_off` of the buffer behind `rsrc` (whose range is `nrec` bytes). The message is fed in 4-byte words after
 // `pad` leading zero bytes (leading zeros do not change the remainder).
 template <int EW, int S>
-__device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, int64_t msg_off, uint32_t k,
+__device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec, int64_t msg_off, uint32_t k,
                                          const uint32_t *__restrict__ s_lut, uint32_t (&rem)[EW])
 {
 #pragma unroll
@@ -168,19 +168,30 @@ __device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, int64_t ms
     uint32_t nw = P >> 2;
     uint32_t prev = buf_load_u32(rsrc, off);
     uint32_t t = 0;
-    for (; t + 4 <= nw; t += 4) {
-        uint32_t c0 = buf_load_u32(rsrc, off + 4 * (t + 1));
-        uint32_t c1 = buf_load_u32(rsrc, off + 4 * (t + 2));
-        uint32_t c2 = buf_load_u32(rsrc, off + 4 * (t + 3));
-        uint32_t c3 = buf_load_u32(rsrc, off + 4 * (t + 4));
-        uint32_t w0 = alignbyte(c0, prev, sh);
+    // 32 bytes per iteration, as two 16-byte loads where all of them are in range (each lane
+    // reads its own row, so wider loads mean fewer L1 lines fetched per byte); single
+    // words at the very end of the buffer
+    for (; t + 8 <= nw; t += 8) {
+        uint32_t c[8];
+        const uint32_t o = off + 4 * (t + 1);
+        if (o + 32 <= nrec) {
+            const uint4 q0 = buf_load_u32x4(rsrc, o);
+            const uint4 q1 = buf_load_u32x4(rsrc, o + 16);
+            c[0] = q0.x; c[1] = q0.y; c[2] = q0.z; c[3] = q0.w;
+            c[4] = q1.x; c[5] = q1.y; c[6] = q1.z; c[7] = q1.w;
+        } else {
+#pragma unroll
+            for (int j = 0; j < 8; ++j)
+                c[j] = buf_load_u32(rsrc, o + 4 * j);
+        }
+        uint32_t w0 = alignbyte(c[0], prev, sh);
         if (t == 0)
             w0 &= headmask;
         lut_step<EW, S>(rem, w0, s_lut);
-        lut_step<EW, S>(rem, alignbyte(c1, c0, sh), s_lut);
-        lut_step<EW, S>(rem, alignbyte(c2, c1, sh), s_lut);
-        lut_step<EW, S>(rem, alignbyte(c3, c2, sh), s_lut);
-        prev = c3;
+#pragma unroll
+        for (int j = 1; j < 8; ++j)
+            lut_step<EW, S>(rem, alignbyte(c[j], c[j - 1], sh), s_lut);
+        prev = c[7];
     }
     for (; t < nw; ++t) {
         uint32_t c = buf_load_u32(rsrc, off + 4 * (t + 1));
@@ -202,8 +213,8 @@ __device__ inline uint8_t rem_byte(const uint32_t (&rem)[EW], uint32_t j)
     return (uint8_t)(v >> (8 * (j & 3)));
 }
 
-template <int EW, int S>
-__global__ __launch_bounds__(256) void rs256_encode_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
+template <int EW, int S, int WG>
+__global__ __launch_bounds__(WG) void rs256_encode_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
                                                             uint32_t count, uint32_t k, uint8_t *__restrict__ out,
                                                             uint32_t ecc, const uint32_t *__restrict__ lut)
 {
@@ -212,11 +223,12 @@ __global__ __launch_bounds__(256) void rs256_encode_kernel(const uint8_t *__rest
         s_lut[i] = lut[i];
     __syncthreads();
 
-    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    const uint32_t nrec = (msg_bytes + 3u) & ~3u;
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, nrec);
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += stride) {
         uint32_t rem[EW];
-        message_remainder<EW, S>(rsrc, (int64_t)cw * k, k, s_lut, rem);
+        message_remainder<EW, S>(rsrc, nrec, (int64_t)cw * k, k, s_lut, rem);
         uint8_t *o = out + (size_t)cw * ecc;
         if ((ecc & 3) == 0) {
             uint32_t *o4 = reinterpret_cast<uint32_t *>(o);
@@ -313,7 +325,7 @@ __host__ __device__ inline uint32_t repair_row_stride(uint32_t ecc)
 // status (1: no errors or repaired, 0: not repairable, buffers untouched). `row` is this
 // lane's LDS scratch of repair_row_stride(E) bytes.
 template <int EW, int S>
-__device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, int64_t msg_off, uint8_t *pm, uint8_t *pe,
+__device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec, int64_t msg_off, uint8_t *pm, uint8_t *pe,
                                      uint32_t k, uint32_t E, const uint32_t *__restrict__ s_lut, const GfLds &gf,
                                      uint8_t *row)
 {
@@ -324,7 +336,7 @@ __device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, int64_t msg_of
     uint16_t *llam = reinterpret_cast<uint16_t *>(row + ((3 * E + 1) & ~1u));
     // ---- R(x) = C(x) mod g(x): remainder of the message XOR the received parity
     uint32_t rem[EW];
-    message_remainder<EW, S>(rsrc, msg_off, k, s_lut, rem);
+    message_remainder<EW, S>(rsrc, nrec, msg_off, k, s_lut, rem);
     if (xor_parity<EW>(rem, pe, E) == 0)  // all syndromes zero (reed_solomon.hpp:565-566)
         return 1;
     syndromes_from_remainder<EW>(rem, E, gf, Sy);
@@ -453,10 +465,11 @@ __global__ __launch_bounds__(WG) void rs256_repair_kernel(uint8_t *__restrict__ 
     __syncthreads();
     GfLds gf{s_exp2, s_log2};
     uint8_t *row = s_rows + threadIdx.x * repair_row_stride(ecc);
-    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    const uint32_t nrec = (msg_bytes + 3u) & ~3u;
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, nrec);
     uint32_t stride = gridDim.x * blockDim.x;
     for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += stride)
-        status[cw] = repair_one<EW, S>(rsrc, (int64_t)cw * k, msgs + (size_t)cw * k, eccs + (size_t)cw * ecc, k, ecc,
+        status[cw] = repair_one<EW, S>(rsrc, nrec, (int64_t)cw * k, msgs + (size_t)cw * k, eccs + (size_t)cw * ecc, k, ecc,
                                        s_lut, gf, row);
 }
 
@@ -478,10 +491,11 @@ __global__ __launch_bounds__(256) void rs256_synds_kernel(const uint8_t *__restr
         s_log2[i] = tab.log2[i];
     __syncthreads();
     GfLds gf{s_exp2, s_log2};
-    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    const uint32_t nrec = (msg_bytes + 3u) & ~3u;
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, nrec);
     for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += gridDim.x * blockDim.x) {
         uint32_t rem[EW];
-        message_remainder<EW, S>(rsrc, (int64_t)cw * k, k, s_lut, rem);
+        message_remainder<EW, S>(rsrc, nrec, (int64_t)cw * k, k, s_lut, rem);
         xor_parity<EW>(rem, eccs + (size_t)cw * ecc, ecc);
         syndromes_from_remainder<EW>(rem, ecc, gf, synds + (size_t)cw * ecc);
     }
@@ -510,8 +524,8 @@ __host__ __device__ inline uint32_t rec_lam_off() { return 20; }
 __host__ __device__ inline uint32_t rec_om_off(uint32_t) { return 4; }
 __host__ __device__ inline uint32_t rec_stride(uint32_t) { return 48; }
 
-template <int EW, int S>
-__global__ __launch_bounds__(256) void rs256_synd_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
+template <int EW, int S, int WG>
+__global__ __launch_bounds__(WG) void rs256_synd_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
                                                           const uint8_t *__restrict__ eccs, uint32_t count,
                                                           uint32_t k, uint32_t E, uint8_t *__restrict__ status,
                                                           uint32_t *__restrict__ synd, const uint32_t *__restrict__ lut,
@@ -525,10 +539,11 @@ __global__ __launch_bounds__(256) void rs256_synd_kernel(const uint8_t *__restri
     for (uint32_t i = threadIdx.x; i < E * 32 * EW; i += blockDim.x)
         s_st[i] = stab[i];
     __syncthreads();
-    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    const uint32_t nrec = (msg_bytes + 3u) & ~3u;
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, nrec);
     for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += gridDim.x * blockDim.x) {
         uint32_t rem[EW];
-        message_remainder<EW, S>(rsrc, (int64_t)cw * k, k, s_lut, rem);
+        message_remainder<EW, S>(rsrc, nrec, (int64_t)cw * k, k, s_lut, rem);
         if (xor_parity<EW>(rem, eccs + (size_t)cw * E, E) == 0) {
             status[cw] = 1;
             continue;
@@ -713,19 +728,23 @@ __global__ __launch_bounds__(WG) void rs256_bm_fixed_kernel(uint32_t count, uint
             llam[i] = i == 0 ? 0u : (uint32_t)ZLOG;
             lBp[i] = i == 1 ? 0u : (uint32_t)ZLOG;
         }
+        // L never decreases, and a codeword whose L exceeds E/2 fails whatever follows; while
+        // L <= E/2, deg Lambda <= L and every x^m B that enters an update has degree <= the new
+        // L. So terms above E/2 (above E/2 + 1 for B') never reach a successful result.
+        constexpr int H = E / 2;
         uint32_t L = 0, lb = 0;
 #pragma unroll
         for (int n = 0; n < E; ++n) {
             uint32_t d = S[n];
 #pragma unroll
-            for (int i = 1; i <= n; ++i)
+            for (int i = 1; i <= (n < H ? n : H); ++i)
                 d ^= s_exp2[llam[i] + lS[n - i]];
             bool nz = d != 0;
             bool grow = nz && (2 * L <= (uint32_t)n);
             uint32_t ld = s_log2[d];
             uint32_t lc = ld + 255 - lb;
             lc = lc >= 255 ? lc - 255 : lc;
-            const int hi = (n + 1 < E - 1) ? n + 1 : E - 1;
+            const int hi = n + 1 < H ? n + 1 : H;
             uint32_t old_llam[E];
 #pragma unroll
             for (int i = 0; i < E; ++i)
@@ -736,7 +755,7 @@ __global__ __launch_bounds__(WG) void rs256_bm_fixed_kernel(uint32_t count, uint
                 lam[i] ^= t;
                 llam[i] = s_log2[lam[i]];
             }
-            const int bh = (n + 2 < E - 1) ? n + 2 : E - 1;
+            const int bh = n + 2 < H + 1 ? n + 2 : (H + 1 < E - 1 ? H + 1 : E - 1);
 #pragma unroll
             for (int i = bh; i >= 1; --i)
                 lBp[i] = grow ? old_llam[i - 1] : lBp[i - 1];
@@ -963,15 +982,15 @@ struct PlanClass {
 __global__ void plan_reset_kernel(uint32_t *queue) { *queue = 0; }
 
 template <int EW, int S, bool REPAIR>
-__device__ inline void plan_item(__amdgpu_buffer_rsrc_t rsrc, uint8_t *msgs, uint8_t *eccs, uint8_t *status,
+__device__ inline void plan_item(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec, uint8_t *msgs, uint8_t *eccs, uint8_t *status,
                                  const PlanItem &it, const uint32_t *s_lut, const GfLds &gf, uint8_t *row)
 {
     if constexpr (REPAIR) {
-        status[it.idx] = repair_one<EW, S>(rsrc, (int64_t)it.msg_off, msgs + it.msg_off, eccs + it.ecc_off, it.k,
+        status[it.idx] = repair_one<EW, S>(rsrc, nrec, (int64_t)it.msg_off, msgs + it.msg_off, eccs + it.ecc_off, it.k,
                                            it.E, s_lut, gf, row);
     } else {
         uint32_t rem[EW];
-        message_remainder<EW, S>(rsrc, (int64_t)it.msg_off, it.k, s_lut, rem);
+        message_remainder<EW, S>(rsrc, nrec, (int64_t)it.msg_off, it.k, s_lut, rem);
         uint8_t *o = eccs + it.ecc_off;
         if ((it.E & 3) == 0 && (it.ecc_off & 3) == 0) {
             uint32_t *o4 = reinterpret_cast<uint32_t *>(o);
@@ -1008,7 +1027,8 @@ __global__ __launch_bounds__(WG) void rs256_plan_kernel(uint8_t *__restrict__ ms
     }
     GfLds gf{s_exp2, s_log2};
     uint8_t *row = s_rows + threadIdx.x * row_stride;
-    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, (msg_bytes + 3u) & ~3u);
+    const uint32_t nrec = (msg_bytes + 3u) & ~3u;
+    __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, nrec);
     uint32_t loaded = 0xffffffffu;
     for (;;) {
         if (threadIdx.x == 0)
@@ -1029,17 +1049,17 @@ __global__ __launch_bounds__(WG) void rs256_plan_kernel(uint8_t *__restrict__ ms
         if (threadIdx.x < ch.count) {
             const PlanItem it = items[ch.first + threadIdx.x];
             switch (cl.EW) {
-            case 1: plan_item<1, 4, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
-            case 2: plan_item<2, 4, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
-            case 4: plan_item<4, 4, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
-            case 8: plan_item<8, 4, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
-            case 16: plan_item<16, 2, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row); break;
+            case 1: plan_item<1, 4, REPAIR>(rsrc, nrec, msgs, eccs, status, it, s_lut, gf, row); break;
+            case 2: plan_item<2, 4, REPAIR>(rsrc, nrec, msgs, eccs, status, it, s_lut, gf, row); break;
+            case 4: plan_item<4, 4, REPAIR>(rsrc, nrec, msgs, eccs, status, it, s_lut, gf, row); break;
+            case 8: plan_item<8, 4, REPAIR>(rsrc, nrec, msgs, eccs, status, it, s_lut, gf, row); break;
+            case 16: plan_item<16, 2, REPAIR>(rsrc, nrec, msgs, eccs, status, it, s_lut, gf, row); break;
             default:
                 if constexpr (MAXEW >= 32) {
                     if (cl.EW == 32)
-                        plan_item<32, 1, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row);
+                        plan_item<32, 1, REPAIR>(rsrc, nrec, msgs, eccs, status, it, s_lut, gf, row);
                     else
-                        plan_item<64, 1, REPAIR>(rsrc, msgs, eccs, status, it, s_lut, gf, row);
+                        plan_item<64, 1, REPAIR>(rsrc, nrec, msgs, eccs, status, it, s_lut, gf, row);
                 }
                 break;
             }
@@ -1114,17 +1134,24 @@ std::vector<uint32_t> build_lut(const ffrs_rs256 &rs)
     return lut;
 }
 
+// Workgroup sizes of the LFSR kernels. More resident waves do not help them: with 512- and
+// 768-lane groups (twice and three times the waves per CU) the ecc-32 kernels ran 5% and
+// 33% slower on MI355X, the scattered per-lane row reads contending for the L1 cache.
+constexpr int lfsr_wg(int) { return 256; }
+constexpr int synd_wg(int) { return 256; }
+
 template <int EW, int S>
 int launch_encode(ffrs_rs256 *rs, const uint8_t *d_msg, size_t count, uint32_t k, uint8_t *d_ecc)
 {
+    constexpr int WG = lfsr_wg(EW);
     ffrs_ctx *ctx = rs->ctx;
     size_t lds = (size_t)S * 256 * lut_row(EW) * 4;
-    size_t tiles = (count + 255) / 256;
-    int wgs_per_cu = (int)std::min<size_t>(8, std::max<size_t>(1, (160 * 1024) / lds));
+    size_t tiles = (count + WG - 1) / WG;
+    int wgs_per_cu = (int)std::min<size_t>(2048 / WG, std::max<size_t>(1, (160 * 1024) / lds));
     int grid = persistent_grid(ctx, wgs_per_cu, tiles);
-    if (int r = allow_lds(rs256_encode_kernel<EW, S>, lds))
+    if (int r = allow_lds(rs256_encode_kernel<EW, S, WG>, lds))
         return r;
-    hipLaunchKernelGGL((rs256_encode_kernel<EW, S>), dim3(grid), dim3(256), lds, ctx->stream, d_msg,
+    hipLaunchKernelGGL((rs256_encode_kernel<EW, S, WG>), dim3(grid), dim3(WG), lds, ctx->stream, d_msg,
                        (uint32_t)(count * k), (uint32_t)count, k, d_ecc, rs->ecc, rs->d_lut);
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
@@ -1231,11 +1258,13 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     uint8_t *d_rec = (uint8_t *)rs->d_work + ((synd_bytes + 255) & ~(size_t)255);
     Rs256Tables tab{rs->d_lut, rs->d_exp2, rs->d_log2};
 
+    constexpr int WG1 = synd_wg(EW);
     size_t lds1 = (size_t)S * 256 * lut_row(EW) * 4 + (size_t)E * 32 * EW * 4;
-    if (int r = allow_lds(rs256_synd_kernel<EW, S>, lds1))
+    if (int r = allow_lds(rs256_synd_kernel<EW, S, WG1>, lds1))
         return r;
-    int g1 = persistent_grid(ctx, (int)std::max<size_t>(1, (160 * 1024) / lds1), (count + 255) / 256);
-    hipLaunchKernelGGL((rs256_synd_kernel<EW, S>), dim3(g1), dim3(256), lds1, ctx->stream, d_msg,
+    int g1 = persistent_grid(ctx, (int)std::min<size_t>(2048 / WG1, std::max<size_t>(1, (160 * 1024) / lds1)),
+                             (count + WG1 - 1) / WG1);
+    hipLaunchKernelGGL((rs256_synd_kernel<EW, S, WG1>), dim3(g1), dim3(WG1), lds1, ctx->stream, d_msg,
                        (uint32_t)(count * k), d_ecc, (uint32_t)count, k, E, d_status, d_synd, rs->d_lut, rs->d_stab);
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);

@@ -228,6 +228,22 @@ __device__ inline uint32_t poly_len(const uint32_t *r, uint32_t max)
     return max;
 }
 
+// The same length computed by the whole group at once: each lane scans its strided
+// elements, then a maximum over the group's lanes (xor shuffles by less than g stay inside
+// the group, whose lanes are an aligned block of g). Every lane of the group must call it.
+__device__ inline uint32_t group_len(const uint32_t *r, uint32_t max, Grp G)
+{
+    uint32_t hi = 0;
+    for (uint32_t j = G.gl; j < max; j += G.g)
+        if (r[j] != 0)
+            hi = j + 1;
+    for (uint32_t s = 1; s < G.g; s <<= 1) {
+        uint32_t o = __shfl_xor(hi, s);
+        hi = o > hi ? o : hi;
+    }
+    return hi;
+}
+
 // ---------------------------------------------------------------- encode
 
 // One codeword per lane group: pntt_message_residue + _mix_ecc_residue
@@ -537,7 +553,7 @@ __device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1, uint32_t *r
     }
     wave_sync();
     uint32_t q_len = 0, t_len = 0, a2_len = 1, a1_len;
-    uint32_t r1_len = poly_len(r1, E);
+    uint32_t r1_len = group_len(r1, E, G);
     uint32_t r2_len = r1_len;
     uint32_t r2l = r2[0];
     uint32_t lanes = r1_len != 0 ? 0xffffffffu : 0u;
@@ -559,7 +575,7 @@ __device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1, uint32_t *r
             tmp[i] = v;
         }
         wave_sync();
-        r1_len = poly_len(tmp, E - 1) & lanes;
+        r1_len = group_len(tmp, E - 1, G) & lanes;
         for (uint32_t i = G.gl; i < E; i += G.g)
             r1[i] = i < r1_len ? tmp[i] : 0u;
         wave_sync();
@@ -605,11 +621,11 @@ __device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1, uint32_t *r
             a1_len = t_len;
             r1_len = nq_len;
             inttr(T, r1, G);
-            r1_len = poly_len(r1, r1_len);
+            r1_len = group_len(r1, r1_len, G);
         }
     }
     inttr(T, a1, G);
-    a1_len = poly_len(a1, a1_len & lanes);
+    a1_len = group_len(a1, a1_len & lanes, G);
     uint32_t inv0 = g_inv(a1[0]);
     wave_sync();
     for (uint32_t i = G.gl; i < E; i += G.g) {
@@ -642,10 +658,7 @@ __device__ inline void repair_group(const Rsi16Dev &T, uint16_t *m, uint16_t *e,
     uint32_t *syn = base, *loc = base + E, *der = base + 2 * E, *chunk = base + 3 * E, *w = base + 4 * E;
 
     pntt_codeword(T, m, e, chunk, syn, G);
-    bool any = false;
-    for (uint32_t j = 0; j < E; ++j)
-        any |= syn[j] != 0;
-    if (!any) {
+    if (group_len(syn, E, G) == 0) {
         if (G.gl == 0)
             *st = FFRS_RSI16_NO_ERRORS;
         return;
@@ -661,7 +674,7 @@ __device__ inline void repair_group(const Rsi16Dev &T, uint16_t *m, uint16_t *e,
     for (uint32_t i = G.gl; i < E; i += G.g)
         der[i] = i + 1 < E ? S::mul(loc[i + 1], i + 1) : 0u;
     wave_sync();
-    uint32_t der_len = poly_len(der, E);
+    uint32_t der_len = group_len(der, E, G);
     if (L > E) {  // py_assert at rsi16v_impl.hpp:454
         if (G.gl == 0)
             *st = FFRS_RSI16_RAISES;
