@@ -400,6 +400,21 @@ def run_mixed(args):
         plan.repair(d_wm, d_wp, d_st)
         plan_i.repair(d_wmi, d_wpi, d_sti)
 
+    # the same two plans with the RSi16 one on a second context (stream), so that the two
+    # persistent launches run side by side; timed on the host around both streams
+    ctx2 = capi.Context(0)
+    plan_i2 = capi.RSi16Plan(ctx2, ns_, eccs_i, moff_i, eoff_i, graph=not args.no_graph)
+
+    def fused2(repair):
+        if repair:
+            plan.repair(d_wm, d_wp, d_st)
+            plan_i2.repair(d_wmi, d_wpi, d_sti)
+        else:
+            plan.encode(d_m, d_p)
+            plan_i2.encode(d_mi, d_pi)
+        ctx.sync()
+        ctx2.sync()
+
     def plain_encode():
         for a, b in zip(gstart[:-1], gstart[1:]):
             k, e = int(lens[a]), int(eccs[a])
@@ -468,6 +483,14 @@ def run_mixed(args):
         fused_repair()
     fe = timed(fused_encode, reps)
     fr = timed(fused_repair, reps, restore)
+    fused2(False)
+    restore()
+    fused2(True)
+    r_two = results()
+    if not all((x == y).all() for x, y in zip(r_two, r_fused)):
+        raise RuntimeError("two-stream fused results differ")
+    fe2 = timed(lambda: fused2(False), reps)
+    fr2 = timed(lambda: fused2(True), reps, restore)
     pe = timed(plain_encode, reps)
     pr = timed(plain_repair, reps, restore)
     pge = pgr = (float("nan"), float("nan"))
@@ -497,11 +520,15 @@ def run_mixed(args):
         "message_bytes": msg_bytes,
         "fused": {"encode_ms": round(fe[1], 4), "repair_ms": round(fr[1], 4),
                   "encode_gbps": gbps(fe[1]), "repair_gbps": gbps(fr[1])},
+        "fused_two_streams": {"encode_ms": round(fe2[0], 4), "repair_ms": round(fr2[0], 4),
+                              "encode_gbps": gbps(fe2[0]), "repair_gbps": gbps(fr2[0]),
+                              "timing": "host wall clock around both streams"},
         "plain_host_issued": {"encode_ms": round(pe[0], 4), "repair_ms": round(pr[0], 4),
                               "encode_gpu_ms": round(pe[1], 4), "repair_gpu_ms": round(pr[1], 4)},
         "plain_graph": ({"encode_ms": round(pge[1], 4), "repair_ms": round(pgr[1], 4)}
                         if not args.no_graph else None),
         "repair_speedup_vs_plain": round(pr[0] / fr[1], 2),
+        "repair_speedup_two_streams_vs_plain": round(pr[0] / fr2[0], 2),
         "repair_speedup_vs_plain_graph": round(pgr[1] / fr[1], 2) if not args.no_graph else None,
         "encode_speedup_vs_plain_graph": round(pge[1] / fe[1], 2) if not args.no_graph else None,
         "graphs": not args.no_graph,
@@ -513,8 +540,10 @@ def run_mixed(args):
         c.close()
     plan.close()
     plan_i.close()
+    plan_i2.close()
     for b in (d_m, d_p, d_mi, d_pi, d_cm, d_cp, d_cmi, d_cpi, d_wm, d_wp, d_wmi, d_wpi, d_st, d_sti):
         b.free()
+    ctx2.close()
     ctx.close()
     return res
 
