@@ -404,31 +404,47 @@ __device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec,
     //      (reed_solomon.hpp:516-538); positions go to the dead syndrome row
     uint8_t *pos = Sy;
     uint32_t roots = 0;
-    for (uint32_t p = 0; p < n; ++p) {
-        uint32_t pinv = p ? 255 - p : 0;  // alpha^-p = alpha^(255 - p)
-        uint32_t e = 0, acc = 0;
+    for (uint32_t p0 = 0; p0 < n; p0 += 4) {  // four positions per pass: one LDS read of each log
+        uint32_t e[4], step[4], acc[4] = {0, 0, 0, 0};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const uint32_t p = p0 + q;
+            step[q] = p ? 255 - p : 0;  // alpha^-p = alpha^(255 - p)
+            e[q] = 0;
+        }
         for (uint32_t i = 0; i <= L; ++i) {
-            acc ^= gf.exp2[llam[i] + e];
-            e += pinv;
-            e = e >= 255 ? e - 255 : e;
+            const uint32_t li = llam[i];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                acc[q] ^= gf.exp2[li + e[q]];
+                e[q] += step[q];
+                e[q] = e[q] >= 255 ? e[q] - 255 : e[q];
+            }
         }
-        if (acc == 0) {
-            if (roots < E)
-                pos[roots] = (uint8_t)p;
-            roots++;
-        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            if (acc[q] == 0 && p0 + q < n) {
+                if (roots < E)
+                    pos[roots] = (uint8_t)(p0 + q);
+                roots++;
+            }
     }
     if (roots != L)
         return 0;
 
     // ---- Forney (reed_solomon.hpp:730-743): e_p = X Omega(X^-1) / Lambda'(X^-1),
     //      X = alpha^p, Lambda' = odd terms of Lambda (characteristic 2)
+    // logs of Omega in the dead locator bytes (255 stands for a zero coefficient)
+    uint8_t *lom = lam;
+    for (uint32_t kk = 0; kk < L; ++kk)
+        lom[kk] = om[kk] ? (uint8_t)gf.log2[om[kk]] : (uint8_t)255;
     for (uint32_t r = 0; r < L; ++r) {
         uint32_t p = pos[r];
         uint32_t pinv = p ? 255 - p : 0;
         uint32_t num = 0, den = 0, e = 0;
         for (uint32_t kk = 0; kk < L; ++kk) {  // Omega_k = 0 for k >= L
-            num ^= gf.exp2[gf.log2[om[kk]] + e];
+            const uint32_t lo = lom[kk];
+            num ^= gf.exp2[(lo == 255 ? (uint32_t)ZLOG : lo) + e];
             if ((kk & 1) == 0)
                 den ^= gf.exp2[llam[kk + 1] + e];
             e += pinv;
