@@ -138,6 +138,7 @@ struct Rsi16Dev {
     const uint32_t *roots_ntt;                   // N
     const uint32_t *mix;                         // E
     const uint32_t *rbo_ecc;                     // E
+    const uint32_t *inv_tab;                     // 65537: inv_tab[a] = g_inv(a)
 };
 
 __device__ inline void wave_sync()
@@ -724,6 +725,518 @@ __device__ inline void repair_group(const Rsi16Dev &T, uint16_t *m, uint16_t *e,
         *st = total > 0 ? FFRS_RSI16_REPAIR_OK : (lanes ? FFRS_RSI16_ERROR_LOCATION_FAIL : FFRS_RSI16_NO_ERRORS);
 }
 
+// ---- repair with two coefficients per lane in registers (ecc_len <= 128)
+//
+// The same operations as repair_group in the same order, for lane groups of g = E / 2:
+// lane gl of a group holds coefficients gl and gl + g of every polynomial. Butterfly
+// partners at distance s < g sit in lane gl ^ s and are exchanged with one cross-lane move
+// (DPP for s <= 8, ds_swizzle for 16, ds_bpermute for 32); the last (or first) stage pairs
+// the lane's own two coefficients. Each lane's twiddles are loaded once per codeword.
+// Permutations (reversal, shifts) gather with ds_bpermute. Lanes of a group take the same
+// branches (every decision depends on group-wide values only), and no move reads a lane
+// outside the group (an aligned block of g lanes).
+
+// Field operations for operands in [0, 65536] (every value the register repair handles:
+// u16 symbols, table entries and results of these operations). The results equal
+// ScalarOps': a product of two such operands is below 2^33, so its 16-bit halves come from
+// the full-rate 24-bit multiplies, and min(t, t + p) replaces the sign test (for t < 2^31
+// the sum is larger; for a wrapped t the sum is the reduced value).
+struct F {
+    __device__ static inline uint32_t add(uint32_t a, uint32_t b)
+    {
+        const uint32_t r = a + b, t = r - P16;
+        return t < r ? t : r;
+    }
+    __device__ static inline uint32_t sub(uint32_t a, uint32_t b)
+    {
+        const uint32_t r = a - b, t = r + P16;
+        return t < r ? t : r;
+    }
+    __device__ static inline uint32_t mul(uint32_t a, uint32_t b)
+    {
+        __builtin_assume(a <= 0x10000u);
+        __builtin_assume(b <= 0x10000u);
+        const uint64_t x = (uint64_t)a * (uint64_t)b;
+        const uint32_t t = ((uint32_t)x & 0xffffu) - (uint32_t)(x >> 16), u = t + P16;
+        return t < u ? t : u;
+    }
+};
+
+struct P2 {
+    uint32_t a, b;  // coefficients gl and gl + g
+};
+
+// v of lane (this lane ^ X)
+template <uint32_t X>
+__device__ inline uint32_t xchg(uint32_t v, uint32_t lane)
+{
+    if constexpr (X == 1)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, true);  // quad_perm 1,0,3,2
+    else if constexpr (X == 2)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, true);  // quad_perm 2,3,0,1
+    else if constexpr (X == 4) {
+        // row_half_mirror reads lane ^ 7, then quad_perm 3,2,1,0 reads lane ^ 3
+        const int h = __builtin_amdgcn_mov_dpp((int)v, 0x141, 0xF, 0xF, true);
+        return (uint32_t)__builtin_amdgcn_mov_dpp(h, 0x1B, 0xF, 0xF, true);
+    } else if constexpr (X == 8)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x128, 0xF, 0xF, true);  // row_ror 8
+    else if constexpr (X == 16)
+        return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);  // bit mode, xor 16
+    else
+        return (uint32_t)__builtin_amdgcn_ds_bpermute((int)((lane ^ X) << 2), (int)v);
+}
+
+__device__ inline uint32_t lane_read(uint32_t v, uint32_t lane)
+{
+    return (uint32_t)__builtin_amdgcn_ds_bpermute((int)(lane << 2), (int)v);
+}
+
+// xchg with s a constant after unrolling
+__device__ __attribute__((always_inline)) inline uint32_t xchg_s(uint32_t v, uint32_t s, uint32_t lane)
+{
+    switch (s) {
+    case 1:
+        return xchg<1>(v, lane);
+    case 2:
+        return xchg<2>(v, lane);
+    case 4:
+        return xchg<4>(v, lane);
+    case 8:
+        return xchg<8>(v, lane);
+    case 16:
+        return xchg<16>(v, lane);
+    default:
+        return xchg<32>(v, lane);
+    }
+}
+
+template <int E>
+struct Lane2 {
+    static constexpr uint32_t G = E / 2;  // lanes per codeword
+    static constexpr int ST = __builtin_ctz(E);  // butterfly stages
+    uint32_t gl, base, lane;  // lane in the group, first lane of the group, lane in the wave
+    // per-stage twiddles (1 where the lane does not multiply): CT and GS over roots_ecc
+    // (e) and roots_i_ecc (i); stage t pairs at distance 2^t
+    uint32_t ct_e[ST], ct_i[ST], gs_e[ST], gs_i[ST];
+    uint32_t rv_k[2], rv_j[2], rv_w[2];  // reverse_ntt: source index, j and roots_i_ecc[j]
+
+    __device__ void init(const Rsi16Dev &T, uint32_t lane_in_wave)
+    {
+        lane = lane_in_wave;
+        gl = lane % G;
+        base = lane - gl;
+#pragma unroll
+        for (int t = 0; t < ST; ++t) {
+            const uint32_t s = 1u << t;
+            if (s < G) {
+                const uint32_t j = gl & (s - 1), ef = E / (2 * s);
+                const bool upper = (gl & s) != 0;
+                ct_e[t] = upper ? T.roots_ecc[ef * j] : 1u;
+                ct_i[t] = upper ? T.roots_i_ecc[ef * j] : 1u;
+                // GS: stage s uses roots[j << ef'] with ef' = log2(G / s)
+                const uint32_t sh = (uint32_t)(ST - 1 - t);
+                gs_e[t] = upper ? T.roots_ecc[j << sh] : 1u;
+                gs_i[t] = upper ? T.roots_i_ecc[j << sh] : 1u;
+            } else {  // s = G: the lane's own pair, j = gl
+                ct_e[t] = gs_e[t] = T.roots_ecc[gl];
+                ct_i[t] = gs_i[t] = T.roots_i_ecc[gl];
+            }
+        }
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t i = gl + (h ? G : 0);
+            const uint32_t j = T.rbo_ecc[i];
+            rv_j[h] = j;
+            rv_k[h] = T.rbo_ecc[(E - j) & (E - 1)];
+            rv_w[h] = T.roots_i_ecc[j];
+        }
+    }
+};
+
+// value of coefficient idx (the same for the whole group)
+template <int E>
+__device__ inline uint32_t p2_get(const P2 &p, uint32_t idx, const Lane2<E> &L)
+{
+    const uint32_t src = L.base + (idx % Lane2<E>::G);
+    const uint32_t va = lane_read(p.a, src), vb = lane_read(p.b, src);
+    return idx < Lane2<E>::G ? va : vb;
+}
+
+// coefficient k of p, k chosen per lane (k < E)
+template <int E>
+__device__ inline uint32_t p2_gather(const P2 &p, uint32_t k, const Lane2<E> &L)
+{
+    return p2_get<E>(p, k, L);
+}
+
+// degree + 1 of p restricted to [0, max) (group_len)
+template <int E>
+__device__ inline uint32_t p2_len(const P2 &p, uint32_t max, const Lane2<E> &L)
+{
+    constexpr uint32_t G = Lane2<E>::G;
+    uint32_t hi = 0;
+    if (L.gl < max && p.a != 0)
+        hi = L.gl + 1;
+    if (L.gl + G < max && p.b != 0)
+        hi = L.gl + G + 1;
+#pragma unroll
+    for (int t = 0; t < Lane2<E>::ST - 1; ++t) {  // 2^t < G
+        const uint32_t o = xchg_s(hi, 1u << t, L.lane);
+        hi = o > hi ? o : hi;
+    }
+    return hi;
+}
+
+// ct_full (RBO in, natural out, full reduction), ntt.hpp:225-249; w = ct_e or ct_i
+template <int E>
+__device__ inline void p2_ct(P2 &x, const uint32_t (&w)[Lane2<E>::ST], const Lane2<E> &L)
+{
+    constexpr uint32_t G = Lane2<E>::G;
+    constexpr int ST = Lane2<E>::ST;
+#pragma unroll
+    for (int t = 0; t < ST - 1; ++t) {  // s = 2^t < G
+        // the upper lane sends c * w, the lower lane sends a (its w is 1)
+        const bool upper = (L.gl & (1u << t)) != 0;
+        const uint32_t sa = F::mul(x.a, w[t]), sb = F::mul(x.b, w[t]);
+        const uint32_t ra = xchg_s(sa, 1u << t, L.lane), rb = xchg_s(sb, 1u << t, L.lane);
+        x.a = upper ? F::sub(ra, sa) : F::add(sa, ra);
+        x.b = upper ? F::sub(rb, sb) : F::add(sb, rb);
+    }
+    // s = G: coefficients gl and gl + G of this lane, twiddle roots[gl]
+    const uint32_t c = F::mul(x.b, w[ST - 1]);
+    const uint32_t a = x.a;
+    x.a = F::add(a, c);
+    x.b = F::sub(a, c);
+    (void)G;
+}
+
+// gs_full (natural in, RBO out) with full reduction, ntt.hpp:280-302; w = gs_e or gs_i
+template <int E>
+__device__ inline void p2_gs(P2 &x, const uint32_t (&w)[Lane2<E>::ST], const Lane2<E> &L)
+{
+    constexpr int ST = Lane2<E>::ST;
+    {  // s = G
+        const uint32_t a = x.a, c = x.b;
+        x.a = F::add(a, c);
+        x.b = F::mul(F::sub(a, c), w[ST - 1]);
+    }
+#pragma unroll
+    for (int t = ST - 2; t >= 0; --t) {  // s = G / 2 .. 1
+        const bool upper = (L.gl & (1u << t)) != 0;
+        const uint32_t ra = xchg_s(x.a, 1u << t, L.lane), rb = xchg_s(x.b, 1u << t, L.lane);
+        const uint32_t da = upper ? F::sub(ra, x.a) : F::add(x.a, ra);
+        const uint32_t db = upper ? F::sub(rb, x.b) : F::add(x.b, rb);
+        x.a = F::mul(da, w[t]);
+        x.b = F::mul(db, w[t]);
+    }
+}
+
+template <int E>
+__device__ inline void p2_nttr(P2 &x, const Lane2<E> &L)
+{
+    p2_gs<E>(x, L.gs_e, L);
+}
+
+template <int E>
+__device__ inline void p2_inttr(const Rsi16Dev &T, P2 &x, const Lane2<E> &L)
+{
+    p2_ct<E>(x, L.ct_i, L);
+    x.a = F::mul(x.a, T.ecc_inv);
+    x.b = F::mul(x.b, T.ecc_inv);
+}
+
+// NTT::_reverse_ntt_vec (ntt.hpp:344-354) in gather form, as reverse_ntt
+template <int E>
+__device__ inline void p2_reverse_ntt(const Rsi16Dev &T, P2 &v, uint32_t shift, const Lane2<E> &L)
+{
+    uint32_t out[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        uint32_t x = p2_gather<E>(v, L.rv_k[h], L);
+        x = F::mul(x, L.rv_w[h]);
+        x = F::mul(x, T.roots_ecc[(shift * L.rv_j[h]) & (E - 1)]);
+        out[h] = (h == 0 && L.gl == 0) ? v.a : x;
+    }
+    v.a = out[0];
+    v.b = out[1];
+}
+
+// NTT::poly_inv_mod_xn_rev (ntt.hpp:363-469), as poly_inv_mod_xn_rev
+template <int E>
+__device__ inline bool p2_poly_inv(const Rsi16Dev &T, P2 &g, uint32_t g_len, uint32_t g0, uint32_t a_len, P2 &a,
+                                   const Lane2<E> &L)
+{
+    constexpr uint32_t G = Lane2<E>::G;
+    const uint32_t a0 = T.inv_tab[g0];
+    a.a = a.b = a0;
+    uint32_t l = 2;
+    const uint32_t a_max = a_len;
+    while (l < a_len) {
+        if (g_len + l - 2 > (uint32_t)E)
+            return false;
+        a.a = F::mul(a.a, F::sub(2, F::mul(a.a, g.a)));
+        a.b = F::mul(a.b, F::sub(2, F::mul(a.b, g.b)));
+        p2_inttr<E>(T, a, L);
+        if (L.gl >= l)
+            a.a = 0;
+        if (L.gl + G >= l)
+            a.b = 0;
+        p2_nttr<E>(a, L);
+        l *= 2;
+    }
+    g.a = F::mul(a.a, F::sub(2, F::mul(a.a, g.a)));
+    g.b = F::mul(a.b, F::sub(2, F::mul(a.b, g.b)));
+    p2_inttr<E>(T, g, L);
+    uint32_t out[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t i = L.gl + (h ? G : 0);
+        const uint32_t src = i < a_max ? a_max - 1 - i : i;
+        const uint32_t v = p2_gather<E>(g, src & (E - 1), L);
+        out[h] = (i < a_len && src < l) ? v : 0u;
+    }
+    a.a = out[0];
+    a.b = out[1];
+    p2_nttr<E>(a, L);
+    return true;
+}
+
+// NTT::poly_div_ntt (ntt.hpp:471-564), as poly_div_ntt
+template <int E>
+__device__ inline bool p2_poly_div(const Rsi16Dev &T, const P2 &f, uint32_t f_len, const P2 &g, uint32_t g_len,
+                                   uint32_t f0, uint32_t g0, P2 &t, P2 &q, uint32_t *q_len_out, const Lane2<E> &L)
+{
+    constexpr uint32_t G = Lane2<E>::G;
+    const uint32_t q_len = f_len - g_len + 1;
+    if (g_len == 0)
+        return false;
+    if (f_len < g_len) {
+        q.a = q.b = 0;
+        *q_len_out = 0;
+        return true;
+    }
+    t = g;
+    p2_reverse_ntt<E>(T, t, g_len, L);
+    if (!p2_poly_inv<E>(T, t, g_len, g0, q_len, q, L))
+        return false;
+    t.a = F::mul(F::sub(f.a, f0), q.a);
+    t.b = F::mul(F::sub(f.b, f0), q.b);
+    p2_inttr<E>(T, t, L);
+    const uint32_t right_n = q_len < E - g_len ? q_len : E - g_len;
+    uint32_t out[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const uint32_t i = L.gl + (h ? G : 0);
+        const uint32_t src = i < right_n ? g_len + i : (i < q_len ? i - right_n : 0u);
+        const uint32_t v = p2_gather<E>(t, src & (E - 1), L);
+        out[h] = i < q_len ? v : 0u;
+    }
+    q.a = out[0];
+    q.b = out[1];
+    p2_nttr<E>(q, L);
+    *q_len_out = q_len;
+    return true;
+}
+
+// RSi16v<1>::_sugiyama (rsi16v_impl.hpp:271-440), as sugiyama: r1 holds the syndromes on
+// entry and the normalised evaluator on exit. Returns the locator length, or 0xFFFFFFFF
+// where the reference raises.
+template <int E>
+__device__ inline uint32_t p2_sugiyama(const Rsi16Dev &T, P2 &a1, P2 &r1, const Lane2<E> &L)
+{
+    constexpr uint32_t G = Lane2<E>::G, mask = E - 1;
+    P2 q{0, 0}, t{0, 0}, a2{1, 1}, r2 = r1;
+    a1.a = a1.b = 0;
+    uint32_t q_len = 0, t_len = 0, a2_len = 1, a1_len;
+    uint32_t r1_len = p2_len<E>(r1, E, L);
+    uint32_t r2_len = r1_len;
+    uint32_t r2l = p2_get<E>(r2, 0, L);
+    const uint32_t lanes = r1_len != 0 ? 0xffffffffu : 0u;
+    {
+        const uint32_t r1h = p2_get<E>(r1, (r1_len - 1) & mask, L);
+        const uint32_t r1h2 = p2_get<E>(r1, (r1_len - 2) & mask, L);
+        const uint32_t a11 = s_neg(T.inv_tab[r1h]);
+        const uint32_t a10 = F::mul(r1h2, F::mul(a11, a11));
+        if constexpr (G > 1) {  // coefficients 0 and 1 live in lanes 0 and 1
+            if (L.gl == 0)
+                a1.a = a10;
+            else if (L.gl == 1)
+                a1.a = a11;
+        } else {  // G == 1: they are the lane's two coefficients
+            a1.a = a10;
+            a1.b = a11;
+        }
+        a1_len = 2 & lanes;
+        // tmp[i] = a10 r2[i] + a11 r2[i - 1]  (r2 still holds the original r1)
+        const uint32_t prev_a = p2_gather<E>(r2, (L.gl + E - 1) & mask, L);
+        const uint32_t prev_b = p2_gather<E>(r2, (L.gl + G - 1) & mask, L);
+        P2 tmp;
+        tmp.a = F::mul(a10, r2.a);
+        if (L.gl >= 1)
+            tmp.a = F::add(tmp.a, F::mul(a11, prev_a));
+        tmp.b = F::add(F::mul(a10, r2.b), F::mul(a11, prev_b));
+        r1_len = p2_len<E>(tmp, E - 1, L) & lanes;
+        r1.a = L.gl < r1_len ? tmp.a : 0u;
+        r1.b = L.gl + G < r1_len ? tmp.b : 0u;
+        p2_nttr<E>(a1, L);
+        p2_nttr<E>(r2, L);
+    }
+    if (r1_len >= E / 2) {
+        for (uint32_t it = 1; it < E / 2; ++it) {
+            const uint32_t r1h = p2_get<E>(r1, (r1_len - 1) & mask, L);
+            const uint32_t r1l = p2_get<E>(r1, 0, L);
+            p2_nttr<E>(r1, L);
+            if (!p2_poly_div<E>(T, r2, r2_len, r1, r1_len, r2l, r1h, t, q, &q_len, L))
+                return 0xffffffffu;
+            {
+                const uint32_t qa = q.a, qb = q.b;
+                t.a = F::sub(a2.a, F::mul(qa, a1.a));
+                t.b = F::sub(a2.b, F::mul(qb, a1.b));
+                q.a = F::sub(r2.a, F::mul(qa, r1.a));
+                q.b = F::sub(r2.b, F::mul(qb, r1.b));
+            }
+            t_len = q_len + a1_len - 1;
+            t_len = a2_len > t_len ? a2_len : t_len;
+            uint32_t nq_len = q_len + r1_len - 1;
+            nq_len = r2_len > nq_len ? r2_len : nq_len;
+            a2 = a1;
+            r2 = r1;
+            a2_len = a1_len;
+            r2_len = r1_len;
+            r2l = r1l;
+            if (!(r1_len >= E / 2)) {
+                p2_inttr<E>(T, r1, L);
+                break;
+            }
+            a1 = t;
+            r1 = q;
+            a1_len = t_len;
+            r1_len = nq_len;
+            p2_inttr<E>(T, r1, L);
+            r1_len = p2_len<E>(r1, r1_len, L);
+        }
+    }
+    p2_inttr<E>(T, a1, L);
+    a1_len = p2_len<E>(a1, a1_len & lanes, L);
+    const uint32_t inv0 = T.inv_tab[p2_get<E>(a1, 0, L)];
+    a1.a = F::mul(a1.a, inv0);
+    a1.b = F::mul(a1.b, inv0);
+    r1.a = F::mul(r1.a, inv0);
+    r1.b = F::mul(r1.b, inv0);
+    return a1_len & lanes;
+}
+
+// repair_group with the polynomials in registers; base holds REG_WORDS * E words (evaluator
+// and derivative copies for Forney, the root list and its count).
+constexpr uint32_t REG_WORDS = 4;
+
+template <int E>
+__device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t *e, uint32_t *base, const Lane2<E> &L,
+                                        uint8_t *st)
+{
+    constexpr uint32_t G = Lane2<E>::G;
+    const uint32_t i0 = L.gl, i1 = L.gl + G;
+    // NTT::pntt (ntt.hpp:133-152) of m || e, one chunk of E symbols at a time
+    P2 syn{0, 0};
+    const uint32_t nb = T.n / E;
+    for (uint32_t c = 0; c < nb; ++c) {
+        const uint32_t x0 = c * E + i0, x1 = c * E + i1;
+        P2 x{x0 < T.k ? (uint32_t)m[x0] : (uint32_t)e[x0 - T.k], x1 < T.k ? (uint32_t)m[x1] : (uint32_t)e[x1 - T.k]};
+        p2_ct<E>(x, L.ct_e, L);
+        const uint32_t va = F::mul(x.a, T.pntt_shift[x0]), vb = F::mul(x.b, T.pntt_shift[x1]);
+        syn.a = c ? F::add(syn.a, va) : va;
+        syn.b = c ? F::add(syn.b, vb) : vb;
+    }
+    if (p2_len<E>(syn, E, L) == 0) {
+        if (L.gl == 0)
+            *st = FFRS_RSI16_NO_ERRORS;
+        return;
+    }
+    P2 loc{0, 0};
+    const uint32_t Ln = p2_sugiyama<E>(T, loc, syn, L);
+    if (Ln == 0xffffffffu) {
+        if (L.gl == 0)
+            *st = FFRS_RSI16_RAISES;
+        return;
+    }
+    const uint32_t eval_len = E - Ln + 1;
+    // _deriv (rsi16v_impl.hpp:608-616): der[i] = (i + 1) loc[i + 1]
+    P2 der;
+    {
+        const uint32_t n0 = p2_gather<E>(loc, (i0 + 1) & (E - 1), L), n1 = p2_gather<E>(loc, (i1 + 1) & (E - 1), L);
+        der.a = i0 + 1 < (uint32_t)E ? F::mul(n0, i0 + 1) : 0u;
+        der.b = i1 + 1 < (uint32_t)E ? F::mul(n1, i1 + 1) : 0u;
+    }
+    const uint32_t der_len = p2_len<E>(der, E, L);
+    if (Ln > (uint32_t)E) {  // py_assert at rsi16v_impl.hpp:454
+        if (L.gl == 0)
+            *st = FFRS_RSI16_RAISES;
+        return;
+    }
+    const uint32_t lanes = Ln != 0 ? 0xffffffffu : 0u;
+    const uint32_t dmax = der_len & lanes, emax = eval_len & lanes;
+    uint32_t *ev = base, *dv = base + E, *list = base + 2 * E, *nroots = base + 3 * E;
+    ev[i0] = syn.a;
+    ev[i1] = syn.b;
+    dv[i0] = der.a;
+    dv[i1] = der.b;
+    if (L.gl == 0)
+        *nroots = 0;
+    wave_sync();
+    // NTT::pintt_ecc (ntt.hpp:206-220) chunk by chunk; zeros listed, corrected afterwards
+    for (uint32_t c = 0; c < nb; ++c) {
+        P2 ch{F::mul(loc.a, T.pintt_shift[c * E + i0]), F::mul(loc.b, T.pintt_shift[c * E + i1])};
+        p2_gs<E>(ch, L.gs_i, L);
+        if (lanes) {
+            if (ch.a == 0) {
+                const uint32_t slot = atomicAdd(nroots, 1u);
+                if (slot < (uint32_t)E)
+                    list[slot] = c * E + i0;
+            }
+            if (ch.b == 0) {
+                const uint32_t slot = atomicAdd(nroots, 1u);
+                if (slot < (uint32_t)E)
+                    list[slot] = c * E + i1;
+            }
+        }
+    }
+    wave_sync();
+    const uint32_t total = *nroots;
+    const uint32_t listed = total < (uint32_t)E ? total : (uint32_t)E;
+    for (uint32_t r = L.gl; r < listed; r += G) {
+        const uint32_t pos = list[r];
+        const uint32_t ri = rbo_bits(pos, T.nbits);
+        const uint32_t x = T.roots_ntt[ri];
+        const uint32_t xi = T.roots_ntt[(T.N - ri) & (T.N - 1)];
+        const uint32_t num = eval_poly(ev, emax, xi);
+        const uint32_t den = eval_poly(dv, dmax, xi);
+        // _forney (rsi16v_impl.hpp:500-512) with the table conventions of g_div
+        const uint32_t q = num == 0 ? 0u : (den == 0 ? num : F::mul(num, T.inv_tab[den]));
+        const uint32_t err = F::mul(q, x);
+        uint16_t *at = pos < T.k ? &m[pos] : &e[pos - T.k];
+        *at = (uint16_t)(F::add((uint32_t)*at, err) & 0xffff);
+    }
+    if (L.gl == 0)
+        *st = total > 0 ? FFRS_RSI16_REPAIR_OK : (lanes ? FFRS_RSI16_ERROR_LOCATION_FAIL : FFRS_RSI16_NO_ERRORS);
+}
+
+template <int E>
+__global__ __launch_bounds__(64) void rsi16_repair_reg_kernel(uint16_t *__restrict__ msgs,
+                                                               uint16_t *__restrict__ eccs, uint32_t count,
+                                                               uint8_t *__restrict__ status, Rsi16Dev T)
+{
+    extern __shared__ uint32_t s_mem[];
+    constexpr uint32_t G = Lane2<E>::G, cpw = 64 / G;
+    const uint32_t grp = threadIdx.x / G;
+    const uint32_t cw = blockIdx.x * cpw + grp;
+    if (cw >= count)
+        return;
+    Lane2<E> L;
+    L.init(T, threadIdx.x);
+    repair_group_reg<E>(T, msgs + (size_t)cw * T.k, eccs + (size_t)cw * E, s_mem + grp * REG_WORDS * E, L,
+                        status + cw);
+}
+
 __global__ __launch_bounds__(64) void rsi16_repair_kernel(uint16_t *__restrict__ msgs, uint16_t *__restrict__ eccs,
                                                            uint32_t count, uint8_t *__restrict__ status, Rsi16Dev T,
                                                            uint32_t g)
@@ -1027,11 +1540,43 @@ int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t 
     return FFRS_OK;
 }
 
+template <int E>
+int launch_repair_reg(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count, uint8_t *d_status)
+{
+    constexpr uint32_t cpw = 128 / E;
+    const size_t lds = (size_t)cpw * REG_WORDS * E * 4;
+    if (int r = allow_lds(rsi16_repair_reg_kernel<E>, lds))
+        return r;
+    hipLaunchKernelGGL(rsi16_repair_reg_kernel<E>, dim3((unsigned)((count + cpw - 1) / cpw)), dim3(64), lds,
+                       rs->ctx->stream, d_msg, d_ecc, (uint32_t)count, d_status, rs->dev);
+    FFRS_HIP(hipGetLastError());
+    mark_kernel(rs->ctx);
+    return FFRS_OK;
+}
+
 int launch_repair(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count, uint8_t *d_status)
 {
     marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
+    if (rs->E >= 2 && rs->E <= 128 && rs->g_rep == rs->E / 2) {  // registers, two coefficients per lane
+        switch (rs->E) {
+        case 2:
+            return launch_repair_reg<2>(rs, d_msg, d_ecc, count, d_status);
+        case 4:
+            return launch_repair_reg<4>(rs, d_msg, d_ecc, count, d_status);
+        case 8:
+            return launch_repair_reg<8>(rs, d_msg, d_ecc, count, d_status);
+        case 16:
+            return launch_repair_reg<16>(rs, d_msg, d_ecc, count, d_status);
+        case 32:
+            return launch_repair_reg<32>(rs, d_msg, d_ecc, count, d_status);
+        case 64:
+            return launch_repair_reg<64>(rs, d_msg, d_ecc, count, d_status);
+        default:
+            return launch_repair_reg<128>(rs, d_msg, d_ecc, count, d_status);
+        }
+    }
     uint32_t g = rs->g_rep, cpw = 64 / g;
     size_t lds = (size_t)cpw * REPAIR_WORDS * rs->E * 4;
     dim3 grid((unsigned)((count + cpw - 1) / cpw));
@@ -1118,7 +1663,20 @@ int ffrs_rsi16_create(ffrs_ctx *ctx, size_t block_len, size_t ecc_len, uint32_t 
     rs->nbits = nbits;
     rs->primitive = primitive;
     rs->root = root;
-    size_t words = 6 * (size_t)E + 2 * (size_t)E * 2 + n + 2 * (size_t)N;
+    // field inverses with the table convention inv(0) = 1, from powers of the primitive root 3
+    std::vector<uint32_t> inv_tab(P16);
+    {
+        std::vector<uint32_t> pw(P16 - 1);
+        uint32_t x = 1;
+        for (uint32_t i = 0; i < P16 - 1; ++i) {
+            pw[i] = x;
+            x = S::mul(x, 3);
+        }
+        inv_tab[0] = 1;
+        for (uint32_t i = 0; i < P16 - 1; ++i)
+            inv_tab[pw[i]] = pw[(P16 - 1 - i) & 0xffff];
+    }
+    size_t words = 6 * (size_t)E + 2 * (size_t)E * 2 + n + 2 * (size_t)N + P16;
     if (int r = ctx_set_device(ctx)) {
         delete rs;
         return r;
@@ -1152,6 +1710,7 @@ int ffrs_rsi16_create(ffrs_ctx *ctx, size_t block_len, size_t ecc_len, uint32_t 
     d.roots_ntt = put(roots_ntt);
     d.mix = put(mix);
     d.rbo_ecc = put(rbo_ecc);
+    d.inv_tab = put(inv_tab);
     herr = hipMemcpy(rs->d_tables, host.data(), host.size() * 4, hipMemcpyHostToDevice);
     if (herr != hipSuccess) {
         ffrs_rsi16_destroy(rs);
