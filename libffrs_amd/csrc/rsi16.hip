@@ -1038,16 +1038,52 @@ __device__ inline bool p2_poly_div(const Rsi16Dev &T, const P2 &f, uint32_t f_le
     return true;
 }
 
-// RSi16v<1>::_sugiyama (rsi16v_impl.hpp:271-440), as sugiyama: r1 holds the syndromes on
-// entry and the normalised evaluator on exit. Returns the locator length, or 0xFFFFFFFF
-// where the reference raises.
+// poly_div_ntt's outcome (true unless it raises) from the lengths alone: g_len == 0 raises,
+// f_len < g_len returns, and poly_inv_mod_xn_rev raises where its Newton loop meets
+// g_len + l - 2 > E (ntt.hpp:363-564).
+__device__ inline bool div_ok(uint32_t E, uint32_t f_len, uint32_t g_len)
+{
+    if (g_len == 0)
+        return false;
+    if (f_len < g_len)
+        return true;
+    const uint32_t q_len = f_len - g_len + 1;
+    for (uint32_t l = 2; l < q_len; l *= 2)
+        if (g_len + l - 2 > E)
+            return false;
+    return true;
+}
+
+// p[(i - 1) mod E] for both of the lane's coefficients
+template <int E>
+__device__ inline P2 p2_prev(const P2 &p, const Lane2<E> &L)
+{
+    constexpr uint32_t G = Lane2<E>::G;
+    const uint32_t src = L.base + (L.gl + G - 1) % G;
+    const uint32_t va = lane_read(p.a, src), vb = lane_read(p.b, src);
+    return P2{L.gl ? va : vb, L.gl ? vb : va};
+}
+
+// RSi16v<1>::_sugiyama (rsi16v_impl.hpp:271-440): r1 holds the syndromes on entry and the
+// normalised evaluator on exit. Returns the locator length, or 0xFFFFFFFF where the
+// reference raises.
+//
+// The reference keeps a1, a2 and r2 as E-point transforms and multiplies pointwise, which
+// is multiplication modulo x^E - 1 of the coefficient vectors. Here every polynomial stays
+// in coefficient form and the same products are formed as cyclic convolutions, so every
+// value is the same field element. In the usual step the quotient has two terms
+// (q_len == 2): poly_inv_mod_xn_rev then reduces to a = (-a0^2 g[g_len - 2],
+// 2 a0 - a0^2 g0) with a0 = 1 / g0 (its Newton loop does not run), two coefficients of
+// (f - f0) * a give the quotient, and the updates are two-tap convolutions. Any other step
+// goes through the transforms exactly as the reference does. The last division, whose
+// result the reference discards, only decides whether it raises (div_ok).
 template <int E>
 __device__ inline uint32_t p2_sugiyama(const Rsi16Dev &T, P2 &a1, P2 &r1, const Lane2<E> &L)
 {
     constexpr uint32_t G = Lane2<E>::G, mask = E - 1;
-    P2 q{0, 0}, t{0, 0}, a2{1, 1}, r2 = r1;
+    P2 a2{L.gl == 0 ? 1u : 0u, 0u}, r2 = r1;  // a2: the transform's all-ones vector
     a1.a = a1.b = 0;
-    uint32_t q_len = 0, t_len = 0, a2_len = 1, a1_len;
+    uint32_t q_len = 0, a2_len = 1, a1_len;
     uint32_t r1_len = p2_len<E>(r1, E, L);
     uint32_t r2_len = r1_len;
     uint32_t r2l = p2_get<E>(r2, 0, L);
@@ -1068,34 +1104,67 @@ __device__ inline uint32_t p2_sugiyama(const Rsi16Dev &T, P2 &a1, P2 &r1, const 
         }
         a1_len = 2 & lanes;
         // tmp[i] = a10 r2[i] + a11 r2[i - 1]  (r2 still holds the original r1)
-        const uint32_t prev_a = p2_gather<E>(r2, (L.gl + E - 1) & mask, L);
-        const uint32_t prev_b = p2_gather<E>(r2, (L.gl + G - 1) & mask, L);
+        const P2 prev = p2_prev<E>(r2, L);
         P2 tmp;
         tmp.a = F::mul(a10, r2.a);
         if (L.gl >= 1)
-            tmp.a = F::add(tmp.a, F::mul(a11, prev_a));
-        tmp.b = F::add(F::mul(a10, r2.b), F::mul(a11, prev_b));
+            tmp.a = F::add(tmp.a, F::mul(a11, prev.a));
+        tmp.b = F::add(F::mul(a10, r2.b), F::mul(a11, prev.b));
         r1_len = p2_len<E>(tmp, E - 1, L) & lanes;
         r1.a = L.gl < r1_len ? tmp.a : 0u;
         r1.b = L.gl + G < r1_len ? tmp.b : 0u;
-        p2_nttr<E>(a1, L);
-        p2_nttr<E>(r2, L);
     }
     if (r1_len >= E / 2) {
         for (uint32_t it = 1; it < E / 2; ++it) {
+            if (r1_len < E / 2) {  // the reference divides once more and stops
+                if (!div_ok(E, r2_len, r1_len))
+                    return 0xffffffffu;
+                break;
+            }
             const uint32_t r1h = p2_get<E>(r1, (r1_len - 1) & mask, L);
             const uint32_t r1l = p2_get<E>(r1, 0, L);
-            p2_nttr<E>(r1, L);
-            if (!p2_poly_div<E>(T, r2, r2_len, r1, r1_len, r2l, r1h, t, q, &q_len, L))
-                return 0xffffffffu;
-            {
-                const uint32_t qa = q.a, qb = q.b;
-                t.a = F::sub(a2.a, F::mul(qa, a1.a));
-                t.b = F::sub(a2.b, F::mul(qb, a1.b));
-                q.a = F::sub(r2.a, F::mul(qa, r1.a));
-                q.b = F::sub(r2.b, F::mul(qb, r1.b));
+            const uint32_t g_len = r1_len, f_len = r2_len;
+            P2 t, rn;
+            if (f_len == g_len + 1) {  // q_len == 2 (g_len >= E / 2 >= 1 here)
+                q_len = 2;
+                const uint32_t a0 = T.inv_tab[r1h];
+                const uint32_t aa = F::mul(a0, a0);
+                const uint32_t c0 = F::sub(0, F::mul(aa, p2_get<E>(r1, (g_len - 2) & mask, L)));
+                const uint32_t c1 = F::sub(F::mul(2, a0), F::mul(aa, r1h));
+                // quotient coefficients t[idx] of t = (r2 - f0) * (c0 + c1 x) mod x^E - 1
+                const uint32_t right_n = (uint32_t)E - g_len < 2 ? (uint32_t)E - g_len : 2u;
+                const uint32_t idx0 = right_n >= 1 ? g_len : 0u;
+                const uint32_t idx1 = right_n >= 2 ? g_len + 1 : 1u - right_n;
+                uint32_t v[4];
+                const uint32_t at[4] = {idx0, (idx0 - 1) & mask, idx1, (idx1 - 1) & mask};
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t x = p2_get<E>(r2, at[u], L);
+                    v[u] = at[u] == 0 ? F::sub(x, r2l) : x;
+                }
+                const uint32_t q0 = F::add(F::mul(c0, v[0]), F::mul(c1, v[1]));
+                const uint32_t q1 = F::add(F::mul(c0, v[2]), F::mul(c1, v[3]));
+                const P2 pa = p2_prev<E>(a1, L), pr = p2_prev<E>(r1, L);
+                t.a = F::sub(a2.a, F::add(F::mul(q0, a1.a), F::mul(q1, pa.a)));
+                t.b = F::sub(a2.b, F::add(F::mul(q0, a1.b), F::mul(q1, pa.b)));
+                rn.a = F::sub(r2.a, F::add(F::mul(q0, r1.a), F::mul(q1, pr.a)));
+                rn.b = F::sub(r2.b, F::add(F::mul(q0, r1.b), F::mul(q1, pr.b)));
+            } else {  // through the transforms
+                P2 R1 = r1, R2 = r2, A1 = a1, A2 = a2, scratch, Q;
+                p2_nttr<E>(R1, L);
+                p2_nttr<E>(R2, L);
+                p2_nttr<E>(A1, L);
+                p2_nttr<E>(A2, L);
+                if (!p2_poly_div<E>(T, R2, r2_len, R1, r1_len, r2l, r1h, scratch, Q, &q_len, L))
+                    return 0xffffffffu;
+                t.a = F::sub(A2.a, F::mul(Q.a, A1.a));
+                t.b = F::sub(A2.b, F::mul(Q.b, A1.b));
+                rn.a = F::sub(R2.a, F::mul(Q.a, R1.a));
+                rn.b = F::sub(R2.b, F::mul(Q.b, R1.b));
+                p2_inttr<E>(T, t, L);
+                p2_inttr<E>(T, rn, L);
             }
-            t_len = q_len + a1_len - 1;
+            uint32_t t_len = q_len + a1_len - 1;
             t_len = a2_len > t_len ? a2_len : t_len;
             uint32_t nq_len = q_len + r1_len - 1;
             nq_len = r2_len > nq_len ? r2_len : nq_len;
@@ -1104,19 +1173,12 @@ __device__ inline uint32_t p2_sugiyama(const Rsi16Dev &T, P2 &a1, P2 &r1, const 
             a2_len = a1_len;
             r2_len = r1_len;
             r2l = r1l;
-            if (!(r1_len >= E / 2)) {
-                p2_inttr<E>(T, r1, L);
-                break;
-            }
             a1 = t;
-            r1 = q;
+            r1 = rn;
             a1_len = t_len;
-            r1_len = nq_len;
-            p2_inttr<E>(T, r1, L);
-            r1_len = p2_len<E>(r1, r1_len, L);
+            r1_len = p2_len<E>(rn, nq_len, L);
         }
     }
-    p2_inttr<E>(T, a1, L);
     a1_len = p2_len<E>(a1, a1_len & lanes, L);
     const uint32_t inv0 = T.inv_tab[p2_get<E>(a1, 0, L)];
     a1.a = F::mul(a1.a, inv0);

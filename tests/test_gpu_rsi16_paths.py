@@ -1,0 +1,88 @@
+"""RSi16 repair kernels against each other: the register kernel (two coefficients per
+lane, the default for ecc_len <= 128) and the LDS kernel (any other lanes-per-codeword
+choice) give the same statuses and symbols on random codewords with errors up to and
+beyond the correction radius, and on structured error patterns whose syndromes are
+sparse (steps where the remainder degree drops by more than one); a sample is checked
+against the oracle."""
+import numpy as np
+import pytest
+
+from libffrs_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+CODES = [(8, 2, 8192), (16, 4, 8192), (64, 8, 16384), (256, 16, 16384), (256, 32, 32768), (512, 64, 8192),
+         (1024, 128, 4096)]
+
+
+def _structured(n, ecc, k, s):
+    """Rows of zeros / small values with offsets at regular positions."""
+    rows = []
+    for step in (1, 2, n // ecc, ecc, n // 2):
+        for start in (0, 1, k - 1, k):
+            if start >= n:
+                continue
+            cw = np.zeros(n, np.int64)
+            cw[start::step] += 1 + (start % 3)
+            rows.append(cw)
+    cw = np.zeros(n, np.int64)
+    cw[k:] = 7  # parity only
+    rows.append(cw)
+    cw = np.zeros(n, np.int64)
+    cw[:ecc] = np.arange(1, ecc + 1)
+    rows.append(cw)
+    return np.array(rows, np.uint16)
+
+
+def _run(ctx, codec, cm, cp):
+    count = cm.shape[0]
+    dcm, dcp, dst = ctx.alloc(cm.nbytes), ctx.alloc(cp.nbytes), ctx.alloc(count)
+    dcm.upload(cm)
+    dcp.upload(cp)
+    codec.repair_device(dcm, dcp, count, dst)
+    om, op, st = np.empty_like(cm), np.empty_like(cp), np.empty(count, np.uint8)
+    dcm.download(om)
+    dcp.download(op)
+    dst.download(st)
+    for b in (dcm, dcp, dst):
+        b.free()
+    return om, op, st
+
+
+@pytest.mark.parametrize("n,ecc,count", CODES)
+def test_register_and_lds_repair_agree(n, ecc, count):
+    from libffrs_amd import capi
+    from oracle import oracle as O
+
+    k = n - ecc
+    ctx = capi.Context(0)
+    codec = capi.RSi16Codec(ctx, n, ecc)
+    msgs, counts, s = W.rsi16_plain(count, n, ecc, ecc, seed=71 + ecc)  # 0..ecc errors
+    dm, dp = ctx.alloc(msgs.nbytes), ctx.alloc(count * ecc * 2)
+    dm.upload(np.ascontiguousarray(msgs))
+    codec.encode_device(dm, count, dp)
+    par = np.empty((count, ecc), np.uint16)
+    dp.download(par)
+    dm.free()
+    dp.free()
+    cws = np.concatenate([msgs, par], 1)
+    W.rsi16_corrupt(cws, counts, s)
+    cws = np.concatenate([cws, _structured(n, ecc, k, s)])
+    cm, cp = np.ascontiguousarray(cws[:, :k]), np.ascontiguousarray(cws[:, k:])
+
+    g_default = codec.groups()[1]
+    assert g_default == ecc // 2
+    reg = _run(ctx, codec, cm, cp)
+    codec.set_groups(0, 2 if ecc == 2 else ecc // 4)
+    lds = _run(ctx, codec, cm, cp)
+    codec.close()
+    ctx.close()
+    for a, b in zip(reg, lds):
+        assert (a == b).all()
+    st = reg[2]
+    assert (st == 2).any() and (st != 2).any()
+
+    rows = np.r_[0:64, cm.shape[0] - 24:cm.shape[0]]
+    m, e = np.ascontiguousarray(cm[rows]), np.ascontiguousarray(cp[rows])
+    ost = O.RSi16(n, ecc).repair_batch(m, e)
+    assert (ost == st[rows]).all() and (m == reg[0][rows]).all() and (e == reg[1][rows]).all()
