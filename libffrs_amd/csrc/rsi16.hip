@@ -538,31 +538,60 @@ __device__ inline bool poly_div_ntt(const Rsi16Dev &T, const uint32_t *f, uint32
     return true;
 }
 
+// poly_div_ntt's outcome (true unless it raises) from the lengths alone: g_len == 0 raises,
+// f_len < g_len returns, and poly_inv_mod_xn_rev raises where its Newton loop meets
+// g_len + l - 2 > E (ntt.hpp:363-564).
+__device__ inline bool div_ok(uint32_t E, uint32_t f_len, uint32_t g_len)
+{
+    if (g_len == 0)
+        return false;
+    if (f_len < g_len)
+        return true;
+    const uint32_t q_len = f_len - g_len + 1;
+    for (uint32_t l = 2; l < q_len; l *= 2)
+        if (g_len + l - 2 > E)
+            return false;
+    return true;
+}
+
+// Scratch words (times E) of sugiyama.
+constexpr uint32_t SUGI_WORDS = 5;
+
 // RSi16v<1>::_sugiyama (rsi16v_impl.hpp:271-440). r1 holds the syndromes on entry and
-// the normalised evaluator on exit; w holds 5 E scratch words. Returns the locator
-// length, or 0xFFFFFFFF where the reference raises.
-__device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1, uint32_t *r1, uint32_t *w, Grp G)
+// the normalised evaluator on exit; w holds SUGI_WORDS * E scratch words. Returns the
+// locator length, or 0xFFFFFFFF where the reference raises.
+//
+// The reference keeps a1, a2 and r2 as E-point transforms and multiplies pointwise, which
+// is multiplication modulo x^E - 1 of the coefficient vectors. Here every polynomial stays
+// in coefficient form and the same products are formed as cyclic convolutions, so every
+// value is the same field element. In the usual step the quotient has two terms
+// (q_len == 2): poly_inv_mod_xn_rev then reduces to a = (-a0^2 g[g_len - 2],
+// 2 a0 - a0^2 g0) with a0 = 1 / g0 (its Newton loop does not run), two coefficients of
+// (f - f0) * a give the quotient, and the updates are two-tap convolutions. Any other step
+// goes through the transforms exactly as the reference does. The last division, whose
+// result the reference discards, only decides whether it raises (div_ok). Lengths are
+// scanned below E.
+__device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1_out, uint32_t *r1_out, uint32_t *w, Grp G)
 {
     const uint32_t E = T.E, mask = E - 1;
-    uint32_t *q = w, *t = w + E, *a2 = w + 2 * E, *r2 = w + 3 * E, *tmp = w + 4 * E;
+    uint32_t *a1 = a1_out, *r1 = r1_out, *a2 = w, *r2 = w + E;
+    uint32_t *t = w + 2 * E, *q = w + 3 * E, *tmp = w + 4 * E;
     for (uint32_t i = G.gl; i < E; i += G.g) {
-        q[i] = 0;
-        t[i] = 0;
         a1[i] = 0;
-        a2[i] = 1;
+        a2[i] = i == 0 ? 1u : 0u;  // the transform's all-ones vector
         r2[i] = r1[i];
     }
     wave_sync();
-    uint32_t q_len = 0, t_len = 0, a2_len = 1, a1_len;
+    uint32_t q_len = 0, a2_len = 1, a1_len;
     uint32_t r1_len = group_len(r1, E, G);
     uint32_t r2_len = r1_len;
     uint32_t r2l = r2[0];
-    uint32_t lanes = r1_len != 0 ? 0xffffffffu : 0u;
+    const uint32_t lanes = r1_len != 0 ? 0xffffffffu : 0u;
     {
-        uint32_t r1h = r1[(r1_len - 1) & mask];
-        uint32_t r1h2 = r1[(r1_len - 2) & mask];
-        uint32_t a11 = s_neg(g_inv(r1h));
-        uint32_t a10 = S::mul(r1h2, S::mul(a11, a11));
+        const uint32_t r1h = r1[(r1_len - 1) & mask];
+        const uint32_t r1h2 = r1[(r1_len - 2) & mask];
+        const uint32_t a11 = s_neg(T.inv_tab[r1h]);
+        const uint32_t a10 = S::mul(r1h2, S::mul(a11, a11));
         wave_sync();
         if (G.gl == 0) {
             a1[0] = a10;
@@ -580,58 +609,85 @@ __device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1, uint32_t *r
         for (uint32_t i = G.gl; i < E; i += G.g)
             r1[i] = i < r1_len ? tmp[i] : 0u;
         wave_sync();
-        nttr(T, a1, G);
-        nttr(T, r2, G);
     }
-    bool cond = r1_len >= E / 2;
-    if (cond) {
+    if (r1_len >= E / 2) {
         for (uint32_t it = 1; it < E / 2; ++it) {
-            uint32_t r1h = r1[(r1_len - 1) & mask];
-            uint32_t r1l = r1[0];
-            wave_sync();
-            nttr(T, r1, G);
-            if (!poly_div_ntt(T, r2, r2_len, r1, r1_len, r2l, r1h, t, q, tmp, &q_len, G))
-                return 0xffffffffu;
-            for (uint32_t i = G.gl; i < E; i += G.g) {
-                uint32_t qi = q[i];
-                t[i] = S::sub(a2[i], S::mul(qi, a1[i]));
-                q[i] = S::sub(r2[i], S::mul(qi, r1[i]));
+            if (r1_len < E / 2) {  // the reference divides once more and stops
+                if (!div_ok(E, r2_len, r1_len))
+                    return 0xffffffffu;
+                break;
             }
-            wave_sync();
-            t_len = q_len + a1_len - 1;
+            const uint32_t r1h = r1[(r1_len - 1) & mask];
+            const uint32_t r1l = r1[0];
+            const uint32_t g_len = r1_len, f_len = r2_len;
+            if (f_len == g_len + 1) {  // q_len == 2 (g_len >= E / 2 >= 1 here)
+                q_len = 2;
+                const uint32_t a0 = T.inv_tab[r1h];
+                const uint32_t aa = S::mul(a0, a0);
+                const uint32_t c0 = S::sub(0, S::mul(aa, r1[(g_len - 2) & mask]));
+                const uint32_t c1 = S::sub(S::mul(2, a0), S::mul(aa, r1h));
+                // quotient coefficients t[idx] of t = (r2 - f0) * (c0 + c1 x) mod x^E - 1
+                const uint32_t right_n = E - g_len < 2 ? E - g_len : 2u;
+                const uint32_t idx0 = right_n >= 1 ? g_len : 0u;
+                const uint32_t idx1 = right_n >= 2 ? g_len + 1 : 1u - right_n;
+                const uint32_t at[4] = {idx0, (idx0 - 1) & mask, idx1, (idx1 - 1) & mask};
+                uint32_t v[4];
+                for (int u = 0; u < 4; ++u) {
+                    const uint32_t x = r2[at[u]];
+                    v[u] = at[u] == 0 ? S::sub(x, r2l) : x;
+                }
+                const uint32_t q0 = S::add(S::mul(c0, v[0]), S::mul(c1, v[1]));
+                const uint32_t q1 = S::add(S::mul(c0, v[2]), S::mul(c1, v[3]));
+                wave_sync();
+                // a2 - q a1 and r2 - q r1 in place of a2 and r2
+                for (uint32_t i = G.gl; i < E; i += G.g) {
+                    const uint32_t j = (i - 1) & mask;
+                    a2[i] = S::sub(a2[i], S::add(S::mul(q0, a1[i]), S::mul(q1, a1[j])));
+                    r2[i] = S::sub(r2[i], S::add(S::mul(q0, r1[i]), S::mul(q1, r1[j])));
+                }
+                wave_sync();
+            } else {  // through the transforms; a1 and r1 are transformed back afterwards
+                nttr(T, r1, G);
+                nttr(T, a1, G);
+                nttr(T, a2, G);
+                nttr(T, r2, G);
+                if (!poly_div_ntt(T, r2, r2_len, r1, r1_len, r2l, r1h, t, q, tmp, &q_len, G))
+                    return 0xffffffffu;
+                for (uint32_t i = G.gl; i < E; i += G.g) {
+                    const uint32_t qi = q[i];
+                    a2[i] = S::sub(a2[i], S::mul(qi, a1[i]));
+                    r2[i] = S::sub(r2[i], S::mul(qi, r1[i]));
+                }
+                wave_sync();
+                inttr(T, a2, G);
+                inttr(T, r2, G);
+                inttr(T, a1, G);
+                inttr(T, r1, G);
+            }
+            uint32_t t_len = q_len + a1_len - 1;
             t_len = a2_len > t_len ? a2_len : t_len;
             uint32_t nq_len = q_len + r1_len - 1;
             nq_len = r2_len > nq_len ? r2_len : nq_len;
-            for (uint32_t i = G.gl; i < E; i += G.g) {
-                a2[i] = a1[i];
-                r2[i] = r1[i];
-            }
-            wave_sync();
+            uint32_t *sw = a1;  // a2 <- a1, a1 <- the new remainder's cofactor
+            a1 = a2;
+            a2 = sw;
+            sw = r1;
+            r1 = r2;
+            r2 = sw;
             a2_len = a1_len;
             r2_len = r1_len;
             r2l = r1l;
-            if (!(r1_len >= E / 2)) {
-                inttr(T, r1, G);
-                break;
-            }
-            for (uint32_t i = G.gl; i < E; i += G.g) {
-                a1[i] = t[i];
-                r1[i] = q[i];
-            }
-            wave_sync();
             a1_len = t_len;
-            r1_len = nq_len;
-            inttr(T, r1, G);
-            r1_len = group_len(r1, r1_len, G);
+            r1_len = group_len(r1, nq_len < E ? nq_len : E, G);
         }
     }
-    inttr(T, a1, G);
-    a1_len = group_len(a1, a1_len & lanes, G);
-    uint32_t inv0 = g_inv(a1[0]);
+    const uint32_t a1_max = a1_len & lanes;
+    a1_len = group_len(a1, a1_max < E ? a1_max : E, G);
+    const uint32_t inv0 = T.inv_tab[a1[0]];
     wave_sync();
     for (uint32_t i = G.gl; i < E; i += G.g) {
-        a1[i] = S::mul(a1[i], inv0);
-        r1[i] = S::mul(r1[i], inv0);
+        a1_out[i] = S::mul(a1[i], inv0);
+        r1_out[i] = S::mul(r1[i], inv0);
     }
     wave_sync();
     return a1_len & lanes;
@@ -648,8 +704,8 @@ __device__ inline uint32_t eval_poly(const uint32_t *r, uint32_t len, uint32_t x
 
 // ---------------------------------------------------------------- repair
 
-// Per group LDS: syn/evaluator E, locator E, derivative E, roots chunk E, Sugiyama 5E.
-constexpr uint32_t REPAIR_WORDS = 9;
+// Per group LDS: syn/evaluator E, locator E, derivative E, roots chunk E, Sugiyama scratch.
+constexpr uint32_t REPAIR_WORDS = 4 + SUGI_WORDS;
 
 // Repair of one codeword by one lane group (rsi16v_impl.hpp:122-168); lane 0 of the
 // group writes the status to *st. base holds REPAIR_WORDS * E words.
@@ -1038,22 +1094,6 @@ __device__ inline bool p2_poly_div(const Rsi16Dev &T, const P2 &f, uint32_t f_le
     return true;
 }
 
-// poly_div_ntt's outcome (true unless it raises) from the lengths alone: g_len == 0 raises,
-// f_len < g_len returns, and poly_inv_mod_xn_rev raises where its Newton loop meets
-// g_len + l - 2 > E (ntt.hpp:363-564).
-__device__ inline bool div_ok(uint32_t E, uint32_t f_len, uint32_t g_len)
-{
-    if (g_len == 0)
-        return false;
-    if (f_len < g_len)
-        return true;
-    const uint32_t q_len = f_len - g_len + 1;
-    for (uint32_t l = 2; l < q_len; l *= 2)
-        if (g_len + l - 2 > E)
-            return false;
-    return true;
-}
-
 // p[(i - 1) mod E] for both of the lane's coefficients
 template <int E>
 __device__ inline P2 p2_prev(const P2 &p, const Lane2<E> &L)
@@ -1345,7 +1385,7 @@ __global__ __launch_bounds__(64) void rsi16_sugiyama_kernel(const uint32_t *__re
     uint32_t cw = blockIdx.x * (64 / g) + grp;
     if (cw >= count)
         return;
-    uint32_t *r1 = s_mem + grp * 7 * E, *a1 = r1 + E, *w = r1 + 2 * E;
+    uint32_t *r1 = s_mem + grp * (2 + SUGI_WORDS) * E, *a1 = r1 + E, *w = r1 + 2 * E;
     for (uint32_t j = G.gl; j < E; j += G.g)
         r1[j] = synds[(size_t)cw * E + j];
     wave_sync();
@@ -2063,7 +2103,7 @@ int ffrs_rsi16_sugiyama(ffrs_rsi16 *rs, const uint32_t *synds, size_t count, uin
         return r;
     FFRS_HIP(hipMemcpyAsync(d_s, synds, sb, hipMemcpyHostToDevice, ctx->stream));
     uint32_t g = group_size(rs->E), cpw = 64 / g;
-    size_t lds = (size_t)cpw * 7 * rs->E * 4;
+    size_t lds = (size_t)cpw * (2 + SUGI_WORDS) * rs->E * 4;
     uint32_t *lo = (uint32_t *)d_o, *eo = lo + count * rs->E;
     if (int r = allow_lds(rsi16_sugiyama_kernel, lds))
         return r;
