@@ -1510,6 +1510,14 @@ struct Rsi16PlanClass {
 
 __global__ void rsi16_plan_reset_kernel(uint32_t *queue) { *queue = 0; }
 
+template <int E>
+__device__ inline void plan_repair_reg(const Rsi16Dev &T, uint16_t *m, uint16_t *e, uint32_t *base, uint8_t *st)
+{
+    Lane2<E> L;
+    L.init(T, threadIdx.x);
+    repair_group_reg<E>(T, m, e, base, L, st);
+}
+
 template <int MODE>  // 0: encode (vector words), 1: encode (scalar words), 2: repair
 __global__ __launch_bounds__(64) void rsi16_plan_kernel(uint16_t *__restrict__ msgs, uint16_t *__restrict__ eccs,
                                                          uint8_t *__restrict__ status,
@@ -1523,7 +1531,7 @@ __global__ __launch_bounds__(64) void rsi16_plan_kernel(uint16_t *__restrict__ m
         if (threadIdx.x == 0)
             s_chunk = atomicAdd(queue, 1u);
         __syncthreads();
-        const uint32_t c = s_chunk;
+        const uint32_t c = __builtin_amdgcn_readfirstlane(s_chunk);  // one wave: the class fields go to SGPRs
         if (c >= nchunks)
             break;
         const Rsi16PlanChunk ch = chunks[c];
@@ -1533,10 +1541,37 @@ __global__ __launch_bounds__(64) void rsi16_plan_kernel(uint16_t *__restrict__ m
         const uint32_t grp = threadIdx.x / g;
         if (grp < ch.count) {
             const Rsi16PlanItem it = items[ch.first + grp];
-            if constexpr (MODE == 2)
-                repair_group(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * REPAIR_WORDS * E, G,
-                             status + it.idx);
-            else if constexpr (MODE == 0)
+            if constexpr (MODE == 2) {
+                uint16_t *m = msgs + it.msg_off, *e = eccs + it.ecc_off;
+                uint32_t *base = s_mem + grp * REPAIR_WORDS * E;
+                if (E <= 128 && g == E / 2) {  // the register kernel's code, as rsi16_repair_reg_kernel
+                    switch (E) {
+                    case 2:
+                        plan_repair_reg<2>(cl.T, m, e, base, status + it.idx);
+                        break;
+                    case 4:
+                        plan_repair_reg<4>(cl.T, m, e, base, status + it.idx);
+                        break;
+                    case 8:
+                        plan_repair_reg<8>(cl.T, m, e, base, status + it.idx);
+                        break;
+                    case 16:
+                        plan_repair_reg<16>(cl.T, m, e, base, status + it.idx);
+                        break;
+                    case 32:
+                        plan_repair_reg<32>(cl.T, m, e, base, status + it.idx);
+                        break;
+                    case 64:
+                        plan_repair_reg<64>(cl.T, m, e, base, status + it.idx);
+                        break;
+                    default:
+                        plan_repair_reg<128>(cl.T, m, e, base, status + it.idx);
+                        break;
+                    }
+                } else {
+                    repair_group(cl.T, m, e, base, G, status + it.idx);
+                }
+            } else if constexpr (MODE == 0)
                 encode_group<VectorOps>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * 2 * E, G);
             else
                 encode_group<ScalarOps>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * 2 * E, G);
