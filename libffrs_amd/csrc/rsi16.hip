@@ -32,6 +32,15 @@ constexpr uint32_t RSI16_MAX_ECC = 4096;
 
 // ---------------------------------------------------------------- field operations
 
+// Every product operand is at most p = 65537 (canonical values, mod_p results, table
+// entries, u16 symbols): telling the compiler so lets it use the full-rate 24-bit multiply,
+// whose low 32 bits are the 32-bit product for such operands.
+__host__ __device__ inline void ffrs_u24(uint32_t a, uint32_t b)
+{
+    __builtin_assume(a <= P16);
+    __builtin_assume(b <= P16);
+}
+
 struct ScalarOps {  // integral branches
     __host__ __device__ static inline uint32_t add(uint32_t a, uint32_t b)
     {
@@ -45,6 +54,7 @@ struct ScalarOps {  // integral branches
     }
     __host__ __device__ static inline uint32_t mul(uint32_t a, uint32_t b)
     {
+        ffrs_u24(a, b);
         uint32_t r = a * b;
         if (r == 0 && a && b)
             return 1;
@@ -53,6 +63,7 @@ struct ScalarOps {  // integral branches
     }
     __host__ __device__ static inline uint32_t mul_residue(uint32_t a, uint32_t b)
     {
+        ffrs_u24(a, b);
         uint32_t r = a * b;
         if (r == 0 && a && b)
             return 1;
@@ -74,6 +85,7 @@ struct VectorOps {  // SIMD-lane branches
     __host__ __device__ static inline uint32_t sub(uint32_t a, uint32_t b) { return a - b + (b > a ? P16 : 0u); }
     __host__ __device__ static inline uint32_t mul(uint32_t a, uint32_t b)
     {
+        ffrs_u24(a, b);
         uint32_t r = a * b;
         uint32_t ovf = (r == 0 && (a & b) != 0) ? 1u : 0u;
         uint32_t lo = r & 0xffff, hi = r >> 16;
@@ -81,6 +93,7 @@ struct VectorOps {  // SIMD-lane branches
     }
     __host__ __device__ static inline uint32_t mul_residue(uint32_t a, uint32_t b)
     {
+        ffrs_u24(a, b);
         uint32_t r = a * b;
         uint32_t ovf = (r == 0 && (a & b) != 0) ? 1u : 0u;
         return (r & 0xffff) - (r >> 16) + ovf;
