@@ -815,8 +815,12 @@ __global__ __launch_bounds__(WG) void rs256_bm_fixed_kernel(uint32_t count, uint
 // that holds it (the other three bytes get 0), issued without waiting for a result.
 __device__ inline void xor_byte(uint8_t *p, uint32_t v)
 {
+    // p points into device memory: a global (not flat) atomic, which LDS waits do not
+    // have to drain
+    typedef __attribute__((address_space(1))) uint32_t global_u32;
     uintptr_t a = reinterpret_cast<uintptr_t>(p);
-    atomicXor(reinterpret_cast<unsigned int *>(a & ~(uintptr_t)3), v << (8 * (a & 3)));
+    global_u32 *w = (global_u32 *)(a & ~(uintptr_t)3);
+    __hip_atomic_fetch_xor(w, v << (8 * (a & 3)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // Chien search one codeword per wave, then Forney one codeword per lane
@@ -1304,7 +1308,7 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
 
-    constexpr int WG3 = 768;
+    constexpr int WG3 = 960;  // 15 waves: the tables and root slots just fit in 160 KB
     size_t lds3 = CHIEN_TABLE_BYTES + (size_t)WG3 * CHIEN_LMAX * 2 + EXP2_LEN + 512;
     if (int r = allow_lds(rs256_chien_kernel<WG3>, lds3))
         return r;
