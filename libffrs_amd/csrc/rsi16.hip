@@ -794,16 +794,17 @@ __device__ inline void repair_group(const Rsi16Dev &T, uint16_t *m, uint16_t *e,
         *st = total > 0 ? FFRS_RSI16_REPAIR_OK : (lanes ? FFRS_RSI16_ERROR_LOCATION_FAIL : FFRS_RSI16_NO_ERRORS);
 }
 
-// ---- repair with two coefficients per lane in registers (ecc_len <= 128)
+// ---- repair with the polynomials in registers (ecc_len <= 256)
 //
-// The same operations as repair_group in the same order, for lane groups of g = E / 2:
-// lane gl of a group holds coefficients gl and gl + g of every polynomial. Butterfly
-// partners at distance s < g sit in lane gl ^ s and are exchanged with one cross-lane move
-// (DPP for s <= 8, ds_swizzle for 16, ds_bpermute for 32); the last (or first) stage pairs
-// the lane's own two coefficients. Each lane's twiddles are loaded once per codeword.
-// Permutations (reversal, shifts) gather with ds_bpermute. Lanes of a group take the same
-// branches (every decision depends on group-wide values only), and no move reads a lane
-// outside the group (an aligned block of g lanes).
+// The same operations as repair_group in the same order. A group of G = min(E / 2, 64)
+// lanes serves one codeword and each lane holds C = E / G coefficients of every polynomial:
+// lane gl holds coefficients gl + h G, h < C. Butterfly partners at distance s < G sit in
+// lane gl ^ s (same h) and are exchanged with one cross-lane move (DPP for s <= 8,
+// ds_swizzle for 16, ds_bpermute for 32); partners at distance s >= G are two of the lane's
+// own coefficients. Each lane's twiddles are loaded once per codeword. Permutations gather
+// with ds_bpermute. Lanes of a group take the same branches (every decision depends on
+// group-wide values only), and no move reads a lane outside the group (an aligned block of
+// G lanes).
 
 // Field operations for operands in [0, 65536] (every value the register repair handles:
 // u16 symbols, table entries and results of these operations). The results equal
@@ -829,10 +830,6 @@ struct F {
         const uint32_t t = ((uint32_t)x & 0xffffu) - (uint32_t)(x >> 16), u = t + P16;
         return t < u ? t : u;
     }
-};
-
-struct P2 {
-    uint32_t a, b;  // coefficients gl and gl + g
 };
 
 // v of lane (this lane ^ X)
@@ -880,292 +877,345 @@ __device__ __attribute__((always_inline)) inline uint32_t xchg_s(uint32_t v, uin
 }
 
 template <int E>
-struct Lane2 {
-    static constexpr uint32_t G = E / 2;  // lanes per codeword
-    static constexpr int ST = __builtin_ctz(E);  // butterfly stages
+struct RegShape {
+    static constexpr uint32_t G = E / 2 < 64 ? E / 2 : 64;  // lanes per codeword
+    static constexpr int C = E / (int)G;                    // coefficients per lane
+    static constexpr int LE = __builtin_ctz(E);             // butterfly stages
+    static constexpr int SX = __builtin_ctz(G);             // stages with partners in other lanes
+    static constexpr int LC = __builtin_ctz(C);             // stages within the lane
+};
+
+template <int E>
+struct PR {
+    uint32_t v[RegShape<E>::C];  // coefficient gl + h G in v[h]
+};
+
+template <int E>
+struct LaneR {
+    using Sh = RegShape<E>;
     uint32_t gl, base, lane;  // lane in the group, first lane of the group, lane in the wave
-    // per-stage twiddles (1 where the lane does not multiply): CT and GS over roots_ecc
-    // (e) and roots_i_ecc (i); stage t pairs at distance 2^t
-    uint32_t ct_e[ST], ct_i[ST], gs_e[ST], gs_i[ST];
-    uint32_t rv_k[2], rv_j[2], rv_w[2];  // reverse_ntt: source index, j and roots_i_ecc[j]
+    // Stage t (s = 2^t) of both butterfly orders uses roots[j << (LE - 1 - t)], j = i & (s - 1).
+    // Stages with partners in other lanes (t < SX): j = gl & (s - 1), kept where this lane is
+    // the upper partner and 1 elsewhere. Stages within the lane (s = G 2^u): the pair
+    // (h, h + 2^u) with bit u of h clear uses entry 2^u - 1 + (h & (2^u - 1)).
+    uint32_t x_e[Sh::SX > 0 ? Sh::SX : 1], x_i[Sh::SX > 0 ? Sh::SX : 1];
+    uint32_t w_e[Sh::C - 1], w_i[Sh::C - 1];
 
     __device__ void init(const Rsi16Dev &T, uint32_t lane_in_wave)
     {
         lane = lane_in_wave;
-        gl = lane % G;
+        gl = lane % Sh::G;
         base = lane - gl;
 #pragma unroll
-        for (int t = 0; t < ST; ++t) {
-            const uint32_t s = 1u << t;
-            if (s < G) {
-                const uint32_t j = gl & (s - 1), ef = E / (2 * s);
-                const bool upper = (gl & s) != 0;
-                ct_e[t] = upper ? T.roots_ecc[ef * j] : 1u;
-                ct_i[t] = upper ? T.roots_i_ecc[ef * j] : 1u;
-                // GS: stage s uses roots[j << ef'] with ef' = log2(G / s)
-                const uint32_t sh = (uint32_t)(ST - 1 - t);
-                gs_e[t] = upper ? T.roots_ecc[j << sh] : 1u;
-                gs_i[t] = upper ? T.roots_i_ecc[j << sh] : 1u;
-            } else {  // s = G: the lane's own pair, j = gl
-                ct_e[t] = gs_e[t] = T.roots_ecc[gl];
-                ct_i[t] = gs_i[t] = T.roots_i_ecc[gl];
-            }
+        for (int t = 0; t < Sh::SX; ++t) {
+            const uint32_t s = 1u << t, j = gl & (s - 1);
+            const bool upper = (gl & s) != 0;
+            const uint32_t idx = j << (Sh::LE - 1 - t);
+            x_e[t] = upper ? T.roots_ecc[idx] : 1u;
+            x_i[t] = upper ? T.roots_i_ecc[idx] : 1u;
         }
 #pragma unroll
-        for (int h = 0; h < 2; ++h) {
-            const uint32_t i = gl + (h ? G : 0);
-            const uint32_t j = T.rbo_ecc[i];
-            rv_j[h] = j;
-            rv_k[h] = T.rbo_ecc[(E - j) & (E - 1)];
-            rv_w[h] = T.roots_i_ecc[j];
-        }
+        for (int u = 0; u < Sh::LC; ++u)
+#pragma unroll
+            for (int hl = 0; hl < (1 << u); ++hl) {
+                const uint32_t j = gl + (uint32_t)hl * Sh::G;
+                const uint32_t idx = j << (Sh::LE - 1 - (Sh::SX + u));
+                w_e[(1 << u) - 1 + hl] = T.roots_ecc[idx];
+                w_i[(1 << u) - 1 + hl] = T.roots_i_ecc[idx];
+            }
     }
 };
 
-// value of coefficient idx (the same for the whole group)
+// coefficient idx (the same for the whole group)
 template <int E>
-__device__ inline uint32_t p2_get(const P2 &p, uint32_t idx, const Lane2<E> &L)
+__device__ inline uint32_t pr_get(const PR<E> &p, uint32_t idx, const LaneR<E> &L)
 {
-    const uint32_t src = L.base + (idx % Lane2<E>::G);
-    const uint32_t va = lane_read(p.a, src), vb = lane_read(p.b, src);
-    return idx < Lane2<E>::G ? va : vb;
+    constexpr uint32_t G = RegShape<E>::G;
+    const uint32_t slot = idx / G;
+    // select by the bits of slot (a lookup into p.v would go through scratch memory)
+    uint32_t w[RegShape<E>::C];
+#pragma unroll
+    for (int h = 0; h < RegShape<E>::C; ++h)
+        w[h] = p.v[h];
+#pragma unroll
+    for (int b = 0; (1 << b) < RegShape<E>::C; ++b)
+#pragma unroll
+        for (int h = 0; h + (1 << b) < RegShape<E>::C; h += 2 << b)
+            w[h] = ((slot >> b) & 1) ? w[h + (1 << b)] : w[h];
+    return lane_read(w[0], L.base + idx % G);
 }
 
-// coefficient k of p, k chosen per lane (k < E)
+// coefficient k of p, k < E chosen per lane
 template <int E>
-__device__ inline uint32_t p2_gather(const P2 &p, uint32_t k, const Lane2<E> &L)
+__device__ inline uint32_t pr_gather(const PR<E> &p, uint32_t k, const LaneR<E> &L)
 {
-    return p2_get<E>(p, k, L);
+    constexpr uint32_t G = RegShape<E>::G;
+    const uint32_t src = L.base + k % G, slot = k / G;
+    uint32_t w[RegShape<E>::C];
+#pragma unroll
+    for (int h = 0; h < RegShape<E>::C; ++h)
+        w[h] = lane_read(p.v[h], src);
+#pragma unroll
+    for (int b = 0; (1 << b) < RegShape<E>::C; ++b)
+#pragma unroll
+        for (int h = 0; h + (1 << b) < RegShape<E>::C; h += 2 << b)
+            w[h] = ((slot >> b) & 1) ? w[h + (1 << b)] : w[h];
+    return w[0];
+}
+
+// p[(i - 1) mod E] for each of the lane's coefficients
+template <int E>
+__device__ inline PR<E> pr_prev(const PR<E> &p, const LaneR<E> &L)
+{
+    constexpr uint32_t G = RegShape<E>::G;
+    constexpr int C = RegShape<E>::C;
+    const uint32_t src = L.base + (L.gl + G - 1) % G;
+    uint32_t r[C];
+#pragma unroll
+    for (int h = 0; h < C; ++h)
+        r[h] = lane_read(p.v[h], src);
+    PR<E> o;
+#pragma unroll
+    for (int h = 0; h < C; ++h)
+        o.v[h] = L.gl ? r[h] : r[(h + C - 1) % C];
+    return o;
 }
 
 // degree + 1 of p restricted to [0, max) (group_len)
 template <int E>
-__device__ inline uint32_t p2_len(const P2 &p, uint32_t max, const Lane2<E> &L)
+__device__ inline uint32_t pr_len(const PR<E> &p, uint32_t max, const LaneR<E> &L)
 {
-    constexpr uint32_t G = Lane2<E>::G;
+    constexpr uint32_t G = RegShape<E>::G;
     uint32_t hi = 0;
-    if (L.gl < max && p.a != 0)
-        hi = L.gl + 1;
-    if (L.gl + G < max && p.b != 0)
-        hi = L.gl + G + 1;
 #pragma unroll
-    for (int t = 0; t < Lane2<E>::ST - 1; ++t) {  // 2^t < G
+    for (int h = 0; h < RegShape<E>::C; ++h) {
+        const uint32_t i = L.gl + (uint32_t)h * G;
+        if (i < max && p.v[h] != 0)
+            hi = i + 1;
+    }
+#pragma unroll
+    for (int t = 0; t < RegShape<E>::SX; ++t) {
         const uint32_t o = xchg_s(hi, 1u << t, L.lane);
         hi = o > hi ? o : hi;
     }
     return hi;
 }
 
-// ct_full (RBO in, natural out, full reduction), ntt.hpp:225-249; w = ct_e or ct_i
+// ct_full (RBO in, natural out, full reduction), ntt.hpp:225-249
 template <int E>
-__device__ inline void p2_ct(P2 &x, const uint32_t (&w)[Lane2<E>::ST], const Lane2<E> &L)
+__device__ inline void pr_ct(PR<E> &x, const uint32_t *xw, const uint32_t *ww, const LaneR<E> &L)
 {
-    constexpr uint32_t G = Lane2<E>::G;
-    constexpr int ST = Lane2<E>::ST;
+    constexpr int C = RegShape<E>::C;
 #pragma unroll
-    for (int t = 0; t < ST - 1; ++t) {  // s = 2^t < G
+    for (int t = 0; t < RegShape<E>::SX; ++t) {  // s = 2^t < G
         // the upper lane sends c * w, the lower lane sends a (its w is 1)
         const bool upper = (L.gl & (1u << t)) != 0;
-        const uint32_t sa = F::mul(x.a, w[t]), sb = F::mul(x.b, w[t]);
-        const uint32_t ra = xchg_s(sa, 1u << t, L.lane), rb = xchg_s(sb, 1u << t, L.lane);
-        x.a = upper ? F::sub(ra, sa) : F::add(sa, ra);
-        x.b = upper ? F::sub(rb, sb) : F::add(sb, rb);
-    }
-    // s = G: coefficients gl and gl + G of this lane, twiddle roots[gl]
-    const uint32_t c = F::mul(x.b, w[ST - 1]);
-    const uint32_t a = x.a;
-    x.a = F::add(a, c);
-    x.b = F::sub(a, c);
-    (void)G;
-}
-
-// gs_full (natural in, RBO out) with full reduction, ntt.hpp:280-302; w = gs_e or gs_i
-template <int E>
-__device__ inline void p2_gs(P2 &x, const uint32_t (&w)[Lane2<E>::ST], const Lane2<E> &L)
-{
-    constexpr int ST = Lane2<E>::ST;
-    {  // s = G
-        const uint32_t a = x.a, c = x.b;
-        x.a = F::add(a, c);
-        x.b = F::mul(F::sub(a, c), w[ST - 1]);
+#pragma unroll
+        for (int h = 0; h < C; ++h) {
+            const uint32_t sa = F::mul(x.v[h], xw[t]);
+            const uint32_t ra = xchg_s(sa, 1u << t, L.lane);
+            x.v[h] = upper ? F::sub(ra, sa) : F::add(sa, ra);
+        }
     }
 #pragma unroll
-    for (int t = ST - 2; t >= 0; --t) {  // s = G / 2 .. 1
+    for (int u = 0; u < RegShape<E>::LC; ++u)  // s = G 2^u
+#pragma unroll
+        for (int h = 0; h < C; ++h)
+            if (!(h & (1 << u))) {
+                const uint32_t a = x.v[h];
+                const uint32_t c = F::mul(x.v[h + (1 << u)], ww[(1 << u) - 1 + (h & ((1 << u) - 1))]);
+                x.v[h] = F::add(a, c);
+                x.v[h + (1 << u)] = F::sub(a, c);
+            }
+}
+
+// gs_full (natural in, RBO out) with full reduction, ntt.hpp:280-302
+template <int E>
+__device__ inline void pr_gs(PR<E> &x, const uint32_t *xw, const uint32_t *ww, const LaneR<E> &L)
+{
+    constexpr int C = RegShape<E>::C;
+#pragma unroll
+    for (int u = RegShape<E>::LC - 1; u >= 0; --u)  // s = G 2^u
+#pragma unroll
+        for (int h = 0; h < C; ++h)
+            if (!(h & (1 << u))) {
+                const uint32_t a = x.v[h], c = x.v[h + (1 << u)];
+                x.v[h] = F::add(a, c);
+                x.v[h + (1 << u)] = F::mul(F::sub(a, c), ww[(1 << u) - 1 + (h & ((1 << u) - 1))]);
+            }
+#pragma unroll
+    for (int t = RegShape<E>::SX - 1; t >= 0; --t) {  // s = 2^t < G
         const bool upper = (L.gl & (1u << t)) != 0;
-        const uint32_t ra = xchg_s(x.a, 1u << t, L.lane), rb = xchg_s(x.b, 1u << t, L.lane);
-        const uint32_t da = upper ? F::sub(ra, x.a) : F::add(x.a, ra);
-        const uint32_t db = upper ? F::sub(rb, x.b) : F::add(x.b, rb);
-        x.a = F::mul(da, w[t]);
-        x.b = F::mul(db, w[t]);
+#pragma unroll
+        for (int h = 0; h < C; ++h) {
+            const uint32_t r = xchg_s(x.v[h], 1u << t, L.lane);
+            const uint32_t d = upper ? F::sub(r, x.v[h]) : F::add(x.v[h], r);
+            x.v[h] = F::mul(d, xw[t]);
+        }
     }
 }
 
 template <int E>
-__device__ inline void p2_nttr(P2 &x, const Lane2<E> &L)
+__device__ inline void pr_nttr(PR<E> &x, const LaneR<E> &L)
 {
-    p2_gs<E>(x, L.gs_e, L);
+    pr_gs<E>(x, L.x_e, L.w_e, L);
 }
 
 template <int E>
-__device__ inline void p2_inttr(const Rsi16Dev &T, P2 &x, const Lane2<E> &L)
+__device__ inline void pr_inttr(const Rsi16Dev &T, PR<E> &x, const LaneR<E> &L)
 {
-    p2_ct<E>(x, L.ct_i, L);
-    x.a = F::mul(x.a, T.ecc_inv);
-    x.b = F::mul(x.b, T.ecc_inv);
+    pr_ct<E>(x, L.x_i, L.w_i, L);
+#pragma unroll
+    for (int h = 0; h < RegShape<E>::C; ++h)
+        x.v[h] = F::mul(x.v[h], T.ecc_inv);
 }
 
 // NTT::_reverse_ntt_vec (ntt.hpp:344-354) in gather form, as reverse_ntt
 template <int E>
-__device__ inline void p2_reverse_ntt(const Rsi16Dev &T, P2 &v, uint32_t shift, const Lane2<E> &L)
+__device__ inline void pr_reverse_ntt(const Rsi16Dev &T, PR<E> &v, uint32_t shift, const LaneR<E> &L)
 {
-    uint32_t out[2];
+    PR<E> out;
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        uint32_t x = p2_gather<E>(v, L.rv_k[h], L);
-        x = F::mul(x, L.rv_w[h]);
-        x = F::mul(x, T.roots_ecc[(shift * L.rv_j[h]) & (E - 1)]);
-        out[h] = (h == 0 && L.gl == 0) ? v.a : x;
+    for (int h = 0; h < RegShape<E>::C; ++h) {
+        const uint32_t i = L.gl + (uint32_t)h * RegShape<E>::G;
+        const uint32_t j = T.rbo_ecc[i];
+        const uint32_t k = T.rbo_ecc[(E - j) & (E - 1)];
+        uint32_t x = pr_gather<E>(v, k, L);
+        x = F::mul(x, T.roots_i_ecc[j]);
+        x = F::mul(x, T.roots_ecc[(shift * j) & (E - 1)]);
+        out.v[h] = i == 0 ? v.v[h] : x;
     }
-    v.a = out[0];
-    v.b = out[1];
+    v = out;
 }
 
 // NTT::poly_inv_mod_xn_rev (ntt.hpp:363-469), as poly_inv_mod_xn_rev
 template <int E>
-__device__ inline bool p2_poly_inv(const Rsi16Dev &T, P2 &g, uint32_t g_len, uint32_t g0, uint32_t a_len, P2 &a,
-                                   const Lane2<E> &L)
+__device__ inline bool pr_poly_inv(const Rsi16Dev &T, PR<E> &g, uint32_t g_len, uint32_t g0, uint32_t a_len, PR<E> &a,
+                                   const LaneR<E> &L)
 {
-    constexpr uint32_t G = Lane2<E>::G;
+    constexpr int C = RegShape<E>::C;
+    constexpr uint32_t G = RegShape<E>::G;
     const uint32_t a0 = T.inv_tab[g0];
-    a.a = a.b = a0;
+#pragma unroll
+    for (int h = 0; h < C; ++h)
+        a.v[h] = a0;
     uint32_t l = 2;
     const uint32_t a_max = a_len;
     while (l < a_len) {
         if (g_len + l - 2 > (uint32_t)E)
             return false;
-        a.a = F::mul(a.a, F::sub(2, F::mul(a.a, g.a)));
-        a.b = F::mul(a.b, F::sub(2, F::mul(a.b, g.b)));
-        p2_inttr<E>(T, a, L);
-        if (L.gl >= l)
-            a.a = 0;
-        if (L.gl + G >= l)
-            a.b = 0;
-        p2_nttr<E>(a, L);
+#pragma unroll
+        for (int h = 0; h < C; ++h)
+            a.v[h] = F::mul(a.v[h], F::sub(2, F::mul(a.v[h], g.v[h])));
+        pr_inttr<E>(T, a, L);
+#pragma unroll
+        for (int h = 0; h < C; ++h)
+            if (L.gl + (uint32_t)h * G >= l)
+                a.v[h] = 0;
+        pr_nttr<E>(a, L);
         l *= 2;
     }
-    g.a = F::mul(a.a, F::sub(2, F::mul(a.a, g.a)));
-    g.b = F::mul(a.b, F::sub(2, F::mul(a.b, g.b)));
-    p2_inttr<E>(T, g, L);
-    uint32_t out[2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t i = L.gl + (h ? G : 0);
+    for (int h = 0; h < C; ++h)
+        g.v[h] = F::mul(a.v[h], F::sub(2, F::mul(a.v[h], g.v[h])));
+    pr_inttr<E>(T, g, L);
+#pragma unroll
+    for (int h = 0; h < C; ++h) {
+        const uint32_t i = L.gl + (uint32_t)h * G;
         const uint32_t src = i < a_max ? a_max - 1 - i : i;
-        const uint32_t v = p2_gather<E>(g, src & (E - 1), L);
-        out[h] = (i < a_len && src < l) ? v : 0u;
+        const uint32_t v = pr_gather<E>(g, src & (E - 1), L);
+        a.v[h] = (i < a_len && src < l) ? v : 0u;
     }
-    a.a = out[0];
-    a.b = out[1];
-    p2_nttr<E>(a, L);
+    pr_nttr<E>(a, L);
     return true;
 }
 
 // NTT::poly_div_ntt (ntt.hpp:471-564), as poly_div_ntt
 template <int E>
-__device__ inline bool p2_poly_div(const Rsi16Dev &T, const P2 &f, uint32_t f_len, const P2 &g, uint32_t g_len,
-                                   uint32_t f0, uint32_t g0, P2 &t, P2 &q, uint32_t *q_len_out, const Lane2<E> &L)
+__device__ inline bool pr_poly_div(const Rsi16Dev &T, const PR<E> &f, uint32_t f_len, const PR<E> &g, uint32_t g_len,
+                                   uint32_t f0, uint32_t g0, PR<E> &t, PR<E> &q, uint32_t *q_len_out,
+                                   const LaneR<E> &L)
 {
-    constexpr uint32_t G = Lane2<E>::G;
+    constexpr int C = RegShape<E>::C;
+    constexpr uint32_t G = RegShape<E>::G;
     const uint32_t q_len = f_len - g_len + 1;
     if (g_len == 0)
         return false;
     if (f_len < g_len) {
-        q.a = q.b = 0;
+#pragma unroll
+        for (int h = 0; h < C; ++h)
+            q.v[h] = 0;
         *q_len_out = 0;
         return true;
     }
     t = g;
-    p2_reverse_ntt<E>(T, t, g_len, L);
-    if (!p2_poly_inv<E>(T, t, g_len, g0, q_len, q, L))
+    pr_reverse_ntt<E>(T, t, g_len, L);
+    if (!pr_poly_inv<E>(T, t, g_len, g0, q_len, q, L))
         return false;
-    t.a = F::mul(F::sub(f.a, f0), q.a);
-    t.b = F::mul(F::sub(f.b, f0), q.b);
-    p2_inttr<E>(T, t, L);
-    const uint32_t right_n = q_len < E - g_len ? q_len : E - g_len;
-    uint32_t out[2];
 #pragma unroll
-    for (int h = 0; h < 2; ++h) {
-        const uint32_t i = L.gl + (h ? G : 0);
+    for (int h = 0; h < C; ++h)
+        t.v[h] = F::mul(F::sub(f.v[h], f0), q.v[h]);
+    pr_inttr<E>(T, t, L);
+    const uint32_t right_n = q_len < E - g_len ? q_len : E - g_len;
+#pragma unroll
+    for (int h = 0; h < C; ++h) {
+        const uint32_t i = L.gl + (uint32_t)h * G;
         const uint32_t src = i < right_n ? g_len + i : (i < q_len ? i - right_n : 0u);
-        const uint32_t v = p2_gather<E>(t, src & (E - 1), L);
-        out[h] = i < q_len ? v : 0u;
+        const uint32_t v = pr_gather<E>(t, src & (E - 1), L);
+        q.v[h] = i < q_len ? v : 0u;
     }
-    q.a = out[0];
-    q.b = out[1];
-    p2_nttr<E>(q, L);
+    pr_nttr<E>(q, L);
     *q_len_out = q_len;
     return true;
 }
 
-// p[(i - 1) mod E] for both of the lane's coefficients
+// RSi16v<1>::_sugiyama (rsi16v_impl.hpp:271-440) in coefficient form, as sugiyama: r1 holds
+// the syndromes on entry and the normalised evaluator on exit. Returns the locator length,
+// or 0xFFFFFFFF where the reference raises.
 template <int E>
-__device__ inline P2 p2_prev(const P2 &p, const Lane2<E> &L)
+__device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, const LaneR<E> &L)
 {
-    constexpr uint32_t G = Lane2<E>::G;
-    const uint32_t src = L.base + (L.gl + G - 1) % G;
-    const uint32_t va = lane_read(p.a, src), vb = lane_read(p.b, src);
-    return P2{L.gl ? va : vb, L.gl ? vb : va};
-}
-
-// RSi16v<1>::_sugiyama (rsi16v_impl.hpp:271-440): r1 holds the syndromes on entry and the
-// normalised evaluator on exit. Returns the locator length, or 0xFFFFFFFF where the
-// reference raises.
-//
-// The reference keeps a1, a2 and r2 as E-point transforms and multiplies pointwise, which
-// is multiplication modulo x^E - 1 of the coefficient vectors. Here every polynomial stays
-// in coefficient form and the same products are formed as cyclic convolutions, so every
-// value is the same field element. In the usual step the quotient has two terms
-// (q_len == 2): poly_inv_mod_xn_rev then reduces to a = (-a0^2 g[g_len - 2],
-// 2 a0 - a0^2 g0) with a0 = 1 / g0 (its Newton loop does not run), two coefficients of
-// (f - f0) * a give the quotient, and the updates are two-tap convolutions. Any other step
-// goes through the transforms exactly as the reference does. The last division, whose
-// result the reference discards, only decides whether it raises (div_ok).
-template <int E>
-__device__ inline uint32_t p2_sugiyama(const Rsi16Dev &T, P2 &a1, P2 &r1, const Lane2<E> &L)
-{
-    constexpr uint32_t G = Lane2<E>::G, mask = E - 1;
-    P2 a2{L.gl == 0 ? 1u : 0u, 0u}, r2 = r1;  // a2: the transform's all-ones vector
-    a1.a = a1.b = 0;
+    constexpr uint32_t G = RegShape<E>::G, mask = E - 1;
+    constexpr int C = RegShape<E>::C;
+    PR<E> a2, r2 = r1;  // a2: the transform's all-ones vector
+#pragma unroll
+    for (int h = 0; h < C; ++h) {
+        a2.v[h] = (h == 0 && L.gl == 0) ? 1u : 0u;
+        a1.v[h] = 0;
+    }
     uint32_t q_len = 0, a2_len = 1, a1_len;
-    uint32_t r1_len = p2_len<E>(r1, E, L);
+    uint32_t r1_len = pr_len<E>(r1, E, L);
     uint32_t r2_len = r1_len;
-    uint32_t r2l = p2_get<E>(r2, 0, L);
+    uint32_t r2l = pr_get<E>(r2, 0, L);
     const uint32_t lanes = r1_len != 0 ? 0xffffffffu : 0u;
     {
-        const uint32_t r1h = p2_get<E>(r1, (r1_len - 1) & mask, L);
-        const uint32_t r1h2 = p2_get<E>(r1, (r1_len - 2) & mask, L);
+        const uint32_t r1h = pr_get<E>(r1, (r1_len - 1) & mask, L);
+        const uint32_t r1h2 = pr_get<E>(r1, (r1_len - 2) & mask, L);
         const uint32_t a11 = s_neg(T.inv_tab[r1h]);
         const uint32_t a10 = F::mul(r1h2, F::mul(a11, a11));
         if constexpr (G > 1) {  // coefficients 0 and 1 live in lanes 0 and 1
             if (L.gl == 0)
-                a1.a = a10;
+                a1.v[0] = a10;
             else if (L.gl == 1)
-                a1.a = a11;
+                a1.v[0] = a11;
         } else {  // G == 1: they are the lane's two coefficients
-            a1.a = a10;
-            a1.b = a11;
+            a1.v[0] = a10;
+            a1.v[1] = a11;
         }
         a1_len = 2 & lanes;
         // tmp[i] = a10 r2[i] + a11 r2[i - 1]  (r2 still holds the original r1)
-        const P2 prev = p2_prev<E>(r2, L);
-        P2 tmp;
-        tmp.a = F::mul(a10, r2.a);
-        if (L.gl >= 1)
-            tmp.a = F::add(tmp.a, F::mul(a11, prev.a));
-        tmp.b = F::add(F::mul(a10, r2.b), F::mul(a11, prev.b));
-        r1_len = p2_len<E>(tmp, E - 1, L) & lanes;
-        r1.a = L.gl < r1_len ? tmp.a : 0u;
-        r1.b = L.gl + G < r1_len ? tmp.b : 0u;
+        const PR<E> prev = pr_prev<E>(r2, L);
+        PR<E> tmp;
+#pragma unroll
+        for (int h = 0; h < C; ++h) {
+            tmp.v[h] = F::mul(a10, r2.v[h]);
+            if (h > 0 || L.gl >= 1)
+                tmp.v[h] = F::add(tmp.v[h], F::mul(a11, prev.v[h]));
+        }
+        r1_len = pr_len<E>(tmp, E - 1, L) & lanes;
+#pragma unroll
+        for (int h = 0; h < C; ++h)
+            r1.v[h] = L.gl + (uint32_t)h * G < r1_len ? tmp.v[h] : 0u;
     }
     if (r1_len >= E / 2) {
         for (uint32_t it = 1; it < E / 2; ++it) {
@@ -1174,15 +1224,15 @@ __device__ inline uint32_t p2_sugiyama(const Rsi16Dev &T, P2 &a1, P2 &r1, const 
                     return 0xffffffffu;
                 break;
             }
-            const uint32_t r1h = p2_get<E>(r1, (r1_len - 1) & mask, L);
-            const uint32_t r1l = p2_get<E>(r1, 0, L);
+            const uint32_t r1h = pr_get<E>(r1, (r1_len - 1) & mask, L);
+            const uint32_t r1l = pr_get<E>(r1, 0, L);
             const uint32_t g_len = r1_len, f_len = r2_len;
-            P2 t, rn;
+            PR<E> t, rn;
             if (f_len == g_len + 1) {  // q_len == 2 (g_len >= E / 2 >= 1 here)
                 q_len = 2;
                 const uint32_t a0 = T.inv_tab[r1h];
                 const uint32_t aa = F::mul(a0, a0);
-                const uint32_t c0 = F::sub(0, F::mul(aa, p2_get<E>(r1, (g_len - 2) & mask, L)));
+                const uint32_t c0 = F::sub(0, F::mul(aa, pr_get<E>(r1, (g_len - 2) & mask, L)));
                 const uint32_t c1 = F::sub(F::mul(2, a0), F::mul(aa, r1h));
                 // quotient coefficients t[idx] of t = (r2 - f0) * (c0 + c1 x) mod x^E - 1
                 const uint32_t right_n = (uint32_t)E - g_len < 2 ? (uint32_t)E - g_len : 2u;
@@ -1192,30 +1242,32 @@ __device__ inline uint32_t p2_sugiyama(const Rsi16Dev &T, P2 &a1, P2 &r1, const 
                 const uint32_t at[4] = {idx0, (idx0 - 1) & mask, idx1, (idx1 - 1) & mask};
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const uint32_t x = p2_get<E>(r2, at[u], L);
+                    const uint32_t x = pr_get<E>(r2, at[u], L);
                     v[u] = at[u] == 0 ? F::sub(x, r2l) : x;
                 }
                 const uint32_t q0 = F::add(F::mul(c0, v[0]), F::mul(c1, v[1]));
                 const uint32_t q1 = F::add(F::mul(c0, v[2]), F::mul(c1, v[3]));
-                const P2 pa = p2_prev<E>(a1, L), pr = p2_prev<E>(r1, L);
-                t.a = F::sub(a2.a, F::add(F::mul(q0, a1.a), F::mul(q1, pa.a)));
-                t.b = F::sub(a2.b, F::add(F::mul(q0, a1.b), F::mul(q1, pa.b)));
-                rn.a = F::sub(r2.a, F::add(F::mul(q0, r1.a), F::mul(q1, pr.a)));
-                rn.b = F::sub(r2.b, F::add(F::mul(q0, r1.b), F::mul(q1, pr.b)));
+                const PR<E> pa = pr_prev<E>(a1, L), pr = pr_prev<E>(r1, L);
+#pragma unroll
+                for (int h = 0; h < C; ++h) {
+                    t.v[h] = F::sub(a2.v[h], F::add(F::mul(q0, a1.v[h]), F::mul(q1, pa.v[h])));
+                    rn.v[h] = F::sub(r2.v[h], F::add(F::mul(q0, r1.v[h]), F::mul(q1, pr.v[h])));
+                }
             } else {  // through the transforms
-                P2 R1 = r1, R2 = r2, A1 = a1, A2 = a2, scratch, Q;
-                p2_nttr<E>(R1, L);
-                p2_nttr<E>(R2, L);
-                p2_nttr<E>(A1, L);
-                p2_nttr<E>(A2, L);
-                if (!p2_poly_div<E>(T, R2, r2_len, R1, r1_len, r2l, r1h, scratch, Q, &q_len, L))
+                PR<E> R1 = r1, R2 = r2, A1 = a1, A2 = a2, scratch, Q;
+                pr_nttr<E>(R1, L);
+                pr_nttr<E>(R2, L);
+                pr_nttr<E>(A1, L);
+                pr_nttr<E>(A2, L);
+                if (!pr_poly_div<E>(T, R2, r2_len, R1, r1_len, r2l, r1h, scratch, Q, &q_len, L))
                     return 0xffffffffu;
-                t.a = F::sub(A2.a, F::mul(Q.a, A1.a));
-                t.b = F::sub(A2.b, F::mul(Q.b, A1.b));
-                rn.a = F::sub(R2.a, F::mul(Q.a, R1.a));
-                rn.b = F::sub(R2.b, F::mul(Q.b, R1.b));
-                p2_inttr<E>(T, t, L);
-                p2_inttr<E>(T, rn, L);
+#pragma unroll
+                for (int h = 0; h < C; ++h) {
+                    t.v[h] = F::sub(A2.v[h], F::mul(Q.v[h], A1.v[h]));
+                    rn.v[h] = F::sub(R2.v[h], F::mul(Q.v[h], R1.v[h]));
+                }
+                pr_inttr<E>(T, t, L);
+                pr_inttr<E>(T, rn, L);
             }
             uint32_t t_len = q_len + a1_len - 1;
             t_len = a2_len > t_len ? a2_len : t_len;
@@ -1229,15 +1281,16 @@ __device__ inline uint32_t p2_sugiyama(const Rsi16Dev &T, P2 &a1, P2 &r1, const 
             a1 = t;
             r1 = rn;
             a1_len = t_len;
-            r1_len = p2_len<E>(rn, nq_len, L);
+            r1_len = pr_len<E>(rn, nq_len, L);
         }
     }
-    a1_len = p2_len<E>(a1, a1_len & lanes, L);
-    const uint32_t inv0 = T.inv_tab[p2_get<E>(a1, 0, L)];
-    a1.a = F::mul(a1.a, inv0);
-    a1.b = F::mul(a1.b, inv0);
-    r1.a = F::mul(r1.a, inv0);
-    r1.b = F::mul(r1.b, inv0);
+    a1_len = pr_len<E>(a1, a1_len & lanes, L);
+    const uint32_t inv0 = T.inv_tab[pr_get<E>(a1, 0, L)];
+#pragma unroll
+    for (int h = 0; h < C; ++h) {
+        a1.v[h] = F::mul(a1.v[h], inv0);
+        r1.v[h] = F::mul(r1.v[h], inv0);
+    }
     return a1_len & lanes;
 }
 
@@ -1246,29 +1299,35 @@ __device__ inline uint32_t p2_sugiyama(const Rsi16Dev &T, P2 &a1, P2 &r1, const 
 constexpr uint32_t REG_WORDS = 4;
 
 template <int E>
-__device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t *e, uint32_t *base, const Lane2<E> &L,
+__device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t *e, uint32_t *base, const LaneR<E> &L,
                                         uint8_t *st)
 {
-    constexpr uint32_t G = Lane2<E>::G;
-    const uint32_t i0 = L.gl, i1 = L.gl + G;
+    constexpr uint32_t G = RegShape<E>::G;
+    constexpr int C = RegShape<E>::C;
     // NTT::pntt (ntt.hpp:133-152) of m || e, one chunk of E symbols at a time
-    P2 syn{0, 0};
+    PR<E> syn;
     const uint32_t nb = T.n / E;
     for (uint32_t c = 0; c < nb; ++c) {
-        const uint32_t x0 = c * E + i0, x1 = c * E + i1;
-        P2 x{x0 < T.k ? (uint32_t)m[x0] : (uint32_t)e[x0 - T.k], x1 < T.k ? (uint32_t)m[x1] : (uint32_t)e[x1 - T.k]};
-        p2_ct<E>(x, L.ct_e, L);
-        const uint32_t va = F::mul(x.a, T.pntt_shift[x0]), vb = F::mul(x.b, T.pntt_shift[x1]);
-        syn.a = c ? F::add(syn.a, va) : va;
-        syn.b = c ? F::add(syn.b, vb) : vb;
+        PR<E> x;
+#pragma unroll
+        for (int h = 0; h < C; ++h) {
+            const uint32_t i = c * E + L.gl + (uint32_t)h * G;
+            x.v[h] = i < T.k ? (uint32_t)m[i] : (uint32_t)e[i - T.k];
+        }
+        pr_ct<E>(x, L.x_e, L.w_e, L);
+#pragma unroll
+        for (int h = 0; h < C; ++h) {
+            const uint32_t v = F::mul(x.v[h], T.pntt_shift[c * E + L.gl + (uint32_t)h * G]);
+            syn.v[h] = c ? F::add(syn.v[h], v) : v;
+        }
     }
-    if (p2_len<E>(syn, E, L) == 0) {
+    if (pr_len<E>(syn, E, L) == 0) {
         if (L.gl == 0)
             *st = FFRS_RSI16_NO_ERRORS;
         return;
     }
-    P2 loc{0, 0};
-    const uint32_t Ln = p2_sugiyama<E>(T, loc, syn, L);
+    PR<E> loc;
+    const uint32_t Ln = pr_sugiyama<E>(T, loc, syn, L);
     if (Ln == 0xffffffffu) {
         if (L.gl == 0)
             *st = FFRS_RSI16_RAISES;
@@ -1276,13 +1335,14 @@ __device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t
     }
     const uint32_t eval_len = E - Ln + 1;
     // _deriv (rsi16v_impl.hpp:608-616): der[i] = (i + 1) loc[i + 1]
-    P2 der;
-    {
-        const uint32_t n0 = p2_gather<E>(loc, (i0 + 1) & (E - 1), L), n1 = p2_gather<E>(loc, (i1 + 1) & (E - 1), L);
-        der.a = i0 + 1 < (uint32_t)E ? F::mul(n0, i0 + 1) : 0u;
-        der.b = i1 + 1 < (uint32_t)E ? F::mul(n1, i1 + 1) : 0u;
+    PR<E> der;
+#pragma unroll
+    for (int h = 0; h < C; ++h) {
+        const uint32_t i = L.gl + (uint32_t)h * G;
+        const uint32_t nx = pr_gather<E>(loc, (i + 1) & (E - 1), L);
+        der.v[h] = i + 1 < (uint32_t)E ? F::mul(nx, i + 1) : 0u;
     }
-    const uint32_t der_len = p2_len<E>(der, E, L);
+    const uint32_t der_len = pr_len<E>(der, E, L);
     if (Ln > (uint32_t)E) {  // py_assert at rsi16v_impl.hpp:454
         if (L.gl == 0)
             *st = FFRS_RSI16_RAISES;
@@ -1291,28 +1351,29 @@ __device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t
     const uint32_t lanes = Ln != 0 ? 0xffffffffu : 0u;
     const uint32_t dmax = der_len & lanes, emax = eval_len & lanes;
     uint32_t *ev = base, *dv = base + E, *list = base + 2 * E, *nroots = base + 3 * E;
-    ev[i0] = syn.a;
-    ev[i1] = syn.b;
-    dv[i0] = der.a;
-    dv[i1] = der.b;
+#pragma unroll
+    for (int h = 0; h < C; ++h) {
+        ev[L.gl + (uint32_t)h * G] = syn.v[h];
+        dv[L.gl + (uint32_t)h * G] = der.v[h];
+    }
     if (L.gl == 0)
         *nroots = 0;
     wave_sync();
     // NTT::pintt_ecc (ntt.hpp:206-220) chunk by chunk; zeros listed, corrected afterwards
     for (uint32_t c = 0; c < nb; ++c) {
-        P2 ch{F::mul(loc.a, T.pintt_shift[c * E + i0]), F::mul(loc.b, T.pintt_shift[c * E + i1])};
-        p2_gs<E>(ch, L.gs_i, L);
+        PR<E> ch;
+#pragma unroll
+        for (int h = 0; h < C; ++h)
+            ch.v[h] = F::mul(loc.v[h], T.pintt_shift[c * E + L.gl + (uint32_t)h * G]);
+        pr_gs<E>(ch, L.x_i, L.w_i, L);
         if (lanes) {
-            if (ch.a == 0) {
-                const uint32_t slot = atomicAdd(nroots, 1u);
-                if (slot < (uint32_t)E)
-                    list[slot] = c * E + i0;
-            }
-            if (ch.b == 0) {
-                const uint32_t slot = atomicAdd(nroots, 1u);
-                if (slot < (uint32_t)E)
-                    list[slot] = c * E + i1;
-            }
+#pragma unroll
+            for (int h = 0; h < C; ++h)
+                if (ch.v[h] == 0) {
+                    const uint32_t slot = atomicAdd(nroots, 1u);
+                    if (slot < (uint32_t)E)
+                        list[slot] = c * E + L.gl + (uint32_t)h * G;
+                }
         }
     }
     wave_sync();
@@ -1341,12 +1402,12 @@ __global__ __launch_bounds__(64) void rsi16_repair_reg_kernel(uint16_t *__restri
                                                                uint8_t *__restrict__ status, Rsi16Dev T)
 {
     extern __shared__ uint32_t s_mem[];
-    constexpr uint32_t G = Lane2<E>::G, cpw = 64 / G;
+    constexpr uint32_t G = RegShape<E>::G, cpw = 64 / G;
     const uint32_t grp = threadIdx.x / G;
     const uint32_t cw = blockIdx.x * cpw + grp;
     if (cw >= count)
         return;
-    Lane2<E> L;
+    LaneR<E> L;
     L.init(T, threadIdx.x);
     repair_group_reg<E>(T, msgs + (size_t)cw * T.k, eccs + (size_t)cw * E, s_mem + grp * REG_WORDS * E, L,
                         status + cw);
@@ -1526,7 +1587,7 @@ __global__ void rsi16_plan_reset_kernel(uint32_t *queue) { *queue = 0; }
 template <int E>
 __device__ inline void plan_repair_reg(const Rsi16Dev &T, uint16_t *m, uint16_t *e, uint32_t *base, uint8_t *st)
 {
-    Lane2<E> L;
+    LaneR<E> L;
     L.init(T, threadIdx.x);
     repair_group_reg<E>(T, m, e, base, L, st);
 }
@@ -1693,7 +1754,7 @@ int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t 
 template <int E>
 int launch_repair_reg(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count, uint8_t *d_status)
 {
-    constexpr uint32_t cpw = 128 / E;
+    constexpr uint32_t cpw = 64 / RegShape<E>::G;
     const size_t lds = (size_t)cpw * REG_WORDS * E * 4;
     if (int r = allow_lds(rsi16_repair_reg_kernel<E>, lds))
         return r;
@@ -1709,7 +1770,7 @@ int launch_repair(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count
     marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
-    if (rs->E >= 2 && rs->E <= 128 && rs->g_rep == rs->E / 2) {  // registers, two coefficients per lane
+    if (rs->E >= 2 && rs->E <= 256 && rs->g_rep == std::min<uint32_t>(64, rs->E / 2)) {  // registers
         switch (rs->E) {
         case 2:
             return launch_repair_reg<2>(rs, d_msg, d_ecc, count, d_status);
@@ -1723,8 +1784,10 @@ int launch_repair(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count
             return launch_repair_reg<32>(rs, d_msg, d_ecc, count, d_status);
         case 64:
             return launch_repair_reg<64>(rs, d_msg, d_ecc, count, d_status);
-        default:
+        case 128:
             return launch_repair_reg<128>(rs, d_msg, d_ecc, count, d_status);
+        default:
+            return launch_repair_reg<256>(rs, d_msg, d_ecc, count, d_status);
         }
     }
     uint32_t g = rs->g_rep, cpw = 64 / g;
