@@ -244,7 +244,7 @@ def run_plain(name, args):
             rep_names[1] = "rs256_bm_kernel"
     else:
         enc_names = ["rsi16_encode_lane_kernel" if ecc <= 32 else "rsi16_encode_kernel"]
-        rep_names = ["rsi16_repair_reg_kernel" if ecc <= 128 else "rsi16_repair_kernel"]
+        rep_names = ["rsi16_repair_reg_kernel" if ecc <= 256 else "rsi16_repair_kernel"]
     ke_ms = np.mean(np.array(kes), axis=0)
     kr_ms = np.mean(np.array(krs), axis=0)
     if len(ke_ms) != len(enc_names) or len(kr_ms) != len(rep_names):
