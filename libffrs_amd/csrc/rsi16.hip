@@ -23,6 +23,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 
 namespace ffrs {
@@ -353,23 +354,94 @@ __device__ inline void gs_reg(uint32_t (&x)[E], const uint32_t *__restrict__ roo
     }
 }
 
-// encode_group with one codeword per lane: the message is read in 16-byte pieces (for
-// ecc_len >= 8 a codeword row and every chunk of it start 16-byte aligned), the twiddles
-// and shifts are wave-uniform scalar loads.
-template <class O, int E>
-__global__ __launch_bounds__(256) void rsi16_encode_lane_kernel(const uint16_t *__restrict__ msgs, uint32_t count,
-                                                                uint16_t *__restrict__ out, Rsi16Dev T)
+// encode_group for one codeword in one lane: the twiddles and shifts are wave-uniform
+// scalar loads; with ALIGNED (16-byte aligned rows, ecc_len >= 8) the message is read in
+// 16-byte pieces (every chunk of such a row starts 16-byte aligned) and the parity written
+// the same way.
+template <class O, int E, bool ALIGNED>
+__device__ inline void encode_lane(const Rsi16Dev &T, const uint16_t *__restrict__ m, uint16_t *__restrict__ o)
 {
-    const uint32_t cw = blockIdx.x * 256 + threadIdx.x;
-    if (cw >= count)
-        return;
     const uint32_t nb = T.n / E;
-    const uint16_t *m = msgs + (size_t)cw * T.k;
+    // message chunks loaded PF at a time before any is transformed (for short ecc_len each
+    // chunk is little work, and long rows would otherwise wait on one load per chunk)
+    constexpr uint32_t PF = E <= 8 ? 8 : (E <= 16 ? 4 : 2);
     uint32_t acc[E];
-    for (uint32_t c = 0; c + 1 < nb; ++c) {
+    for (uint32_t c0 = 0; c0 + 1 < nb; c0 += PF) {
+        uint32_t xs[PF][E];
+#pragma unroll
+        for (uint32_t u = 0; u < PF; ++u) {
+            if (c0 + u + 1 < nb) {
+                const uint16_t *mc = m + (c0 + u) * E;
+                if constexpr (ALIGNED) {
+#pragma unroll
+                    for (int q = 0; q < E / 8; ++q) {
+                        const uint4 v = reinterpret_cast<const uint4 *>(mc)[q];
+                        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+                        for (int h = 0; h < 4; ++h) {
+                            xs[u][8 * q + 2 * h] = w[h] & 0xffffu;
+                            xs[u][8 * q + 2 * h + 1] = w[h] >> 16;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int j = 0; j < E; ++j)
+                        xs[u][j] = mc[j];
+                }
+            }
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < PF; ++u) {
+            const uint32_t c = c0 + u;
+            if (c + 1 < nb) {
+                ct_residue_reg<O, E>(xs[u], T.roots_ecc);
+                const uint32_t *sh = T.pntt_shift + c * E;
+#pragma unroll
+                for (int j = 0; j < E; ++j) {
+                    const uint32_t v = O::mul_residue(O::mod_p(xs[u][j]), sh[j]);
+                    acc[j] = c ? acc[j] + v : v;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+        acc[j] = O::mul(O::mod_p(acc[j]), T.mix[j]);
+    gs_reg<O, E>(acc, T.roots_i_ecc);
+    if constexpr (ALIGNED) {
+#pragma unroll
+        for (int q = 0; q < E / 8; ++q) {
+            uint4 v;
+            v.x = (acc[8 * q + 0] & 0xffffu) | (acc[8 * q + 1] << 16);
+            v.y = (acc[8 * q + 2] & 0xffffu) | (acc[8 * q + 3] << 16);
+            v.z = (acc[8 * q + 4] & 0xffffu) | (acc[8 * q + 5] << 16);
+            v.w = (acc[8 * q + 6] & 0xffffu) | (acc[8 * q + 7] << 16);
+            reinterpret_cast<uint4 *>(o)[q] = v;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            o[j] = (uint16_t)(acc[j] & 0xffff);
+    }
+}
+
+// encode_lane with the message chunks of one codeword spread over L lanes (gl = lane in the
+// group): each lane sums its chunks' shifted residues, the L partial sums are added across
+// the group (integer sums mod 2^32, so the order does not change them), and every lane
+// finishes the same words and stores its share. For long rows with short ecc_len.
+template <class O, int E, bool ALIGNED>
+__device__ inline void encode_split(const Rsi16Dev &T, const uint16_t *__restrict__ m, uint16_t *__restrict__ o,
+                                    uint32_t gl, uint32_t L)
+{
+    const uint32_t nb = T.n / E;
+    uint32_t acc[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j)
+        acc[j] = 0;
+    for (uint32_t c = gl; c + 1 < nb; c += L) {
         uint32_t x[E];
         const uint16_t *mc = m + c * E;
-        if constexpr (E >= 8) {
+        if constexpr (ALIGNED) {
 #pragma unroll
             for (int q = 0; q < E / 8; ++q) {
                 const uint4 v = reinterpret_cast<const uint4 *>(mc)[q];
@@ -388,31 +460,32 @@ __global__ __launch_bounds__(256) void rsi16_encode_lane_kernel(const uint16_t *
         ct_residue_reg<O, E>(x, T.roots_ecc);
         const uint32_t *sh = T.pntt_shift + c * E;
 #pragma unroll
-        for (int j = 0; j < E; ++j) {
-            const uint32_t v = O::mul_residue(O::mod_p(x[j]), sh[j]);
-            acc[j] = c ? acc[j] + v : v;
-        }
+        for (int j = 0; j < E; ++j)
+            acc[j] += O::mul_residue(O::mod_p(x[j]), sh[j]);
     }
+    for (uint32_t s = 1; s < L; s <<= 1)
+#pragma unroll
+        for (int j = 0; j < E; ++j)
+            acc[j] += (uint32_t)__shfl_xor((int)acc[j], (int)s);
 #pragma unroll
     for (int j = 0; j < E; ++j)
         acc[j] = O::mul(O::mod_p(acc[j]), T.mix[j]);
     gs_reg<O, E>(acc, T.roots_i_ecc);
-    uint16_t *o = out + (size_t)cw * E;
-    if constexpr (E >= 8) {
 #pragma unroll
-        for (int q = 0; q < E / 8; ++q) {
-            uint4 v;
-            v.x = (acc[8 * q + 0] & 0xffffu) | (acc[8 * q + 1] << 16);
-            v.y = (acc[8 * q + 2] & 0xffffu) | (acc[8 * q + 3] << 16);
-            v.z = (acc[8 * q + 4] & 0xffffu) | (acc[8 * q + 5] << 16);
-            v.w = (acc[8 * q + 6] & 0xffffu) | (acc[8 * q + 7] << 16);
-            reinterpret_cast<uint4 *>(o)[q] = v;
-        }
-    } else {
-#pragma unroll
-        for (int j = 0; j < E; ++j)
+    for (int j = 0; j < E; ++j)
+        if ((uint32_t)j % L == gl)
             o[j] = (uint16_t)(acc[j] & 0xffff);
-    }
+}
+
+// One codeword per lane (ecc_len <= 32; rows 16-byte aligned for ecc_len >= 8).
+template <class O, int E>
+__global__ __launch_bounds__(256) void rsi16_encode_lane_kernel(const uint16_t *__restrict__ msgs, uint32_t count,
+                                                                uint16_t *__restrict__ out, Rsi16Dev T)
+{
+    const uint32_t cw = blockIdx.x * 256 + threadIdx.x;
+    if (cw >= count)
+        return;
+    encode_lane<O, E, (E >= 8)>(T, msgs + (size_t)cw * T.k, out + (size_t)cw * E);
 }
 
 // ---------------------------------------------------------------- syndromes
@@ -1304,22 +1377,33 @@ __device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t
 {
     constexpr uint32_t G = RegShape<E>::G;
     constexpr int C = RegShape<E>::C;
-    // NTT::pntt (ntt.hpp:133-152) of m || e, one chunk of E symbols at a time
+    // NTT::pntt (ntt.hpp:133-152) of m || e, one chunk of E symbols at a time; the symbols
+    // and shifts of PF chunks are loaded before any is transformed, so their latencies overlap
+    constexpr uint32_t PF = C >= 4 ? 1 : 4;
     PR<E> syn;
     const uint32_t nb = T.n / E;
-    for (uint32_t c = 0; c < nb; ++c) {
-        PR<E> x;
+    for (uint32_t c0 = 0; c0 < nb; c0 += PF) {
+        PR<E> x[PF], sh[PF];
 #pragma unroll
-        for (int h = 0; h < C; ++h) {
-            const uint32_t i = c * E + L.gl + (uint32_t)h * G;
-            x.v[h] = i < T.k ? (uint32_t)m[i] : (uint32_t)e[i - T.k];
-        }
-        pr_ct<E>(x, L.x_e, L.w_e, L);
+        for (uint32_t u = 0; u < PF; ++u)
+            if (c0 + u < nb) {
 #pragma unroll
-        for (int h = 0; h < C; ++h) {
-            const uint32_t v = F::mul(x.v[h], T.pntt_shift[c * E + L.gl + (uint32_t)h * G]);
-            syn.v[h] = c ? F::add(syn.v[h], v) : v;
-        }
+                for (int h = 0; h < C; ++h) {
+                    const uint32_t i = (c0 + u) * E + L.gl + (uint32_t)h * G;
+                    x[u].v[h] = i < T.k ? (uint32_t)m[i] : (uint32_t)e[i - T.k];
+                    sh[u].v[h] = T.pntt_shift[i];
+                }
+            }
+#pragma unroll
+        for (uint32_t u = 0; u < PF; ++u)
+            if (c0 + u < nb) {
+                pr_ct<E>(x[u], L.x_e, L.w_e, L);
+#pragma unroll
+                for (int h = 0; h < C; ++h) {
+                    const uint32_t v = F::mul(x[u].v[h], sh[u].v[h]);
+                    syn.v[h] = c0 + u ? F::add(syn.v[h], v) : v;
+                }
+            }
     }
     if (pr_len<E>(syn, E, L) == 0) {
         if (L.gl == 0)
@@ -1359,22 +1443,36 @@ __device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t
     if (L.gl == 0)
         *nroots = 0;
     wave_sync();
-    // NTT::pintt_ecc (ntt.hpp:206-220) chunk by chunk; zeros listed, corrected afterwards
-    for (uint32_t c = 0; c < nb; ++c) {
-        PR<E> ch;
+    // NTT::pintt_ecc (ntt.hpp:206-220) chunk by chunk, shifts of PF chunks loaded ahead;
+    // zeros listed, corrected afterwards
+    for (uint32_t c0 = 0; c0 < nb; c0 += PF) {
+        PR<E> sh[PF];
 #pragma unroll
-        for (int h = 0; h < C; ++h)
-            ch.v[h] = F::mul(loc.v[h], T.pintt_shift[c * E + L.gl + (uint32_t)h * G]);
-        pr_gs<E>(ch, L.x_i, L.w_i, L);
-        if (lanes) {
+        for (uint32_t u = 0; u < PF; ++u)
+            if (c0 + u < nb) {
 #pragma unroll
-            for (int h = 0; h < C; ++h)
-                if (ch.v[h] == 0) {
-                    const uint32_t slot = atomicAdd(nroots, 1u);
-                    if (slot < (uint32_t)E)
-                        list[slot] = c * E + L.gl + (uint32_t)h * G;
+                for (int h = 0; h < C; ++h)
+                    sh[u].v[h] = T.pintt_shift[(c0 + u) * E + L.gl + (uint32_t)h * G];
+            }
+#pragma unroll
+        for (uint32_t u = 0; u < PF; ++u)
+            if (c0 + u < nb) {
+                const uint32_t c = c0 + u;
+                PR<E> ch;
+#pragma unroll
+                for (int h = 0; h < C; ++h)
+                    ch.v[h] = F::mul(loc.v[h], sh[u].v[h]);
+                pr_gs<E>(ch, L.x_i, L.w_i, L);
+                if (lanes) {
+#pragma unroll
+                    for (int h = 0; h < C; ++h)
+                        if (ch.v[h] == 0) {
+                            const uint32_t slot = atomicAdd(nroots, 1u);
+                            if (slot < (uint32_t)E)
+                                list[slot] = c * E + L.gl + (uint32_t)h * G;
+                        }
                 }
-        }
+            }
     }
     wave_sync();
     const uint32_t total = *nroots;
@@ -1579,20 +1677,22 @@ struct Rsi16PlanChunk {
 
 struct Rsi16PlanClass {
     Rsi16Dev T;
-    uint32_t g;
+    uint32_t g;      // lanes per codeword of the LDS code
+    uint32_t l_enc;  // lanes per codeword of the per-lane encode code (encode_split when > 1)
 };
 
-__global__ void rsi16_plan_reset_kernel(uint32_t *queue) { *queue = 0; }
-
-template <int E>
-__device__ inline void plan_repair_reg(const Rsi16Dev &T, uint16_t *m, uint16_t *e, uint32_t *base, uint8_t *st)
+// zeroes the work counters of a plan's launches (one per chunk range)
+__global__ void rsi16_plan_reset_kernel(uint32_t *queue, uint32_t n)
 {
-    LaneR<E> L;
-    L.init(T, threadIdx.x);
-    repair_group_reg<E>(T, m, e, base, L, st);
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x)
+        queue[i] = 0;
 }
 
-template <int MODE>  // 0: encode (vector words), 1: encode (scalar words), 2: repair
+// One persistent launch per chunk range: every wave takes the next chunk of its range from
+// the range's counter. E = 0: any ecc_len, one codeword per lane group with the polynomials
+// in LDS (repair_group / encode_group). E > 0, repair: the register code of
+// rsi16_repair_reg_kernel<E>. E > 0, encode: one codeword per lane (encode_lane).
+template <int MODE, int E>  // MODE 0: encode (vector words), 1: encode (scalar words), 2: repair
 __global__ __launch_bounds__(64) void rsi16_plan_kernel(uint16_t *__restrict__ msgs, uint16_t *__restrict__ eccs,
                                                          uint8_t *__restrict__ status,
                                                          const Rsi16PlanItem *__restrict__ items,
@@ -1610,50 +1710,64 @@ __global__ __launch_bounds__(64) void rsi16_plan_kernel(uint16_t *__restrict__ m
             break;
         const Rsi16PlanChunk ch = chunks[c];
         const Rsi16PlanClass &cl = classes[ch.cls];
-        const uint32_t g = cl.g, E = cl.T.E;
-        Grp G{threadIdx.x % g, g};
-        const uint32_t grp = threadIdx.x / g;
-        if (grp < ch.count) {
-            const Rsi16PlanItem it = items[ch.first + grp];
-            if constexpr (MODE == 2) {
-                uint16_t *m = msgs + it.msg_off, *e = eccs + it.ecc_off;
-                uint32_t *base = s_mem + grp * REPAIR_WORDS * E;
-                if (E <= 128 && g == E / 2) {  // the register kernel's code, as rsi16_repair_reg_kernel
-                    switch (E) {
-                    case 2:
-                        plan_repair_reg<2>(cl.T, m, e, base, status + it.idx);
-                        break;
-                    case 4:
-                        plan_repair_reg<4>(cl.T, m, e, base, status + it.idx);
-                        break;
-                    case 8:
-                        plan_repair_reg<8>(cl.T, m, e, base, status + it.idx);
-                        break;
-                    case 16:
-                        plan_repair_reg<16>(cl.T, m, e, base, status + it.idx);
-                        break;
-                    case 32:
-                        plan_repair_reg<32>(cl.T, m, e, base, status + it.idx);
-                        break;
-                    case 64:
-                        plan_repair_reg<64>(cl.T, m, e, base, status + it.idx);
-                        break;
-                    default:
-                        plan_repair_reg<128>(cl.T, m, e, base, status + it.idx);
-                        break;
+        if constexpr (E == 0) {
+            const uint32_t g = cl.g, Ec = cl.T.E;
+            Grp G{threadIdx.x % g, g};
+            const uint32_t grp = threadIdx.x / g;
+            if (grp < ch.count) {
+                const Rsi16PlanItem it = items[ch.first + grp];
+                if constexpr (MODE == 2)
+                    repair_group(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * REPAIR_WORDS * Ec, G,
+                                 status + it.idx);
+                else if constexpr (MODE == 0)
+                    encode_group<VectorOps>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * 2 * Ec, G);
+                else
+                    encode_group<ScalarOps>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * 2 * Ec, G);
+            }
+        } else if constexpr (MODE == 2) {
+            constexpr uint32_t G = RegShape<E>::G;
+            const uint32_t grp = threadIdx.x / G;
+            if (grp < ch.count) {
+                const Rsi16PlanItem it = items[ch.first + grp];
+                LaneR<E> L;
+                L.init(cl.T, threadIdx.x);
+                repair_group_reg<E>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * REG_WORDS * E, L,
+                                    status + it.idx);
+            }
+        } else {
+            using O = typename std::conditional<MODE == 0, VectorOps, ScalarOps>::type;
+            const uint32_t L = cl.l_enc, grp = threadIdx.x / L, gl = threadIdx.x % L;
+            if (grp < ch.count) {
+                const Rsi16PlanItem it = items[ch.first + grp];
+                const uint16_t *m = msgs + it.msg_off;
+                uint16_t *o = eccs + it.ecc_off;
+                bool aligned = false;
+                if constexpr (E >= 8)
+                    aligned = (((uintptr_t)m | (uintptr_t)o) & 15) == 0;
+                if (L == 1) {
+                    if constexpr (E >= 8) {
+                        if (aligned)
+                            encode_lane<O, E, true>(cl.T, m, o);
+                        else
+                            encode_lane<O, E, false>(cl.T, m, o);
+                    } else {
+                        encode_lane<O, E, false>(cl.T, m, o);
                     }
                 } else {
-                    repair_group(cl.T, m, e, base, G, status + it.idx);
+                    if constexpr (E >= 8) {
+                        if (aligned)
+                            encode_split<O, E, true>(cl.T, m, o, gl, L);
+                        else
+                            encode_split<O, E, false>(cl.T, m, o, gl, L);
+                    } else {
+                        encode_split<O, E, false>(cl.T, m, o, gl, L);
+                    }
                 }
-            } else if constexpr (MODE == 0)
-                encode_group<VectorOps>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * 2 * E, G);
-            else
-                encode_group<ScalarOps>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * 2 * E, G);
+            }
         }
         __syncthreads();
     }
 }
-
 
 }  // namespace ffrs
 
@@ -2261,10 +2375,17 @@ struct ffrs_rsi16_plan {
     ffrs_ctx *ctx = nullptr;
     std::vector<ffrs_rsi16 *> codecs;
     Rsi16PlanItem *d_items = nullptr;
-    Rsi16PlanChunk *d_chunks = nullptr;
+    Rsi16PlanChunk *d_chunks = nullptr;      // repair chunks, then encode chunks
     Rsi16PlanClass *d_classes = nullptr;
-    uint32_t *d_queue = nullptr;
-    uint32_t count = 0, nchunks = 0, lds_enc = 0, lds_rep = 0;
+    uint32_t *d_queue = nullptr;             // one counter per range
+    struct Range {
+        uint32_t key;    // kernel specialisation: ecc_len, or 0 for the LDS code
+        uint32_t first;  // first chunk in d_chunks
+        uint32_t n;      // chunks
+        uint32_t lds;    // bytes of dynamic LDS
+    };
+    std::vector<Range> rep_ranges, enc_ranges;
+    uint32_t count = 0, nchunks = 0;
     uint64_t msg_end = 0, ecc_end = 0;
     bool use_graph = false;
     struct Graph {
@@ -2290,19 +2411,69 @@ void rsi16_plan_free(ffrs_rsi16_plan *p)
     delete p;
 }
 
+template <int MODE, int E>
+int rsi16_plan_launch_range(ffrs_rsi16_plan *p, const ffrs_rsi16_plan::Range &r, uint32_t *queue, uint16_t *msgs,
+                            uint16_t *eccs, uint8_t *status)
+{
+    ffrs_ctx *ctx = p->ctx;
+    auto kern = rsi16_plan_kernel<MODE, E>;
+    if (int rc = allow_lds(kern, r.lds))
+        return rc;
+    int per_cu = (int)std::max<size_t>(1, std::min<size_t>(32, (160 * 1024) / std::max<size_t>(r.lds, 1)));
+    int grid = persistent_grid(ctx, per_cu, r.n);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), r.lds, ctx->stream, msgs, eccs, status, p->d_items,
+                       p->d_chunks + r.first, r.n, p->d_classes, queue);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+template <int MODE>
+int rsi16_plan_launch_key(ffrs_rsi16_plan *p, const ffrs_rsi16_plan::Range &r, uint32_t *queue, uint16_t *msgs,
+                          uint16_t *eccs, uint8_t *status)
+{
+    switch (r.key) {
+    case 2:
+        return rsi16_plan_launch_range<MODE, 2>(p, r, queue, msgs, eccs, status);
+    case 4:
+        return rsi16_plan_launch_range<MODE, 4>(p, r, queue, msgs, eccs, status);
+    case 8:
+        return rsi16_plan_launch_range<MODE, 8>(p, r, queue, msgs, eccs, status);
+    case 16:
+        return rsi16_plan_launch_range<MODE, 16>(p, r, queue, msgs, eccs, status);
+    case 32:
+        return rsi16_plan_launch_range<MODE, 32>(p, r, queue, msgs, eccs, status);
+    default:
+        break;
+    }
+    if constexpr (MODE == 2) {
+        switch (r.key) {
+        case 64:
+            return rsi16_plan_launch_range<2, 64>(p, r, queue, msgs, eccs, status);
+        case 128:
+            return rsi16_plan_launch_range<2, 128>(p, r, queue, msgs, eccs, status);
+        case 256:
+            return rsi16_plan_launch_range<2, 256>(p, r, queue, msgs, eccs, status);
+        default:
+            break;
+        }
+    }
+    return rsi16_plan_launch_range<MODE, 0>(p, r, queue, msgs, eccs, status);
+}
+
 int rsi16_plan_enqueue(ffrs_rsi16_plan *p, int mode, uint16_t *msgs, uint16_t *eccs, uint8_t *status)
 {
     ffrs_ctx *ctx = p->ctx;
-    size_t lds = mode == 2 ? p->lds_rep : p->lds_enc;
-    auto kern = mode == 2 ? rsi16_plan_kernel<2> : mode == 0 ? rsi16_plan_kernel<0> : rsi16_plan_kernel<1>;
-    if (int r = allow_lds(kern, lds))
-        return r;
-    int per_cu = (int)std::max<size_t>(1, std::min<size_t>(32, (160 * 1024) / std::max<size_t>(lds, 1)));
-    int grid = persistent_grid(ctx, per_cu, p->nchunks);
-    hipLaunchKernelGGL(rsi16_plan_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, p->d_queue);
-    hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, ctx->stream, msgs, eccs, status, p->d_items, p->d_chunks,
-                       p->nchunks, p->d_classes, p->d_queue);
+    const auto &ranges = mode == 2 ? p->rep_ranges : p->enc_ranges;
+    uint32_t *queue = p->d_queue + (mode == 2 ? 0 : p->rep_ranges.size());
+    hipLaunchKernelGGL(rsi16_plan_reset_kernel, dim3(1), dim3(64), 0, ctx->stream, queue, (uint32_t)ranges.size());
     FFRS_HIP(hipGetLastError());
+    for (size_t i = 0; i < ranges.size(); ++i) {
+        int r = mode == 2 ? rsi16_plan_launch_key<2>(p, ranges[i], queue + i, msgs, eccs, status)
+                : mode == 0 ? rsi16_plan_launch_key<0>(p, ranges[i], queue + i, msgs, eccs, status)
+                            : rsi16_plan_launch_key<1>(p, ranges[i], queue + i, msgs, eccs, status);
+        if (r)
+            return r;
+    }
     return FFRS_OK;
 }
 
@@ -2378,9 +2549,12 @@ int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_le
             keys.push_back(key);
             p->codecs.push_back(codec);
             uint32_t g = codec->g_rep;
-            classes.push_back({codec->dev, g});
-            p->lds_enc = std::max<uint32_t>(p->lds_enc, (64 / g) * 2 * codec->E * 4);
-            p->lds_rep = std::max<uint32_t>(p->lds_rep, (64 / g) * REPAIR_WORDS * codec->E * 4);
+            // encode lanes per codeword: at least four message chunks per lane
+            uint32_t l_enc = 1;
+            const uint32_t chunks_per_cw = codec->n / codec->E - 1;
+            while (l_enc < 64 && 8 * l_enc <= chunks_per_cw)
+                l_enc *= 2;
+            classes.push_back({codec->dev, g, l_enc});
         }
         cls[i] = (uint32_t)c;
     }
@@ -2397,18 +2571,40 @@ int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_le
         return cls[a] < cls[b];
     });
     std::vector<Rsi16PlanItem> items(count);
-    std::vector<Rsi16PlanChunk> chunks;
     for (size_t j = 0; j < count; ++j) {
         uint32_t i = order[j], c = cls[i];
         const ffrs_rsi16 *codec = p->codecs[c];
         items[j] = {msg_off[i], ecc_off[i], i, c};
         p->msg_end = std::max<uint64_t>(p->msg_end, msg_off[i] + codec->k);
         p->ecc_end = std::max<uint64_t>(p->ecc_end, ecc_off[i] + codec->E);
-        uint32_t cpw = 64 / classes[c].g;
-        if (chunks.empty() || chunks.back().cls != c || chunks.back().count == cpw)
-            chunks.push_back({(uint32_t)j, 0, (uint16_t)c});
-        chunks.back().count++;
     }
+    // Chunks hold codewords of one class: a wave's worth for the kernel that runs them.
+    // Consecutive chunks with the same kernel form one range (one launch).
+    std::vector<Rsi16PlanChunk> chunks;
+    auto build = [&](bool repair, std::vector<ffrs_rsi16_plan::Range> &ranges) {
+        for (size_t j = 0; j < count; ++j) {
+            const uint32_t c = items[j].cls;
+            const ffrs_rsi16 *codec = p->codecs[c];
+            const uint32_t E = codec->E, g = classes[c].g;
+            const uint32_t G = E / 2 < 64 ? E / 2 : 64;
+            const bool reg = repair ? (E >= 2 && E <= 256 && g == G) : (E >= 2 && E <= 32);
+            const uint32_t key = reg ? E : 0u;
+            const uint32_t cpw = repair ? (reg ? 64 / G : 64 / g) : (reg ? 64 / classes[c].l_enc : 64 / g);
+            const uint32_t lds = repair ? (reg ? (64 / G) * REG_WORDS * E * 4 : (64 / g) * REPAIR_WORDS * E * 4)
+                                        : (reg ? 0u : (64 / g) * 2 * E * 4);
+            const bool new_range = ranges.empty() || ranges.back().key != key;
+            if (new_range)
+                ranges.push_back({key, (uint32_t)chunks.size(), 0, 0});
+            if (new_range || chunks.back().cls != c || chunks.back().count == cpw) {
+                chunks.push_back({(uint32_t)j, 0, (uint16_t)c});
+                ranges.back().n++;
+            }
+            chunks.back().count++;
+            ranges.back().lds = std::max(ranges.back().lds, lds);
+        }
+    };
+    build(true, p->rep_ranges);
+    build(false, p->enc_ranges);
     p->nchunks = (uint32_t)chunks.size();
     std::lock_guard<std::mutex> lk(ctx->mu);
     hipError_t e = hipSetDevice(ctx->device);
@@ -2423,7 +2619,7 @@ int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_le
     up(&p->d_chunks, chunks);
     up(&p->d_classes, classes);
     if (e == hipSuccess)
-        e = hipMalloc((void **)&p->d_queue, 4);
+        e = hipMalloc((void **)&p->d_queue, 4 * std::max<size_t>(1, p->rep_ranges.size() + p->enc_ranges.size()));
     if (e != hipSuccess) {
         rsi16_plan_free(p);
         return fail(FFRS_ERR_NOMEM, "plan upload failed: %s", hipGetErrorString(e));
