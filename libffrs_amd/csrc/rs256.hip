@@ -265,21 +265,38 @@ struct GfLds {
 // S_i = R(alpha^i) for i < E, where R(x) = sum_j r_j x^(E-1-j) is held in remainder
 // order (r_0 = highest). Horner over j per syndrome; EXP2 is zero above index 509, so a
 // zero accumulator (LOG2 = ZLOG) contributes nothing.
+// Q syndromes advance together, so Q lookup chains are in flight at a time (fewer for wide
+// remainders, whose words already fill the registers).
 template <int EW>
 __device__ inline void syndromes_from_remainder(const uint32_t (&rem)[EW], uint32_t E, const GfLds &gf,
                                                 uint8_t *Sy)
 {
-    for (uint32_t i = 0; i < E; ++i) {
-        uint32_t acc = 0;
+    constexpr int Q = EW >= 32 ? 4 : 8;
+    for (uint32_t i0 = 0; i0 < E; i0 += Q) {
+        uint32_t acc[Q];
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            acc[q] = 0;
 #pragma unroll
         for (int d = 0; d < EW; ++d) {
 #pragma unroll
             for (int b = 0; b < 4; ++b) {
-                if ((uint32_t)(d * 4 + b) < E)
-                    acc = (uint32_t)gf.exp2[gf.log2[acc] + i] ^ ((rem[d] >> (8 * b)) & 0xffu);
+                if ((uint32_t)(d * 4 + b) < E) {
+                    const uint32_t r = (rem[d] >> (8 * b)) & 0xffu;
+                    uint32_t la[Q];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+                        la[q] = gf.log2[acc[q]];
+#pragma unroll
+                    for (int q = 0; q < Q; ++q)
+                        acc[q] = (uint32_t)gf.exp2[la[q] + i0 + q] ^ r;
+                }
             }
         }
-        Sy[i] = (uint8_t)acc;
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+            if (i0 + q < E)
+                Sy[i0 + q] = (uint8_t)acc[q];
     }
 }
 
@@ -356,8 +373,22 @@ __device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec,
     for (uint32_t nn = 0; nn < E; ++nn) {
         uint32_t d = Sy[nn];
         uint32_t top = L < lenL - 1 ? L : lenL - 1;
-        for (uint32_t i = 1; i <= top; ++i)
-            d ^= gf.exp2[gf.log2[lam[i]] + lS[nn - i]];
+        for (uint32_t i = 1; i <= top; i += 4) {  // four terms' lookups in flight
+            uint32_t lv[4], ls[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool in = i + q <= top;
+                lv[q] = in ? (uint32_t)lam[i + q] : 0u;
+                ls[q] = in ? (uint32_t)lS[nn - i - q] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                lv[q] = gf.log2[lv[q]];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (i + q <= top)
+                    d ^= gf.exp2[lv[q] + ls[q]];
+        }
         if (d == 0) {
             m += 1;
             continue;
@@ -366,13 +397,29 @@ __device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec,
         bool grow = 2 * L <= nn;
         uint32_t hiB = m + lenB < E ? m + lenB : E;  // x^m B truncated to degrees < E
         uint32_t hi = grow ? (hiB > lenL ? hiB : lenL) : hiB;
-        // lam <- lam - c x^m B ; if grow, B <- old lam (downwards keeps B[i-m] unread-after-write)
-        for (uint32_t ii = hi; ii-- > 0;) {
-            uint32_t old = lam[ii];
-            uint32_t sb = (ii >= m && ii < hiB) ? (uint32_t)Bx[ii - m] : 0u;
-            lam[ii] = (uint8_t)(old ^ gf.mul_log(lc, sb));
-            if (grow)
-                Bx[ii] = (uint8_t)old;
+        // lam <- lam - c x^m B ; if grow, B <- old lam. Downwards, four indices at a time: a
+        // group reads B[ii - m] below every index written so far (m >= 1), so its reads can
+        // all precede its writes.
+        for (uint32_t g0 = hi; g0 > 0; g0 = g0 > 4 ? g0 - 4 : 0) {
+            uint32_t old[4], sb[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const uint32_t ii = g0 - 1 - t;
+                const bool in = g0 > (uint32_t)t;
+                old[t] = in ? (uint32_t)lam[ii] : 0u;
+                sb[t] = (in && ii >= m && ii < hiB) ? (uint32_t)Bx[ii - m] : 0u;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                sb[t] = gf.mul_log(lc, sb[t]);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                if (g0 > (uint32_t)t) {
+                    const uint32_t ii = g0 - 1 - t;
+                    lam[ii] = (uint8_t)(old[t] ^ sb[t]);
+                    if (grow)
+                        Bx[ii] = (uint8_t)old[t];
+                }
         }
         if (grow) {
             lenB = lenL;
@@ -393,8 +440,22 @@ __device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec,
     uint8_t *om = Bx;
     for (uint32_t kk = 0; kk < L; ++kk) {
         uint32_t v = 0;
-        for (uint32_t i = 0; i <= kk; ++i)
-            v ^= gf.exp2[gf.log2[lam[i]] + lS[kk - i]];
+        for (uint32_t i = 0; i <= kk; i += 4) {  // four terms' lookups in flight
+            uint32_t lv[4], ls[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const bool in = i + q <= kk;
+                lv[q] = in ? (uint32_t)lam[i + q] : 0u;
+                ls[q] = in ? (uint32_t)lS[kk - i - q] : 0u;
+            }
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                lv[q] = gf.log2[lv[q]];
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (i + q <= kk)
+                    v ^= gf.exp2[lv[q] + ls[q]];
+        }
         om[kk] = (uint8_t)v;
     }
     for (uint32_t i = 0; i <= L; ++i)
@@ -412,14 +473,19 @@ __device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec,
             step[q] = p ? 255 - p : 0;  // alpha^-p = alpha^(255 - p)
             e[q] = 0;
         }
-        for (uint32_t i = 0; i <= L; ++i) {
-            const uint32_t li = llam[i];
+        for (uint32_t i = 0; i <= L; i += 4) {  // four terms' logs read together
+            uint32_t li[4];
 #pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                acc[q] ^= gf.exp2[li + e[q]];
-                e[q] += step[q];
-                e[q] = e[q] >= 255 ? e[q] - 255 : e[q];
-            }
+            for (int t = 0; t < 4; ++t)
+                li[t] = i + t <= L ? (uint32_t)llam[i + t] : (uint32_t)ZLOG;
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    acc[q] ^= gf.exp2[li[t] + e[q]];
+                    e[q] += step[q];
+                    e[q] = e[q] >= 255 ? e[q] - 255 : e[q];
+                }
         }
 #pragma unroll
         for (int q = 0; q < 4; ++q)
@@ -442,13 +508,21 @@ __device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec,
         uint32_t p = pos[r];
         uint32_t pinv = p ? 255 - p : 0;
         uint32_t num = 0, den = 0, e = 0;
-        for (uint32_t kk = 0; kk < L; ++kk) {  // Omega_k = 0 for k >= L
-            const uint32_t lo = lom[kk];
-            num ^= gf.exp2[(lo == 255 ? (uint32_t)ZLOG : lo) + e];
-            if ((kk & 1) == 0)
-                den ^= gf.exp2[llam[kk + 1] + e];
-            e += pinv;
-            e = e >= 255 ? e - 255 : e;
+        for (uint32_t k0 = 0; k0 < L; k0 += 4) {  // Omega_k = 0 for k >= L; four terms at a time
+            uint32_t lo[4], ld[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const uint32_t kk = k0 + t;
+                lo[t] = kk < L ? (uint32_t)lom[kk] : 255u;
+                ld[t] = (kk < L && (kk & 1) == 0) ? (uint32_t)llam[kk + 1] : (uint32_t)ZLOG;
+            }
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                num ^= gf.exp2[(lo[t] == 255 ? (uint32_t)ZLOG : lo[t]) + e];
+                den ^= gf.exp2[ld[t] + e];
+                e += pinv;
+                e = e >= 255 ? e - 255 : e;
+            }
         }
         uint32_t X = gf.exp2[p];
         uint32_t mag = gf.mul(X, gf.div(num, den));
