@@ -395,8 +395,14 @@ __device__ inline uint8_t repair_one(__amdgpu_buffer_rsrc_t rsrc, uint32_t nrec,
         }
         uint32_t lc = gf.log2[gf.div(d, b)];
         bool grow = 2 * L <= nn;
+        // L never decreases and the codeword fails once L > E / 2 (checked after the loop),
+        // so stop there; while L <= E / 2, deg Lambda <= L and deg x^m B <= the new L, so
+        // the coefficients above E / 2 stay zero and the update can skip them
+        if (grow && nn + 1 - L > E / 2)
+            return 0;
         uint32_t hiB = m + lenB < E ? m + lenB : E;  // x^m B truncated to degrees < E
         uint32_t hi = grow ? (hiB > lenL ? hiB : lenL) : hiB;
+        hi = hi < E / 2 + 1 ? hi : E / 2 + 1;
         // lam <- lam - c x^m B ; if grow, B <- old lam. Downwards, four indices at a time: a
         // group reads B[ii - m] below every index written so far (m >= 1), so its reads can
         // all precede its writes.
