@@ -243,7 +243,8 @@ def run_plain(name, args):
         if ecc <= 32 and ecc not in (2, 4, 8, 16, 32):
             rep_names[1] = "rs256_bm_kernel"
     else:
-        enc_names = ["rsi16_encode_lane_kernel" if ecc <= 32 else "rsi16_encode_kernel"]
+        enc_names = ["rsi16_encode_lane_kernel" if ecc <= 32 else
+                     ("rsi16_encode_reg_kernel" if ecc <= 256 else "rsi16_encode_kernel")]
         rep_names = ["rsi16_repair_reg_kernel" if ecc <= 256 else "rsi16_repair_kernel"]
     ke_ms = np.mean(np.array(kes), axis=0)
     kr_ms = np.mean(np.array(krs), axis=0)

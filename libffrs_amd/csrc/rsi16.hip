@@ -1502,6 +1502,93 @@ __device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t
         *st = total > 0 ? FFRS_RSI16_REPAIR_OK : (lanes ? FFRS_RSI16_ERROR_LOCATION_FAIL : FFRS_RSI16_NO_ERRORS);
 }
 
+// encode_group with the values in registers (lane layout of LaneR): the residue butterflies
+// of ct_residue (only lanes with a non-zero twiddle index multiply, so every lazy value is
+// the same integer as there), the shifted sums, the mix and gs_full, all for one codeword
+// per lane group.
+template <class O, int E>
+__device__ inline void encode_group_reg(const Rsi16Dev &T, const uint16_t *__restrict__ m, uint16_t *__restrict__ o,
+                                        const LaneR<E> &L)
+{
+    constexpr uint32_t G = RegShape<E>::G;
+    constexpr int C = RegShape<E>::C;
+    constexpr uint32_t PF = C >= 4 ? 1 : 4;
+    const uint32_t nb = T.n / E;
+    PR<E> acc;
+    for (uint32_t c0 = 0; c0 + 1 < nb; c0 += PF) {
+        PR<E> x[PF], sh[PF];
+#pragma unroll
+        for (uint32_t u = 0; u < PF; ++u)
+            if (c0 + u + 1 < nb) {
+#pragma unroll
+                for (int h = 0; h < C; ++h) {
+                    const uint32_t i = (c0 + u) * E + L.gl + (uint32_t)h * G;
+                    x[u].v[h] = m[i];
+                    sh[u].v[h] = T.pntt_shift[i];
+                }
+            }
+#pragma unroll
+        for (uint32_t u = 0; u < PF; ++u) {
+            if (c0 + u + 1 >= nb)
+                continue;
+            PR<E> &y = x[u];
+            // ct_residue (ntt.hpp:252-275): partners in other lanes
+#pragma unroll
+            for (int t = 0; t < RegShape<E>::SX; ++t) {
+                const uint32_t sbit = 1u << t;
+                const bool upper = (L.gl & sbit) != 0;
+                const bool mult = upper && (L.gl & (sbit - 1)) != 0;
+#pragma unroll
+                for (int h = 0; h < C; ++h) {
+                    const uint32_t sa = mult ? O::mul_residue(O::mod_p(y.v[h]), L.x_e[t]) : y.v[h];
+                    const uint32_t ra = xchg_s(sa, sbit, L.lane);
+                    y.v[h] = upper ? ra - sa : sa + ra;
+                }
+            }
+            // partners within the lane
+#pragma unroll
+            for (int uu = 0; uu < RegShape<E>::LC; ++uu)
+#pragma unroll
+                for (int h = 0; h < C; ++h)
+                    if (!(h & (1 << uu))) {
+                        const int hl = h & ((1 << uu) - 1);
+                        const uint32_t a = y.v[h], cc = y.v[h + (1 << uu)];
+                        const bool mult = L.gl != 0 || hl != 0;
+                        const uint32_t mm = mult ? O::mul_residue(O::mod_p(cc), L.w_e[(1 << uu) - 1 + hl]) : cc;
+                        y.v[h] = a + mm;
+                        y.v[h + (1 << uu)] = a - mm;
+                    }
+#pragma unroll
+            for (int h = 0; h < C; ++h) {
+                const uint32_t v = O::mul_residue(O::mod_p(y.v[h]), sh[u].v[h]);
+                acc.v[h] = c0 + u ? acc.v[h] + v : v;
+            }
+        }
+    }
+#pragma unroll
+    for (int h = 0; h < C; ++h)
+        acc.v[h] = O::mul(O::mod_p(acc.v[h]), T.mix[L.gl + (uint32_t)h * G]);
+    // gs_full on canonical values: O's add, sub and mul equal F's there
+    pr_gs<E>(acc, L.x_i, L.w_i, L);
+#pragma unroll
+    for (int h = 0; h < C; ++h)
+        o[L.gl + (uint32_t)h * G] = (uint16_t)(acc.v[h] & 0xffff);
+}
+
+template <class O, int E>
+__global__ __launch_bounds__(64) void rsi16_encode_reg_kernel(const uint16_t *__restrict__ msgs, uint32_t count,
+                                                               uint16_t *__restrict__ out, Rsi16Dev T)
+{
+    constexpr uint32_t G = RegShape<E>::G, cpw = 64 / G;
+    const uint32_t grp = threadIdx.x / G;
+    const uint32_t cw = blockIdx.x * cpw + grp;
+    if (cw >= count)
+        return;
+    LaneR<E> L;
+    L.init(T, threadIdx.x);
+    encode_group_reg<O, E>(T, msgs + (size_t)cw * T.k, out + (size_t)cw * E, L);
+}
+
 template <int E>
 __global__ __launch_bounds__(64) void rsi16_repair_reg_kernel(uint16_t *__restrict__ msgs,
                                                                uint16_t *__restrict__ eccs, uint32_t count,
@@ -1742,6 +1829,16 @@ __global__ __launch_bounds__(64) void rsi16_plan_kernel(uint16_t *__restrict__ m
                 repair_group_reg<E>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, s_mem + grp * REG_WORDS * E, L,
                                     status + it.idx);
             }
+        } else if constexpr (E > 32) {  // encode, the register code of rsi16_encode_reg_kernel<E>
+            using O = typename std::conditional<MODE == 0, VectorOps, ScalarOps>::type;
+            constexpr uint32_t G = RegShape<E>::G;
+            const uint32_t grp = threadIdx.x / G;
+            if (grp < ch.count) {
+                const Rsi16PlanItem it = items[ch.first + grp];
+                LaneR<E> L;
+                L.init(cl.T, threadIdx.x);
+                encode_group_reg<O, E>(cl.T, msgs + it.msg_off, eccs + it.ecc_off, L);
+            }
         } else {
             using O = typename std::conditional<MODE == 0, VectorOps, ScalarOps>::type;
             const uint32_t L = cl.l_enc, grp = threadIdx.x / L, gl = threadIdx.x % L;
@@ -1845,6 +1942,22 @@ int launch_encode_lane(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint
     return FFRS_OK;
 }
 
+template <int E>
+int launch_encode_reg(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vec)
+{
+    constexpr uint32_t cpw = 64 / RegShape<E>::G;
+    const dim3 grid((unsigned)((count + cpw - 1) / cpw));
+    if (vec)
+        hipLaunchKernelGGL((rsi16_encode_reg_kernel<VectorOps, E>), grid, dim3(64), 0, rs->ctx->stream, d_msg,
+                           (uint32_t)count, d_ecc, rs->dev);
+    else
+        hipLaunchKernelGGL((rsi16_encode_reg_kernel<ScalarOps, E>), grid, dim3(64), 0, rs->ctx->stream, d_msg,
+                           (uint32_t)count, d_ecc, rs->dev);
+    FFRS_HIP(hipGetLastError());
+    mark_kernel(rs->ctx);
+    return FFRS_OK;
+}
+
 int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vec, uint32_t I = 1)
 {
     marks_begin(rs->ctx);
@@ -1856,6 +1969,26 @@ int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t 
         return vec ? launch_encode_lane<VectorOps>(rs, d_msg, count, d_ecc)
                    : launch_encode_lane<ScalarOps>(rs, d_msg, count, d_ecc);
     uint32_t g = rs->g_enc == 1 && rs->E <= 32 ? group_size(rs->E) : rs->g_enc, cpw = 64 / g;
+    if (I == 1 && rs->E >= 2 && rs->E <= 256 && g == group_size(rs->E)) {  // the default shape: registers
+        switch (rs->E) {
+        case 2:
+            return launch_encode_reg<2>(rs, d_msg, count, d_ecc, vec);
+        case 4:
+            return launch_encode_reg<4>(rs, d_msg, count, d_ecc, vec);
+        case 8:
+            return launch_encode_reg<8>(rs, d_msg, count, d_ecc, vec);
+        case 16:
+            return launch_encode_reg<16>(rs, d_msg, count, d_ecc, vec);
+        case 32:
+            return launch_encode_reg<32>(rs, d_msg, count, d_ecc, vec);
+        case 64:
+            return launch_encode_reg<64>(rs, d_msg, count, d_ecc, vec);
+        case 128:
+            return launch_encode_reg<128>(rs, d_msg, count, d_ecc, vec);
+        default:
+            return launch_encode_reg<256>(rs, d_msg, count, d_ecc, vec);
+        }
+    }
     size_t lds = (size_t)cpw * 2 * rs->E * 4;
     dim3 grid((unsigned)((count + cpw - 1) / cpw));
     if (int r = allow_lds(rsi16_encode_kernel<VectorOps>, lds))
@@ -2453,17 +2586,15 @@ int rsi16_plan_launch_key(ffrs_rsi16_plan *p, const ffrs_rsi16_plan::Range &r, u
     default:
         break;
     }
-    if constexpr (MODE == 2) {
-        switch (r.key) {
-        case 64:
-            return rsi16_plan_launch_range<2, 64>(p, r, queue, msgs, eccs, status);
-        case 128:
-            return rsi16_plan_launch_range<2, 128>(p, r, queue, msgs, eccs, status);
-        case 256:
-            return rsi16_plan_launch_range<2, 256>(p, r, queue, msgs, eccs, status);
-        default:
-            break;
-        }
+    switch (r.key) {
+    case 64:
+        return rsi16_plan_launch_range<MODE, 64>(p, r, queue, msgs, eccs, status);
+    case 128:
+        return rsi16_plan_launch_range<MODE, 128>(p, r, queue, msgs, eccs, status);
+    case 256:
+        return rsi16_plan_launch_range<MODE, 256>(p, r, queue, msgs, eccs, status);
+    default:
+        break;
     }
     return rsi16_plan_launch_range<MODE, 0>(p, r, queue, msgs, eccs, status);
 }
@@ -2595,9 +2726,10 @@ int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_le
             const ffrs_rsi16 *codec = p->codecs[c];
             const uint32_t E = codec->E, g = classes[c].g;
             const uint32_t G = reg_group(E);
-            const bool reg = repair ? (E >= 2 && E <= 256 && g == group_size(E)) : (E >= 2 && E <= 32);
+            const bool dflt = E >= 2 && E <= 256 && g == group_size(E);
+            const bool reg = repair ? dflt : (E >= 2 && E <= 32) || dflt;
             const uint32_t key = reg ? E : 0u;
-            const uint32_t cpw = repair ? (reg ? 64 / G : 64 / g) : (reg ? 64 / classes[c].l_enc : 64 / g);
+            const uint32_t cpw = !reg ? 64 / g : repair ? 64 / G : (E <= 32 ? 64 / classes[c].l_enc : 64 / G);
             const uint32_t lds = repair ? (reg ? (64 / G) * REG_WORDS * E * 4 : (64 / g) * REPAIR_WORDS * E * 4)
                                         : (reg ? 0u : (64 / g) * 2 * E * 4);
             const bool new_range = ranges.empty() || ranges.back().key != key;
