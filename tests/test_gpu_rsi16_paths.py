@@ -86,3 +86,44 @@ def test_register_and_lds_repair_agree(n, ecc, count):
     m, e = np.ascontiguousarray(cm[rows]), np.ascontiguousarray(cp[rows])
     ost = O.RSi16(n, ecc).repair_batch(m, e)
     assert (ost == st[rows]).all() and (m == reg[0][rows]).all() and (e == reg[1][rows]).all()
+
+
+ENC_CODES = [(8, 2), (16, 4), (64, 8), (256, 16), (256, 32), (512, 64), (1024, 128), (2048, 256)]
+
+
+@pytest.mark.parametrize("n,ecc", ENC_CODES)
+@pytest.mark.parametrize("vector", [True, False], ids=["vector_words", "scalar_words"])
+def test_encode_kernels_agree(n, ecc, vector):
+    """Parity words of the default encode (one codeword per lane up to ecc 32, registers up to
+    256) equal the LDS kernel's (another lanes-per-codeword choice), on uniform messages and on
+    messages of only 0 and 65535 (zero partial sums, the lazy-residue edge); a sample matches
+    the oracle."""
+    from libffrs_amd import capi
+    from oracle import oracle as O
+
+    k = n - ecc
+    count = max(64, min(8192, (1 << 22) // (2 * n)))
+    s = W.Stream(97 + ecc)
+    msgs = s.u16(count * k).reshape(count, k)
+    edge = np.where(s.ints(0, 1, (count // 2) * k).reshape(count // 2, k) == 1, 65535, 0).astype(np.uint16)
+    msgs[: count // 2] = edge
+    ctx = capi.Context(0)
+    codec = capi.RSi16Codec(ctx, n, ecc)
+    dm, dp = ctx.alloc(msgs.nbytes), ctx.alloc(count * ecc * 2)
+    dm.upload(np.ascontiguousarray(msgs))
+    outs = []
+    for g in (None, 2 if ecc <= 4 else min(64, ecc) // 4):
+        if g is not None:
+            codec.set_groups(g, 0)
+        codec.encode_device(dm, count, dp, vector)
+        par = np.empty((count, ecc), np.uint16)
+        dp.download(par)
+        outs.append(par)
+    dm.free()
+    dp.free()
+    codec.close()
+    ctx.close()
+    assert (outs[0] == outs[1]).all()
+    rows = np.r_[0:16, count - 16:count]
+    ref = O.RSi16(n, ecc).encode(np.ascontiguousarray(msgs[rows]), vector_lanes=vector)
+    assert (ref == outs[0][rows]).all()
