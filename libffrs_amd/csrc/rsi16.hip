@@ -824,7 +824,8 @@ __device__ inline void repair_group(const Rsi16Dev &T, uint16_t *m, uint16_t *e,
         return;
     }
     uint32_t lanes = L != 0 ? 0xffffffffu : 0u;
-    uint32_t dmax = der_len & lanes, emax = eval_len & lanes;
+    // Horner from the highest non-zero coefficient below emax: the zeros above it add nothing
+    uint32_t dmax = der_len & lanes, emax = group_len(syn, (eval_len & lanes) < E ? (eval_len & lanes) : E, G);
     uint32_t nb = T.n / E;
     // NTT::pintt_ecc (ntt.hpp:206-220), one chunk of E positions at a time. The zeros are
     // listed first (the Sugiyama scratch is free now) and corrected afterwards with one
@@ -1441,7 +1442,8 @@ __device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t
         return;
     }
     const uint32_t lanes = Ln != 0 ? 0xffffffffu : 0u;
-    const uint32_t dmax = der_len & lanes, emax = eval_len & lanes;
+    // Horner from the highest non-zero coefficient below emax: the zeros above it add nothing
+    const uint32_t dmax = der_len & lanes, emax = pr_len<E>(syn, eval_len & lanes, L);
     uint32_t *ev = base, *dv = base + E, *list = base + 2 * E, *nroots = base + 3 * E;
 #pragma unroll
     for (int h = 0; h < C; ++h) {
