@@ -9,17 +9,20 @@
 //                 ntt.hpp:343-564 (reversal, Newton inverse, NTT-domain division)
 //   known pos.    libffrs/rsi16v_impl.hpp:171-231, 532-563
 //
-// Layout: every kernel runs in single-wave workgroups. A wave is split into lane
-// groups of g = min(64, ecc/2) lanes; each group owns one codeword and keeps its
-// working polynomials in LDS. Butterfly stages are spread over the group's lanes and
-// ordered with wavefront-scope fences (a group never waits on another group, so the
-// groups of one wave may follow different control flow).
+// Layout: a wave is split into lane groups that each own one codeword. The default
+// kernels (ecc_len <= 256) keep every polynomial in registers, ecc/G coefficients per
+// lane for groups of G = min(64, ecc/2) lanes, and exchange butterfly partners across
+// lanes with DPP / ds_swizzle / ds_bpermute; encode for ecc_len <= 32 runs one codeword
+// per lane. The LDS kernels (other group sizes, larger ecc_len, interleaved encode) keep
+// the polynomials in LDS with stages ordered by wavefront-scope fences. A group never
+// waits on another, so the groups of one wave may follow different control flow.
 //
 // Exactness: the encode path reproduces the reference's lazy residue arithmetic
 // word for word (the u32 values, not just their residues), because the parity depends
-// on them in corner cases; `VEC` selects the SIMD-lane or the scalar variant of
-// mod_p / mul / mul_residue / sub. The repair path uses the scalar full-reduction
-// operations, whose outputs are canonical field elements.
+// on them in corner cases; VectorOps / ScalarOps select the SIMD-lane or the scalar
+// variant of mod_p / mul / mul_residue / sub. The repair path uses the scalar
+// full-reduction operations, whose outputs are canonical field elements; its Sugiyama
+// solver computes the reference's values in coefficient form (see pr_sugiyama).
 #include "common.hpp"
 
 #include <algorithm>
