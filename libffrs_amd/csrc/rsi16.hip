@@ -907,6 +907,22 @@ struct F {
         const uint32_t t = ((uint32_t)x & 0xffffu) - (uint32_t)(x >> 16), u = t + P16;
         return t < u ? t : u;
     }
+    // a b (a <= p, b <= 65536) folded once: lo16 - hi of the product, in [-65537, 65535]
+    __device__ static inline int32_t part(uint32_t a, uint32_t b)
+    {
+        __builtin_assume(a <= P16);
+        __builtin_assume(b <= 0x10000u);
+        const uint64_t x = (uint64_t)a * (uint64_t)b;
+        return (int32_t)((uint32_t)x & 0xffffu) - (int32_t)(uint32_t)(x >> 16);
+    }
+    // c + r0 + r1 reduced (c <= 65536, r0 and r1 from part): the sum plus 2p lies in
+    // [0, 327680], and one more fold leaves [-5, 65535]
+    __device__ static inline uint32_t fold3(uint32_t c, int32_t r0, int32_t r1)
+    {
+        const uint32_t y = (uint32_t)((int32_t)c + r0 + r1 + 2 * (int32_t)P16);
+        const uint32_t t = (y & 0xffffu) - (y >> 16), u = t + P16;
+        return t < u ? t : u;
+    }
 };
 
 // v of lane (this lane ^ X)
@@ -1333,10 +1349,11 @@ __device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, 
                 const uint32_t q0 = F::add(F::mul(c0, v[0]), F::mul(c1, v[1]));
                 const uint32_t q1 = F::add(F::mul(c0, v[2]), F::mul(c1, v[3]));
                 const PR<E> pa = pr_prev<E>(a1, L), pr = pr_prev<E>(r1, L);
+                const uint32_t n0 = P16 - q0, n1 = P16 - q1;  // a2 - q0 a1 - q1 pa = a2 + n0 a1 + n1 pa
 #pragma unroll
                 for (int h = 0; h < C; ++h) {
-                    t.v[h] = F::sub(a2.v[h], F::add(F::mul(q0, a1.v[h]), F::mul(q1, pa.v[h])));
-                    rn.v[h] = F::sub(r2.v[h], F::add(F::mul(q0, r1.v[h]), F::mul(q1, pr.v[h])));
+                    t.v[h] = F::fold3(a2.v[h], F::part(n0, a1.v[h]), F::part(n1, pa.v[h]));
+                    rn.v[h] = F::fold3(r2.v[h], F::part(n0, r1.v[h]), F::part(n1, pr.v[h]));
                 }
             } else {  // through the transforms
                 PR<E> R1 = r1, R2 = r2, A1 = a1, A2 = a2, scratch, Q;
