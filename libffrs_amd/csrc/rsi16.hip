@@ -1289,9 +1289,14 @@ __device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, 
     uint32_t r2_len = r1_len;
     uint32_t r2l = pr_get<E>(r2, 0, L);
     const uint32_t lanes = r1_len != 0 ? 0xffffffffu : 0u;
+    // r2[r2_len - 1] and r2[r2_len - 2], carried from the step that read them while r2 was r1
+    uint32_t h1, h2;
+    bool carried = true;
     {
         const uint32_t r1h = pr_get<E>(r1, (r1_len - 1) & mask, L);
         const uint32_t r1h2 = pr_get<E>(r1, (r1_len - 2) & mask, L);
+        h1 = r1h;  // r2 is this polynomial from here on
+        h2 = r1h2;
         const uint32_t a11 = s_neg(T.inv_tab[r1h]);
         const uint32_t a10 = F::mul(r1h2, F::mul(a11, a11));
         if constexpr (G > 1) {  // coefficients 0 and 1 live in lanes 0 and 1
@@ -1333,7 +1338,8 @@ __device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, 
                 q_len = 2;
                 const uint32_t a0 = T.inv_tab[r1h];
                 const uint32_t aa = F::mul(a0, a0);
-                const uint32_t c0 = F::sub(0, F::mul(aa, pr_get<E>(r1, (g_len - 2) & mask, L)));
+                const uint32_t r1s = pr_get<E>(r1, (g_len - 2) & mask, L);
+                const uint32_t c0 = F::sub(0, F::mul(aa, r1s));
                 const uint32_t c1 = F::sub(F::mul(2, a0), F::mul(aa, r1h));
                 // quotient coefficients t[idx] of t = (r2 - f0) * (c0 + c1 x) mod x^E - 1
                 const uint32_t right_n = (uint32_t)E - g_len < 2 ? (uint32_t)E - g_len : 2u;
@@ -1341,11 +1347,23 @@ __device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, 
                 const uint32_t idx1 = right_n >= 2 ? g_len + 1 : 1u - right_n;
                 uint32_t v[4];
                 const uint32_t at[4] = {idx0, (idx0 - 1) & mask, idx1, (idx1 - 1) & mask};
+                // the two leading coefficients of r2 are known when idx0 = r2_len - 1 (then
+                // idx0 >= E / 2 >= 2, so neither is coefficient 0)
+                const bool known = carried && right_n >= 1;
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const uint32_t x = pr_get<E>(r2, at[u], L);
+                    uint32_t x;
+                    if (u < 2 && known)
+                        x = u == 0 ? h1 : h2;
+                    else if (u == 3 && right_n >= 2)
+                        x = known ? h1 : pr_get<E>(r2, at[u], L);  // idx1 - 1 = idx0
+                    else
+                        x = pr_get<E>(r2, at[u], L);
                     v[u] = at[u] == 0 ? F::sub(x, r2l) : x;
                 }
+                h1 = r1h;
+                h2 = r1s;
+                carried = true;
                 const uint32_t q0 = F::add(F::mul(c0, v[0]), F::mul(c1, v[1]));
                 const uint32_t q1 = F::add(F::mul(c0, v[2]), F::mul(c1, v[3]));
                 const PR<E> pa = pr_prev<E>(a1, L), pr = pr_prev<E>(r1, L);
@@ -1356,6 +1374,8 @@ __device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, 
                     rn.v[h] = F::fold3(r2.v[h], F::part(n0, r1.v[h]), F::part(n1, pr.v[h]));
                 }
             } else {  // through the transforms
+                h1 = r1h;
+                carried = false;  // r1[g_len - 2] is not read on this path
                 PR<E> R1 = r1, R2 = r2, A1 = a1, A2 = a2, scratch, Q;
                 pr_nttr<E>(R1, L);
                 pr_nttr<E>(R2, L);
