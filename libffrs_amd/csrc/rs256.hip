@@ -90,10 +90,83 @@ struct Rs256Tables {
 // row reads start at every 16-byte bank group of the LDS instead of every other one.
 __host__ __device__ constexpr int lut_row(int EW) { return (EW % 8 == 0) ? EW + 4 : EW; }
 
+// Replicated two-slice layout of the 32-byte-row table (S == LUT_REP, EW == 8): chunk h (16
+// bytes) of row b of slice s is stored LUT_COPIES times, copy c at byte offset
+// (s << 16) | (b << 8) | (h << 7) | (c << 4), i.e. in 16-byte bank group 8h + c of the 64
+// banks. The LDS serves a 16-byte read sixteen lanes at a time (one lane per bank group when
+// there are no conflicts). Lane l reads copy l % 8, and lanes whose bit 3 is set read chunk 1
+// with their first read and chunk 0 with their second, the others the reverse: the sixteen
+// lanes served together then always sit in sixteen different bank groups, whatever rows the
+// data selects. (Unreplicated rows collide about three ways per group of sixteen.)
+// Rows and the remainder are held halfword-paired: with h_0..h_15 the remainder's 16-bit
+// symbol pairs (h_0 = bytes 0 and 1, the leading ones), dword j holds h_j in its low half and
+// h_(j+8) in its high half. A two-byte LFSR shift then moves whole dwords (register renaming)
+// and one 16-bit shift, instead of a byte-align per dword. Lanes that read chunk 1 first keep
+// the two halves of their remainder swapped (registers 0..3 hold dwords 4..7), so that every
+// lane XORs its first read into registers 0..3.
+constexpr int LUT_REP = 0;
+constexpr int LUT_COPIES = 8;
+__host__ __device__ constexpr int lut_dwords(int EW, int S)
+{
+    return S == LUT_REP ? 2 * 256 * 2 * LUT_COPIES * 4 : S * 256 * lut_row(EW);
+}
+
+// Lanes of the replicated layout that read chunk 1 first (bit 3 of the lane index).
+__device__ inline bool lut_swapped() { return (threadIdx.x & 8u) != 0; }
+
+// a ^ b ^ c in one instruction (truth table 0x96)
+__device__ inline uint32_t xor3(uint32_t a, uint32_t b, uint32_t c) { return __builtin_amdgcn_bitop3_b32(a, b, c, 0x96); }
+
+// LDS home of an LFSR table of layout (EW, S): the replicated layout lives in a static array
+// (a known LDS address, so row offsets fold into the read instructions), every other layout at
+// the start of the kernel's dynamic LDS `dyn`.
+template <int EW, int S>
+__device__ inline uint32_t *lfsr_table(uint32_t *dyn)
+{
+    if constexpr (S == LUT_REP) {
+        __shared__ uint4 s_rep[lut_dwords(8, LUT_REP) / 4];
+        return reinterpret_cast<uint32_t *>(s_rep);
+    } else {
+        return dyn;
+    }
+}
+// Dynamic LDS bytes that the table of layout (EW, S) takes.
+__host__ __device__ constexpr int lfsr_dyn_dwords(int EW, int S) { return S == LUT_REP ? 0 : lut_dwords(EW, S); }
+
 template <int EW, int S>
 __device__ inline void lut_step(uint32_t (&rem)[EW], uint32_t w, const uint32_t *__restrict__ s_lut)
 {
-    if constexpr (S == 4) {
+    if constexpr (S == LUT_REP) {
+        static_assert(EW == 8, "replicated layout holds 32-byte rows");
+        // byte permutes build the offsets: this lane's copy and first chunk in byte 0, the row
+        // in byte 1, the slice in byte 2; the second chunk is 128 bytes away
+        const bool swapped = lut_swapped();
+        const uint32_t lane_off1 = ((threadIdx.x % LUT_COPIES) << 4) | (swapped ? 0x80u : 0u) | 0x10000u;
+        const uint32_t sh3 = swapped ? 16u : 0u, sh7 = 16u - sh3;
+        const char *lds0 = reinterpret_cast<const char *>(s_lut);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t x = (swapped ? rem[4] : rem[0]) ^ (h ? w >> 16 : w);
+            // slice 1 row x.byte0 (the first byte needs one more shift), slice 0 row x.byte1
+            const uint32_t o1 = __builtin_amdgcn_perm(x, lane_off1, 0x0c020400u);
+            const uint32_t o0 = __builtin_amdgcn_perm(x, lane_off1, 0x0c0c0500u);
+            const uint4 a1 = *reinterpret_cast<const uint4 *>(lds0 + o1);
+            const uint4 b1 = *reinterpret_cast<const uint4 *>(lds0 + (o1 ^ 0x80u));
+            const uint4 a0 = *reinterpret_cast<const uint4 *>(lds0 + o0);
+            const uint4 b0 = *reinterpret_cast<const uint4 *>(lds0 + (o0 ^ 0x80u));
+            // h_8 moves to the low half of dword 7 (register 7, or 3 when swapped); dword 3
+            // takes dword 4 unshifted (register 3, or 7 when swapped)
+            const uint32_t t3 = rem[4] >> sh3, t7 = rem[0] >> sh7;
+            rem[0] = xor3(rem[1], a1.x, a0.x);
+            rem[1] = xor3(rem[2], a1.y, a0.y);
+            rem[2] = xor3(rem[3], a1.z, a0.z);
+            rem[3] = xor3(t3, a1.w, a0.w);
+            rem[4] = xor3(rem[5], b1.x, b0.x);
+            rem[5] = xor3(rem[6], b1.y, b0.y);
+            rem[6] = xor3(rem[7], b1.z, b0.z);
+            rem[7] = xor3(t7, b1.w, b0.w);
+        }
+    } else if constexpr (S == 4) {
         uint32_t x = rem[0] ^ w;
         const uint32_t *t3 = s_lut + (3 * 256 + (x & 0xff)) * lut_row(EW);
         const uint32_t *t2 = s_lut + (2 * 256 + ((x >> 8) & 0xff)) * lut_row(EW);
@@ -168,6 +241,50 @@ __device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, uint32_t n
     uint32_t nw = P >> 2;
     uint32_t prev = buf_load_u32(rsrc, off);
     uint32_t t = 0;
+    if constexpr (S == LUT_REP) {
+        // 64 bytes per iteration, the next 64 loaded before the current ones are fed: a lane
+        // visits each cache line of its message at most twice, and the loads overlap the
+        // table reads
+        constexpr int CW = 16;
+        auto load = [&](uint32_t tt, uint32_t (&c)[CW]) {
+            const uint32_t o = off + 4 * (tt + 1);
+            if (o + 4 * CW <= nrec) {
+#pragma unroll
+                for (int q = 0; q < CW / 4; ++q) {
+                    const uint4 v = buf_load_u32x4(rsrc, o + 16 * q);
+                    c[4 * q] = v.x; c[4 * q + 1] = v.y; c[4 * q + 2] = v.z; c[4 * q + 3] = v.w;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < CW; ++j)
+                    c[j] = buf_load_u32(rsrc, o + 4 * j);
+            }
+        };
+        if (t + CW <= nw) {
+            uint32_t cur[CW];
+            load(t, cur);
+            for (;;) {
+                const bool more = t + 2 * CW <= nw;
+                uint32_t nxt[CW];
+                if (more)
+                    load(t + CW, nxt);
+                uint32_t w0 = alignbyte(cur[0], prev, sh);
+                if (t == 0)
+                    w0 &= headmask;
+                lut_step<EW, S>(rem, w0, s_lut);
+#pragma unroll
+                for (int j = 1; j < CW; ++j)
+                    lut_step<EW, S>(rem, alignbyte(cur[j], cur[j - 1], sh), s_lut);
+                prev = cur[CW - 1];
+                t += CW;
+                if (!more)
+                    break;
+#pragma unroll
+                for (int j = 0; j < CW; ++j)
+                    cur[j] = nxt[j];
+            }
+        }
+    }
     // 32 bytes per iteration, as two 16-byte loads where all of them are in range (each lane
     // reads its own row, so wider loads mean fewer L1 lines fetched per byte); single
     // words at the very end of the buffer
@@ -201,6 +318,26 @@ __device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, uint32_t n
         lut_step<EW, S>(rem, w, s_lut);
         prev = c;
     }
+    if constexpr (S == LUT_REP) {
+        // swapped lanes: registers back in dword order
+        const bool swapped = lut_swapped();
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            const uint32_t lo = rem[d], hi = rem[d + 4];
+            rem[d] = swapped ? hi : lo;
+            rem[d + 4] = swapped ? lo : hi;
+        }
+        // halfword-paired -> byte order: dword m = h_2m | h_(2m+1) << 16
+        uint32_t o[8];
+#pragma unroll
+        for (int m = 0; m < 4; ++m) {
+            o[m] = __builtin_amdgcn_perm(rem[2 * m + 1], rem[2 * m], 0x05040100u);
+            o[m + 4] = __builtin_amdgcn_perm(rem[2 * m + 1], rem[2 * m], 0x07060302u);
+        }
+#pragma unroll
+        for (int d = 0; d < 8; ++d)
+            rem[d] = o[d];
+    }
 }
 
 template <int EW>
@@ -218,9 +355,11 @@ __global__ __launch_bounds__(WG) void rs256_encode_kernel(const uint8_t *__restr
                                                             uint32_t count, uint32_t k, uint8_t *__restrict__ out,
                                                             uint32_t ecc, const uint32_t *__restrict__ lut)
 {
-    extern __shared__ uint32_t s_lut[];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * lut_row(EW)); i += blockDim.x)
-        s_lut[i] = lut[i];
+    extern __shared__ uint32_t s_dyn[];
+    uint32_t *s_lut = lfsr_table<EW, S>(s_dyn);
+    static_assert(lut_dwords(EW, S) % 4 == 0, "tables are copied in 16-byte units");
+    for (uint32_t i = threadIdx.x; i < (uint32_t)lut_dwords(EW, S) / 4; i += blockDim.x)
+        reinterpret_cast<uint4 *>(s_lut)[i] = reinterpret_cast<const uint4 *>(lut)[i];
     __syncthreads();
 
     const uint32_t nrec = (msg_bytes + 3u) & ~3u;
@@ -628,10 +767,10 @@ __global__ __launch_bounds__(WG) void rs256_synd_kernel(const uint8_t *__restric
                                                           const uint32_t *__restrict__ stab)
 {
     extern __shared__ uint32_t s_mem[];
-    uint32_t *s_lut = s_mem;
-    uint32_t *s_st = s_mem + S * 256 * lut_row(EW);
-    for (uint32_t i = threadIdx.x; i < (uint32_t)(S * 256 * lut_row(EW)); i += blockDim.x)
-        s_lut[i] = lut[i];
+    uint32_t *s_lut = lfsr_table<EW, S>(s_mem);
+    uint32_t *s_st = s_mem + lfsr_dyn_dwords(EW, S);
+    for (uint32_t i = threadIdx.x; i < (uint32_t)lut_dwords(EW, S) / 4; i += blockDim.x)
+        reinterpret_cast<uint4 *>(s_lut)[i] = reinterpret_cast<const uint4 *>(lut)[i];
     for (uint32_t i = threadIdx.x; i < E * 32 * EW; i += blockDim.x)
         s_st[i] = stab[i];
     __syncthreads();
@@ -1181,6 +1320,7 @@ struct ffrs_rs256 {
     uint8_t gen[256] = {0};
     uint8_t roots[256] = {0};
     uint32_t *d_lut = nullptr;
+    uint32_t *d_lutr = nullptr;  // the same table in the replicated two-slice layout (EW == 8)
     uint8_t *d_exp2 = nullptr;
     uint16_t *d_log2 = nullptr;
     uint32_t *d_stab = nullptr;  // syndrome nibble tables (ecc <= 32)
@@ -1234,25 +1374,56 @@ std::vector<uint32_t> build_lut(const ffrs_rs256 &rs)
     return lut;
 }
 
+// The replicated two-slice layout of lut_dwords(8, LUT_REP) dwords (ecc 29..32).
+std::vector<uint32_t> build_lut_rep(const ffrs_rs256 &rs)
+{
+    unsigned E = rs.ecc;
+    std::vector<uint32_t> t((size_t)lut_dwords(8, LUT_REP), 0);
+    for (unsigned b = 0; b < 256; ++b) {
+        uint8_t r[256] = {0};
+        r[0] = (uint8_t)b;
+        lfsr_zero_step(rs, r);
+        for (unsigned s = 0; s < 2; ++s) {
+            // halfword-paired: dword j = symbols pair j | pair (j + 8) << 16
+            uint32_t row[8] = {0};
+            for (unsigned j = 0; j < E; ++j) {
+                const unsigned pair = j / 2;
+                row[pair % 8] |= (uint32_t)r[j] << (8 * (j % 2) + 16 * (pair / 8));
+            }
+            for (unsigned h = 0; h < 2; ++h)
+                for (unsigned c = 0; c < (unsigned)LUT_COPIES; ++c)
+                    memcpy(&t[((((size_t)s * 256 + b) * 2 + h) * LUT_COPIES + c) * 4], row + 4 * h, 16);
+            lfsr_zero_step(rs, r);
+        }
+    }
+    return t;
+}
+
 // Workgroup sizes of the LFSR kernels. More resident waves do not help them: with 512- and
 // 768-lane groups (twice and three times the waves per CU) the ecc-32 kernels ran 5% and
 // 33% slower on MI355X, the scattered per-lane row reads contending for the L1 cache.
 constexpr int lfsr_wg(int) { return 256; }
+// The replicated table fills 128 KB of LDS, one workgroup per CU, which therefore takes 1024
+// lanes (measured at ecc 32 on MI355X: encode 124 us with 1024 lanes, 126 with 512, 190 with
+// 256; syndromes 150, 159 and 255 us).
+constexpr int lfsr_wg(int EW, int S) { return S == LUT_REP ? 1024 : lfsr_wg(EW); }
+template <int S>
+const uint32_t *lut_of(const ffrs_rs256 *rs) { return S == LUT_REP ? rs->d_lutr : rs->d_lut; }
 constexpr int synd_wg(int) { return 256; }
 
 template <int EW, int S>
 int launch_encode(ffrs_rs256 *rs, const uint8_t *d_msg, size_t count, uint32_t k, uint8_t *d_ecc)
 {
-    constexpr int WG = lfsr_wg(EW);
+    constexpr int WG = lfsr_wg(EW, S);
     ffrs_ctx *ctx = rs->ctx;
-    size_t lds = (size_t)S * 256 * lut_row(EW) * 4;
+    size_t lds = (size_t)lfsr_dyn_dwords(EW, S) * 4;
     size_t tiles = (count + WG - 1) / WG;
-    int wgs_per_cu = (int)std::min<size_t>(2048 / WG, std::max<size_t>(1, (160 * 1024) / lds));
+    int wgs_per_cu = (int)std::min<size_t>(2048 / WG, std::max<size_t>(1, (160 * 1024) / (lut_dwords(EW, S) * 4)));
     int grid = persistent_grid(ctx, wgs_per_cu, tiles);
     if (int r = allow_lds(rs256_encode_kernel<EW, S, WG>, lds))
         return r;
     hipLaunchKernelGGL((rs256_encode_kernel<EW, S, WG>), dim3(grid), dim3(WG), lds, ctx->stream, d_msg,
-                       (uint32_t)(count * k), (uint32_t)count, k, d_ecc, rs->ecc, rs->d_lut);
+                       (uint32_t)(count * k), (uint32_t)count, k, d_ecc, rs->ecc, lut_of<S>(rs));
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
     return FFRS_OK;
@@ -1358,14 +1529,14 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     uint8_t *d_rec = (uint8_t *)rs->d_work + ((synd_bytes + 255) & ~(size_t)255);
     Rs256Tables tab{rs->d_lut, rs->d_exp2, rs->d_log2};
 
-    constexpr int WG1 = synd_wg(EW);
-    size_t lds1 = (size_t)S * 256 * lut_row(EW) * 4 + (size_t)E * 32 * EW * 4;
+    constexpr int WG1 = S == LUT_REP ? lfsr_wg(EW, S) : synd_wg(EW);
+    size_t lds1 = (size_t)lfsr_dyn_dwords(EW, S) * 4 + (size_t)E * 32 * EW * 4;
     if (int r = allow_lds(rs256_synd_kernel<EW, S, WG1>, lds1))
         return r;
-    int g1 = persistent_grid(ctx, (int)std::min<size_t>(2048 / WG1, std::max<size_t>(1, (160 * 1024) / lds1)),
+    int g1 = persistent_grid(ctx, (int)std::min<size_t>(2048 / WG1, std::max<size_t>(1, (160 * 1024) / (lds1 + (lut_dwords(EW, S) - lfsr_dyn_dwords(EW, S)) * 4))),
                              (count + WG1 - 1) / WG1);
     hipLaunchKernelGGL((rs256_synd_kernel<EW, S, WG1>), dim3(g1), dim3(WG1), lds1, ctx->stream, d_msg,
-                       (uint32_t)(count * k), d_ecc, (uint32_t)count, k, E, d_status, d_synd, rs->d_lut, rs->d_stab);
+                       (uint32_t)(count * k), d_ecc, (uint32_t)count, k, E, d_status, d_synd, lut_of<S>(rs), rs->d_stab);
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
 
@@ -1419,7 +1590,7 @@ int encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *d_msg, size
         case 1: r = launch_encode<1, 4>(rs, m, c, k, e); break;
         case 2: r = launch_encode<2, 4>(rs, m, c, k, e); break;
         case 4: r = launch_encode<4, 4>(rs, m, c, k, e); break;
-        case 8: r = launch_encode<8, 4>(rs, m, c, k, e); break;
+        case 8: r = rs->d_lutr ? launch_encode<8, LUT_REP>(rs, m, c, k, e) : launch_encode<8, 4>(rs, m, c, k, e); break;
         case 16: r = launch_encode<16, 2>(rs, m, c, k, e); break;
         case 32: r = launch_encode<32, 1>(rs, m, c, k, e); break;
         default: r = launch_encode<64, 1>(rs, m, c, k, e); break;
@@ -1451,7 +1622,11 @@ int repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d
         case 1: r = pipe ? launch_repair_pipeline<1, 4>(rs, m, e, c, k, st) : launch_repair<1, 4, 256>(rs, m, e, c, k, st); break;
         case 2: r = pipe ? launch_repair_pipeline<2, 4>(rs, m, e, c, k, st) : launch_repair<2, 4, 256>(rs, m, e, c, k, st); break;
         case 4: r = pipe ? launch_repair_pipeline<4, 4>(rs, m, e, c, k, st) : launch_repair<4, 4, 256>(rs, m, e, c, k, st); break;
-        case 8: r = pipe ? launch_repair_pipeline<8, 4>(rs, m, e, c, k, st) : launch_repair<8, 4, 128>(rs, m, e, c, k, st); break;
+        case 8:
+            r = !pipe ? launch_repair<8, 4, 128>(rs, m, e, c, k, st)
+                : rs->d_lutr ? launch_repair_pipeline<8, LUT_REP>(rs, m, e, c, k, st)
+                : launch_repair_pipeline<8, 4>(rs, m, e, c, k, st);
+            break;
         case 16: r = launch_repair<16, 2, 128>(rs, m, e, c, k, st); break;
         case 32: r = launch_repair<32, 1, 64>(rs, m, e, c, k, st); break;
         default: r = launch_repair<64, 1, 64>(rs, m, e, c, k, st); break;
@@ -1531,6 +1706,12 @@ int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsig
         e = hipMalloc(&rs->d_log2, 512);
     if (e == hipSuccess)
         e = hipMemcpy(rs->d_lut, lut.data(), lut.size() * 4, hipMemcpyHostToDevice);
+    if (e == hipSuccess && rs->EW == 8) {
+        std::vector<uint32_t> lr = build_lut_rep(*rs);
+        e = hipMalloc(&rs->d_lutr, lr.size() * 4);
+        if (e == hipSuccess)
+            e = hipMemcpy(rs->d_lutr, lr.data(), lr.size() * 4, hipMemcpyHostToDevice);
+    }
     if (e == hipSuccess)
         e = hipMemcpy(rs->d_exp2, exp2, EXP2_LEN, hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -1561,6 +1742,8 @@ void ffrs_rs256_destroy(ffrs_rs256 *rs)
     (void)hipSetDevice(rs->ctx->device);
     if (rs->d_lut)
         (void)hipFree(rs->d_lut);
+    if (rs->d_lutr)
+        (void)hipFree(rs->d_lutr);
     if (rs->d_exp2)
         (void)hipFree(rs->d_exp2);
     if (rs->d_log2)
