@@ -388,19 +388,25 @@ public:
             return RepairStatus::NoErrors;
         PY_ASSERT(m.size == message_len * interleave, std::to_string(m.size));
         PY_ASSERT(e.size == ecc_len * interleave, std::to_string(e.size));
-        if (interleave != 1)
-            throw std::runtime_error("repair with interleave > 1 is not supported by this build (the reference repairs "
-                                     "columns in coupled SIMD groups, pyrsi16.hpp:391-522)");
-        uint8_t st = 0;
-        int r;
+        if (interleave != 1 && vector_lanes())
+            throw std::runtime_error("repair with interleave > 1 and SIMD words is not supported by this build (the "
+                                     "reference repairs those columns in coupled SIMD groups, pyrsi16.hpp:391-522); "
+                                     "with simd_x4/x8/x16 False it repairs column by column");
+        std::vector<uint32_t> pos;
         if (error_pos) {
             PY_ASSERT(error_pos->size() <= ecc_len, "too many error positions");
-            std::vector<uint32_t> pos(error_pos->size());
+            pos.resize(error_pos->size());
             for (size_t i = 0; i < pos.size(); ++i) {
                 if ((*error_pos)[i] >= block_len)
                     throw py::value_error("error position " + std::to_string((*error_pos)[i]) + " >= block_len");
                 pos[i] = (uint32_t)(*error_pos)[i];
             }
+        }
+        if (interleave != 1)
+            return repair_columns(m.data, e.data, error_pos ? &pos : nullptr);
+        uint8_t st = 0;
+        int r;
+        if (error_pos) {
             py::gil_scoped_release nogil;
             r = ffrs_rsi16_repair_known(codec, m.data, e.data, pos.data(), pos.size(), &st);
         } else {
@@ -409,6 +415,50 @@ public:
         }
         check(r, "RSi16.repair");
         return status_of(st);
+    }
+
+    // Interleaved repair with the scalar words (pyrsi16.hpp:391-453 with SIMD_W = 1): column c
+    // is the codeword of symbols i * interleave + c. The reference repairs the columns one
+    // after another and writes each back after its repair; the call returns the largest
+    // status, and a column whose repair raises ends it with the earlier columns written back.
+    // Here all columns are repaired in one device call (known positions: one call per column)
+    // and written back up to the first one that raises.
+    RepairStatus repair_columns(uint16_t *msg, uint16_t *ecc, const std::vector<uint32_t> *pos)
+    {
+        const size_t il = interleave, k = message_len, E = ecc_len;
+        std::vector<uint16_t> cm(il * k), ce(il * E);
+        std::vector<uint8_t> st(il, 0);
+        for (size_t i = 0; i < k; ++i)
+            for (size_t c = 0; c < il; ++c)
+                cm[c * k + i] = msg[i * il + c];
+        for (size_t i = 0; i < E; ++i)
+            for (size_t c = 0; c < il; ++c)
+                ce[c * E + i] = ecc[i * il + c];
+        int r = FFRS_OK;
+        {
+            py::gil_scoped_release nogil;
+            if (pos) {
+                for (size_t c = 0; c < il && r == FFRS_OK; ++c) {
+                    r = ffrs_rsi16_repair_known(codec, &cm[c * k], &ce[c * E], pos->data(), pos->size(), &st[c]);
+                    if (st[c] == FFRS_RSI16_RAISES)
+                        break;
+                }
+            } else {
+                r = ffrs_rsi16_repair(codec, cm.data(), ce.data(), il, st.data());
+            }
+        }
+        check(r, "RSi16.repair");
+        uint8_t res = FFRS_RSI16_NO_ERRORS;
+        size_t done = 0;
+        for (; done < il && st[done] != FFRS_RSI16_RAISES; ++done)
+            res = std::max(res, st[done]);
+        for (size_t i = 0; i < k; ++i)
+            for (size_t c = 0; c < done; ++c)
+                msg[i * il + c] = cm[c * k + i];
+        for (size_t i = 0; i < E; ++i)
+            for (size_t c = 0; c < done; ++c)
+                ecc[i * il + c] = ce[c * E + i];
+        return status_of(done < il ? (uint8_t)FFRS_RSI16_RAISES : res);
     }
 
     // Extension: repair `count` codewords (message / ecc buffers back to back) in one pass.

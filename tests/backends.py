@@ -61,6 +61,27 @@ class OracleBackend:
     def rsi16_encode_interleaved(self, n, ecc, il, blocks, vector=True):
         return _interleaved_by_columns(lambda c: self.ri(n, ecc).encode(c, vector), n, ecc, il, blocks)
 
+    def rsi16_repair_interleaved(self, n, ecc, il, msg, parity, positions=None):
+        """The scalar-word interleaved repair as the reference runs it (pyrsi16.hpp:391-453 with
+        SIMD_W = 1): column after column, each written back after its repair, the status the
+        maximum; a column whose repair raises ends the call (255) with the columns before it
+        written back."""
+        k = n - ecc
+        M = msg.reshape(k, il)
+        P = parity.reshape(ecc, il)
+        status = 0
+        for c in range(il):
+            m = np.ascontiguousarray(M[:, c])
+            p = np.ascontiguousarray(P[:, c])
+            st = (self.ri(n, ecc).repair_known(m, p, positions) if positions is not None
+                  else self.ri(n, ecc).repair(m, p))
+            if st == 255:
+                return 255
+            M[:, c] = m
+            P[:, c] = p
+            status = max(status, st)
+        return status
+
     def rsi16_sugiyama(self, n, ecc, synds):
         L, loc, ev = self.ri(n, ecc).sugiyama(synds)
         return None if L == 0xFFFFFFFF else (loc, ev)
@@ -178,6 +199,16 @@ class GpuBackend:
             self._ri[key] = self.L.RSi16(n, ecc, il, **kw)
         out = self._ri[key].encode(np.ascontiguousarray(blocks, np.uint16).tobytes())
         return np.frombuffer(bytes(out), np.uint16).reshape(-1, ecc * il).copy()
+
+    def rsi16_repair_interleaved(self, n, ecc, il, msg, parity, positions=None):
+        key = (n, ecc, il, False)
+        if key not in self._ri:
+            self._ri[key] = self.L.RSi16(n, ecc, il, simd_x4=False, simd_x8=False, simd_x16=False)
+        kw = {} if positions is None else dict(error_pos=[int(v) for v in positions])
+        try:
+            return int(self._ri[key].repair(msg.view(np.uint8), parity.view(np.uint8), **kw))
+        except RuntimeError:
+            return 255
 
     def rsi16_synds(self, n, ecc, msg, parity):
         return np.array(self.ri(n, ecc)._synd(np.ascontiguousarray(msg, np.uint16).tobytes(),

@@ -279,9 +279,68 @@ def interleave_fixture(L):
     return out
 
 
+IL_REPAIR_CASES = ((64, 8, 3), (256, 32, 5), (64, 16, 33), (1024, 128, 2))
+
+
+def interleave_repair_fixture(L):
+    """(f) interleaved repair with the scalar words (simd_x4/x8/x16 False), where the reference
+    repairs the columns one after another (pyrsi16.hpp:391-453 with SIMD_W = 1): per trial the
+    corrupted interleaved message and ecc, the error positions handed over (-1 padded; none
+    in the first half of the trials), the returned status (255: the call raises) and both
+    buffers after the call. Columns carry 0 .. ecc/2 + 2 errors, so some lie beyond the
+    radius."""
+    out = {}
+    rng = np.random.default_rng(23)
+    T = 8
+    for n, ecc, il in IL_REPAIR_CASES:
+        k = n - ecc
+        ri = L.RSi16(n, ecc, il, simd_x4=False, simd_x8=False, simd_x16=False)
+        msgs = rng.integers(0, 65536, size=(T, k * il), dtype=np.uint16)
+        pars = np.stack([np.frombuffer(ri.encode(m.tobytes()), np.uint16) for m in msgs])
+        m_in, e_in = msgs.copy(), pars.copy()
+        positions = np.full((T, ecc), -1, np.int64)
+        for t in range(T):
+            known = t >= T // 2
+            if known:
+                npos = int(rng.integers(1, ecc + 1))
+                pos = np.sort(rng.choice(n, size=npos, replace=False))
+                positions[t, :npos] = pos
+            for c in range(il):
+                if known:
+                    cand = positions[t][positions[t] >= 0]
+                    ne = int(rng.integers(0, len(cand) + 1))
+                    where = rng.choice(cand, size=ne, replace=False)
+                else:
+                    ne = int(rng.integers(0, ecc // 2 + 3))
+                    where = rng.choice(n, size=ne, replace=False)
+                for p in where:
+                    v = np.uint16(rng.integers(1, 65536))
+                    if p < k:
+                        m_in[t, p * il + c] ^= v
+                    else:
+                        e_in[t, (p - k) * il + c] ^= v
+        status = np.zeros(T, np.uint8)
+        m_out, e_out = m_in.copy(), e_in.copy()
+        for t in range(T):
+            mb, eb = bytearray(m_in[t].tobytes()), bytearray(e_in[t].tobytes())
+            pos = [int(p) for p in positions[t] if p >= 0]
+            try:
+                status[t] = int(ri.repair(mb, eb, pos) if pos else ri.repair(mb, eb))
+            except RuntimeError:
+                status[t] = 255
+            m_out[t] = np.frombuffer(mb, np.uint16)
+            e_out[t] = np.frombuffer(eb, np.uint16)
+        out[f"rsi16_ilrepair_{n}_{ecc}_{il}"] = dict(msgs=m_in, eccs=e_in, positions=positions, status=status,
+                                                     msgs_out=m_out, eccs_out=e_out)
+    return out
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["interleave"]:
         for key, v in interleave_fixture(load_reference()).items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    elif sys.argv[1:] == ["interleave_repair"]:
+        for key, v in interleave_repair_fixture(load_reference()).items():
             np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
     elif sys.argv[1:] == ["radius"]:
         from libffrs_amd import workloads as W  # noqa: F811

@@ -206,3 +206,19 @@ def check_rsi16_interleaved(be):
                 assert (got.reshape(-1) == p).all(), (n, ecc, il, vector)
             got = be.rsi16_encode_interleaved(n, ecc, il, fx["msgs"], vector)
             assert (got.reshape(fx[key].shape) == fx[key]).all(), (n, ecc, il, vector, "multi-block")
+
+
+def check_rsi16_interleaved_repair(be):
+    """Interleaved repair with the scalar words against the reference: status and both buffers
+    after the call, with and without error positions."""
+    from make_golden import IL_REPAIR_CASES
+    for n, ecc, il in IL_REPAIR_CASES:
+        fx = fixture(f"rsi16_ilrepair_{n}_{ecc}_{il}")
+        for t in range(len(fx["status"])):
+            m = fx["msgs"][t].copy()
+            e = fx["eccs"][t].copy()
+            pos = fx["positions"][t]
+            pos = pos[pos >= 0]
+            st = be.rsi16_repair_interleaved(n, ecc, il, m, e, pos if len(pos) else None)
+            assert st == fx["status"][t], (n, ecc, il, t, st, fx["status"][t])
+            assert (m == fx["msgs_out"][t]).all() and (e == fx["eccs_out"][t]).all(), (n, ecc, il, t)

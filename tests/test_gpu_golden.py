@@ -55,9 +55,14 @@ def test_rsi16_interleaved(gpu):
     R.check_rsi16_interleaved(gpu)
 
 
-def test_rsi16_interleaved_repair_raises(gpu):
+def test_rsi16_interleaved_repair(gpu):
+    R.check_rsi16_interleaved_repair(gpu)
+
+
+def test_rsi16_interleaved_repair_simd_words_raises(gpu):
+    """With the SIMD words the reference repairs columns in coupled SIMD groups; not built."""
     import numpy as np
-    ri = gpu.L.RSi16(64, 8, 3)
+    ri = gpu.L.RSi16(64, 8, 3, simd_x4=True)
     assert ri.message_len == 56 * 3 and ri.ecc_len == 8 * 3 and ri.interleave == 3
     m = np.zeros(56 * 3, np.uint16)
     e = np.zeros(8 * 3, np.uint16)

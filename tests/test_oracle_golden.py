@@ -38,3 +38,8 @@ def test_rsi16_known(oracle_backend):
 
 def test_rsi16_interleaved(oracle_backend):
     R.check_rsi16_interleaved(oracle_backend)
+
+
+def test_rsi16_interleaved_repair(oracle_backend):
+    """The reference's scalar-word interleaved repair equals its column-by-column repair."""
+    R.check_rsi16_interleaved_repair(oracle_backend)
