@@ -221,6 +221,29 @@ __device__ inline void lut_step(uint32_t (&rem)[EW], uint32_t w, const uint32_t 
     }
 }
 
+// A remainder of the replicated layout (halfword-paired, halves swapped on swapped lanes)
+// back in byte order.
+__device__ inline void rep_to_bytes(uint32_t (&rem)[8])
+{
+    const bool swapped = lut_swapped();
+#pragma unroll
+    for (int d = 0; d < 4; ++d) {
+        const uint32_t lo = rem[d], hi = rem[d + 4];
+        rem[d] = swapped ? hi : lo;
+        rem[d + 4] = swapped ? lo : hi;
+    }
+    // halfword-paired -> byte order: dword m = h_2m | h_(2m+1) << 16
+    uint32_t o[8];
+#pragma unroll
+    for (int m = 0; m < 4; ++m) {
+        o[m] = __builtin_amdgcn_perm(rem[2 * m + 1], rem[2 * m], 0x05040100u);
+        o[m + 4] = __builtin_amdgcn_perm(rem[2 * m + 1], rem[2 * m], 0x07060302u);
+    }
+#pragma unroll
+    for (int d = 0; d < 8; ++d)
+        rem[d] = o[d];
+}
+
 // Remainder of x^ecc * M(x) by g(x) for the message of `k` bytes at byte offset
 // `msg_off` of the buffer behind `rsrc` (whose range is `nrec` bytes). The message is fed in 4-byte words after
 // `pad` leading zero bytes (leading zeros do not change the remainder).
@@ -242,9 +265,9 @@ __device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, uint32_t n
     uint32_t prev = buf_load_u32(rsrc, off);
     uint32_t t = 0;
     if constexpr (S == LUT_REP) {
-        // 64 bytes per iteration, the next 64 loaded before the current ones are fed: a lane
-        // visits each cache line of its message at most twice, and the loads overlap the
-        // table reads
+        // 64 bytes per iteration, the 128 bytes after them already in flight while they are fed:
+        // a lane visits each cache line of its message at most twice, and the loads have two
+        // iterations of table reads to arrive
         constexpr int CW = 16;
         auto load = [&](uint32_t tt, uint32_t (&c)[CW]) {
             const uint32_t o = off + 4 * (tt + 1);
@@ -260,29 +283,57 @@ __device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, uint32_t n
                     c[j] = buf_load_u32(rsrc, o + 4 * j);
             }
         };
+        // the last nw % CW words, loaded up front with the first chunks
+        const uint32_t full = nw - nw % CW;
+        uint32_t ct[CW];
+        if (full < nw)
+            load(full, ct);
         if (t + CW <= nw) {
-            uint32_t cur[CW];
-            load(t, cur);
+            uint32_t c0[CW], c1[CW];
+            load(t, c0);
+            bool have1 = t + 2 * CW <= nw;
+            if (have1)
+                load(t + CW, c1);
             for (;;) {
-                const bool more = t + 2 * CW <= nw;
-                uint32_t nxt[CW];
-                if (more)
-                    load(t + CW, nxt);
-                uint32_t w0 = alignbyte(cur[0], prev, sh);
+                const bool have2 = t + 3 * CW <= nw;
+                uint32_t c2[CW];
+                if (have2) {
+                    load(t + 2 * CW, c2);
+                } else {
+#pragma unroll
+                    for (int j = 0; j < CW; ++j)
+                        c2[j] = 0;
+                }
+                uint32_t w0 = alignbyte(c0[0], prev, sh);
                 if (t == 0)
                     w0 &= headmask;
                 lut_step<EW, S>(rem, w0, s_lut);
 #pragma unroll
                 for (int j = 1; j < CW; ++j)
-                    lut_step<EW, S>(rem, alignbyte(cur[j], cur[j - 1], sh), s_lut);
-                prev = cur[CW - 1];
+                    lut_step<EW, S>(rem, alignbyte(c0[j], c0[j - 1], sh), s_lut);
+                prev = c0[CW - 1];
                 t += CW;
-                if (!more)
+                if (!have1)
                     break;
 #pragma unroll
-                for (int j = 0; j < CW; ++j)
-                    cur[j] = nxt[j];
+                for (int j = 0; j < CW; ++j) {
+                    c0[j] = c1[j];
+                    c1[j] = c2[j];
+                }
+                have1 = have2;
             }
+        }
+        if (full < nw) {
+#pragma unroll
+            for (int j = 0; j < CW; ++j) {
+                if (full + j < nw) {
+                    uint32_t w = alignbyte(ct[j], j ? ct[j - 1] : prev, sh);
+                    if (full + j == 0)
+                        w &= headmask;
+                    lut_step<EW, S>(rem, w, s_lut);
+                }
+            }
+            t = nw;
         }
     }
     // 32 bytes per iteration, as two 16-byte loads where all of them are in range (each lane
@@ -318,26 +369,8 @@ __device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, uint32_t n
         lut_step<EW, S>(rem, w, s_lut);
         prev = c;
     }
-    if constexpr (S == LUT_REP) {
-        // swapped lanes: registers back in dword order
-        const bool swapped = lut_swapped();
-#pragma unroll
-        for (int d = 0; d < 4; ++d) {
-            const uint32_t lo = rem[d], hi = rem[d + 4];
-            rem[d] = swapped ? hi : lo;
-            rem[d + 4] = swapped ? lo : hi;
-        }
-        // halfword-paired -> byte order: dword m = h_2m | h_(2m+1) << 16
-        uint32_t o[8];
-#pragma unroll
-        for (int m = 0; m < 4; ++m) {
-            o[m] = __builtin_amdgcn_perm(rem[2 * m + 1], rem[2 * m], 0x05040100u);
-            o[m + 4] = __builtin_amdgcn_perm(rem[2 * m + 1], rem[2 * m], 0x07060302u);
-        }
-#pragma unroll
-        for (int d = 0; d < 8; ++d)
-            rem[d] = o[d];
-    }
+    if constexpr (S == LUT_REP)
+        rep_to_bytes(rem);
 }
 
 template <int EW>
