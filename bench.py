@@ -238,10 +238,10 @@ def run_plain(name, args):
     value = msg_bytes / (ms_step * 1e-3) / 1e9
     if rs256:
         enc_names = ["rs256_encode_kernel"]
-        rep_names = (["rs256_synd_kernel", "rs256_bm_fixed_kernel", "rs256_chien_kernel"] if ecc <= 32
+        # ecc 2, 4, 8, 16, 32 run BM inside the Chien kernel; other ecc <= 32 launch it on its own
+        rep_names = (["rs256_synd_kernel", "rs256_chien_kernel"] if ecc in (2, 4, 8, 16, 32)
+                     else ["rs256_synd_kernel", "rs256_bm_kernel", "rs256_chien_kernel"] if ecc <= 32
                      else ["rs256_repair_kernel"])
-        if ecc <= 32 and ecc not in (2, 4, 8, 16, 32):
-            rep_names[1] = "rs256_bm_kernel"
     else:
         enc_names = ["rsi16_encode_lane_kernel" if ecc <= 32 else
                      ("rsi16_encode_reg_kernel" if ecc <= 256 else "rsi16_encode_kernel")]

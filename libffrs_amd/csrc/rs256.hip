@@ -958,109 +958,89 @@ __global__ __launch_bounds__(WG) void rs256_bm_kernel(uint32_t count, uint32_t E
 // iteration n deg Lambda <= n + 1, which bounds the unrolled work. Same results as the
 // reference's reversed-storage version (reed_solomon.hpp:655-702), whose truncation of
 // degrees >= E never matters while L <= E / 2.
-template <int E, int WG>
-__global__ __launch_bounds__(WG) void rs256_bm_fixed_kernel(uint32_t count, uint8_t *__restrict__ status,
-                                                             const uint32_t *__restrict__ synd,
-                                                             uint8_t *__restrict__ rec, Rs256Tables tab)
+// Degree-bounded BM of one codeword from its packed syndromes sw[0..EW) (the tables in LDS).
+// Returns false when L > E/2 (not repairable); otherwise the 48-byte record in words[]:
+// L in byte 0, Omega_0..Omega_(E/2-1) from byte rec_om_off, Lambda_0..Lambda_(E/2) from byte
+// rec_lam_off.
+template <int E>
+__device__ inline bool bm_fixed_one(const uint32_t *__restrict__ sw, const uint8_t *s_exp2, const uint16_t *s_log2,
+                                    uint32_t (&words)[12])
 {
     constexpr int EW = (E + 3) / 4;
-    extern __shared__ uint32_t s_mem[];
-    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_mem);
-    uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
-    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
-        s_exp2[i] = tab.exp2[i];
-    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
-        s_log2[i] = tab.log2[i];
-    __syncthreads();
-    const uint32_t RS = rec_stride(E), LO = rec_lam_off(), OO = rec_om_off(E);
-
-    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += gridDim.x * blockDim.x) {
-        if (status[cw] != RS_PENDING)
-            continue;
-        uint32_t S[E], lS[E];
-        const uint32_t *sw = synd + (size_t)cw * EW;
+    const uint32_t LO = rec_lam_off(), OO = rec_om_off(E);
+    // the syndromes stay packed four to a register (S_n is extracted where it is used)
+    uint32_t sv[EW], lS[E];
 #pragma unroll
-        for (int w = 0; w < EW; ++w) {
-            uint32_t v = sw[w];
+    for (int w = 0; w < EW; ++w) {
+        sv[w] = sw[w];
 #pragma unroll
-            for (int b = 0; b < 4; ++b)
-                if (w * 4 + b < E) {
-                    S[w * 4 + b] = (v >> (8 * b)) & 0xffu;
-                    lS[w * 4 + b] = s_log2[S[w * 4 + b]];
-                }
-        }
-        uint32_t lam[E], llam[E], lBp[E];
-#pragma unroll
-        for (int i = 0; i < E; ++i) {
-            lam[i] = i == 0 ? 1u : 0u;
-            llam[i] = i == 0 ? 0u : (uint32_t)ZLOG;
-            lBp[i] = i == 1 ? 0u : (uint32_t)ZLOG;
-        }
-        // L never decreases, and a codeword whose L exceeds E/2 fails whatever follows; while
-        // L <= E/2, deg Lambda <= L and every x^m B that enters an update has degree <= the new
-        // L. So terms above E/2 (above E/2 + 1 for B') never reach a successful result.
-        constexpr int H = E / 2;
-        uint32_t L = 0, lb = 0;
-#pragma unroll
-        for (int n = 0; n < E; ++n) {
-            uint32_t d = S[n];
-#pragma unroll
-            for (int i = 1; i <= (n < H ? n : H); ++i)
-                d ^= s_exp2[llam[i] + lS[n - i]];
-            bool nz = d != 0;
-            bool grow = nz && (2 * L <= (uint32_t)n);
-            uint32_t ld = s_log2[d];
-            uint32_t lc = ld + 255 - lb;
-            lc = lc >= 255 ? lc - 255 : lc;
-            const int hi = n + 1 < H ? n + 1 : H;
-            uint32_t old_llam[E];
-#pragma unroll
-            for (int i = 0; i < E; ++i)
-                old_llam[i] = llam[i];
-#pragma unroll
-            for (int i = 1; i <= hi; ++i) {
-                uint32_t t = nz ? (uint32_t)s_exp2[lc + lBp[i]] : 0u;
-                lam[i] ^= t;
-                llam[i] = s_log2[lam[i]];
-            }
-            const int bh = n + 2 < H + 1 ? n + 2 : (H + 1 < E - 1 ? H + 1 : E - 1);
-#pragma unroll
-            for (int i = bh; i >= 1; --i)
-                lBp[i] = grow ? old_llam[i - 1] : lBp[i - 1];
-            lBp[0] = ZLOG;
-            if (grow) {
-                L = (uint32_t)n + 1 - L;
-                lb = ld;
-            }
-        }
-        if (L > (uint32_t)E / 2) {
-            status[cw] = 0;
-            continue;
-        }
-        uint8_t *r = rec + (size_t)cw * RS;
-        uint32_t words[64];
-#pragma unroll
-        for (int w = 0; w < 64; ++w)
-            words[w] = 0;
-        words[0] = L;
-#pragma unroll
-        for (int i = 0; i <= E / 2; ++i)
-            words[(LO + i) >> 2] |= (i <= (int)E ? lam[i < E ? i : 0] : 0u) << (8 * ((LO + i) & 3));
-#pragma unroll
-        for (int kk = 0; kk < E / 2; ++kk) {
-            uint32_t v = 0;
-#pragma unroll
-            for (int i = 0; i <= kk; ++i)
-                v ^= s_exp2[llam[i] + lS[kk - i]];
-            v = (uint32_t)kk < L ? v : 0u;
-            words[(OO + kk) >> 2] |= v << (8 * ((OO + kk) & 3));
-        }
-        uint32_t *rw = reinterpret_cast<uint32_t *>(r);
-#pragma unroll
-        for (int w = 0; w < 64; ++w)
-            if ((uint32_t)w * 4 < RS)
-                rw[w] = words[w];
+        for (int b = 0; b < 4; ++b)
+            if (w * 4 + b < E)
+                lS[w * 4 + b] = s_log2[(sv[w] >> (8 * b)) & 0xffu];
     }
+    uint32_t lam[E], llam[E], lBp[E];
+#pragma unroll
+    for (int i = 0; i < E; ++i) {
+        lam[i] = i == 0 ? 1u : 0u;
+        llam[i] = i == 0 ? 0u : (uint32_t)ZLOG;
+        lBp[i] = i == 1 ? 0u : (uint32_t)ZLOG;
+    }
+    // L never decreases, and a codeword whose L exceeds E/2 fails whatever follows; while
+    // L <= E/2, deg Lambda <= L and every x^m B that enters an update has degree <= the new
+    // L. So terms above E/2 (above E/2 + 1 for B') never reach a successful result.
+    constexpr int H = E / 2;
+    uint32_t L = 0, lb = 0;
+#pragma unroll
+    for (int n = 0; n < E; ++n) {
+        uint32_t d = (sv[n >> 2] >> (8 * (n & 3))) & 0xffu;
+#pragma unroll
+        for (int i = 1; i <= (n < H ? n : H); ++i)
+            d ^= s_exp2[llam[i] + lS[n - i]];
+        bool nz = d != 0;
+        bool grow = nz && (2 * L <= (uint32_t)n);
+        uint32_t ld = s_log2[d];
+        uint32_t lc = ld + 255 - lb;
+        lc = lc >= 255 ? lc - 255 : lc;
+        const int hi = n + 1 < H ? n + 1 : H;
+        uint32_t old_llam[E];
+#pragma unroll
+        for (int i = 0; i < E; ++i)
+            old_llam[i] = llam[i];
+#pragma unroll
+        for (int i = 1; i <= hi; ++i) {
+            uint32_t t = nz ? (uint32_t)s_exp2[lc + lBp[i]] : 0u;
+            lam[i] ^= t;
+            llam[i] = s_log2[lam[i]];
+        }
+        const int bh = n + 2 < H + 1 ? n + 2 : (H + 1 < E - 1 ? H + 1 : E - 1);
+#pragma unroll
+        for (int i = bh; i >= 1; --i)
+            lBp[i] = grow ? old_llam[i - 1] : lBp[i - 1];
+        lBp[0] = ZLOG;
+        if (grow) {
+            L = (uint32_t)n + 1 - L;
+            lb = ld;
+        }
+    }
+    if (L > (uint32_t)E / 2)
+        return false;
+#pragma unroll
+    for (int w = 0; w < 12; ++w)
+        words[w] = 0;
+    words[0] = L;
+#pragma unroll
+    for (int i = 0; i <= E / 2; ++i)
+        words[(LO + i) >> 2] |= (i <= (int)E ? lam[i < E ? i : 0] : 0u) << (8 * ((LO + i) & 3));
+#pragma unroll
+    for (int kk = 0; kk < E / 2; ++kk) {
+        uint32_t v = 0;
+#pragma unroll
+        for (int i = 0; i <= kk; ++i)
+            v ^= s_exp2[llam[i] + lS[kk - i]];
+        v = (uint32_t)kk < L ? v : 0u;
+        words[(OO + kk) >> 2] |= v << (8 * ((OO + kk) & 3));
+    }
+    return true;
 }
 
 // XOR a byte in global memory without reading it: an atomic XOR on the aligned dword
@@ -1087,10 +1067,15 @@ __device__ inline void xor_byte(uint8_t *p, uint32_t v)
 // div(Omega(X^-1), Lambda_odd(X^-1)), or X Omega(X^-1) when Lambda_odd = 0 (its
 // div(a, 0) = a). Omega(X^-1) = sum_k Omega_k alpha^(-p k) is summed from independent
 // log-domain terms.
-template <int WG>
+// EB > 0: the kernel first runs the degree-bounded BM (bm_fixed_one<EB>) of each pending codeword
+// of a tile, one codeword per lane, from its syndromes `synd`, and keeps the record in registers
+// (the lookup-bound BM and the issue-bound Chien loop of other waves then share each CU);
+// EB == 0: the records come from `rec`.
+template <int WG, int EB>
 __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ msgs, uint8_t *__restrict__ eccs,
                                                           uint32_t count, uint32_t k, uint32_t E,
                                                           uint8_t *__restrict__ status, const uint8_t *__restrict__ rec,
+                                                          const uint32_t *__restrict__ synd,
                                                           const uint32_t *__restrict__ ctab, Rs256Tables tab)
 {
     extern __shared__ uint32_t s_mem[];
@@ -1118,12 +1103,26 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
     for (uint32_t base = blockIdx.x * WG + wave * 64; base < count; base += gridDim.x * WG) {
         uint32_t cw = base + lane;
         bool pend = cw < count && status[cw] == RS_PENDING;
+        // the tile's records, one per lane; per codeword read back by readlane
+        uint32_t recw[10];
+        if constexpr (EB > 0) {
+            uint32_t words[12];
+#pragma unroll
+            for (int w = 0; w < 12; ++w)
+                words[w] = 0;
+            if (pend && !bm_fixed_one<EB>(synd + (size_t)cw * ((EB + 3) / 4), s_exp2, s_log2, words)) {
+                status[cw] = 0;
+                pend = false;
+            }
+#pragma unroll
+            for (int w = 0; w < 10; ++w)
+                recw[w] = words[w];
+        }
         uint64_t pend_mask = __ballot(pend);
         if (!pend_mask)
             continue;
-        // the tile's records, one per lane (coalesced); per codeword read back by readlane
-        uint32_t recw[10];
-        {
+        if constexpr (EB == 0) {
+            // coalesced record loads
             const uint4 *r4 = reinterpret_cast<const uint4 *>(rec32 + (size_t)(cw < count ? cw : 0) * RSW);
             uint4 a = r4[0], b = r4[1], c = r4[2];
             recw[0] = a.x; recw[1] = a.y; recw[2] = a.z; recw[3] = a.w;
@@ -1573,35 +1572,39 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
 
-    if (E == 2 || E == 4 || E == 8 || E == 16 || E == 32) {
-        size_t lds2 = EXP2_LEN + 512;
-        int g2 = persistent_grid(ctx, 4, (count + 255) / 256);
-        auto k2 = E == 2 ? rs256_bm_fixed_kernel<2, 256> : E == 4 ? rs256_bm_fixed_kernel<4, 256>
-                  : E == 8 ? rs256_bm_fixed_kernel<8, 256> : E == 16 ? rs256_bm_fixed_kernel<16, 256>
-                  : rs256_bm_fixed_kernel<32, 256>;
-        hipLaunchKernelGGL(k2, dim3(g2), dim3(256), lds2, ctx->stream, (uint32_t)count, d_status,
-                           (const uint32_t *)d_synd, d_rec, tab);
-    } else {
-        size_t lds2 = EXP2_LEN + 512 + (size_t)256 * bm_row_stride(E);
-        if (int r = allow_lds(rs256_bm_kernel<256>, lds2))
+    // 15 waves: the tables and root slots just fit in 160 KB; 11 with the ecc-32 BM inside,
+    // whose registers (145) exceed the 128 that four waves per SIMD leave (measured at 1M x
+    // RS(255,223) on MI355X: 565 us with 11 waves, 581 with 12, 628 with 10, 659 with 8)
+    constexpr int WG3 = 960, WG3B = 704;
+    auto chien = [&](auto kern, int wg) -> int {
+        size_t lds3 = CHIEN_TABLE_BYTES + (size_t)wg * CHIEN_LMAX * 2 + EXP2_LEN + 512;
+        int g3 = persistent_grid(ctx, 1, (count + wg - 1) / wg);
+        if (int r = allow_lds(kern, lds3))
             return r;
-        int g2 = persistent_grid(ctx, (int)std::max<size_t>(1, (160 * 1024) / lds2), (count + 255) / 256);
-        hipLaunchKernelGGL((rs256_bm_kernel<256>), dim3(g2), dim3(256), lds2, ctx->stream, (uint32_t)count, E,
-                           (uint32_t)EW, d_status, (const uint32_t *)d_synd, d_rec, tab);
+        hipLaunchKernelGGL(kern, dim3(g3), dim3(wg), lds3, ctx->stream, d_msg, d_ecc, (uint32_t)count, k, E,
+                           d_status, (const uint8_t *)d_rec, (const uint32_t *)d_synd, rs->d_ctab, tab);
+        FFRS_HIP(hipGetLastError());
+        mark_kernel(ctx);
+        return FFRS_OK;
+    };
+    // ecc 2, 4, 8, 16, 32: BM inside the Chien kernel
+    switch (E) {
+    case 2: return chien(rs256_chien_kernel<WG3, 2>, WG3);
+    case 4: return chien(rs256_chien_kernel<WG3, 4>, WG3);
+    case 8: return chien(rs256_chien_kernel<WG3, 8>, WG3);
+    case 16: return chien(rs256_chien_kernel<WG3, 16>, WG3);
+    case 32: return chien(rs256_chien_kernel<WG3B, 32>, WG3B);
+    default: break;
     }
-    FFRS_HIP(hipGetLastError());
-    mark_kernel(ctx);
-
-    constexpr int WG3 = 960;  // 15 waves: the tables and root slots just fit in 160 KB
-    size_t lds3 = CHIEN_TABLE_BYTES + (size_t)WG3 * CHIEN_LMAX * 2 + EXP2_LEN + 512;
-    if (int r = allow_lds(rs256_chien_kernel<WG3>, lds3))
+    size_t lds2 = EXP2_LEN + 512 + (size_t)256 * bm_row_stride(E);
+    if (int r = allow_lds(rs256_bm_kernel<256>, lds2))
         return r;
-    int g3 = persistent_grid(ctx, 1, (count + WG3 - 1) / WG3);
-    hipLaunchKernelGGL((rs256_chien_kernel<WG3>), dim3(g3), dim3(WG3), lds3, ctx->stream, d_msg, d_ecc,
-                       (uint32_t)count, k, E, d_status, (const uint8_t *)d_rec, rs->d_ctab, tab);
+    int g2 = persistent_grid(ctx, (int)std::max<size_t>(1, (160 * 1024) / lds2), (count + 255) / 256);
+    hipLaunchKernelGGL((rs256_bm_kernel<256>), dim3(g2), dim3(256), lds2, ctx->stream, (uint32_t)count, E,
+                       (uint32_t)EW, d_status, (const uint32_t *)d_synd, d_rec, tab);
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
-    return FFRS_OK;
+    return chien(rs256_chien_kernel<WG3, 0>, WG3);
 }
 
 // Splits launches so that every launch addresses < 2 GiB of messages with 32-bit offsets.
