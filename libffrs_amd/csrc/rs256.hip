@@ -1078,19 +1078,23 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
                                                           const uint32_t *__restrict__ synd,
                                                           const uint32_t *__restrict__ ctab, Rs256Tables tab)
 {
+    // [CHIEN_LMAX][2][16][64] dwords at a known LDS address, so that the table offsets fold
+    // into the read instructions
+    __shared__ uint32_t s_ct[CHIEN_TABLE_BYTES / 4];
     extern __shared__ uint32_t s_mem[];
-    uint32_t *s_ct = s_mem;  // [CHIEN_LMAX][2][16][64] dwords
-    uint16_t *s_roots = reinterpret_cast<uint16_t *>(s_mem + CHIEN_TABLE_BYTES / 4);  // [WG/64][64][16]
+    uint16_t *s_roots = reinterpret_cast<uint16_t *>(s_mem);  // [WG/64][64][16]
     uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_roots + WG * CHIEN_LMAX);
     uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
-    for (uint32_t i = threadIdx.x; i < CHIEN_TABLE_BYTES / 4; i += blockDim.x)
-        s_ct[i] = ctab[i];
+    for (uint32_t i = threadIdx.x; i < CHIEN_TABLE_BYTES / 16; i += blockDim.x)
+        reinterpret_cast<uint4 *>(s_ct)[i] = reinterpret_cast<const uint4 *>(ctab)[i];
     for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
         s_exp2[i] = tab.exp2[i];
     for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
         s_log2[i] = tab.log2[i];
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63;
+    // byte offsets of this lane's column in the first and the second 64 KB of the table
+    const uint32_t lb_lo = lane * 4, lb_hi = lane * 4 + 0x10000u;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     uint16_t *my_roots = s_roots + wave * 64 * CHIEN_LMAX;
     const uint32_t n = k + E;
@@ -1129,6 +1133,19 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
             recw[4] = b.x; recw[5] = b.y; recw[6] = b.z; recw[7] = b.w;
             recw[8] = c.x; recw[9] = c.y;
         }
+        // Lambda_1..Lambda_16 of this lane's codeword as nibble bytes: dword m holds the low
+        // and high nibbles of Lambda_(2m+1), then of Lambda_(2m+2) (each the row of its table)
+        uint32_t nib[8];
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+            const uint32_t q = alignbyte(recw[6 + g], recw[5 + g], 1);  // Lambda_(4g+1..4g+4)
+#pragma unroll
+            for (int hw = 0; hw < 2; ++hw) {
+                const uint32_t t = (q >> (16 * hw)) & 0xffffu;
+                const uint32_t b = (t & 0xffu) | ((t & 0xff00u) << 8);   // bytes a, 0, b, 0
+                nib[2 * g + hw] = (b & 0x000f000fu) | ((b & 0x00f000f0u) << 4);
+            }
+        }
         uint64_t ok_mask = 0;
         // Count the roots of codeword o; when they number exactly L, store them as
         // (p, Lambda_odd(X^-1)) in the codeword's slots.
@@ -1159,27 +1176,33 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
             const uint32_t o = (uint32_t)__builtin_ctzll(pend_mask);
             pend_mask &= pend_mask - 1;
             const uint32_t Lo = __builtin_amdgcn_readlane(recw[0], o) & 0xffu;
-            uint32_t lamw[5];
-#pragma unroll
-            for (int w = 0; w < 5; ++w)
-                lamw[w] = __builtin_amdgcn_readlane(recw[5 + w], o);  // Lambda_0..Lambda_19
             // Lambda_i = 0 for L < i <= 16 (records are zero-filled) and row v = 0 of every
             // table is zero, so terms past L add nothing; whole groups of four past L are skipped.
+            // Row addresses: one byte permute puts the nibble (wave-uniform) in byte 1 over this
+            // lane's column offset; the table and nibble parts are immediate offsets.
             uint32_t v_even = 0x01010101u, v_odd = 0;
+            const char *ct = reinterpret_cast<const char *>(s_ct);
 #pragma unroll
             for (int g = 0; g < 4; ++g) {
                 if (4 * g + 1 > (int)Lo)
                     break;
+                const uint32_t nw[2] = {(uint32_t)__builtin_amdgcn_readlane(nib[2 * g], o),
+                                        (uint32_t)__builtin_amdgcn_readlane(nib[2 * g + 1], o)};
 #pragma unroll
                 for (int j = 0; j < 4; ++j) {
                     const int i = 4 * g + j + 1;
-                    const uint32_t li = (lamw[i >> 2] >> (8 * (i & 3))) & 0xffu;
-                    const uint32_t *row = s_ct + (i - 1) * 32 * 64 + lane;
-                    const uint32_t t = row[(li & 15) * 64] ^ row[(16 + (li >> 4)) * 64];
+                    uint32_t t[2];
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const int tbl = (i - 1) * 2 + h;                 // 4 KB table index
+                        const uint32_t sel = 0x0c020000u | ((4u + (uint32_t)(2 * (j & 1) + h)) << 8);
+                        const uint32_t a = __builtin_amdgcn_perm(nw[j >> 1], tbl >= 16 ? lb_hi : lb_lo, sel);
+                        t[h] = *reinterpret_cast<const uint32_t *>(ct + a + (tbl & 15) * 4096);
+                    }
                     if (i & 1)
-                        v_odd ^= t;
+                        v_odd = xor3(v_odd, t[0], t[1]);
                     else
-                        v_even ^= t;
+                        v_even = xor3(v_even, t[0], t[1]);
                 }
             }
             settle(o, Lo, v_even, v_odd);
@@ -1577,7 +1600,7 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     // RS(255,223) on MI355X: 565 us with 11 waves, 581 with 12, 628 with 10, 659 with 8)
     constexpr int WG3 = 960, WG3B = 704;
     auto chien = [&](auto kern, int wg) -> int {
-        size_t lds3 = CHIEN_TABLE_BYTES + (size_t)wg * CHIEN_LMAX * 2 + EXP2_LEN + 512;
+        size_t lds3 = (size_t)wg * CHIEN_LMAX * 2 + EXP2_LEN + 512;  // plus the static table
         int g3 = persistent_grid(ctx, 1, (count + wg - 1) / wg);
         if (int r = allow_lds(kern, lds3))
             return r;
