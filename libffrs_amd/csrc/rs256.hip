@@ -1100,10 +1100,15 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
     const uint32_t n = k + E;
     const uint32_t *rec32 = reinterpret_cast<const uint32_t *>(rec);
     const uint32_t RSW = rec_stride(E) / 4;
-    uint64_t valid[4];  // lanes whose position 4 lane + q lies in the codeword
+    bool in_cw[4];      // position 4 lane + q lies in the codeword
+    uint64_t valid[4];  // the same as lane masks
+    uint32_t pos[4];    // that position
 #pragma unroll
-    for (int q = 0; q < 4; ++q)
-        valid[q] = __builtin_amdgcn_ballot_w64(lane * 4 + q < n);
+    for (int q = 0; q < 4; ++q) {
+        pos[q] = lane * 4 + q;
+        in_cw[q] = pos[q] < n;
+        valid[q] = __builtin_amdgcn_ballot_w64(in_cw[q]);
+    }
     for (uint32_t base = blockIdx.x * WG + wave * 64; base < count; base += gridDim.x * WG) {
         uint32_t cw = base + lane;
         bool pend = cw < count && status[cw] == RS_PENDING;
@@ -1151,24 +1156,28 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
         // (p, Lambda_odd(X^-1)) in the codeword's slots.
         auto settle = [&](uint32_t o, uint32_t Lo, uint32_t v_even, uint32_t v_odd) {
             const uint32_t val = v_even ^ v_odd;
+            bool root[4];  // this lane's own predicate, which is also the ballot's exec mask
             uint64_t zb[4];
             uint32_t cnt = 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                zb[q] = __builtin_amdgcn_ballot_w64((val & (0xffu << (8 * q))) == 0) & valid[q];
+                const bool z = (val & (0xffu << (8 * q))) == 0;
+                zb[q] = __builtin_amdgcn_ballot_w64(z) & valid[q];
+                root[q] = z && in_cw[q];
                 cnt += (uint32_t)__builtin_popcountll(zb[q]);
             }
             if (cnt != Lo)
                 return;
             ok_mask |= 1ull << o;
-            uint32_t slot = 0;
+            uint32_t slot = o * CHIEN_LMAX;
 #pragma unroll
             for (int q = 0; q < 4; ++q) {
-                const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(zb[q] >> 32),
-                                                                 __builtin_amdgcn_mbcnt_lo((uint32_t)zb[q], 0u));
-                if ((zb[q] >> lane) & 1ull)
-                    my_roots[o * CHIEN_LMAX + slot + below] =
-                        (uint16_t)((lane * 4 + q) | (((v_odd >> (8 * q)) & 0xffu) << 8));
+                if (root[q]) {
+                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(zb[q] >> 32),
+                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)zb[q], 0u));
+                    // position in byte 0, Lambda_odd(X^-1) (byte q of v_odd) in byte 1
+                    my_roots[slot + below] = (uint16_t)__builtin_amdgcn_perm(v_odd, pos[q], 0x0c0c0400u + (uint32_t)q * 0x100u);
+                }
                 slot += (uint32_t)__builtin_popcountll(zb[q]);
             }
         };
