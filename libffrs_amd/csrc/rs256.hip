@@ -1001,6 +1001,9 @@ __device__ inline bool bm_fixed_one(const uint32_t *__restrict__ sw, const uint8
         uint32_t ld = s_log2[d];
         uint32_t lc = ld + 255 - lb;
         lc = lc >= 255 ? lc - 255 : lc;
+        // d == 0: ZLOG puts every product below in the zero region of EXP2 (ZLOG + a log or
+        // ZLOG <= 1022), so no per-term select is needed
+        lc = nz ? lc : (uint32_t)ZLOG;
         const int hi = n + 1 < H ? n + 1 : H;
         uint32_t old_llam[E];
 #pragma unroll
@@ -1008,8 +1011,7 @@ __device__ inline bool bm_fixed_one(const uint32_t *__restrict__ sw, const uint8
             old_llam[i] = llam[i];
 #pragma unroll
         for (int i = 1; i <= hi; ++i) {
-            uint32_t t = nz ? (uint32_t)s_exp2[lc + lBp[i]] : 0u;
-            lam[i] ^= t;
+            lam[i] ^= (uint32_t)s_exp2[lc + lBp[i]];
             llam[i] = s_log2[lam[i]];
         }
         const int bh = n + 2 < H + 1 ? n + 2 : (H + 1 < E - 1 ? H + 1 : E - 1);
