@@ -777,6 +777,8 @@ __global__ __launch_bounds__(256) void rs256_synds_kernel(const uint8_t *__restr
 //   rs256_bm_kernel     lane per codeword: Berlekamp-Massey; L > E/2 -> status 0; else
 //                       a record {L, Lambda_0..L, Omega_0..L-1} (Omega = S Lambda mod x^E
 //                       has degree < L once the LFSR of length L generates all E syndromes)
+//                       (ecc 2, 4, 8, 16, 32: the Chien kernel runs this BM itself per tile
+//                       and keeps the records in registers)
 //   rs256_chien_kernel  Chien search one codeword per wave (4 positions per lane, nibble
 //                       tables CT[i][h][v] of 256 position bytes addressed by wave-uniform
 //                       coefficients), then Forney one codeword per lane.
