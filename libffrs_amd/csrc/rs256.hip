@@ -813,8 +813,27 @@ __global__ __launch_bounds__(WG) void rs256_synd_kernel(const uint8_t *__restric
     __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, nrec);
     for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += gridDim.x * blockDim.x) {
         uint32_t rem[EW];
+        // whole-word parity rows are loaded before the message, so that their latency overlaps
+        // the LFSR
+        const uint8_t *pe = eccs + (size_t)cw * E;
+        uint32_t par[EW];
+        if ((E & 3) == 0) {
+#pragma unroll
+            for (int d = 0; d < EW; ++d)
+                par[d] = (uint32_t)d * 4 < E ? reinterpret_cast<const uint32_t *>(pe)[d] : 0u;
+        }
         message_remainder<EW, S>(rsrc, nrec, (int64_t)cw * k, k, s_lut, rem);
-        if (xor_parity<EW>(rem, eccs + (size_t)cw * E, E) == 0) {
+        uint32_t nz = 0;
+        if ((E & 3) == 0) {
+#pragma unroll
+            for (int d = 0; d < EW; ++d) {
+                rem[d] ^= par[d];
+                nz |= rem[d];
+            }
+        } else {
+            nz = xor_parity<EW>(rem, pe, E);
+        }
+        if (nz == 0) {
             status[cw] = 1;
             continue;
         }
