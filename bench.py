@@ -3,8 +3,8 @@
 Default workload rs256_plain_scale: RS256 ecc_len 32, n 255 (223 message bytes), 1,048,576
 codewords (233,832,448 message bytes); every codeword is corrupted at 0-16 distinct bytes,
 1% at 17-20 (uncorrectable path). Other workloads (--workload): rs256_plain_small (ecc 8,
-4,096 codewords), rsi16_plain (RSi16(256, 32), 262,144 codewords) and rsi16_large
-(RSi16(4096, 256), 16,384 codewords). A step is one encode of all messages plus one repair
+4,096 codewords), rs256_ecc8_scale (ecc 8, 4,194,304 codewords), rsi16_plain (RSi16(256, 32),
+262,144 codewords) and rsi16_large (RSi16(4096, 256), 16,384 codewords). A step is one encode of all messages plus one repair
 of all corrupted codewords, with inputs resident in HBM. The corrupted input is restored
 from a pristine device copy before each step, outside the timed bracket.
 
@@ -68,6 +68,9 @@ WORKLOADS = {
     # RS256 ecc 8 over 4,096 codewords of 247 bytes; 0-4 errors
     "rs256_plain_small": dict(field="rs256", N=4096, n=255, ecc=8, max_err=4, unc=0.0, seed=0,
                               errors="0-4 per codeword", cpu_repair=4096),
+    # the same shape at scale: 4,194,304 codewords (1,036,017,664 message bytes)
+    "rs256_ecc8_scale": dict(field="rs256", N=4194304, n=255, ecc=8, max_err=4, unc=0.0, seed=0,
+                             errors="0-4 per codeword", cpu_repair=262144),
     # RSi16(256, 32) over 262,144 codewords; 0-16 additive errors
     "rsi16_plain": dict(field="rsi16", N=262144, n=256, ecc=32, max_err=16, seed=3,
                         errors="0-16 additive per codeword", cpu_repair=32768),
@@ -77,76 +80,152 @@ WORKLOADS = {
 }
 
 
-def cpu_baseline(wl, msgs, cws, sample):
-    """The reference's CPU build (or the oracle port where it is absent) on the host cores:
-    encode of the whole message buffer in one call, and repair of `sample` corrupted
-    codewords one call each (the reference's API repairs one codeword per call), single
-    thread. Returns message GB/s of one encode + repair pass over the workload."""
+# Worker state of the all-cores CPU baseline: set in the parent before the pool forks, read by
+# the workers through copy-on-write memory (nothing is pickled).
+_CPU = {}
+
+
+def _cpu_encode_shard(i):
+    """Worker i: the reference's encode of its contiguous shard of the messages, one call."""
+    ref, wl, msgs, nw = _CPU["ref"], _CPU["wl"], _CPU["msgs"], _CPU["workers"]
+    n, ecc = wl["n"], wl["ecc"]
+    a, b = len(msgs) * i // nw, len(msgs) * (i + 1) // nw
+    codec = ref.RS256(block_len=n, ecc_len=ecc) if wl["field"] == "rs256" else ref.RSi16(n, ecc)
+    buf = msgs[a:b].tobytes()
+    t = time.perf_counter()
+    codec.encode(buf)
+    return time.perf_counter() - t
+
+
+def _cpu_repair_shard(i):
+    """Worker i: one repair call per corrupted codeword of its shard (the reference's API repairs
+    one codeword per call)."""
+    ref, wl, cws, nw = _CPU["ref"], _CPU["wl"], _CPU["cws"], _CPU["workers"]
     n, ecc = wl["n"], wl["ecc"]
     k = n - ecc
+    a, b = len(cws) * i // nw, len(cws) * (i + 1) // nw
+    codec = ref.RS256(block_len=n, ecc_len=ecc) if wl["field"] == "rs256" else ref.RSi16(n, ecc)
+    mb = [bytearray(r[:k].tobytes()) for r in cws[a:b]]
+    pb = [bytearray(r[k:].tobytes()) for r in cws[a:b]]
+    t = time.perf_counter()
+    for m, q in zip(mb, pb):
+        try:
+            codec.repair(m, q)
+        except RuntimeError:
+            pass
+    return time.perf_counter() - t
+
+
+def _cpu_repair_il16_shard(i):
+    """Worker i: RSi16 repair with interleave 16, the reference's best CPU repair: 16 codewords
+    per call, their columns repaired together by the 16-lane SIMD build."""
+    ref, wl, cws, nw = _CPU["ref"], _CPU["wl"], _CPU["cws"], _CPU["workers"]
+    n, ecc = wl["n"], wl["ecc"]
+    k = n - ecc
+    groups = len(cws) // 16
+    a, b = groups * i // nw, groups * (i + 1) // nw
+    codec = ref.RSi16(n, ecc, 16)
+    bufs = []
+    for g in range(a, b):
+        blk = cws[16 * g:16 * g + 16]  # [16, n]: column c = codeword c
+        bufs.append((bytearray(np.ascontiguousarray(blk[:, :k].T).tobytes()),
+                     bytearray(np.ascontiguousarray(blk[:, k:].T).tobytes())))
+    t = time.perf_counter()
+    for m, q in bufs:
+        try:
+            codec.repair(m, q)
+        except RuntimeError:
+            pass
+    return time.perf_counter() - t
+
+
+def cpu_baseline(wl, msgs, sample):
+    """The reference's own CPU build (oracle/_ref; the oracle port where that build is absent)
+    on the host cores, before anything touches the GPU (the pool's workers are forked): encode
+    of the message buffer and repair of `sample` corrupted codewords, one call each, first on
+    one core, then split over min(16, host CPUs) worker processes (the reference is
+    single-threaded; separate processes on disjoint shards are its all-cores form). For RSi16
+    the repair is also timed through interleave 16 (16 codewords per call, the 16-lane SIMD
+    build). Returns message GB/s of one encode + repair pass over the workload."""
+    from libffrs_amd import workloads as W
+
+    n, ecc = wl["n"], wl["ecc"]
+    k = n - ecc
+    elem = 1 if wl["field"] == "rs256" else 2
     ref = load_reference_cpu()
     kind = "reference" if ref is not None else "port"
-    if wl["field"] == "rs256":
-        if ref is not None:
-            rs = ref.RS256(block_len=n, ecc_len=ecc)
-            t = time.perf_counter()
-            rs.encode(msgs.tobytes())
-            te = time.perf_counter() - t
-            mb = [bytearray(r[:k].tobytes()) for r in cws[:sample]]
-            pb = [bytearray(r[k:].tobytes()) for r in cws[:sample]]
-            t = time.perf_counter()
-            for m, p in zip(mb, pb):
-                rs.repair(m, p)
-            tr = time.perf_counter() - t
-        else:
-            from oracle import oracle as O
-            orc = O.RS256(ecc)
-            t = time.perf_counter()
-            orc.encode_batch(msgs, threads=1)
-            te = time.perf_counter() - t
-            m, p = np.ascontiguousarray(cws[:sample, :k]), np.ascontiguousarray(cws[:sample, k:])
-            t = time.perf_counter()
-            orc.decode_batch(m, p, threads=1)
-            tr = time.perf_counter() - t
-        elem = 1
-    else:
-        if ref is not None:
-            ri = ref.RSi16(n, ecc)  # the SIMD width the host supports (AVX-512: x16)
-            t = time.perf_counter()
-            ri.encode(msgs.tobytes())
-            te = time.perf_counter() - t
-            mb = [bytearray(r[:k].tobytes()) for r in cws[:sample]]
-            pb = [bytearray(r[k:].tobytes()) for r in cws[:sample]]
-            t = time.perf_counter()
-            for m, p in zip(mb, pb):
-                try:
-                    ri.repair(m, p)
-                except RuntimeError:
-                    pass
-            tr = time.perf_counter() - t
-        else:
-            from oracle import oracle as O
-            orc = O.RSi16(n, ecc)
-            t = time.perf_counter()
-            orc.encode(msgs, True, threads=1)
-            te = time.perf_counter() - t
-            m, p = np.ascontiguousarray(cws[:sample, :k]), np.ascontiguousarray(cws[:sample, k:])
-            t = time.perf_counter()
-            orc.repair_batch(m, p, threads=1)
-            tr = time.perf_counter() - t
-        elem = 2
     N = msgs.shape[0]
-    per_byte = te / (N * k * elem) + tr / (sample * k * elem)
-    return {
-        "value": round(1.0 / per_byte / 1e9, 6),
+    # corrupted codewords of the sample, encoded by the same build and corrupted as the workload
+    t0 = time.perf_counter()
+    if ref is not None:
+        codec = ref.RS256(block_len=n, ecc_len=ecc) if elem == 1 else ref.RSi16(n, ecc)
+        par = np.frombuffer(codec.encode(msgs[:sample].tobytes()), msgs.dtype).reshape(sample, ecc)
+    else:
+        from oracle import oracle as O
+        par = (O.RS256(ecc).encode_batch(msgs[:sample]) if elem == 1 else O.RSi16(n, ecc).encode(msgs[:sample], True))
+    cws = np.concatenate([msgs[:sample], par], 1)
+    if elem == 1:
+        _, counts, s = W.rs256_plain(sample, ecc, wl["max_err"], seed=wl["seed"] + 100,
+                                     uncorrectable_frac=wl.get("unc", 0.0))
+        W.rs256_corrupt(cws, counts, s)
+    else:
+        _, counts, s = W.rsi16_plain(sample, n, ecc, wl["max_err"], seed=wl["seed"] + 100)
+        W.rsi16_corrupt(cws, counts, s)
+    log(f"CPU baseline sample of {sample} codewords prepared in {time.perf_counter() - t0:.1f} s")
+    if ref is None:
+        from oracle import oracle as O
+        t = time.perf_counter()
+        if elem == 1:
+            O.RS256(ecc).encode_batch(msgs, threads=1)
+        else:
+            O.RSi16(n, ecc).encode(msgs, True, threads=1)
+        te = time.perf_counter() - t
+        m, q = np.ascontiguousarray(cws[:, :k]), np.ascontiguousarray(cws[:, k:])
+        t = time.perf_counter()
+        (O.RS256(ecc).decode_batch(m, q, threads=1) if elem == 1 else O.RSi16(n, ecc).repair_batch(m, q, threads=1))
+        tr = time.perf_counter() - t
+        per_byte = te / (N * k * elem) + tr / (sample * k * elem)
+        return {"value": round(1.0 / per_byte / 1e9, 6), "unit": "GB/s", "cores": 1, "kind": kind,
+                "sample": f"oracle port: encode of {N} messages + repair of {sample} codewords, one thread"}
+    import multiprocessing as mp
+    _CPU.update(ref=ref, wl=wl, msgs=msgs, cws=cws, workers=1)
+    te1 = _cpu_encode_shard(0)
+    tr1 = _cpu_repair_shard(0)
+    workers = max(1, min(16, os.cpu_count() or 1))
+    _CPU["workers"] = workers
+    with mp.get_context("fork").Pool(workers) as pool:
+        ten = max(pool.map(_cpu_encode_shard, range(workers)))
+        trn = max(pool.map(_cpu_repair_shard, range(workers)))
+        il16 = None
+        if elem == 2 and sample >= 16 * workers:
+            il16 = max(pool.map(_cpu_repair_il16_shard, range(workers)))
+    _CPU.clear()
+    enc_b, rep_b = N * k * elem, sample * k * elem
+
+    def gb(b, t):
+        return round(b / t / 1e9, 6)
+
+    per_byte_1 = te1 / enc_b + tr1 / rep_b
+    per_byte_n = ten / enc_b + trn / rep_b
+    res = {
+        "value": round(1.0 / per_byte_n / 1e9, 6),
         "unit": "GB/s",
-        "cores": 1,
+        "cores": workers,
         "kind": kind,
-        "sample": f"encode of all {N} messages in one call ({te:.2f} s) + repair of the first {sample} "
-                  f"corrupted codewords, one call each ({tr:.2f} s); single thread, {os.cpu_count()} host CPUs",
-        "encode_gbps": round(N * k * elem / te / 1e9, 4),
-        "repair_gbps": round(sample * k * elem / tr / 1e9, 6),
+        "sample": (f"encode of all {N} messages (one call per worker shard) + repair of {sample} corrupted "
+                   f"codewords, one call each; {workers} worker processes on disjoint shards "
+                   f"({os.cpu_count()} host CPUs), and one core alone"),
+        "encode_gbps": gb(enc_b, ten),
+        "repair_gbps": gb(rep_b, trn),
+        "single_core": {"value": round(1.0 / per_byte_1 / 1e9, 6), "encode_gbps": gb(enc_b, te1),
+                        "repair_gbps": gb(rep_b, tr1), "encode_s": round(te1, 3), "repair_s": round(tr1, 3)},
     }
+    if il16 is not None:
+        groups = (sample // 16) * 16
+        rb16 = groups * k * elem
+        res["repair_interleave16_gbps"] = gb(rb16, il16)
+        res["value_interleave16"] = round(1.0 / (ten / enc_b + il16 / rb16) / 1e9, 6)
+    return res
 
 
 def rocprof_traffic(workload):
@@ -238,9 +317,9 @@ def run_plain(name, args):
     value = msg_bytes / (ms_step * 1e-3) / 1e9
     if rs256:
         enc_names = ["rs256_encode_kernel"]
-        # ecc 2, 4, 8, 16, 32 run BM inside the Chien kernel; other ecc <= 32 launch it on its own
-        rep_names = (["rs256_synd_kernel", "rs256_chien_kernel"] if ecc in (2, 4, 8, 16, 32)
-                     else ["rs256_synd_kernel", "rs256_bm_kernel", "rs256_chien_kernel"] if ecc <= 32
+        # ecc 2, 4, 8, 16, 32 run BM over the product table, other ecc <= 32 from the log tables
+        rep_names = (["rs256_synd_kernel", "rs256_bm_mul_kernel", "rs256_chien16_kernel"] if ecc in (2, 4, 8, 16, 32)
+                     else ["rs256_synd_kernel", "rs256_bm_kernel", "rs256_chien16_kernel"] if ecc <= 32
                      else ["rs256_repair_kernel"])
     else:
         enc_names = ["rsi16_encode_lane_kernel" if ecc <= 32 else
@@ -297,9 +376,6 @@ def run_plain(name, args):
         "kernels": kernels,
         "status_counts": {int(v): int(c) for v, c in zip(*np.unique(st, return_counts=True))},
     }
-    if not args.no_cpu_baseline:
-        log("timing the CPU baseline ...")
-        result["cpu_baseline"] = cpu_baseline(wl, msgs, cws, min(N, args.cpu_repair_sample or wl["cpu_repair"]))
     for b in (d_msg, d_par, d_cm, d_cp, d_wm, d_wp, d_st):
         b.free()
     codec.close()
@@ -550,17 +626,19 @@ def run_mixed(args):
 
 
 def run_serving(args):
-    """Serving loop: Poisson arrivals (args.serve_rate per second) of par-style requests, 70%
-    encode and 30% verify+repair, files log-uniform in 4 KiB-64 MiB split into RSi16(1024, 128)
-    blocks (ecc 128: 12.5% parity, the legal neighbour of 10%), repair requests corrupted at
-    0-5% of their symbols (half single errors, half 8-64 symbol bursts). Requests are batched
-    continuously by libffrs_amd.serving.BlockServer; latency runs from submission to the
-    moment the last block's results are back in host memory."""
+    """Serving loop: Poisson arrivals of par-style requests, 70% encode and 30% verify+repair,
+    files log-uniform in 4 KiB-64 MiB split into RSi16(1024, 128) blocks (ecc 128: 12.5% parity,
+    the legal neighbour of 10%), repair requests corrupted at 0-5% of their symbols (half single
+    errors, half 8-64 symbol bursts). Requests are batched continuously by
+    libffrs_amd.serving.BlockServer; latency runs from submission to the moment the last
+    block's results are back in host memory. The same request set is replayed at
+    args.serve_rate and at the rates of args.serve_sweep (offered load rising until the server
+    saturates); the main fields are those of args.serve_rate."""
     from libffrs_amd import workloads as W
     from libffrs_amd.serving import BlockServer
 
-    n_req, rate = args.serve_requests, args.serve_rate
-    t_arr, kinds, sizes, s = W.serving_requests(n_req, rate, seed=6)
+    n_req = args.serve_requests
+    t_arr, kinds, sizes, s = W.serving_requests(n_req, 1.0, seed=6)  # arrival times at 1 request/s
     t0 = time.perf_counter()
     datas = [s.bytes(int(sz)) for sz in sizes]
     srv = BlockServer(block_len=1024, ecc_len=128, slots=args.serve_slots, slot_blocks=args.serve_slot_blocks,
@@ -578,45 +656,113 @@ def run_serving(args):
         originals[i] = r.msgs.copy()
         W.serving_corrupt(msgs, par, s)
         payload.append((kind, msgs.reshape(-1).view(np.uint8), par))
-    log(f"serving: {n_req} requests prepared in {time.perf_counter() - t0:.1f} s "
-        f"({sizes.sum() / 1e9:.2f} GB, rate {rate}/s)")
-    srv.batches.clear()
-    reqs = []
-    start = time.perf_counter()
-    for i, (kind, d, par) in enumerate(payload):
-        delay = start + t_arr[i] - time.perf_counter()
-        if delay > 0:
-            time.sleep(delay)
-        reqs.append(srv.submit(kind, d, par))
-    for r in reqs:
-        r.done.wait()
-    end = max(r.t_done for r in reqs)
-    lat = np.array([r.latency for r in reqs]) * 1e3
-    total = sum(r.nbytes for r in reqs)
-    restored = sum(int((reqs[i].msgs == originals[i]).all(axis=1).sum()) for i in originals)
-    rep_blocks = sum(reqs[i].blocks for i in originals)
-    ok_blocks = sum(int(np.isin(reqs[i].status, (0, 2)).sum()) for i in originals)
-    mixed = sum(1 for b in srv.batches if b[1] > 1)
-    res = {
-        "workload": "serving (Poisson arrivals, 70% encode / 30% verify+repair, RSi16(1024, 128) blocks)",
-        "requests": n_req,
-        "rate_per_s": rate,
-        "offered_gbps": round(float(sizes.mean()) * rate / 1e9, 3),
-        "bytes": int(total),
-        "seconds": round(end - start, 3),
-        "gbps": round(total / (end - start) / 1e9, 3),
-        "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3), "p99": round(float(np.percentile(lat, 99)), 3),
-                       "max": round(float(lat.max()), 3), "mean": round(float(lat.mean()), 3)},
-        "batches": len(srv.batches),
-        "batches_with_several_requests": mixed,
-        "slots": args.serve_slots,
-        "hbm_budget_bytes": srv.hbm_budget,
-        "repair_blocks": rep_blocks,
-        "repair_blocks_ok_status": ok_blocks,
-        "repair_blocks_restored": restored,
-    }
+    log(f"serving: {n_req} requests prepared in {time.perf_counter() - t0:.1f} s ({sizes.sum() / 1e9:.2f} GB)")
+
+    def run_at(rate):
+        srv.batches.clear()
+        reqs = []
+        start = time.perf_counter()
+        for i, (kind, d, par) in enumerate(payload):
+            delay = start + t_arr[i] / rate - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            reqs.append(srv.submit(kind, d, par))
+        for r in reqs:
+            r.done.wait()
+        end = max(r.t_done for r in reqs)
+        lat = np.array([r.latency for r in reqs]) * 1e3
+        total = sum(r.nbytes for r in reqs)
+        restored = sum(int((reqs[i].msgs == originals[i]).all(axis=1).sum()) for i in originals)
+        rep_blocks = sum(reqs[i].blocks for i in originals)
+        ok_blocks = sum(int(np.isin(reqs[i].status, (0, 2)).sum()) for i in originals)
+        return {
+            "rate_per_s": rate,
+            "offered_gbps": round(float(sizes.mean()) * rate / 1e9, 3),
+            "bytes": int(total),
+            "seconds": round(end - start, 3),
+            "gbps": round(total / (end - start) / 1e9, 3),
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 3),
+                           "p99": round(float(np.percentile(lat, 99)), 3),
+                           "max": round(float(lat.max()), 3), "mean": round(float(lat.mean()), 3)},
+            "batches": len(srv.batches),
+            "batches_with_several_requests": sum(1 for b in srv.batches if b[1] > 1),
+            "repair_blocks": rep_blocks,
+            "repair_blocks_ok_status": ok_blocks,
+            "repair_blocks_restored": restored,
+        }
+
+    res = {"workload": "serving (Poisson arrivals, 70% encode / 30% verify+repair, RSi16(1024, 128) blocks)",
+           "requests": n_req, "slots": args.serve_slots, "hbm_free_bytes": srv.hbm_free,
+           "hbm_budget_bytes": srv.hbm_budget}
+    res.update(run_at(args.serve_rate))
+    sweep = [float(x) for x in args.serve_sweep.split(",") if x] if args.serve_sweep else []
+    if sweep:
+        res["sweep"] = []
+        for rate in sweep:
+            r = run_at(rate)
+            res["sweep"].append({k_: r[k_] for k_ in ("rate_per_s", "offered_gbps", "gbps", "latency_ms", "batches",
+                                                      "batches_with_several_requests")})
+            log(f"serving at {rate}/s: offered {r['offered_gbps']} GB/s, achieved {r['gbps']} GB/s, "
+                f"p99 {r['latency_ms']['p99']} ms")
     srv.close()
     return res
+
+
+def run_drop_in(args):
+    """Per-call latency of the drop-in API on the reference's own calling pattern, one codeword
+    per call (host buffers in, results back in host memory): RS256(ecc_len=32).encode / .repair
+    of 223-byte messages and RSi16(256, 32).encode / .repair, next to the reference's CPU build
+    doing the same calls. Microseconds per call, median and p99 over `calls` calls."""
+    import libffrs_amd as L
+    from libffrs_amd import workloads as W
+    ref = load_reference_cpu()
+    calls = args.drop_in_calls
+
+    def time_calls(fn, items):
+        for it in items[:10]:
+            fn(*it)
+        ts = []
+        for it in items:
+            t = time.perf_counter()
+            fn(*it)
+            ts.append(time.perf_counter() - t)
+        ts = np.array(ts) * 1e6
+        return {"median_us": round(float(np.median(ts)), 2), "p99_us": round(float(np.percentile(ts, 99)), 2)}
+
+    out = {"calls": calls}
+    msgs, counts, s = W.rs256_plain(calls, 32, 16, seed=21)
+    gpu_rs = L.RS256(ecc_len=32)
+    par = np.frombuffer(bytes(gpu_rs.encode(msgs.tobytes())), np.uint8).reshape(calls, 32)
+    cws = np.concatenate([msgs, par], 1)
+    W.rs256_corrupt(cws, counts, s)
+
+    def rs_items():
+        return [(bytearray(r[:223].tobytes()), bytearray(r[223:].tobytes())) for r in cws]
+
+    enc_items = [(m.tobytes(),) for m in msgs]
+    out["rs256_ecc32"] = {"gpu_encode": time_calls(gpu_rs.encode, enc_items),
+                          "gpu_repair": time_calls(gpu_rs.repair, rs_items())}
+    if ref is not None:
+        cpu_rs = ref.RS256(ecc_len=32)
+        out["rs256_ecc32"]["reference_cpu_encode"] = time_calls(cpu_rs.encode, enc_items)
+        out["rs256_ecc32"]["reference_cpu_repair"] = time_calls(cpu_rs.repair, rs_items())
+    msgs, counts, s = W.rsi16_plain(calls, 256, 32, 16, seed=22)
+    gpu_ri = L.RSi16(256, 32)
+    par = np.frombuffer(bytes(gpu_ri.encode(msgs.tobytes())), np.uint16).reshape(calls, 32)
+    cws = np.concatenate([msgs, par], 1)
+    W.rsi16_corrupt(cws, counts, s)
+
+    def ri_items():
+        return [(bytearray(r[:224].tobytes()), bytearray(r[224:].tobytes())) for r in cws]
+
+    enc_items = [(m.tobytes(),) for m in msgs]
+    out["rsi16_256_32"] = {"gpu_encode": time_calls(gpu_ri.encode, enc_items),
+                           "gpu_repair": time_calls(gpu_ri.repair, ri_items())}
+    if ref is not None:
+        cpu_ri = ref.RSi16(256, 32)
+        out["rsi16_256_32"]["reference_cpu_encode"] = time_calls(cpu_ri.encode, enc_items)
+        out["rsi16_256_32"]["reference_cpu_repair"] = time_calls(cpu_ri.repair, ri_items())
+    return out
 
 
 def main():
@@ -635,6 +781,10 @@ def main():
     ap.add_argument("--no-serving", action="store_true", help="leave the serving loop out of the bench line")
     ap.add_argument("--serve-requests", type=int, default=300)
     ap.add_argument("--serve-rate", type=float, default=800.0)
+    ap.add_argument("--serve-sweep", default="3200,12800,51200",
+                    help="further request rates (per second) the serving request set is replayed at")
+    ap.add_argument("--drop-in-calls", type=int, default=300)
+    ap.add_argument("--no-drop-in", action="store_true", help="leave the per-call drop-in latencies out")
     ap.add_argument("--serve-slots", type=int, default=4)
     ap.add_argument("--serve-slot-blocks", type=int, default=16384)
     ap.add_argument("--mixed-rs256", type=int, default=200000)
@@ -652,13 +802,31 @@ def main():
     if args.serving:
         print(json.dumps(run_serving(args)), flush=True)
         return
+    cpu = None
+    if not args.no_cpu_baseline:
+        # first, while no GPU state exists (the baseline forks worker processes)
+        wl = dict(WORKLOADS[args.workload])
+        from libffrs_amd import workloads as W
+        N = args.codewords or wl["N"]
+        log("timing the CPU baseline ...")
+        if wl["field"] == "rs256":
+            msgs_cpu = W.rs256_plain(N, wl["ecc"], wl["max_err"], seed=wl["seed"], uncorrectable_frac=wl["unc"])[0]
+        else:
+            msgs_cpu = W.rsi16_plain(N, wl["n"], wl["ecc"], wl["max_err"], seed=wl["seed"])[0]
+        cpu = cpu_baseline(wl, msgs_cpu, min(N, args.cpu_repair_sample or wl["cpu_repair"]))
+        del msgs_cpu
     res = run_plain(args.workload, args)
+    if cpu is not None:
+        res["cpu_baseline"] = cpu
     if not args.no_mixed:
         log("running the mixed-size batch ...")
         res["mixed"] = run_mixed(args)
     if not args.no_serving:
         log("running the serving loop ...")
         res["serving"] = run_serving(args)
+    if not args.no_drop_in:
+        log("timing drop-in calls ...")
+        res["drop_in"] = run_drop_in(args)
     print(json.dumps(res), flush=True)
 
 

@@ -23,7 +23,7 @@
 extern "C" {
 #endif
 
-#define FFRS_ABI_VERSION 1
+#define FFRS_ABI_VERSION 2
 
 typedef enum ffrs_status {
     FFRS_OK = 0,
@@ -165,6 +165,18 @@ int ffrs_rsi16_repair(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, size_t count
  * rsi16v_impl.hpp:171-231. One codeword. */
 int ffrs_rsi16_repair_known(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, const uint32_t *positions,
                             size_t npos, uint8_t *status);
+/* Interleaved repair without error positions with the SIMD words, RSi16(block_len, ecc_len,
+ * interleave) with simd_x4/x8/x16 resolving to `width` = 4, 8 or 16 lanes
+ * (libffrs/pyrsi16.hpp:389-453 -> the vector RSi16v<W>::_repair, rsi16v_impl.hpp:122-168,
+ * 271-497): the columns (symbol i of column c at i * interleave + c, message and ecc alike)
+ * are repaired in groups of `width`, the last one zero-filled past the columns, and the
+ * columns of a group share the reference's vector control flow, so a column's result can
+ * depend on its neighbours. Repaired in place; *status = the largest group status, or
+ * FFRS_RSI16_RAISES where a group's repair raises in the reference (the groups before it are
+ * written back, the others left as they were). With error positions the SIMD and scalar
+ * words give the same result (column by column through ffrs_rsi16_repair_known). */
+int ffrs_rsi16_repair_interleaved_simd(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, size_t interleave,
+                                       unsigned width, uint8_t *status);
 /* py_synd (libffrs/pyrsi16.hpp:633-651): ecc_len syndromes per codeword. */
 int ffrs_rsi16_synds(ffrs_rsi16 *rs, const uint16_t *msg, const uint16_t *ecc, size_t count, uint32_t *synds);
 /* py_sugiyama<1> (libffrs/pyrsi16.hpp:653-674): locator and evaluator (ecc_len values
@@ -202,9 +214,11 @@ int ffrs_rs256_plan_repair(ffrs_rs256_plan *plan, uint8_t *d_msgs, size_t msg_by
                            size_t ecc_bytes, uint8_t *d_status);
 
 /* RSi16: offsets and sizes in u16 symbols; every (block_len, ecc_len) must be accepted by
- * ffrs_rsi16_create (primitive 3). */
-int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_len, const uint16_t *ecc_len,
-                           const uint64_t *msg_off, const uint64_t *ecc_off, ffrs_rsi16_plan **out);
+ * ffrs_rsi16_create with the given primitive (0 stands for 3, the reference's default,
+ * pyrsi16.hpp:269-287); block_len up to 65536. */
+int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint32_t *block_len, const uint32_t *ecc_len,
+                           const uint64_t *msg_off, const uint64_t *ecc_off, uint32_t primitive,
+                           ffrs_rsi16_plan **out);
 void ffrs_rsi16_plan_destroy(ffrs_rsi16_plan *plan);
 int ffrs_rsi16_plan_use_graph(ffrs_rsi16_plan *plan, int enable);
 int ffrs_rsi16_plan_encode(ffrs_rsi16_plan *plan, const uint16_t *d_msgs, size_t msg_elems, uint16_t *d_eccs,

@@ -73,12 +73,13 @@ SIGNATURES = [
     ("ffrs_rsi16_roots", c_int, [c_vp, c_vp, c_vp]),
     ("ffrs_rsi16_encode_device", c_int, [c_vp, c_vp, c_sz, c_vp, c_int]),
     ("ffrs_rsi16_repair_device", c_int, [c_vp, c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_rsi16_repair_interleaved_simd", c_int, [c_vp, c_vp, c_vp, c_sz, ctypes.c_uint, c_vp]),
     ("ffrs_rs256_plan_create", c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
     ("ffrs_rs256_plan_destroy", None, [c_vp]),
     ("ffrs_rs256_plan_use_graph", c_int, [c_vp, c_int]),
     ("ffrs_rs256_plan_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz]),
     ("ffrs_rs256_plan_repair", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_vp]),
-    ("ffrs_rsi16_plan_create", c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
+    ("ffrs_rsi16_plan_create", c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint32, ctypes.POINTER(c_vp)]),
     ("ffrs_rsi16_plan_destroy", None, [c_vp]),
     ("ffrs_rsi16_plan_use_graph", c_int, [c_vp, c_int]),
     ("ffrs_rsi16_plan_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_int]),
@@ -314,13 +315,13 @@ class RS256Plan:
 class RSi16Plan:
     """A mixed-size RSi16 work list (per-codeword block_len, ecc_len and u16 offsets)."""
 
-    def __init__(self, ctx, block_len, ecc_len, msg_off, ecc_off, graph=False):
+    def __init__(self, ctx, block_len, ecc_len, msg_off, ecc_off, graph=False, primitive=3):
         import numpy as np
-        self._keep = [_arr(block_len, np.uint16), _arr(ecc_len, np.uint16), _arr(msg_off, np.uint64),
+        self._keep = [_arr(block_len, np.uint32), _arr(ecc_len, np.uint32), _arr(msg_off, np.uint64),
                       _arr(ecc_off, np.uint64)]
         h = c_vp()
         check(lib().ffrs_rsi16_plan_create(ctx.h, len(self._keep[0]), *[a.ctypes.data for a in self._keep],
-                                           ctypes.byref(h)), "ffrs_rsi16_plan_create")
+                                           primitive, ctypes.byref(h)), "ffrs_rsi16_plan_create")
         self.h, self.ctx = h, ctx
         if graph:
             check(lib().ffrs_rsi16_plan_use_graph(self.h, 1), "use_graph")

@@ -351,6 +351,8 @@ public:
     }
 
     bool vector_lanes() const { return simd_x4 || simd_x8 || simd_x16; }
+    // the lanes of the SIMD instantiation the reference dispatches to (pyrsi16.hpp:524-534)
+    unsigned simd_width() const { return simd_x16 ? 16 : simd_x8 ? 8 : simd_x4 ? 4 : 1; }
 
     py::bytearray encode(py::buffer buffer)
     {
@@ -388,10 +390,6 @@ public:
             return RepairStatus::NoErrors;
         PY_ASSERT(m.size == message_len * interleave, std::to_string(m.size));
         PY_ASSERT(e.size == ecc_len * interleave, std::to_string(e.size));
-        if (interleave != 1 && vector_lanes())
-            throw std::runtime_error("repair with interleave > 1 and SIMD words is not supported by this build (the "
-                                     "reference repairs those columns in coupled SIMD groups, pyrsi16.hpp:391-522); "
-                                     "with simd_x4/x8/x16 False it repairs column by column");
         std::vector<uint32_t> pos;
         if (error_pos) {
             PY_ASSERT(error_pos->size() <= ecc_len, "too many error positions");
@@ -401,6 +399,18 @@ public:
                     throw py::value_error("error position " + std::to_string((*error_pos)[i]) + " >= block_len");
                 pos[i] = (uint32_t)(*error_pos)[i];
             }
+        }
+        if (interleave != 1 && vector_lanes() && !error_pos) {
+            // the columns in coupled groups of the host's SIMD width (pyrsi16.hpp:389-453,
+            // _simd_dispatch :524-534); with error positions the SIMD and scalar words agree
+            uint8_t st = 0;
+            int r;
+            {
+                py::gil_scoped_release nogil;
+                r = ffrs_rsi16_repair_interleaved_simd(codec, m.data, e.data, interleave, simd_width(), &st);
+            }
+            check(r, "RSi16.repair");
+            return status_of(st);
         }
         if (interleave != 1)
             return repair_columns(m.data, e.data, error_pos ? &pos : nullptr);
@@ -417,7 +427,10 @@ public:
         return status_of(st);
     }
 
-    // Interleaved repair with the scalar words (pyrsi16.hpp:391-453 with SIMD_W = 1): column c
+    // Interleaved repair with the scalar words (pyrsi16.hpp:391-453 with SIMD_W = 1), and with
+    // error positions in either words (there the vector instantiations give each column the
+    // result of its own repair: the locator comes from the positions, the same in every lane,
+    // and a column without errors gets zero corrections): column c
     // is the codeword of symbols i * interleave + c. The reference repairs the columns one
     // after another and writes each back after its repair; the call returns the largest
     // status, and a column whose repair raises ends it with the earlier columns written back.

@@ -18,6 +18,7 @@
 #include "common.hpp"
 
 #include <algorithm>
+#include <type_traits>
 #include <cstring>
 
 namespace ffrs {
@@ -973,6 +974,33 @@ __global__ __launch_bounds__(WG) void rs256_bm_kernel(uint32_t count, uint32_t E
     }
 }
 
+// Log / exp tables for the register BM, two layouts:
+//  LogExp1K  EXP2 (1,024 bytes, zero above 509) and LOG2 (u16, log 0 = ZLOG = 511): a product
+//            of logs is one read at the sum; the reads land in 256 dwords of the LDS (up to
+//            four per bank), so lanes collide on banks.
+//  LogExp256 EXP (256 bytes, EXP[255] = 0) and LOG (u16, log 0 = 767): the index of a sum s is
+//            min(s, s - 255, 255), which reduces s mod 255 and sends any sum with a zero operand
+//            (>= 767) to EXP[255]; the 256 bytes are one dword per bank, so lanes never collide.
+struct LogExp1K {
+    const uint8_t *ex;
+    const uint16_t *lg;
+    static constexpr uint32_t Z = ZLOG;
+    __device__ inline uint32_t log(uint32_t a) const { return lg[a]; }
+    __device__ inline uint32_t exp_sum(uint32_t s) const { return ex[s]; }
+};
+struct LogExp256 {
+    const uint8_t *ex;
+    const uint16_t *lg;
+    static constexpr uint32_t Z = 767;
+    __device__ inline uint32_t log(uint32_t a) const { return lg[a]; }
+    __device__ inline uint32_t exp_sum(uint32_t s) const
+    {
+        const uint32_t t = s - 255u;
+        const uint32_t m = s < t ? s : t;
+        return ex[m < 255u ? m : 255u];  // one v_min3_u32
+    }
+};
+
 // Berlekamp-Massey with every polynomial in registers, fully unrolled for a fixed E.
 // B is kept pre-shifted as x^m B (its logs only), so an update is Lambda_i ^= c B'_i and
 // every iteration shifts B' by one: B' <- x Lambda_old when L grows, else x B'. At
@@ -983,11 +1011,11 @@ __global__ __launch_bounds__(WG) void rs256_bm_kernel(uint32_t count, uint32_t E
 // Returns false when L > E/2 (not repairable); otherwise the 48-byte record in words[]:
 // L in byte 0, Omega_0..Omega_(E/2-1) from byte rec_om_off, Lambda_0..Lambda_(E/2) from byte
 // rec_lam_off.
-template <int E>
-__device__ inline bool bm_fixed_one(const uint32_t *__restrict__ sw, const uint8_t *s_exp2, const uint16_t *s_log2,
-                                    uint32_t (&words)[12])
+template <int E, class T>
+__device__ inline bool bm_fixed_one(const uint32_t *__restrict__ sw, const T &tb, uint32_t (&words)[12])
 {
     constexpr int EW = (E + 3) / 4;
+    constexpr uint32_t ZL = T::Z;
     const uint32_t LO = rec_lam_off(), OO = rec_om_off(E);
     // the syndromes stay packed four to a register (S_n is extracted where it is used)
     uint32_t sv[EW], lS[E];
@@ -997,14 +1025,14 @@ __device__ inline bool bm_fixed_one(const uint32_t *__restrict__ sw, const uint8
 #pragma unroll
         for (int b = 0; b < 4; ++b)
             if (w * 4 + b < E)
-                lS[w * 4 + b] = s_log2[(sv[w] >> (8 * b)) & 0xffu];
+                lS[w * 4 + b] = tb.log((sv[w] >> (8 * b)) & 0xffu);
     }
     uint32_t lam[E], llam[E], lBp[E];
 #pragma unroll
     for (int i = 0; i < E; ++i) {
         lam[i] = i == 0 ? 1u : 0u;
-        llam[i] = i == 0 ? 0u : (uint32_t)ZLOG;
-        lBp[i] = i == 1 ? 0u : (uint32_t)ZLOG;
+        llam[i] = i == 0 ? 0u : ZL;
+        lBp[i] = i == 1 ? 0u : ZL;
     }
     // L never decreases, and a codeword whose L exceeds E/2 fails whatever follows; while
     // L <= E/2, deg Lambda <= L and every x^m B that enters an update has degree <= the new
@@ -1016,15 +1044,14 @@ __device__ inline bool bm_fixed_one(const uint32_t *__restrict__ sw, const uint8
         uint32_t d = (sv[n >> 2] >> (8 * (n & 3))) & 0xffu;
 #pragma unroll
         for (int i = 1; i <= (n < H ? n : H); ++i)
-            d ^= s_exp2[llam[i] + lS[n - i]];
+            d ^= tb.exp_sum(llam[i] + lS[n - i]);
         bool nz = d != 0;
         bool grow = nz && (2 * L <= (uint32_t)n);
-        uint32_t ld = s_log2[d];
+        uint32_t ld = tb.log(d);
         uint32_t lc = ld + 255 - lb;
         lc = lc >= 255 ? lc - 255 : lc;
-        // d == 0: ZLOG puts every product below in the zero region of EXP2 (ZLOG + a log or
-        // ZLOG <= 1022), so no per-term select is needed
-        lc = nz ? lc : (uint32_t)ZLOG;
+        // d == 0: the zero log puts every product below at zero, so no per-term select is needed
+        lc = nz ? lc : ZL;
         const int hi = n + 1 < H ? n + 1 : H;
         uint32_t old_llam[E];
 #pragma unroll
@@ -1032,14 +1059,14 @@ __device__ inline bool bm_fixed_one(const uint32_t *__restrict__ sw, const uint8
             old_llam[i] = llam[i];
 #pragma unroll
         for (int i = 1; i <= hi; ++i) {
-            lam[i] ^= (uint32_t)s_exp2[lc + lBp[i]];
-            llam[i] = s_log2[lam[i]];
+            lam[i] ^= tb.exp_sum(lc + lBp[i]);
+            llam[i] = tb.log(lam[i]);
         }
         const int bh = n + 2 < H + 1 ? n + 2 : (H + 1 < E - 1 ? H + 1 : E - 1);
 #pragma unroll
         for (int i = bh; i >= 1; --i)
             lBp[i] = grow ? old_llam[i - 1] : lBp[i - 1];
-        lBp[0] = ZLOG;
+        lBp[0] = ZL;
         if (grow) {
             L = (uint32_t)n + 1 - L;
             lb = ld;
@@ -1059,7 +1086,7 @@ __device__ inline bool bm_fixed_one(const uint32_t *__restrict__ sw, const uint8
         uint32_t v = 0;
 #pragma unroll
         for (int i = 0; i <= kk; ++i)
-            v ^= s_exp2[llam[i] + lS[kk - i]];
+            v ^= tb.exp_sum(llam[i] + lS[kk - i]);
         v = (uint32_t)kk < L ? v : 0u;
         words[(OO + kk) >> 2] |= v << (8 * ((OO + kk) & 3));
     }
@@ -1142,7 +1169,7 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
 #pragma unroll
             for (int w = 0; w < 12; ++w)
                 words[w] = 0;
-            if (pend && !bm_fixed_one<EB>(synd + (size_t)cw * ((EB + 3) / 4), s_exp2, s_log2, words)) {
+            if (pend && !bm_fixed_one<EB>(synd + (size_t)cw * ((EB + 3) / 4), LogExp1K{s_exp2, s_log2}, words)) {
                 status[cw] = 0;
                 pend = false;
             }
@@ -1278,6 +1305,405 @@ __global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ m
 
 
 // ======================================================================
+// Berlekamp-Massey over a product table (ecc 2, 4, 8, 16, 32), lane per codeword:
+// MUL[a][b] = a b (64 KB, a-major) and INV[a] (inv(0) = 1) in LDS, every polynomial in
+// registers and fully unrolled. A product is one table read at (a << 8) | b, so no
+// coefficient returns to the log domain after an update. Same iteration as bm_fixed_one<E>
+// (reed_solomon.hpp:655-702: degree-bounded, B kept as x^m B, L > E/2 fails) and the same
+// 48-byte record: L in byte 0, Omega_0..Omega_(E/2-1) from byte 4, Lambda_0..Lambda_(E/2)
+// from byte 20, zero-filled.
+// ======================================================================
+
+constexpr uint32_t MULT_BYTES = 65536 + 256;
+
+// Offset of MUL[a][b] from the low bytes of a and b (one byte permute; higher bits ignored).
+__device__ inline uint32_t mul_at(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(a, b, 0x0c0c0400u); }
+// (a ^ b) & 0xff in one instruction: written as plain XORs of table bytes, the compiler narrows
+// them to 16-bit operations and masks every result again before it goes into an address.
+__device__ inline uint32_t xr(uint32_t a, uint32_t b) { return __builtin_amdgcn_bitop3_b32(a, b, 0xffu, 0x28); }
+
+template <int E>
+__device__ inline bool bm_mul_one(const uint32_t *__restrict__ sw, const uint8_t *mul, const uint8_t *inv,
+                                  uint32_t (&words)[12])
+{
+    constexpr int EW = (E + 3) / 4;
+    constexpr int H = E / 2;
+    uint32_t S[E];
+#pragma unroll
+    for (int w = 0; w < EW; ++w) {
+        const uint32_t v = sw[w];
+#pragma unroll
+        for (int b = 0; b < 4; ++b)
+            if (w * 4 + b < E)
+                S[w * 4 + b] = (v >> (8 * b)) & 0xffu;
+    }
+    // While L <= E/2, deg Lambda <= L and every x^m B that enters an update has degree <= the
+    // new L, so coefficients above E/2 never reach a successful result.
+    uint32_t lam[H + 1], bp[H + 1];  // Lambda_0..Lambda_H; B' = x^m B, coefficients 0..H
+#pragma unroll
+    for (int i = 0; i <= H; ++i) {
+        lam[i] = i == 0 ? 1u : 0u;
+        bp[i] = i == 1 ? 1u : 0u;
+    }
+    uint32_t L = 0, binv = 1;  // b = 1
+#pragma unroll
+    for (int n = 0; n < E; ++n) {
+        uint32_t d = S[n];
+#pragma unroll
+        for (int i = 1; i <= (n < H ? n : H); ++i)
+            d = xr(d, mul[mul_at(lam[i], S[n - i])]);
+        const bool grow = d != 0 && 2 * L <= (uint32_t)n;
+        // c = d / b; d == 0 gives c = 0 and leaves Lambda as it is
+        const uint32_t c = mul[mul_at(d, binv)];
+        uint32_t old[H + 1];
+#pragma unroll
+        for (int i = 0; i <= H; ++i)
+            old[i] = lam[i];
+        const int hi = n + 1 < H ? n + 1 : H;
+#pragma unroll
+        for (int i = 1; i <= hi; ++i)
+            lam[i] = xr(lam[i], mul[mul_at(c, bp[i])]);
+        const int bh = n + 2 < H ? n + 2 : H;
+#pragma unroll
+        for (int i = bh; i >= 1; --i)
+            bp[i] = grow ? old[i - 1] : bp[i - 1];
+        bp[0] = 0;
+        if (grow) {
+            L = (uint32_t)n + 1 - L;
+            binv = inv[d & 0xffu];
+        }
+    }
+    if (L > (uint32_t)H)
+        return false;
+#pragma unroll
+    for (int w = 0; w < 12; ++w)
+        words[w] = 0;
+    words[0] = L;
+    const uint32_t LO = rec_lam_off(), OO = rec_om_off(E);
+#pragma unroll
+    for (int i = 0; i <= H; ++i)
+        words[(LO + i) >> 2] |= lam[i] << (8 * ((LO + i) & 3));
+    // Omega_k = sum_{i <= k} Lambda_i S_(k-i), k < L (reed_solomon.hpp:708-728)
+#pragma unroll
+    for (int kk = 0; kk < H; ++kk) {
+        uint32_t v = S[kk];
+#pragma unroll
+        for (int i = 1; i <= kk; ++i)
+            v = xr(v, mul[mul_at(lam[i], S[kk - i])]);
+        v = (uint32_t)kk < L ? v : 0u;
+        words[(OO + kk) >> 2] |= v << (8 * ((OO + kk) & 3));
+    }
+    return true;
+}
+
+template <int E, int WG>
+__global__ __launch_bounds__(WG) void rs256_bm_mul_kernel(uint32_t count, uint8_t *__restrict__ status,
+                                                           const uint32_t *__restrict__ synd,
+                                                           uint32_t *__restrict__ rec,
+                                                           const uint32_t *__restrict__ mult)
+{
+    __shared__ __align__(16) uint8_t s_mul[MULT_BYTES];
+    for (uint32_t i = threadIdx.x; i < MULT_BYTES / 16; i += WG)
+        reinterpret_cast<uint4 *>(s_mul)[i] = reinterpret_cast<const uint4 *>(mult)[i];
+    __syncthreads();
+    const uint8_t *mul = s_mul;
+    const uint8_t *inv = s_mul + 65536;
+    constexpr int EW = (E + 3) / 4;
+    for (uint32_t cw = blockIdx.x * WG + threadIdx.x; cw < count; cw += gridDim.x * WG) {
+        if (status[cw] != RS_PENDING)
+            continue;
+        uint32_t words[12];
+        if (!bm_mul_one<E>(synd + (size_t)cw * EW, mul, inv, words)) {
+            status[cw] = 0;
+            continue;
+        }
+        uint4 *r = reinterpret_cast<uint4 *>(rec + (size_t)cw * 12);
+        r[0] = make_uint4(words[0], words[1], words[2], words[3]);
+        r[1] = make_uint4(words[4], words[5], words[6], words[7]);
+        r[2] = make_uint4(words[8], words[9], words[10], words[11]);
+    }
+}
+
+
+// bm_fixed_one<E> (log / exp tables in LDS) as a kernel of its own: lane per pending codeword,
+// the 48-byte record to `rec`, status 0 where L > E/2.
+// LDS copies of the LogExp256 tables from the device's EXP2 / LOG2.
+__device__ inline void stage_logexp256(uint8_t *ex, uint16_t *lg, const Rs256Tables &tab, uint32_t tid, uint32_t nt)
+{
+    for (uint32_t i = tid; i < 256u; i += nt) {
+        ex[i] = i < 255 ? tab.exp2[i] : 0;
+        lg[i] = i ? tab.log2[i] : (uint16_t)LogExp256::Z;
+    }
+}
+
+template <int E, int WG, bool CF>
+__global__ __launch_bounds__(WG) void rs256_bm_log_kernel(uint32_t count, uint8_t *__restrict__ status,
+                                                           const uint32_t *__restrict__ synd,
+                                                           uint32_t *__restrict__ rec, Rs256Tables tab)
+{
+    __shared__ uint8_t s_exp2[EXP2_LEN];
+    __shared__ uint16_t s_log2[256];
+    __shared__ uint8_t s_ex256[256];
+    __shared__ uint16_t s_lg256[256];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += WG)
+        s_exp2[i] = tab.exp2[i];
+    for (uint32_t i = threadIdx.x; i < 256u; i += WG)
+        s_log2[i] = tab.log2[i];
+    stage_logexp256(s_ex256, s_lg256, tab, threadIdx.x, WG);
+    __syncthreads();
+    using TB = typename std::conditional<CF, LogExp256, LogExp1K>::type;
+    const TB tb{CF ? s_ex256 : s_exp2, CF ? s_lg256 : s_log2};
+    for (uint32_t cw = blockIdx.x * WG + threadIdx.x; cw < count; cw += gridDim.x * WG) {
+        if (status[cw] != RS_PENDING)
+            continue;
+        uint32_t words[12];
+        if (!bm_fixed_one<E>(synd + (size_t)cw * ((E + 3) / 4), tb, words)) {
+            status[cw] = 0;
+            continue;
+        }
+        uint4 *r = reinterpret_cast<uint4 *>(rec + (size_t)cw * 12);
+        r[0] = make_uint4(words[0], words[1], words[2], words[3]);
+        r[1] = make_uint4(words[4], words[5], words[6], words[7]);
+        r[2] = make_uint4(words[8], words[9], words[10], words[11]);
+    }
+}
+
+// ======================================================================
+// Chien search and Forney, sixteen lanes per codeword (records of the BM kernels, L <= 16;
+// reed_solomon.hpp:516-538, 704-743). Lane j of a group holds positions 16j .. 16j+15 of its
+// codeword as the sixteen bytes of four dwords. For a coefficient Lambda_i (i = 1..16) with
+// nibbles v_0, v_1, one 16-byte read of row CT[i][h][v_h] per nibble gives
+// (v_h << 4h) alpha^(-ip) at the lane's sixteen positions, so a wave evaluates four codewords
+// per pass, two reads and four XORs per coefficient, with no value read across lanes. A
+// 64-codeword tile is taken in ascending L (the four codewords of a pass have nearly the same
+// degree and the pass stops at the largest). Where the zeros among positions [0, n) number
+// exactly L, Omega(X^-1) at the lane's positions comes from the same tables and each root's
+// magnitude follows: with Lambda'(X^-1) = X Lambda_odd(X^-1) (characteristic 2), the
+// reference's e = X div(Omega(X^-1), Lambda'(X^-1)) is div(Omega, Lambda_odd), or X Omega
+// where Lambda_odd = 0 (its div(a, 0) = a).
+// ======================================================================
+
+// Nibble bytes of coefficients 1..16 of a polynomial whose coefficient c sits at byte
+// `first + c` of the record words r: dword m holds the low and high nibbles of coefficient
+// 2m+1, then of coefficient 2m+2.
+template <int FIRST>
+__device__ inline void record_nibbles(const uint32_t (&r)[12], uint32_t (&nib)[8])
+{
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        const uint32_t q = alignbyte(r[(FIRST + 4 * g + 4) / 4], r[(FIRST + 4 * g) / 4], (FIRST + 1) & 3);
+#pragma unroll
+        for (int hw = 0; hw < 2; ++hw) {
+            const uint32_t t = (q >> (16 * hw)) & 0xffffu;
+            const uint32_t b = (t & 0xffu) | ((t & 0xff00u) << 8);  // bytes a, 0, b, 0
+            nib[2 * g + hw] = (b & 0x000f000fu) | ((b & 0x00f000f0u) << 4);
+        }
+    }
+}
+
+// acc ^= sum over coefficients i = 1..top (top <= IMAX, wave-uniform) of Lambda_i alpha^(-ip) at
+// the lane's sixteen positions, odd coefficients into acc_odd when SPLIT.
+template <bool SPLIT, int IMAX>
+__device__ inline void chien16_eval(const char *ct, const uint32_t (&nib)[8], uint32_t lo_off, uint32_t hi_off,
+                                    uint32_t top, uint32_t (&acc)[4], uint32_t (&acc_odd)[4])
+{
+#pragma unroll
+    for (int gq = 0; gq < 4; ++gq) {
+        if (4 * gq + 1 > (int)top)
+            break;
+#pragma unroll
+        for (int jj = 0; jj < 4; ++jj) {
+            const int i = 4 * gq + jj + 1;
+            if (i > IMAX)
+                break;
+            const uint32_t word = nib[2 * gq + (jj >> 1)];
+            uint4 t[2];
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int tbl = (i - 1) * 2 + h;  // 4 KB table index
+                const uint32_t sel = 0x0c020000u | ((4u + (uint32_t)(2 * (jj & 1) + h)) << 8);
+                const uint32_t a = __builtin_amdgcn_perm(word, tbl >= 16 ? hi_off : lo_off, sel);
+                t[h] = *reinterpret_cast<const uint4 *>(ct + a + (tbl & 15) * 4096);
+            }
+            uint32_t(&d)[4] = (SPLIT && (i & 1)) ? acc_odd : acc;
+            d[0] = xor3(d[0], t[0].x, t[1].x);
+            d[1] = xor3(d[1], t[0].y, t[1].y);
+            d[2] = xor3(d[2], t[0].z, t[1].z);
+            d[3] = xor3(d[3], t[0].w, t[1].w);
+        }
+    }
+}
+
+// Sum over the sixteen lanes of this lane's row (DPP butterflies), in every lane of the row.
+__device__ inline uint32_t row16_sum(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false);  // row_mirror
+    return x;
+}
+
+// EB > 0: the BM records come from bm_fixed_one<EB> run per tile, one pending codeword per lane,
+// on the syndromes `synd` (its LDS lookups overlap the Chien loops of other waves); EB == 0:
+// from `rec`.
+template <int WG, int EB, bool CF = false>
+__global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__ msgs, uint8_t *__restrict__ eccs,
+                                                            uint32_t count, uint32_t k, uint32_t E,
+                                                            uint8_t *__restrict__ status,
+                                                            const uint32_t *__restrict__ rec,
+                                                            const uint32_t *__restrict__ synd,
+                                                            const uint32_t *__restrict__ ctab, Rs256Tables tab)
+{
+    // CT[i-1][h][v]: 256 position bytes (v << 4h) alpha^(-i p), at a known LDS address so that
+    // the table part of every offset is an immediate
+    __shared__ uint4 s_ct4[CHIEN_TABLE_BYTES / 16];
+    __shared__ uint8_t s_exp2[EXP2_LEN];
+    __shared__ uint16_t s_log2[256];
+    __shared__ uint8_t s_ex256[CF ? 256 : 4];
+    __shared__ uint16_t s_lg256[CF ? 256 : 2];
+    for (uint32_t i = threadIdx.x; i < CHIEN_TABLE_BYTES / 16; i += WG)
+        s_ct4[i] = reinterpret_cast<const uint4 *>(ctab)[i];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += WG)
+        s_exp2[i] = tab.exp2[i];
+    for (uint32_t i = threadIdx.x; i < 256u; i += WG)
+        s_log2[i] = tab.log2[i];
+    if constexpr (CF)
+        stage_logexp256(s_ex256, s_lg256, tab, threadIdx.x, WG);
+    __syncthreads();
+    using TB = typename std::conditional<CF, LogExp256, LogExp1K>::type;
+    const TB tb{CF ? s_ex256 : s_exp2, CF ? s_lg256 : s_log2};
+    const char *ct = reinterpret_cast<const char *>(s_ct4);
+    const uint32_t lane = threadIdx.x & 63, j = lane & 15, grp = lane >> 4;
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const uint32_t n = k + E;
+    const uint32_t lo_off = 16 * j, hi_off = 16 * j + 0x10000u;
+    for (uint32_t base = (blockIdx.x * (WG / 64) + wave) * 64; base < count; base += gridDim.x * WG) {
+        const uint32_t cw = base + lane;
+        bool pend = cw < count && status[cw] == RS_PENDING;
+        uint32_t r[12];
+        if constexpr (EB > 0) {
+#pragma unroll
+            for (int w = 0; w < 12; ++w)
+                r[w] = 0;
+            if (pend && !bm_fixed_one<EB>(synd + (size_t)cw * ((EB + 3) / 4), tb, r)) {
+                status[cw] = 0;
+                pend = false;
+            }
+        }
+        const uint64_t pm = __ballot(pend);
+        if (!pm)
+            continue;
+        if (EB > 0) {
+        } else if (pend) {
+            const uint4 *r4 = reinterpret_cast<const uint4 *>(rec + (size_t)cw * 12);
+            const uint4 a = r4[0], b = r4[1], c = r4[2];
+            r[0] = a.x; r[1] = a.y; r[2] = a.z; r[3] = a.w;
+            r[4] = b.x; r[5] = b.y; r[6] = b.z; r[7] = b.w;
+            r[8] = c.x; r[9] = c.y; r[10] = c.z; r[11] = c.w;
+        } else {
+#pragma unroll
+            for (int w = 0; w < 12; ++w)
+                r[w] = 0;
+        }
+        const uint32_t L = r[0] & 0xffu;
+        // bit 7 of byte q of dword d: position 16 j + 4 d + q lies in the codeword (set up after
+        // BM, whose registers it would otherwise add to)
+        uint32_t vmask[4];
+#pragma unroll
+        for (int d = 0; d < 4; ++d) {
+            vmask[d] = 0;
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                vmask[d] |= (16 * j + 4 * d + q < n) ? (0x80u << (8 * q)) : 0u;
+        }
+        uint32_t lnib[8], onib[8];
+        record_nibbles<20>(r, lnib);  // Lambda_1..Lambda_16 (Lambda_0 = 1 at byte 20)
+        r[5] &= 0xffffff00u;          // Omega_16 does not exist (byte 20 holds Lambda_0)
+        record_nibbles<4>(r, onib);   // Omega_1..Omega_15
+        onib[7] |= ((r[1] & 0xffu) << 16) | (L << 24);  // Omega_0 and L in the unused bytes
+        // rank of each pending codeword in ascending (L, lane) order; others after them
+        const uint32_t npend = (uint32_t)__builtin_popcountll(pm);
+        uint32_t rank = 0, acc = 0;
+#pragma unroll
+        for (uint32_t v = 1; v <= (uint32_t)CHIEN_LMAX; ++v) {
+            const bool hit = pend && L == v;
+            const uint64_t m = __ballot(hit);
+            const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+            rank = hit ? acc + below : rank;
+            acc += (uint32_t)__builtin_popcountll(m);
+        }
+        if (!pend)
+            rank = npend + __builtin_amdgcn_mbcnt_hi((uint32_t)(~pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)~pm, 0u));
+        // lane r receives the lane of the codeword ranked r
+        const uint32_t src = (uint32_t)__builtin_amdgcn_ds_permute((int)(rank * 4), (int)lane);
+        const uint32_t npass = (npend + 3) / 4;
+        for (uint32_t ps = 0; ps < npass; ++ps) {
+            const uint32_t rr = 4 * ps + grp;
+            const bool active = rr < npend;
+            const uint32_t o = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((rr & 63) * 4), (int)src) & 63u;
+            uint32_t ln[8], on[8];
+#pragma unroll
+            for (int m = 0; m < 8; ++m) {
+                ln[m] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)lnib[m]);
+                on[m] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)onib[m]);
+            }
+            const uint32_t Lg = on[7] >> 24;
+            const uint32_t last = (npend - 1 < 4 * ps + 3 ? npend - 1 : 4 * ps + 3) - 4 * ps;
+            const uint32_t Lmax = (uint32_t)__builtin_amdgcn_readlane((int)Lg, (int)(16 * last));
+            uint32_t ev[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};  // Lambda_0 = 1
+            uint32_t od[4] = {0, 0, 0, 0};
+            chien16_eval<true, CHIEN_LMAX>(ct, ln, lo_off, hi_off, Lmax, ev, od);
+            uint32_t z[4], cnt = 0;
+#pragma unroll
+            for (int d = 0; d < 4; ++d) {
+                const uint32_t v = ev[d] ^ od[d];
+                const uint32_t t = (v & 0x7f7f7f7fu) + 0x7f7f7f7fu;
+                z[d] = __builtin_amdgcn_bitop3_b32(t, v, vmask[d], 0x02);  // ~t & ~v & mask: zero bytes
+                cnt += (uint32_t)__builtin_popcount(z[d]);
+            }
+            cnt = row16_sum(cnt);
+            const bool ok = active && cnt == Lg;
+            const uint32_t cwo = base + o;
+            if (j == 0 && active)
+                status[cwo] = ok ? 1 : 0;
+            if (!__ballot(ok))
+                continue;
+            // Omega(X^-1) at the lane's positions: Omega_0 + coefficients 1 .. Lmax - 1
+            uint32_t w[4];
+            const uint32_t om0 = ((on[7] >> 16) & 0xffu) * 0x01010101u;
+            w[0] = w[1] = w[2] = w[3] = om0;
+            chien16_eval<false, CHIEN_LMAX - 1>(ct, on, lo_off, hi_off, Lmax - 1, w, w);
+            // roots: bit 8q + d of rm <-> position 16 j + 4 d + q
+            uint32_t rm = ok ? ((z[0] >> 7) | (z[1] >> 6) | (z[2] >> 5) | (z[3] >> 4)) : 0u;
+            uint8_t *mrow = msgs + (size_t)cwo * k;
+            uint8_t *erow = eccs + (size_t)cwo * E;
+            while (rm) {
+                const uint32_t t = (uint32_t)__builtin_ctz(rm);
+                rm &= rm - 1;
+                const uint32_t q = t >> 3, d = t & 7u;
+                const uint32_t p = 16 * j + 4 * d + q;
+                uint32_t wv = w[0], ov = od[0];
+                wv = d == 1 ? w[1] : wv;
+                ov = d == 1 ? od[1] : ov;
+                wv = d == 2 ? w[2] : wv;
+                ov = d == 2 ? od[2] : ov;
+                wv = d == 3 ? w[3] : wv;
+                ov = d == 3 ? od[3] : ov;
+                const uint32_t num = (wv >> (8 * q)) & 0xffu, dod = (ov >> (8 * q)) & 0xffu;
+                if (num) {
+                    const uint32_t lnm = s_log2[num], ld = s_log2[dod];
+                    const uint32_t mag = dod ? (uint32_t)s_exp2[lnm + 255 - ld] : (uint32_t)s_exp2[p + lnm];
+                    const uint32_t at = n - 1 - p;
+                    xor_byte(at < k ? mrow + at : erow + (at - k), mag);
+                }
+            }
+        }
+    }
+}
+
+
+// ======================================================================
 // Mixed-size batches: a plan holds a work list of codewords with their own message and
 // ecc lengths, sorted by descending ecc then descending length and cut into chunks of at most
 // WG codewords of one ecc. One persistent launch serves the whole list: each workgroup
@@ -1396,9 +1822,9 @@ __global__ __launch_bounds__(WG) void rs256_plan_kernel(uint8_t *__restrict__ ms
 
 }  // namespace ffrs
 
-// ---------------------------------------------------------------- C ABI
-
 using namespace ffrs;
+
+// ---------------------------------------------------------------- C ABI
 
 struct ffrs_rs256 {
     ffrs_ctx *ctx = nullptr;
@@ -1412,6 +1838,7 @@ struct ffrs_rs256 {
     uint16_t *d_log2 = nullptr;
     uint32_t *d_stab = nullptr;  // syndrome nibble tables (ecc <= 32)
     uint32_t *d_ctab = nullptr;  // Chien nibble tables (ecc <= 32)
+    uint8_t *d_mult = nullptr;   // product table MUL[a][b] then INV[a] (ecc <= 32)
     void *d_work = nullptr;      // syndromes + BM records of the last repair
     size_t work_size = 0;
     int repair_mode = 0;         // 0 automatic, 1 pipeline (ecc <= 32), 2 one kernel, lane per codeword
@@ -1602,18 +2029,27 @@ int reserve_work(ffrs_rs256 *rs, size_t bytes)
     return FFRS_OK;
 }
 
+// Workspace of the repair pipeline for `count` codewords of ecc E (word class EW): the
+// syndromes (EW dwords per codeword), then the 48-byte BM records from the next 256-byte
+// boundary.
+size_t pipeline_work_bytes(uint32_t E, uint32_t EW, size_t count)
+{
+    const size_t synd_bytes = count * EW * 4;
+    return ((synd_bytes + 255) & ~(size_t)255) + count * rec_stride(E);
+}
+
+// The pipeline over a workspace `work` of at least pipeline_work_bytes(E, EW, count) bytes.
+// Nothing in it needs to start at zero: the syndrome kernel writes the syndromes of every
+// codeword it marks pending, and only pending codewords' entries are read after it.
 template <int EW, int S>
 int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_t count, uint32_t k,
-                           uint8_t *d_status)
+                           uint8_t *d_status, void *work)
 {
     ffrs_ctx *ctx = rs->ctx;
     const uint32_t E = rs->ecc;
     size_t synd_bytes = count * EW * 4;
-    size_t rec_bytes = count * rec_stride(E);
-    if (int r = reserve_work(rs, synd_bytes + rec_bytes + 256))
-        return r;
-    uint32_t *d_synd = (uint32_t *)rs->d_work;
-    uint8_t *d_rec = (uint8_t *)rs->d_work + ((synd_bytes + 255) & ~(size_t)255);
+    uint32_t *d_synd = (uint32_t *)work;
+    uint8_t *d_rec = (uint8_t *)work + ((synd_bytes + 255) & ~(size_t)255);
     Rs256Tables tab{rs->d_lut, rs->d_exp2, rs->d_log2};
 
     constexpr int WG1 = S == LUT_REP ? lfsr_wg(EW, S) : synd_wg(EW);
@@ -1627,39 +2063,112 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
 
-    // 15 waves: the tables and root slots just fit in 160 KB; 11 with the ecc-32 BM inside,
-    // whose registers (145) exceed the 128 that four waves per SIMD leave (measured at 1M x
-    // RS(255,223) on MI355X: 565 us with 11 waves, 581 with 12, 628 with 10, 659 with 8)
-    constexpr int WG3 = 960, WG3B = 704;
-    auto chien = [&](auto kern, int wg) -> int {
-        size_t lds3 = (size_t)wg * CHIEN_LMAX * 2 + EXP2_LEN + 512;  // plus the static table
-        int g3 = persistent_grid(ctx, 1, (count + wg - 1) / wg);
-        if (int r = allow_lds(kern, lds3))
-            return r;
-        hipLaunchKernelGGL(kern, dim3(g3), dim3(wg), lds3, ctx->stream, d_msg, d_ecc, (uint32_t)count, k, E,
-                           d_status, (const uint8_t *)d_rec, (const uint32_t *)d_synd, rs->d_ctab, tab);
+    if (rs->repair_mode == 3) {
+        // BM inside the wave-per-codeword Chien kernel: 15 waves (the tables and root slots just
+        // fit in 160 KB); 11 with the ecc-32 BM inside, whose registers (145) exceed the 128
+        // that four waves per SIMD leave
+        constexpr int WG3 = 960, WG3B = 704;
+        auto chien = [&](auto kern, int wg) -> int {
+            size_t lds3 = (size_t)wg * CHIEN_LMAX * 2 + EXP2_LEN + 512;  // plus the static table
+            int g3 = persistent_grid(ctx, 1, (count + wg - 1) / wg);
+            if (int r = allow_lds(kern, lds3))
+                return r;
+            hipLaunchKernelGGL(kern, dim3(g3), dim3(wg), lds3, ctx->stream, d_msg, d_ecc, (uint32_t)count, k, E,
+                               d_status, (const uint8_t *)d_rec, (const uint32_t *)d_synd, rs->d_ctab, tab);
+            FFRS_HIP(hipGetLastError());
+            mark_kernel(ctx);
+            return FFRS_OK;
+        };
+        switch (E) {
+        case 2: return chien(rs256_chien_kernel<WG3, 2>, WG3);
+        case 4: return chien(rs256_chien_kernel<WG3, 4>, WG3);
+        case 8: return chien(rs256_chien_kernel<WG3, 8>, WG3);
+        case 16: return chien(rs256_chien_kernel<WG3, 16>, WG3);
+        case 32: return chien(rs256_chien_kernel<WG3B, 32>, WG3B);
+        default: break;
+        }
+    }
+
+    // ecc 2, 4, 8, 16, 32: BM inside the sixteen-lane Chien kernel
+    auto fused = [&](auto kern, int wg) -> int {
+        int g = persistent_grid(ctx, 1, (count + wg - 1) / wg);
+        hipLaunchKernelGGL(kern, dim3(g), dim3(wg), 0, ctx->stream, d_msg, d_ecc, (uint32_t)count, k, E, d_status,
+                           (const uint32_t *)d_rec, (const uint32_t *)d_synd, rs->d_ctab, tab);
         FFRS_HIP(hipGetLastError());
         mark_kernel(ctx);
         return FFRS_OK;
     };
-    // ecc 2, 4, 8, 16, 32: BM inside the Chien kernel
-    switch (E) {
-    case 2: return chien(rs256_chien_kernel<WG3, 2>, WG3);
-    case 4: return chien(rs256_chien_kernel<WG3, 4>, WG3);
-    case 8: return chien(rs256_chien_kernel<WG3, 8>, WG3);
-    case 16: return chien(rs256_chien_kernel<WG3, 16>, WG3);
-    case 32: return chien(rs256_chien_kernel<WG3B, 32>, WG3B);
-    default: break;
+    if (rs->repair_mode <= 1 && E == 32)
+        return fused(rs256_chien16_kernel<768, 32>, 768);
+    if (rs->repair_mode == 5 && E == 32)
+        return fused(rs256_chien16_kernel<512, 32>, 512);
+    if (rs->repair_mode == 8 && E == 32)
+        return fused(rs256_chien16_kernel<512, 32, true>, 512);
+    if (rs->repair_mode <= 1 || rs->repair_mode == 5) {
+        switch (E) {
+        case 2: return fused(rs256_chien16_kernel<1024, 2>, 1024);
+        case 4: return fused(rs256_chien16_kernel<1024, 4>, 1024);
+        case 8: return fused(rs256_chien16_kernel<1024, 8>, 1024);
+        case 16: return fused(rs256_chien16_kernel<1024, 16>, 1024);
+        default: break;
+        }
     }
-    size_t lds2 = EXP2_LEN + 512 + (size_t)256 * bm_row_stride(E);
-    if (int r = allow_lds(rs256_bm_kernel<256>, lds2))
-        return r;
-    int g2 = persistent_grid(ctx, (int)std::max<size_t>(1, (160 * 1024) / lds2), (count + 255) / 256);
-    hipLaunchKernelGGL((rs256_bm_kernel<256>), dim3(g2), dim3(256), lds2, ctx->stream, (uint32_t)count, E,
-                       (uint32_t)EW, d_status, (const uint32_t *)d_synd, d_rec, tab);
+    if ((rs->repair_mode == 6 || rs->repair_mode == 7) && E == 32) {
+        int g = persistent_grid(ctx, 3, (count + 255) / 256);
+        auto *bk = rs->repair_mode == 7 ? rs256_bm_log_kernel<32, 256, true> : rs256_bm_log_kernel<32, 256, false>;
+        hipLaunchKernelGGL(bk, dim3(g), dim3(256), 0, ctx->stream, (uint32_t)count,
+                           d_status, (const uint32_t *)d_synd, (uint32_t *)d_rec, tab);
+        FFRS_HIP(hipGetLastError());
+        mark_kernel(ctx);
+        constexpr int WGC = 1024;
+        int gc = persistent_grid(ctx, 1, (count + WGC - 1) / WGC);
+        hipLaunchKernelGGL((rs256_chien16_kernel<WGC, 0>), dim3(gc), dim3(WGC), 0, ctx->stream, d_msg, d_ecc,
+                           (uint32_t)count, k, E, d_status, (const uint32_t *)d_rec, (const uint32_t *)d_synd,
+                           rs->d_ctab, tab);
+        FFRS_HIP(hipGetLastError());
+        mark_kernel(ctx);
+        return FFRS_OK;
+    }
+    // Berlekamp-Massey: over the product table for ecc 2, 4, 8, 16, 32, else from the log tables
+    constexpr int WGB = 512;
+    auto bm_mul = [&](auto kern) -> int {
+        int g = persistent_grid(ctx, 2, (count + WGB - 1) / WGB);  // 64 KB of LDS each: two per CU
+        hipLaunchKernelGGL(kern, dim3(g), dim3(WGB), 0, ctx->stream, (uint32_t)count, d_status,
+                           (const uint32_t *)d_synd, (uint32_t *)d_rec, (const uint32_t *)rs->d_mult);
+        FFRS_HIP(hipGetLastError());
+        mark_kernel(ctx);
+        return FFRS_OK;
+    };
+    int rb;
+    switch (E) {
+    case 2: rb = bm_mul(rs256_bm_mul_kernel<2, WGB>); break;
+    case 4: rb = bm_mul(rs256_bm_mul_kernel<4, WGB>); break;
+    case 8: rb = bm_mul(rs256_bm_mul_kernel<8, WGB>); break;
+    case 16: rb = bm_mul(rs256_bm_mul_kernel<16, WGB>); break;
+    case 32: rb = bm_mul(rs256_bm_mul_kernel<32, WGB>); break;
+    default: {
+        size_t lds2 = EXP2_LEN + 512 + (size_t)256 * bm_row_stride(E);
+        if (int r = allow_lds(rs256_bm_kernel<256>, lds2))
+            return r;
+        int g2 = persistent_grid(ctx, (int)std::max<size_t>(1, (160 * 1024) / lds2), (count + 255) / 256);
+        hipLaunchKernelGGL((rs256_bm_kernel<256>), dim3(g2), dim3(256), lds2, ctx->stream, (uint32_t)count, E,
+                           (uint32_t)EW, d_status, (const uint32_t *)d_synd, d_rec, tab);
+        FFRS_HIP(hipGetLastError());
+        mark_kernel(ctx);
+        rb = FFRS_OK;
+    }
+    }
+    if (rb)
+        return rb;
+    // Chien + Forney: the 128 KB table fills the CU's LDS, one workgroup of sixteen waves per CU
+    constexpr int WGC = 1024;
+    int gc = persistent_grid(ctx, 1, (count + WGC - 1) / WGC);
+    hipLaunchKernelGGL((rs256_chien16_kernel<WGC, 0>), dim3(gc), dim3(WGC), 0, ctx->stream, d_msg, d_ecc,
+                       (uint32_t)count, k, E, d_status, (const uint32_t *)d_rec, (const uint32_t *)d_synd, rs->d_ctab,
+                       tab);
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
-    return chien(rs256_chien_kernel<WG3, 0>, WG3);
+    return FFRS_OK;
 }
 
 // Splits launches so that every launch addresses < 2 GiB of messages with 32-bit offsets.
@@ -1692,31 +2201,42 @@ int encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *d_msg, size
     return FFRS_OK;
 }
 
+// Whether repairs of this codec run the syndrome / BM / Chien pipeline (which needs a
+// workspace) or the one-kernel lane mode (which does not).
+bool uses_pipeline(const ffrs_rs256 *rs) { return rs->EW <= 8 && rs->repair_mode != 2; }
+
+// `work`: the pipeline's workspace, at least pipeline_work_bytes(ecc, EW, min(count,
+// max_batch)) bytes, or nullptr to use (and grow) the codec's own.
 int repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d_ecc, size_t count,
-                  uint8_t *d_status)
+                  uint8_t *d_status, void *work = nullptr)
 {
     marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
     uint32_t k = (uint32_t)message_len;
     size_t step = max_batch(message_len);
+    // The pipeline is faster at every batch size measured (142 codewords of ecc 32:
+    // 274 us against 373 us lane by lane); the lane kernel stays selectable.
+    const bool pipe = uses_pipeline(rs);
+    if (pipe && !work) {
+        if (int r = reserve_work(rs, pipeline_work_bytes(rs->ecc, rs->EW, std::min(step, count))))
+            return r;
+        work = rs->d_work;
+    }
     for (size_t b = 0; b < count; b += step) {
         size_t c = std::min(step, count - b);
         uint8_t *m = d_msg + b * message_len;
         uint8_t *e = d_ecc + b * rs->ecc;
         uint8_t *st = d_status + b;
         int r;
-        // The pipeline is faster at every batch size measured (142 codewords of ecc 32:
-        // 274 us against 373 us lane by lane); the lane kernel stays selectable.
-        bool pipe = rs->repair_mode != 2;
         switch (rs->EW) {
-        case 1: r = pipe ? launch_repair_pipeline<1, 4>(rs, m, e, c, k, st) : launch_repair<1, 4, 256>(rs, m, e, c, k, st); break;
-        case 2: r = pipe ? launch_repair_pipeline<2, 4>(rs, m, e, c, k, st) : launch_repair<2, 4, 256>(rs, m, e, c, k, st); break;
-        case 4: r = pipe ? launch_repair_pipeline<4, 4>(rs, m, e, c, k, st) : launch_repair<4, 4, 256>(rs, m, e, c, k, st); break;
+        case 1: r = pipe ? launch_repair_pipeline<1, 4>(rs, m, e, c, k, st, work) : launch_repair<1, 4, 256>(rs, m, e, c, k, st); break;
+        case 2: r = pipe ? launch_repair_pipeline<2, 4>(rs, m, e, c, k, st, work) : launch_repair<2, 4, 256>(rs, m, e, c, k, st); break;
+        case 4: r = pipe ? launch_repair_pipeline<4, 4>(rs, m, e, c, k, st, work) : launch_repair<4, 4, 256>(rs, m, e, c, k, st); break;
         case 8:
             r = !pipe ? launch_repair<8, 4, 128>(rs, m, e, c, k, st)
-                : rs->d_lutr ? launch_repair_pipeline<8, LUT_REP>(rs, m, e, c, k, st)
-                : launch_repair_pipeline<8, 4>(rs, m, e, c, k, st);
+                : rs->d_lutr ? launch_repair_pipeline<8, LUT_REP>(rs, m, e, c, k, st, work)
+                : launch_repair_pipeline<8, 4>(rs, m, e, c, k, st, work);
             break;
         case 16: r = launch_repair<16, 2, 128>(rs, m, e, c, k, st); break;
         case 32: r = launch_repair<32, 1, 64>(rs, m, e, c, k, st); break;
@@ -1810,9 +2330,22 @@ int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsig
     if (e == hipSuccess && rs->EW <= 8) {
         std::vector<uint32_t> st = build_synd_tables(*rs);
         std::vector<uint32_t> ct = build_chien_tables(*rs);
+        std::vector<uint8_t> mt(MULT_BYTES);
+        for (unsigned a = 0; a < 256; ++a)
+            for (unsigned b = 0; b < 256; ++b)
+                mt[a * 256 + b] = rs->gf.mul((uint8_t)a, (uint8_t)b);
+        mt[65536] = 1;  // inv(0) = 1 (galois.hpp:98-114)
+        for (unsigned a = 1; a < 256; ++a)
+            for (unsigned b = 1; b < 256; ++b)
+                if (mt[a * 256 + b] == 1)
+                    mt[65536 + a] = (uint8_t)b;
         e = hipMalloc(&rs->d_stab, st.size() * 4);
         if (e == hipSuccess)
             e = hipMalloc(&rs->d_ctab, ct.size() * 4);
+        if (e == hipSuccess)
+            e = hipMalloc(&rs->d_mult, mt.size());
+        if (e == hipSuccess)
+            e = hipMemcpy(rs->d_mult, mt.data(), mt.size(), hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(rs->d_stab, st.data(), st.size() * 4, hipMemcpyHostToDevice);
         if (e == hipSuccess)
@@ -1843,6 +2376,8 @@ void ffrs_rs256_destroy(ffrs_rs256 *rs)
         (void)hipFree(rs->d_stab);
     if (rs->d_ctab)
         (void)hipFree(rs->d_ctab);
+    if (rs->d_mult)
+        (void)hipFree(rs->d_mult);
     if (rs->d_work)
         (void)hipFree(rs->d_work);
     delete rs;
@@ -1852,7 +2387,7 @@ int ffrs_rs256_set_repair_mode(ffrs_rs256 *rs, int mode)
 {
     if (!rs)
         return fail(FFRS_ERR_INVALID, "codec is NULL");
-    if (mode < 0 || mode > 2)
+    if (mode < 0 || mode > 8)
         return fail(FFRS_ERR_INVALID, "repair mode %d (0 automatic, 1 pipeline, 2 lane per codeword)", mode);
     rs->repair_mode = mode;
     return FFRS_OK;

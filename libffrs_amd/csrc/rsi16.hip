@@ -1795,6 +1795,369 @@ __global__ __launch_bounds__(64) void rsi16_repair_known_kernel(uint16_t *__rest
         *status = FFRS_RSI16_REPAIR_OK;
 }
 
+// ---------------------------------------------------------------- interleaved repair, SIMD words
+
+// The reference's vector instantiations repair the columns of an interleaved block in groups of
+// W = 4, 8 or 16 run in lockstep (pyrsi16.hpp:389-453 -> RSi16v<W>::_repair, _sugiyama,
+// _find_roots_ntt: rsi16v_impl.hpp:122-168, 271-497; NTT::poly_div_ntt, poly_inv_mod_xn_rev:
+// ntt.hpp:363-564; vector helpers simd.hpp:60-254), and the columns of a group change one
+// another's results: the Sugiyama loop runs while any lane is active, lanes that finished keep
+// their polynomials but have their lengths overwritten and measured again against bounds taken
+// over the group, and Forney evaluates every lane with the group's longest derivative and
+// evaluator. With W = 16 the Newton step of the inverse ignores its lane mask
+// (rsi16v_avx512.cpp, assign_masked). Here one GPU lane plays one SIMD lane: the W lanes of a
+// cluster take the same branches, decided on maxima and any() over the cluster, and each lane
+// runs the per-lane arithmetic (on canonical values the vector and scalar words agree) over
+// its own polynomials, kept in a workspace with the lanes' elements side by side.
+
+struct LaneVec {  // element i of one lane's vector
+    uint32_t *p;
+    uint32_t stride;
+    __device__ uint32_t &operator[](uint32_t i) const { return p[(size_t)i * stride]; }
+};
+
+template <int W>
+__device__ inline uint32_t cl_max(uint32_t v)
+{
+#pragma unroll
+    for (int m = 1; m < W; m <<= 1) {
+        const uint32_t o = (uint32_t)__shfl_xor((int)v, m, W);
+        v = o > v ? o : v;
+    }
+    return v;
+}
+
+template <int W>
+__device__ inline bool cl_any(bool b) { return cl_max<W>(b ? 1u : 0u) != 0; }
+
+// ntt.hpp:280-302 (natural in, RBO out) and ntt.hpp:225-249 (RBO in, natural out), one lane
+__device__ inline void lv_gs(LaneVec x, const uint32_t *roots, uint32_t len)
+{
+    for (uint32_t st = len / 2, ef = 0; st > 0; st /= 2, ++ef)
+        for (uint32_t b = 0; b < len; b += 2 * st) {
+            for (uint32_t i = b; i < b + st; ++i) {
+                const uint32_t a = x[i], c = x[i + st];
+                x[i] = S::add(a, c);
+                const uint32_t d = S::sub(a, c);
+                x[i + st] = i == b ? d : S::mul(d, roots[(i - b) << ef]);
+            }
+        }
+}
+
+__device__ inline void lv_ct(LaneVec x, const uint32_t *roots, uint32_t len)
+{
+    for (uint32_t st = 1, ef = len >> 1; st < len; st *= 2, ef >>= 1)
+        for (uint32_t b = 0; b < len; b += 2 * st) {
+            for (uint32_t i = b; i < b + st; ++i) {
+                const uint32_t a = x[i];
+                const uint32_t m = i == b ? x[i + st] : S::mul(x[i + st], roots[ef * (i - b)]);
+                x[i] = S::add(a, m);
+                x[i + st] = S::sub(a, m);
+            }
+        }
+}
+
+__device__ inline void lv_nttr(const Rsi16Dev &T, LaneVec x) { lv_gs(x, T.roots_ecc, T.E); }
+__device__ inline void lv_inttr(const Rsi16Dev &T, LaneVec x)
+{
+    lv_ct(x, T.roots_i_ecc, T.E);
+    for (uint32_t i = 0; i < T.E; ++i)
+        x[i] = S::mul(x[i], T.ecc_inv);
+}
+
+// simd.hpp:228-254: degree + 1 within the first `bound` coefficients (a bound past the
+// polynomial is clamped to it)
+__device__ inline uint32_t lv_len(LaneVec x, uint32_t bound, uint32_t E)
+{
+    uint32_t m = bound < E ? bound : E;
+    while (m > 0 && x[m - 1] == 0)
+        --m;
+    return m;
+}
+
+// NTT::_reverse_ntt_vec, ntt.hpp:344-354
+__device__ inline void lv_reverse_ntt(const Rsi16Dev &T, LaneVec v, uint32_t shift)
+{
+    const uint32_t E = T.E, mask = E - 1;
+    for (uint32_t i = 1; i < E; ++i) {
+        const uint32_t j = T.rbo_ecc[i], kk = T.rbo_ecc[E - j];
+        if (i < kk) {
+            const uint32_t t = v[i];
+            v[i] = v[kk];
+            v[kk] = t;
+        }
+        v[i] = S::mul(S::mul(v[i], T.roots_i_ecc[j]), T.roots_ecc[(shift * j) & mask]);
+    }
+}
+
+// The per-lane state of the coupled repair, in the workspace.
+struct SimdLane {
+    LaneVec blk, ev, loc, der, q, t, a2, r2, tmp;
+};
+
+// NTT::poly_inv_mod_xn_rev, ntt.hpp:363-469, vector instantiation; false where it raises
+template <int W>
+__device__ inline bool lv_poly_inv(const Rsi16Dev &T, LaneVec g, uint32_t g_len, uint32_t g0, uint32_t a_len, LaneVec a)
+{
+    const uint32_t E = T.E;
+    const uint32_t a0 = g_inv(g0);
+    for (uint32_t i = 0; i < E; ++i)
+        a[i] = a0;
+    uint32_t l = 2;
+    const uint32_t a_max = cl_max<W>(a_len);
+    bool cond = l < a_len;
+    while (cl_any<W>(cond)) {
+        if (cl_max<W>(cond ? g_len : 0u) + l - 2 > E)  // py_assert at ntt.hpp:388-391
+            return false;
+        if (cond || W == 16)  // rsi16v_avx512.cpp: the W = 16 masked assignment updates every lane
+            for (uint32_t i = 0; i < E; ++i)
+                a[i] = S::mul(a[i], S::sub(2, S::mul(a[i], g[i])));
+        lv_inttr(T, a);
+        for (uint32_t i = l; i < E; ++i)
+            a[i] = 0;
+        lv_nttr(T, a);
+        l *= 2;
+        cond = l < a_len;
+    }
+    for (uint32_t i = 0; i < E; ++i)
+        g[i] = S::mul(a[i], S::sub(2, S::mul(a[i], g[i])));
+    lv_inttr(T, g);
+    for (uint32_t i = 0; i < E; ++i)
+        a[i] = i < l ? g[i] : 0u;
+    for (uint32_t i = 0, m = a_max - 1; i < a_max / 2; ++i, --m) {  // std::reverse(&a[0], &a[a_max])
+        const uint32_t t = a[i];
+        a[i] = a[m];
+        a[m] = t;
+    }
+    for (uint32_t i = 0; i < a_len; ++i)  // vec::copy_n(&a[0], a_max - a_len, a_len, &a[0], 0)
+        a[i] = a[a_max - a_len + i];
+    for (uint32_t i = a_len; i < E; ++i)
+        a[i] = 0;
+    lv_nttr(T, a);
+    return true;
+}
+
+// NTT::poly_div_ntt, ntt.hpp:471-564, vector instantiation: q <- f // g (transforms);
+// false where it raises
+template <int W>
+__device__ inline bool lv_poly_div(const Rsi16Dev &T, LaneVec f, uint32_t f_len, LaneVec g, uint32_t g_len, uint32_t f0,
+                                   uint32_t g0, LaneVec t, LaneVec q, uint32_t &q_len)
+{
+    const uint32_t E = T.E;
+    if (cl_any<W>(g_len == 0 && f_len > 0))  // py_assert(!any((g_len == 0) && (f_len > 0)))
+        return false;
+    q_len = (g_len != 0 && f_len >= g_len) ? f_len - g_len + 1 : 0u;
+    for (uint32_t i = 0; i < E; ++i)
+        t[i] = g[i];
+    lv_reverse_ntt(T, t, g_len);
+    if (!lv_poly_inv<W>(T, t, g_len, g0, q_len, q))
+        return false;
+    for (uint32_t i = 0; i < E; ++i)
+        t[i] = S::mul(S::sub(f[i], f0), q[i]);
+    lv_inttr(T, t);
+    const uint32_t room = E - g_len, right_n = q_len < room ? q_len : room;
+    for (uint32_t i = 0; i < right_n; ++i)
+        q[i] = t[g_len + i];
+    for (uint32_t i = 0; i < q_len - right_n; ++i)
+        q[right_n + i] = t[i];
+    for (uint32_t i = q_len; i < E; ++i)
+        q[i] = 0;
+    lv_nttr(T, q);
+    return true;
+}
+
+// RSi16v<W>::_sugiyama, rsi16v_impl.hpp:271-440: L.ev holds the syndromes on entry and the
+// evaluator on exit, L.loc the locator; returns the locator length, or 0xFFFFFFFF where the
+// reference raises.
+template <int W>
+__device__ inline uint32_t lv_sugiyama(const Rsi16Dev &T, const SimdLane &L)
+{
+    const uint32_t E = T.E, mask = E - 1;
+    LaneVec a1 = L.loc, r1 = L.ev, q = L.q, t = L.t, a2 = L.a2, r2 = L.r2;
+    for (uint32_t i = 0; i < E; ++i) {
+        a1[i] = 0;
+        a2[i] = 1;
+        q[i] = 0;
+        t[i] = 0;
+    }
+    uint32_t q_len = 0, t_len = 0, a2_len = 1;
+    uint32_t r1_len = lv_len(r1, E, E);
+    for (uint32_t i = 0; i < E; ++i)
+        r2[i] = r1[i];
+    uint32_t r2_len = r1_len, r2l = r2[0];
+    const uint32_t lanes = r1_len != 0 ? 0xffffffffu : 0u;
+    uint32_t a1_len;
+    {  // the degree-one first step (:308-335)
+        const uint32_t h = r1[(r1_len - 1) & mask], h2 = r1[(r1_len - 2) & mask];
+        const uint32_t c1 = s_neg(g_inv(h));
+        const uint32_t c0 = S::mul(h2, S::mul(c1, c1));
+        a1[0] = c0;
+        a1[1] = c1;
+        a1_len = 2 & lanes;
+        for (uint32_t i = 0; i < E; ++i)
+            r1[i] = S::mul(c0, r1[i]);
+        for (uint32_t i = 1; i < E; ++i)
+            r1[i] = S::add(r1[i], S::mul(c1, r2[i - 1]));
+        r1_len = lv_len(r1, E - 1, E) & lanes;
+        for (uint32_t i = r1_len; i < E; ++i)
+            r1[i] = 0;
+        lv_nttr(T, a1);
+        lv_nttr(T, r2);
+    }
+    bool cond = r1_len >= E / 2;
+    if (cl_any<W>(cond)) {
+        for (uint32_t it = 1; it < E / 2; ++it) {
+            const uint32_t r1h = r1[(r1_len - 1) & mask], r1l = r1[0];
+            lv_nttr(T, r1);
+            if (!lv_poly_div<W>(T, r2, cond ? r2_len : 0u, r1, r1_len, r2l, r1h, t, q, q_len))
+                return 0xffffffffu;
+            for (uint32_t i = 0; i < E; ++i) {
+                const uint32_t qi = q[i];
+                t[i] = S::sub(a2[i], S::mul(qi, a1[i]));  // a2 - q a1 (poly_mul_ntt, _vec_sub)
+                q[i] = S::sub(r2[i], S::mul(qi, r1[i]));  // r2 - q r1
+            }
+            t_len = q_len + a1_len - 1;
+            t_len = a2_len > t_len ? a2_len : t_len;
+            q_len = q_len + r1_len - 1;
+            q_len = r2_len > q_len ? r2_len : q_len;
+            for (uint32_t i = 0; i < E; ++i) {
+                a2[i] = a1[i];
+                r2[i] = r1[i];
+            }
+            a2_len = a1_len;
+            r2_len = r1_len;
+            r2l = r1l;
+            cond = r1_len >= E / 2;
+            if (!cl_any<W>(cond)) {
+                lv_inttr(T, r1);
+                break;
+            }
+            if (cond)  // vec::copy_n_masked
+                for (uint32_t i = 0; i < E; ++i) {
+                    a1[i] = t[i];
+                    r1[i] = q[i];
+                }
+            a1_len = t_len;  // assigned in every lane
+            r1_len = q_len;
+            lv_inttr(T, r1);
+            r1_len = lv_len(r1, cl_max<W>(cond ? r1_len : 0u), E);
+        }
+    }
+    lv_inttr(T, a1);
+    a1_len = lv_len(a1, cl_max<W>(a1_len & lanes), E);
+    const uint32_t inv0 = g_inv(a1[0]);
+    for (uint32_t i = 0; i < E; ++i) {
+        a1[i] = S::mul(a1[i], inv0);
+        r1[i] = S::mul(r1[i], inv0);
+    }
+    return a1_len & lanes;
+}
+
+__device__ inline uint32_t lv_eval(LaneVec r, uint32_t len, uint32_t x)
+{
+    uint32_t s = 0;
+    for (uint32_t i = len - 1; i < len; --i)
+        s = S::add(S::mul(s, x), r[i]);
+    return s;
+}
+
+// One lane per SIMD lane: group `g0 + lane / W` of an interleaved block (columns
+// (g0 + lane / W) * W + lane % W, those past the interleave zero), its codeword loaded into the
+// workspace, repaired there; gstatus[group] = the group's status (the maximum over its lanes)
+// or FFRS_RSI16_RAISES.
+template <int W>
+__global__ __launch_bounds__(64) void rsi16_repair_simd_kernel(Rsi16Dev T, const uint16_t *__restrict__ msg,
+                                                               const uint16_t *__restrict__ ecc, uint32_t il, uint32_t g0,
+                                                               uint32_t lanes_total, uint32_t *__restrict__ work,
+                                                               uint8_t *__restrict__ gstatus)
+{
+    const uint32_t lane = blockIdx.x * 64 + threadIdx.x;
+    if (lane >= lanes_total)  // lanes_total is a multiple of W: clusters are whole
+        return;
+    const uint32_t n = T.n, k = T.k, E = T.E;
+    const uint32_t grp = g0 + lane / W, col = grp * W + lane % W;
+    const bool valid = col < il;
+    auto row = [&](uint32_t r) { return LaneVec{work + (size_t)r * lanes_total + lane, lanes_total}; };
+    SimdLane L{row(0), row(n), row(n + E), row(n + 2 * E), row(n + 3 * E), row(n + 4 * E), row(n + 5 * E),
+               row(n + 6 * E), row(n + 7 * E)};
+    for (uint32_t i = 0; i < n; ++i)
+        L.blk[i] = !valid ? 0u : (i < k ? msg[(size_t)i * il + col] : ecc[(size_t)(i - k) * il + col]);
+    // NTT::pntt, ntt.hpp:133-152: the syndromes
+    for (uint32_t c = 0; c < n / E; ++c) {
+        for (uint32_t i = 0; i < E; ++i)
+            L.tmp[i] = L.blk[c * E + i];
+        lv_ct(L.tmp, T.roots_ecc, E);
+        for (uint32_t i = 0; i < E; ++i) {
+            const uint32_t v = S::mul(L.tmp[i], T.pntt_shift[c * E + i]);
+            L.ev[i] = c == 0 ? v : S::add(L.ev[i], v);
+        }
+    }
+    bool nz = false;
+    for (uint32_t i = 0; i < E; ++i)
+        nz |= L.ev[i] != 0;
+    uint8_t st = FFRS_RSI16_NO_ERRORS;
+    if (cl_any<W>(nz)) {
+        const uint32_t Lloc = lv_sugiyama<W>(T, L);
+        if (cl_any<W>(Lloc == 0xffffffffu) || cl_max<W>(Lloc) > E) {  // rsi16v_impl.hpp:454
+            st = FFRS_RSI16_RAISES;
+        } else {
+            const bool active = Lloc != 0;
+            const uint32_t elen = active ? E - Lloc + 1 : 0u;
+            for (uint32_t i = 0; i + 1 < E; ++i)  // _deriv, rsi16v_impl.hpp:608-616
+                L.der[i] = S::mul(L.loc[i + 1], i + 1);
+            L.der[E - 1] = 0;
+            const uint32_t dlen = active ? lv_len(L.der, E, E) : 0u;
+            const uint32_t dmax = cl_max<W>(dlen), emax = cl_max<W>(elen);
+            // _find_roots_ntt with NTT::pintt_ecc chunk by chunk (ntt.hpp:206-220)
+            uint32_t count = 0;
+            for (uint32_t c = 0; c < n / E; ++c) {
+                for (uint32_t i = 0; i < E; ++i)
+                    L.tmp[i] = S::mul(L.loc[i], T.pintt_shift[c * E + i]);
+                lv_gs(L.tmp, T.roots_i_ecc, E);
+                for (uint32_t i = 0; i < E; ++i) {
+                    if (active && L.tmp[i] == 0) {
+                        const uint32_t m = c * E + i;
+                        const uint32_t x = T.roots_ntt[rbo_bits(m, T.nbits)];
+                        const uint32_t xi = g_inv(x);
+                        const uint32_t num = lv_eval(L.ev, emax, xi), den = lv_eval(L.der, dmax, xi);
+                        L.blk[m] = S::add(L.blk[m], S::mul(g_div(num, den), x));  // _forney, :500-512
+                        ++count;
+                    }
+                }
+            }
+            const uint8_t lst = count > 0 ? FFRS_RSI16_REPAIR_OK
+                                          : (active ? FFRS_RSI16_ERROR_LOCATION_FAIL : FFRS_RSI16_NO_ERRORS);
+            st = (uint8_t)cl_max<W>(lst);
+        }
+    }
+    if (lane % W == 0)
+        gstatus[grp] = st;
+}
+
+// Writes the repaired columns of groups [g0, g_end) back (values & 0xFFFF), from the workspace
+// of the launch that repaired groups g0.. (lanes_total lanes).
+template <int W>
+__global__ __launch_bounds__(256) void rsi16_simd_writeback_kernel(const Rsi16Dev T, uint16_t *__restrict__ msg,
+                                                                   uint16_t *__restrict__ ecc, uint32_t il, uint32_t g0,
+                                                                   uint32_t g_end, uint32_t lanes_total,
+                                                                   const uint32_t *__restrict__ work)
+{
+    const uint32_t lane = blockIdx.x * 64 + (threadIdx.x & 63);
+    if (lane >= lanes_total)
+        return;
+    const uint32_t grp = g0 + lane / W, col = grp * W + lane % W;
+    if (grp >= g_end || col >= il)
+        return;
+    const uint32_t n = T.n, k = T.k;
+    for (uint32_t i = threadIdx.x >> 6; i < n; i += 4) {
+        const uint16_t v = (uint16_t)work[(size_t)i * lanes_total + lane];
+        if (i < k)
+            msg[(size_t)i * il + col] = v;
+        else
+            ecc[(size_t)(i - k) * il + col] = v;
+    }
+}
+
 // ---------------------------------------------------------------- mixed-size plans
 
 // A plan's work list is sorted by (block_len, ecc_len) class and cut into chunks of the
@@ -1929,6 +2292,8 @@ struct ffrs_rsi16 {
     uint32_t *d_tables = nullptr;
     Rsi16Dev dev{};
     uint32_t g_enc = 0, g_rep = 0;  // lanes per codeword of the encode and repair kernels
+    void *d_work = nullptr;          // workspace of the SIMD-word interleaved repair
+    size_t work_size = 0;
 };
 
 namespace {
@@ -2239,6 +2604,8 @@ void ffrs_rsi16_destroy(ffrs_rsi16 *rs)
     (void)hipSetDevice(rs->ctx->device);
     if (rs->d_tables)
         (void)hipFree(rs->d_tables);
+    if (rs->d_work)
+        (void)hipFree(rs->d_work);
     delete rs;
 }
 
@@ -2370,6 +2737,81 @@ int ffrs_rsi16_encode_interleaved(ffrs_rsi16 *rs, const uint16_t *msg, size_t bl
         return r;
     FFRS_HIP(hipMemcpyAsync(ecc, d_e, eb, hipMemcpyDeviceToHost, ctx->stream));
     FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_repair_interleaved_simd(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *ecc, size_t interleave,
+                                       unsigned width, uint8_t *status)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (width != 4 && width != 8 && width != 16)
+        return fail(FFRS_ERR_INVALID, "SIMD width must be 4, 8 or 16, got %u", width);
+    if (interleave == 0 || interleave > 0xffffu)
+        return fail(FFRS_ERR_INVALID, "interleave must be in [1, 65535], got %zu", interleave);
+    if (!msg || !ecc || !status)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    const uint32_t W = width, il = (uint32_t)interleave;
+    const size_t mb = (size_t)rs->k * il * 2, eb = (size_t)rs->E * il * 2;
+    const uint32_t groups = (il + W - 1) / W;
+    const size_t rows = (size_t)rs->n + 8 * (size_t)rs->E;  // per-lane workspace, u32 each
+    // lanes per launch: every group of the call, or as many as 256 MB of workspace hold
+    size_t lanes_cap = std::max<size_t>(64, ((size_t)256 << 20) / (rows * 4) / 64 * 64);
+    const uint32_t gpb = (uint32_t)std::min<size_t>(groups, lanes_cap / W);  // groups per launch
+    void *d_m, *d_e, *d_s;
+    if (int r = stage_reserve(ctx, 0, mb, &d_m))
+        return r;
+    if (int r = stage_reserve(ctx, 1, eb, &d_e))
+        return r;
+    if (int r = stage_reserve(ctx, 2, groups, &d_s))
+        return r;
+    const size_t wbytes = (size_t)gpb * W * rows * 4;
+    if (rs->work_size < wbytes) {
+        if (rs->d_work)
+            FFRS_HIP(hipFree(rs->d_work));
+        rs->d_work = nullptr;
+        rs->work_size = 0;
+        hipError_t e = hipMalloc(&rs->d_work, wbytes);
+        if (e != hipSuccess)
+            return fail(FFRS_ERR_NOMEM, "repair workspace of %zu bytes: %s", wbytes, hipGetErrorString(e));
+        rs->work_size = wbytes;
+    }
+    FFRS_HIP(hipMemcpyAsync(d_m, msg, mb, hipMemcpyHostToDevice, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(d_e, ecc, eb, hipMemcpyHostToDevice, ctx->stream));
+    std::vector<uint8_t> gst(groups);
+    uint8_t res = FFRS_RSI16_NO_ERRORS;
+    bool raised = false;
+    for (uint32_t g0 = 0; g0 < groups && !raised; g0 += gpb) {
+        const uint32_t ng = std::min(gpb, groups - g0), lanes = ng * W;
+        const dim3 grid((lanes + 63) / 64);
+        auto *rk = W == 4 ? rsi16_repair_simd_kernel<4> : W == 8 ? rsi16_repair_simd_kernel<8>
+                                                                  : rsi16_repair_simd_kernel<16>;
+        hipLaunchKernelGGL(rk, grid, dim3(64), 0, ctx->stream, rs->dev, (const uint16_t *)d_m,
+                           (const uint16_t *)d_e, il, g0, lanes, (uint32_t *)rs->d_work, (uint8_t *)d_s);
+        FFRS_HIP(hipGetLastError());
+        FFRS_HIP(hipMemcpyAsync(gst.data() + g0, (uint8_t *)d_s + g0, ng, hipMemcpyDeviceToHost, ctx->stream));
+        FFRS_HIP(hipStreamSynchronize(ctx->stream));
+        // groups are written back in order up to the first that raises (pyrsi16.hpp:410-430)
+        uint32_t end = g0;
+        for (; end < g0 + ng && gst[end] != FFRS_RSI16_RAISES; ++end)
+            res = std::max(res, gst[end]);
+        raised = end < g0 + ng;
+        if (end > g0) {
+            auto *wk = W == 4 ? rsi16_simd_writeback_kernel<4> : W == 8 ? rsi16_simd_writeback_kernel<8>
+                                                                        : rsi16_simd_writeback_kernel<16>;
+            hipLaunchKernelGGL(wk, grid, dim3(256), 0, ctx->stream, rs->dev, (uint16_t *)d_m, (uint16_t *)d_e, il, g0,
+                               end, lanes, (const uint32_t *)rs->d_work);
+            FFRS_HIP(hipGetLastError());
+        }
+    }
+    FFRS_HIP(hipMemcpyAsync(msg, d_m, mb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(ecc, d_e, eb, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    *status = raised ? (uint8_t)FFRS_RSI16_RAISES : res;
     return FFRS_OK;
 }
 
@@ -2705,11 +3147,13 @@ int rsi16_plan_run(ffrs_rsi16_plan *p, int mode, uint16_t *msgs, size_t msg_elem
 
 extern "C" {
 
-int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_len, const uint16_t *ecc_len,
-                           const uint64_t *msg_off, const uint64_t *ecc_off, ffrs_rsi16_plan **out)
+int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint32_t *block_len, const uint32_t *ecc_len,
+                           const uint64_t *msg_off, const uint64_t *ecc_off, uint32_t primitive, ffrs_rsi16_plan **out)
 {
     if (!ctx || !out || (count && (!block_len || !ecc_len || !msg_off || !ecc_off)))
         return fail(FFRS_ERR_INVALID, "NULL argument");
+    if (primitive == 0)
+        primitive = 3;
     if (count > 0xffffffffu)
         return fail(FFRS_ERR_INVALID, "too many codewords in one plan");
     auto *p = new ffrs_rsi16_plan();
@@ -2723,7 +3167,7 @@ int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint16_t *block_le
         size_t c = std::find(keys.begin(), keys.end(), key) - keys.begin();
         if (c == keys.size()) {
             ffrs_rsi16 *codec = nullptr;
-            if (int r = ffrs_rsi16_create(ctx, block_len[i], ecc_len[i], 3, &codec)) {
+            if (int r = ffrs_rsi16_create(ctx, block_len[i], ecc_len[i], primitive, &codec)) {
                 rsi16_plan_free(p);
                 return r;
             }

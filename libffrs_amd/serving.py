@@ -8,8 +8,10 @@ ffrs/par/repair.py:97).
 
 Continuous batching: requests are cut into pieces of at most `piece_blocks` blocks and
 queued in arrival order. Each of `slots` workers owns a HIP stream (its own context), a
-fixed device arena and pinned host staging of `slot_blocks` blocks; a free worker takes
-queued pieces, from any requests, until its arena is full (admission by HBM budget), then
+device arena and pinned host staging of `slot_blocks` blocks; the arenas together take at
+most `hbm_fraction` of the HBM the device reports free when the server starts (the HBM
+budget: `slot_blocks` is lowered to fit it). A free worker takes queued pieces, from any
+requests, until its arena is full (admission by the HBM budget), then
 copies them in, runs one encode launch and one repair launch over all of them, and copies
 the results out. Requests that arrive while other batches are on the GPU are admitted by
 the next free worker, so copies and kernels of different batches overlap on the device.
@@ -86,11 +88,18 @@ class _Slot:
 class BlockServer:
     """RSi16 block coding server; see the module docstring. Defaults: RSi16(1024, 128)."""
 
-    def __init__(self, block_len=1024, ecc_len=128, slots=3, slot_blocks=8192, piece_blocks=2048, device=0):
+    def __init__(self, block_len=1024, ecc_len=128, slots=3, slot_blocks=8192, piece_blocks=2048, device=0,
+                 hbm_fraction=0.25):
         self.n, self.ecc, self.k = block_len, ecc_len, block_len - ecc_len
+        per_block = 2 * block_len + 1  # message + parity symbols and a status byte
+        probe = capi.Context(device)
+        self.hbm_free = int(probe.info()[2])
+        probe.close()
+        self.hbm_budget_cap = int(self.hbm_free * hbm_fraction)
+        slot_blocks = max(1, min(slot_blocks, self.hbm_budget_cap // (slots * per_block)))
         self.piece_blocks = max(1, min(piece_blocks, slot_blocks))
         self._slots = [_Slot(device, block_len, ecc_len, slot_blocks) for _ in range(slots)]
-        self.hbm_budget = slots * slot_blocks * (2 * block_len + 1)
+        self.hbm_budget = slots * slot_blocks * per_block
         self._queue = collections.deque()
         self._cv = threading.Condition()
         self._stop = False

@@ -1146,3 +1146,372 @@ int orc_rsi16_repair_known(const orc_rsi16 *rs, uint16_t *msg, uint16_t *ecc, co
     free(temp);
     return st;
 }
+
+/* =====================================================================
+ * RSi16 repair with the SIMD words: W = 4, 8 or 16 columns in lockstep, as the vector
+ * instantiations RSi16v<W>::_repair (rsi16v_impl.hpp:122-168), _sugiyama (:271-440) and
+ * _find_roots_ntt (:443-497), NTT::poly_div_ntt / poly_inv_mod_xn_rev (ntt.hpp:363-564)
+ * and the vector helpers of simd.hpp:60-254 run them for _repair_interleaved
+ * (pyrsi16.hpp:389-453). Lane j's polynomial of E coefficients is at x + j * E; per-lane
+ * lengths are arrays of W. Field arithmetic on canonical values is the same in the vector
+ * and scalar words, so the per-lane work reuses the scalar routines; what the W lanes share
+ * is every loop bound and branch the reference takes on a vector: any() / max() over the
+ * lanes, masked copies, and lengths assigned to all lanes.
+ * ===================================================================== */
+
+#define VW_MAX 16
+
+static uint32_t vl_max(const uint32_t *a, unsigned W)
+{
+    uint32_t m = 0;
+    for (unsigned j = 0; j < W; ++j)
+        m = a[j] > m ? a[j] : m;
+    return m;
+}
+
+static int vl_any(const uint32_t *a, unsigned W)
+{
+    for (unsigned j = 0; j < W; ++j)
+        if (a[j])
+            return 1;
+    return 0;
+}
+
+static inline uint32_t vl_cond(int c) { return c ? 0xffffffffu : 0u; }
+
+/* simd.hpp:228-254 with a shared bound (clamped to E, where the reference would read past
+ * the polynomial) */
+static uint32_t vl_poly_len(const uint32_t *r, uint32_t bound, size_t E)
+{
+    return poly_len(r, bound < (uint32_t)E ? bound : (uint32_t)E);
+}
+
+/* ntt.hpp:363-469, vector instantiation. The Newton step's vec::assign_masked honours its
+ * condition for W = 4 and 8 (rsi16v_sse2.cpp:47-51, rsi16v_avx2.cpp) and ignores it for
+ * W = 16 (rsi16v_avx512.cpp: `condition[j] || true`), where every lane is updated. */
+static void vl_poly_inv_mod_xn_rev(const orc_rsi16 *rs, unsigned W, uint32_t *g, const uint32_t *g_len,
+                                   const uint32_t *g0, const uint32_t *a_len, uint32_t *a, int *err)
+{
+    size_t E = rs->ecc;
+    uint32_t cond[VW_MAX];
+    for (unsigned j = 0; j < W; ++j) {
+        uint32_t a0 = g_inv(rs, g0[j]);
+        for (size_t i = 0; i < E; ++i)
+            a[j * E + i] = a0;
+    }
+    uint32_t l = 2;
+    uint32_t a_max = vl_max(a_len, W);
+    for (unsigned j = 0; j < W; ++j)
+        cond[j] = vl_cond(l < a_len[j]);
+    while (vl_any(cond, W)) {
+        uint32_t gm[VW_MAX];
+        for (unsigned j = 0; j < W; ++j)
+            gm[j] = g_len[j] & cond[j];
+        if (vl_max(gm, W) + l - 2 > E) { /* py_assert at ntt.hpp:388-391 */
+            *err = 1;
+            return;
+        }
+        for (unsigned j = 0; j < W; ++j) {
+            uint32_t *aj = a + j * E, *gj = g + j * E;
+            if (cond[j] || W == 16)
+                for (size_t i = 0; i < E; ++i)
+                    aj[i] = s_mul(aj[i], s_sub(2, s_mul(aj[i], gj[i])));
+            inttr(rs, aj);
+            for (size_t i = l; i < E; ++i)
+                aj[i] = 0;
+            nttr(rs, aj);
+        }
+        l *= 2;
+        for (unsigned j = 0; j < W; ++j)
+            cond[j] = vl_cond(l < a_len[j]);
+    }
+    for (unsigned j = 0; j < W; ++j) {
+        uint32_t *aj = a + j * E, *gj = g + j * E;
+        for (size_t i = 0; i < E; ++i)
+            gj[i] = s_sub(2, s_mul(aj[i], gj[i]));
+        for (size_t i = 0; i < E; ++i)
+            gj[i] = s_mul(aj[i], gj[i]);
+        inttr(rs, gj);
+        memcpy(aj, gj, E * 4);
+        for (size_t i = l; i < E; ++i)
+            aj[i] = 0;
+        for (uint32_t i = 0, m = a_max - 1; i < a_max / 2; ++i, --m) { /* std::reverse(&a[0], &a[a_max]) */
+            uint32_t t = aj[i];
+            aj[i] = aj[m];
+            aj[m] = t;
+        }
+        for (uint32_t i = 0; i < a_len[j]; ++i) /* vec::copy_n(&a[0], a_max - a_len, a_len, &a[0], 0) */
+            aj[i] = aj[a_max - a_len[j] + i];
+        for (size_t i = a_len[j]; i < E; ++i)
+            aj[i] = 0;
+        nttr(rs, aj);
+    }
+}
+
+/* ntt.hpp:471-564, vector instantiation; q_len[] receives the quotient lengths */
+static void vl_poly_div_ntt(const orc_rsi16 *rs, unsigned W, const uint32_t *f, const uint32_t *f_len,
+                            const uint32_t *g, const uint32_t *g_len, const uint32_t *f0, const uint32_t *g0,
+                            uint32_t *t, uint32_t *q, uint32_t *q_len, int *err)
+{
+    size_t E = rs->ecc;
+    for (unsigned j = 0; j < W; ++j)
+        if (g_len[j] == 0 && f_len[j] > 0) { /* py_assert(!any((g_len == 0) && (f_len > 0))) */
+            *err = 1;
+            return;
+        }
+    for (unsigned j = 0; j < W; ++j)
+        q_len[j] = (f_len[j] - g_len[j] + 1) & vl_cond(g_len[j] != 0 && f_len[j] >= g_len[j]);
+    for (unsigned j = 0; j < W; ++j) {
+        memcpy(t + j * E, g + j * E, E * 4);
+        reverse_ntt(rs, t + j * E, g_len[j]);
+    }
+    vl_poly_inv_mod_xn_rev(rs, W, t, g_len, g0, q_len, q, err);
+    if (*err)
+        return;
+    for (unsigned j = 0; j < W; ++j) {
+        uint32_t *tj = t + j * E, *qj = q + j * E;
+        const uint32_t *fj = f + j * E;
+        for (size_t i = 0; i < E; ++i)
+            tj[i] = s_mul(s_sub(fj[i], f0[j]), qj[i]);
+        inttr(rs, tj);
+        uint32_t room = (uint32_t)E - g_len[j];
+        uint32_t right_n = q_len[j] < room ? q_len[j] : room;
+        for (uint32_t i = 0; i < right_n; ++i)
+            qj[i] = tj[g_len[j] + i];
+        for (uint32_t i = 0; i < q_len[j] - right_n; ++i)
+            qj[right_n + i] = tj[i];
+        for (size_t i = q_len[j]; i < E; ++i)
+            qj[i] = 0;
+        nttr(rs, qj);
+    }
+}
+
+/* rsi16v_impl.hpp:271-440, vector instantiation. r1 holds the W syndrome vectors on entry
+ * and the evaluators on exit; temp has 4 * E * W entries. L[] receives the locator
+ * lengths. */
+static void vl_sugiyama(const orc_rsi16 *rs, unsigned W, uint32_t *a1, uint32_t *r1, uint32_t *temp, uint32_t *L,
+                        int *err)
+{
+    size_t E = rs->ecc, EW = E * W;
+    uint32_t mask = (uint32_t)E - 1;
+    memset(temp, 0, 4 * EW * 4);
+    memset(a1, 0, EW * 4);
+    uint32_t *q = temp, *t = temp + EW, *a2 = temp + 2 * EW, *r2 = temp + 3 * EW;
+    uint32_t q_len[VW_MAX], t_len[VW_MAX], a2_len[VW_MAX], a1_len[VW_MAX], r1_len[VW_MAX], r2_len[VW_MAX];
+    uint32_t r2l[VW_MAX], lanes[VW_MAX], cond[VW_MAX], r1h[VW_MAX], r1l[VW_MAX], f_len[VW_MAX], bound[VW_MAX];
+    for (unsigned j = 0; j < W; ++j) {
+        uint32_t *a1j = a1 + j * E, *r1j = r1 + j * E, *a2j = a2 + j * E, *r2j = r2 + j * E;
+        q_len[j] = 0;
+        t_len[j] = 0;
+        a2_len[j] = 1;
+        for (size_t i = 0; i < E; ++i)
+            a2j[i] = 1;
+        r1_len[j] = poly_len(r1j, (uint32_t)E);
+        memcpy(r2j, r1j, E * 4);
+        r2_len[j] = r1_len[j];
+        r2l[j] = r2j[0];
+        lanes[j] = vl_cond(r1_len[j] != 0);
+        /* the degree-1 first step (:308-335) */
+        uint32_t h = r1j[(r1_len[j] - 1) & mask];
+        uint32_t h2 = r1j[(r1_len[j] - 2) & mask];
+        a1j[1] = s_neg(g_inv(rs, h));
+        a1j[0] = s_mul(h2, s_mul(a1j[1], a1j[1]));
+        a1_len[j] = 2 & lanes[j];
+        for (size_t i = 0; i < E; ++i)
+            r1j[i] = s_mul(a1j[0], r1j[i]);
+        for (size_t i = 1; i < E; ++i)
+            r1j[i] = s_add(r1j[i], s_mul(a1j[1], r2j[i - 1]));
+        r1_len[j] = poly_len(r1j, (uint32_t)E - 1) & lanes[j];
+        for (size_t i = r1_len[j]; i < E; ++i)
+            r1j[i] = 0;
+        nttr(rs, a1j);
+        nttr(rs, r2j);
+        cond[j] = vl_cond(r1_len[j] >= (uint32_t)(E / 2));
+    }
+    if (vl_any(cond, W)) {
+        for (size_t it = 1; it < E / 2; ++it) {
+            for (unsigned j = 0; j < W; ++j) {
+                uint32_t *r1j = r1 + j * E;
+                r1h[j] = r1j[(r1_len[j] - 1) & mask];
+                r1l[j] = r1j[0];
+                nttr(rs, r1j);
+                f_len[j] = r2_len[j] & cond[j];
+            }
+            vl_poly_div_ntt(rs, W, r2, f_len, r1, r1_len, r2l, r1h, t, q, q_len, err);
+            if (*err)
+                return;
+            for (unsigned j = 0; j < W; ++j) {
+                uint32_t *qj = q + j * E, *tj = t + j * E, *a1j = a1 + j * E, *r1j = r1 + j * E;
+                uint32_t *a2j = a2 + j * E, *r2j = r2 + j * E;
+                for (size_t i = 0; i < E; ++i) /* poly_mul_ntt, ntt.hpp:356-361 */
+                    tj[i] = s_mul(qj[i], a1j[i]);
+                t_len[j] = q_len[j] + a1_len[j] - 1;
+                for (size_t i = 0; i < E; ++i)
+                    qj[i] = s_mul(qj[i], r1j[i]);
+                q_len[j] = q_len[j] + r1_len[j] - 1;
+                for (size_t i = 0; i < E; ++i) /* _vec_sub, rsi16v_impl.hpp:578-584 */
+                    tj[i] = s_sub(a2j[i], tj[i]);
+                t_len[j] = a2_len[j] > t_len[j] ? a2_len[j] : t_len[j];
+                for (size_t i = 0; i < E; ++i)
+                    qj[i] = s_sub(r2j[i], qj[i]);
+                q_len[j] = r2_len[j] > q_len[j] ? r2_len[j] : q_len[j];
+                memcpy(a2j, a1j, E * 4);
+                a2_len[j] = a1_len[j];
+                memcpy(r2j, r1j, E * 4);
+                r2_len[j] = r1_len[j];
+                r2l[j] = r1l[j];
+                cond[j] = vl_cond(r1_len[j] >= (uint32_t)(E / 2));
+            }
+            if (!vl_any(cond, W)) {
+                for (unsigned j = 0; j < W; ++j)
+                    inttr(rs, r1 + j * E);
+                break;
+            }
+            for (unsigned j = 0; j < W; ++j) {
+                if (cond[j]) { /* vec::copy_n_masked */
+                    memcpy(a1 + j * E, t + j * E, E * 4);
+                    memcpy(r1 + j * E, q + j * E, E * 4);
+                }
+                a1_len[j] = t_len[j]; /* assigned in every lane */
+                r1_len[j] = q_len[j];
+                inttr(rs, r1 + j * E);
+                bound[j] = r1_len[j] & cond[j];
+            }
+            uint32_t b = vl_max(bound, W);
+            for (unsigned j = 0; j < W; ++j)
+                r1_len[j] = vl_poly_len(r1 + j * E, b, E);
+        }
+    }
+    for (unsigned j = 0; j < W; ++j) {
+        inttr(rs, a1 + j * E);
+        bound[j] = a1_len[j] & lanes[j];
+    }
+    uint32_t b = vl_max(bound, W);
+    for (unsigned j = 0; j < W; ++j) {
+        uint32_t *a1j = a1 + j * E, *r1j = r1 + j * E;
+        a1_len[j] = vl_poly_len(a1j, b, E);
+        uint32_t inv0 = g_inv(rs, a1j[0]);
+        for (size_t i = 0; i < E; ++i)
+            a1j[i] = s_mul(a1j[i], inv0);
+        for (size_t i = 0; i < E; ++i)
+            r1j[i] = s_mul(r1j[i], inv0);
+        L[j] = a1_len[j] & lanes[j];
+    }
+}
+
+/* rsi16v_impl.hpp:122-168 + 443-497 for W lanes: blk holds lane j's codeword at
+ * blk + j * n. Returns the group's status (the maximum over the lanes) or ORC_RSI16_ASSERT. */
+static int vl_repair_group(const orc_rsi16 *rs, unsigned W, uint32_t *blk)
+{
+    size_t E = rs->ecc, n = rs->n, N = rs->N, EW = E * W;
+    uint32_t *synds = malloc(N * 4);
+    uint32_t *ev = malloc(EW * 4), *loc = malloc(EW * 4), *der = malloc(EW * 4), *tmp = malloc(4 * EW * 4);
+    uint32_t *roots = malloc(N * 4);
+    uint32_t L[VW_MAX], elen[VW_MAX], dlen[VW_MAX], lanes[VW_MAX];
+    int any = 0, err = 0, st = ORC_NO_ERRORS;
+    for (unsigned j = 0; j < W; ++j) {
+        memcpy(synds, blk + j * n, n * 4);
+        pntt(rs, synds);
+        memcpy(ev + j * E, synds, E * 4);
+        for (size_t i = 0; i < E; ++i)
+            any |= synds[i] != 0;
+    }
+    if (!any)
+        goto done;
+    vl_sugiyama(rs, W, loc, ev, tmp, L, &err);
+    if (err) {
+        st = ORC_RSI16_ASSERT;
+        goto done;
+    }
+    for (unsigned j = 0; j < W; ++j) {
+        uint32_t *dj = der + j * E;
+        lanes[j] = vl_cond(L[j] != 0);
+        elen[j] = ((uint32_t)E - L[j] + 1) & lanes[j];
+        memcpy(dj, loc + j * E, E * 4);
+        for (uint32_t i = 0; i < E - 1; ++i) /* _deriv, rsi16v_impl.hpp:608-616 */
+            dj[i] = s_mul(dj[i + 1], i + 1);
+        dj[E - 1] = 0;
+        dlen[j] = poly_len(dj, (uint32_t)E);
+    }
+    if (vl_max(L, W) > E) { /* py_assert at rsi16v_impl.hpp:454 */
+        st = ORC_RSI16_ASSERT;
+        goto done;
+    }
+    uint32_t dm[VW_MAX], em[VW_MAX];
+    for (unsigned j = 0; j < W; ++j) {
+        dm[j] = dlen[j] & lanes[j];
+        em[j] = elen[j] & lanes[j];
+    }
+    uint32_t dmax = vl_max(dm, W), emax = vl_max(em, W);
+    for (unsigned j = 0; j < W; ++j) {
+        /* NTT::pintt_ecc, ntt.hpp:206-220 */
+        memset(roots, 0, N * 4);
+        memcpy(roots, loc + j * E, E * 4);
+        for (size_t i = 1; i < n / E; ++i) {
+            memcpy(roots + i * E, roots, E * 4);
+            for (size_t m = 0; m < E; ++m)
+                roots[i * E + m] = s_mul(roots[i * E + m], rs->pintt_shift[i * E + m]);
+            gs_butterfly(rs->roots_i_ecc, roots + i * E, E, 0);
+        }
+        for (size_t m = 0; m < E; ++m)
+            roots[m] = s_mul(roots[m], rs->pintt_shift[m]);
+        gs_butterfly(rs->roots_i_ecc, roots, E, 0);
+        uint32_t count = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (roots[i] == 0 && lanes[j]) {
+                count++;
+                uint32_t x = rs->roots_ntt[rs_rbo(rs, i)];
+                blk[j * n + i] = s_add(blk[j * n + i], forney(rs, der + j * E, dmax, ev + j * E, emax, x));
+            }
+        }
+        int lst = count > 0 ? ORC_REPAIR_OK : (lanes[j] ? ORC_ERROR_LOCATION_FAIL : ORC_NO_ERRORS);
+        st = lst > st ? lst : st;
+    }
+done:
+    free(synds);
+    free(ev);
+    free(loc);
+    free(der);
+    free(tmp);
+    free(roots);
+    return st;
+}
+
+/* PyRSi16::_repair_interleaved<W> (pyrsi16.hpp:389-453) without error positions: groups of W
+ * columns (column c is symbols i * interleave + c), a last group zero-filled past the
+ * columns; each group is written back after its repair and the call returns the largest
+ * status; a group whose repair raises ends the call (ORC_RSI16_ASSERT) with the groups
+ * before it written back. */
+int orc_rsi16_repair_interleaved_simd(const orc_rsi16 *rs, unsigned W, uint16_t *msg, uint16_t *ecc,
+                                      size_t interleave)
+{
+    size_t n = rs->n, k = rs->k, E = rs->ecc;
+    if (W != 4 && W != 8 && W != 16)
+        return ORC_RSI16_ASSERT;
+    uint32_t *blk = malloc(n * W * 4);
+    int res = ORC_NO_ERRORS;
+    for (size_t c0 = 0; c0 < interleave; c0 += W) {
+        size_t cols = interleave - c0 < W ? interleave - c0 : W;
+        memset(blk, 0, n * W * 4);
+        for (size_t j = 0; j < cols; ++j) {
+            for (size_t i = 0; i < k; ++i)
+                blk[j * n + i] = msg[i * interleave + c0 + j];
+            for (size_t i = 0; i < E; ++i)
+                blk[j * n + k + i] = ecc[i * interleave + c0 + j];
+        }
+        int st = vl_repair_group(rs, W, blk);
+        if (st == ORC_RSI16_ASSERT) {
+            res = st;
+            break;
+        }
+        for (size_t j = 0; j < cols; ++j) {
+            for (size_t i = 0; i < k; ++i)
+                msg[i * interleave + c0 + j] = (uint16_t)blk[j * n + i];
+            for (size_t i = 0; i < E; ++i)
+                ecc[i * interleave + c0 + j] = (uint16_t)blk[j * n + k + i];
+        }
+        res = st > res ? st : res;
+    }
+    free(blk);
+    return res;
+}

@@ -101,6 +101,10 @@ uint32_t orc_rsi16_sugiyama(const orc_rsi16 *rs, const uint32_t *synds, uint32_t
  * (pyrsi16.hpp:588-631 -> rsi16v_impl.hpp:122-168). Returns a status above;
  * msg / ecc are updated in place exactly as the reference writes them back. */
 int orc_rsi16_repair(const orc_rsi16 *rs, uint16_t *msg, uint16_t *ecc);
+/* Interleaved repair with the SIMD words, W = 4/8/16 columns per coupled group
+ * (pyrsi16.hpp:389-453, rsi16v_impl.hpp:122-168, 271-497 vector instantiations). */
+int orc_rsi16_repair_interleaved_simd(const orc_rsi16 *rs, unsigned W, uint16_t *msg, uint16_t *ecc,
+                                      size_t interleave);
 void orc_rsi16_repair_batch(const orc_rsi16 *rs, uint16_t *msgs, uint16_t *eccs, size_t count,
                             uint8_t *status, int threads);
 

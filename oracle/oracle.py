@@ -73,6 +73,9 @@ def lib():
         L.orc_rsi16_repair_batch.argtypes = [ctypes.c_void_p, c_u16p, c_u16p, ctypes.c_size_t, c_u8p, ctypes.c_int]
         L.orc_rsi16_repair_known.argtypes = [ctypes.c_void_p, c_u16p, c_u16p, c_szp, ctypes.c_size_t]
         L.orc_rsi16_repair_known.restype = ctypes.c_int
+        L.orc_rsi16_repair_interleaved_simd.argtypes = [ctypes.c_void_p, ctypes.c_uint, c_u16p, c_u16p,
+                                                        ctypes.c_size_t]
+        L.orc_rsi16_repair_interleaved_simd.restype = ctypes.c_int
         _lib = L
     return _lib
 
@@ -215,6 +218,13 @@ class RSi16:
         lib().orc_rsi16_repair_batch(self._h, _p(msgs, c_u16p), _p(eccs, c_u16p), count, _p(st, c_u8p),
                                      threads or default_threads())
         return st
+
+    def repair_interleaved_simd(self, width, msg, ecc, interleave):
+        """Interleaved repair with the SIMD words: columns in coupled groups of `width`
+        (4, 8, 16), in place on the interleaved uint16 buffers; returns the call's status
+        (255: the reference raises, with the groups before the raising one written back)."""
+        assert msg.flags.c_contiguous and ecc.flags.c_contiguous
+        return lib().orc_rsi16_repair_interleaved_simd(self._h, width, _p(msg, c_u16p), _p(ecc, c_u16p), interleave)
 
     def repair_known(self, msg, ecc, positions):
         pos = np.ascontiguousarray(positions, np.uint64).astype(np.uintp)

@@ -82,6 +82,11 @@ class OracleBackend:
             status = max(status, st)
         return status
 
+    def rsi16_repair_interleaved_simd(self, n, ecc, il, width, msg, parity, positions=None):
+        if positions is not None:  # the vector words give every column its own repair here
+            return self.rsi16_repair_interleaved(n, ecc, il, msg, parity, positions)
+        return self.ri(n, ecc).repair_interleaved_simd(width, msg, parity, il)
+
     def rsi16_sugiyama(self, n, ecc, synds):
         L, loc, ev = self.ri(n, ecc).sugiyama(synds)
         return None if L == 0xFFFFFFFF else (loc, ev)
@@ -204,6 +209,20 @@ class GpuBackend:
         key = (n, ecc, il, False)
         if key not in self._ri:
             self._ri[key] = self.L.RSi16(n, ecc, il, simd_x4=False, simd_x8=False, simd_x16=False)
+        kw = {} if positions is None else dict(error_pos=[int(v) for v in positions])
+        try:
+            return int(self._ri[key].repair(msg.view(np.uint8), parity.view(np.uint8), **kw))
+        except RuntimeError:
+            return 255
+
+    def rsi16_repair_interleaved_simd(self, n, ecc, il, width, msg, parity, positions=None):
+        """RSi16(n, ecc, il) with the SIMD words of `width` lanes (flags set explicitly, so the
+        result does not depend on the host CPU)."""
+        flags = {16: dict(simd_x16=True), 8: dict(simd_x16=False, simd_x8=True),
+                 4: dict(simd_x16=False, simd_x8=False, simd_x4=True)}[width]
+        key = (n, ecc, il, "simd", width)
+        if key not in self._ri:
+            self._ri[key] = self.L.RSi16(n, ecc, il, **flags)
         kw = {} if positions is None else dict(error_pos=[int(v) for v in positions])
         try:
             return int(self._ri[key].repair(msg.view(np.uint8), parity.view(np.uint8), **kw))

@@ -166,3 +166,142 @@ def rsi16_known_positions(pos, s, n=256):
         p = (p + cand[:int(extra[i])])[:32]
         allpos[i, :len(p)] = p
     return allpos
+
+
+# ------------------------------------------------------------------ shapes of the reference's own tests
+
+# tests/test_lib_rs256.py:73-82 of the reference: block_len 239 and 255, ecc_len 1-16, 31-33, 63-65,
+# 127-129, 253, 254 (where block_len > ecc_len)
+RS256_REF_SHAPES = tuple((bl, e) for bl in (239, 255)
+                         for e in list(range(1, 17)) + [31, 32, 33, 63, 64, 65, 127, 128, 129, 253, 254] if bl > e)
+
+
+def rs256_reftest_case(block_len, ecc, count=24):
+    """`count` messages of one reference test shape, with 0 .. ecc/2 + 2 errors each (some
+    codewords lie beyond the decoding radius)."""
+    k = block_len - ecc
+    s = W.Stream(5000 + block_len * 256 + ecc)
+    msgs = s.bytes(count * k).reshape(count, k)
+    counts = s.ints(0, min(ecc // 2 + 2, block_len), count)
+    return msgs, counts, s
+
+
+# tests/test_lib_rsi16.py:399-455 of the reference (TestRSPower2, TestRSMult, TestRSLarge)
+RSI16_REF_SHAPES = ((4, 2), (8, 2), (16, 4), (16, 8), (128, 2), (256, 32), (208, 8), (768, 8), (4096, 8),
+                    (4096, 256), (2048, 128), (4096, 2048), (32768, 256), (65536, 256))
+
+
+def rsi16_reftest_case(n, ecc):
+    """Messages of one reference test shape (64 codewords for short blocks, down to 4 for
+    n = 65536) with 0 .. ecc/2 + 1 additive errors each."""
+    count = int(max(4, min(64, (1 << 18) // n)))
+    s = W.Stream(7000 + n + ecc)
+    msgs = s.u16(count * (n - ecc)).reshape(count, n - ecc)
+    counts = s.ints(0, min(ecc // 2 + 1, n), count)
+    return msgs, counts, s
+
+
+# ------------------------------------------------------------------ SIMD-word parity corner
+
+# Shapes with a mix constant -(w^(-rbo(k) j) / ecc) equal to 65536 (= -1) for some j (the
+# mix of libffrs/rsi16v_impl.hpp:68-86 of the reference). There the SIMD words of the encode
+# (galois.hpp:375-393: a residue that is 0 mod p and non-negative reduces to p, not 0, and
+# p * 65536 wraps to 65536 in 32 bits) give other parity than the scalar words whenever the
+# message residue at that j is 0 mod p, e.g. for the all-zero message.
+RSI16_DIV_SHAPES = ((22, 2), (24, 4), (32, 8), (48, 8), (128, 8), (256, 128), (512, 128), (512, 256),
+                    (4096, 1024))
+
+
+def rsi16_mix_minus_one(n, ecc):
+    """Indices j < ecc whose mix constant is 65536, for RSi16(n, ecc) with primitive 3."""
+    p = 65537
+    N = 1 << (n - 1).bit_length()
+    bits = N.bit_length() - 1
+    w = pow(3, 65536 // N, p)
+    wm = pow(w, rbo(n - ecc, bits), p)
+    wmi = pow(wm, p - 2, p)
+    inv_e = pow(ecc, p - 2, p)
+    return [j for j in range(ecc) if (-pow(wmi, j, p) * inv_e) % p == 65536]
+
+
+def rsi16_message_residue(n, ecc, msg, j):
+    """sum_m msg_m w^(j rbo(m)) mod p over the message symbols (the residue the mix scales)."""
+    p = 65537
+    N = 1 << (n - 1).bit_length()
+    bits = N.bit_length() - 1
+    w = pow(3, 65536 // N, p)
+    acc = 0
+    for m, v in enumerate(msg):
+        if v:
+            acc = (acc + int(v) * pow(w, (j * rbo(m, bits)) % N, p)) % p
+    return acc
+
+
+def rsi16_divergence_case(n, ecc):
+    """Messages for RSi16(n, ecc): all zero, all 65535, messages whose residue at a
+    mix-minus-one index j is 0 mod p (symbol 0, whose locator is 1, adjusted), and random
+    ones."""
+    k = n - ecc
+    s = W.Stream(9000 + n * 16 + ecc)
+    js = rsi16_mix_minus_one(n, ecc)
+    rows = [np.zeros(k, np.uint16), np.full(k, 65535, np.uint16)]
+    want = 6 if k <= 512 else 3
+    made = 0
+    while made < want:
+        m = s.u16(k)
+        j = js[made % len(js)]
+        r = rsi16_message_residue(n, ecc, m, j)
+        v = (int(m[0]) - r) % 65537
+        if v == 65536:
+            continue
+        m[0] = v
+        rows.append(m)
+        made += 1
+    rows += [s.u16(k), s.u16(k)]
+    return np.stack(rows)
+
+
+# ------------------------------------------------------------------ interleaved repair, SIMD words
+
+# (block_len, ecc_len, interleave) of the coupled-group repair fixtures; groups of 16, 8 and 4
+# columns, with a zero-filled last group where the interleave is not a multiple of the width
+RSI16_ILSIMD_CASES = ((64, 8, 16), (64, 8, 21), (256, 32, 16), (256, 32, 7), (64, 16, 40), (1024, 128, 17),
+                      (16, 4, 9), (4096, 256, 5))
+RSI16_ILSIMD_WIDTHS = (16, 8, 4)
+RSI16_ILSIMD_TRIALS = 4
+
+
+def rsi16_ilsimd_messages(n, ecc, il, width, trial):
+    """The interleaved message of one trial, and the stream that corrupts it."""
+    s = W.Stream(11000 + 977 * n + 131 * ecc + 17 * il + 5 * width + trial)
+    return s.u16((n - ecc) * il), s
+
+
+def rsi16_ilsimd_corrupt(n, ecc, il, msg, par, s, trial):
+    """Additive errors per column: 0 .. ecc/2 + 2 in even trials (some columns beyond the
+    radius), 0 .. ecc/2 in odd ones."""
+    k = n - ecc
+    cws = np.concatenate([msg.reshape(k, il), par.reshape(ecc, il)], 0).T.copy()  # [il, n]
+    counts = s.ints(0, ecc // 2 + (2 if trial % 2 == 0 else 0), il)
+    W.rsi16_corrupt(cws, counts, s)
+    return np.ascontiguousarray(cws[:, :k].T).reshape(-1), np.ascontiguousarray(cws[:, k:].T).reshape(-1)
+
+
+RSI16_ILSIMD_KNOWN_TRIALS = 2
+
+
+def rsi16_ilsimd_known(n, ecc, il, msg, par, s):
+    """Known-position trials: 1 .. ecc positions, and in every column additive errors at a
+    random subset of them. Returns (message, ecc, positions)."""
+    k = n - ecc
+    npos = int(s.ints(1, ecc, 1)[0])
+    pos = np.sort(s.distinct_positions(np.array([npos]), n)[0]).astype(np.int64)
+    cws = np.concatenate([msg.reshape(k, il), par.reshape(ecc, il)], 0).T.astype(np.int64)  # [il, n]
+    for c in range(il):
+        ne = int(s.ints(0, npos, 1)[0])
+        where = pos[:ne]
+        v = (cws[c, where] + s.ints(1, 65536, ne)) % 65537
+        v[v == 65536] = 0
+        cws[c, where] = v
+    cws = cws.astype(np.uint16)
+    return (np.ascontiguousarray(cws[:, :k].T).reshape(-1), np.ascontiguousarray(cws[:, k:].T).reshape(-1), pos)

@@ -231,6 +231,10 @@ def main():
 
     out.update(radius_fixture(L))
     out.update(interleave_fixture(L))
+    out.update(interleave_repair_fixture(L))
+    out.update(reftest_fixture(L))
+    out.update(divergence_fixture(L))
+    out.update(ilsimd_fixture(L))
 
     for key, d in out.items():
         np.savez_compressed(os.path.join(HERE, key + ".npz"), **d)
@@ -335,12 +339,120 @@ def interleave_repair_fixture(L):
     return out
 
 
+def reftest_fixture(L):
+    """(g) every shape of the reference's own RS256 and RSi16 tests: per shape a batch of
+    messages (regenerated from a seed; digests stored), the parity CRCs (RSi16: SIMD and
+    scalar words), and the status and repaired-codeword CRCs after 0 .. ecc/2 + 2 (RS256) or
+    0 .. ecc/2 + 1 (RSi16) errors per codeword."""
+    rs = {}
+    for bl, ecc in C.RS256_REF_SHAPES:
+        k = bl - ecc
+        msgs, counts, s = C.rs256_reftest_case(bl, ecc)
+        ref = L.RS256(block_len=bl, ecc_len=ecc)
+        par = np.frombuffer(ref.encode(msgs.tobytes()), np.uint8).reshape(-1, ecc)
+        cws = np.concatenate([msgs, par], 1)
+        W.rs256_corrupt(cws, counts, s)
+        st, rep = ref_repair_rs256(ref, cws, k)
+        t = f"{bl}_{ecc}"
+        rs.update({f"in_sha_{t}": C.sha(msgs, counts), f"parity_crc_{t}": C.row_crc(par),
+                   f"corrupt_sha_{t}": C.sha(cws), f"status_{t}": st, f"repaired_crc_{t}": C.row_crc(rep)})
+    ri = {}
+    for n, ecc in C.RSI16_REF_SHAPES:
+        k = n - ecc
+        msgs, counts, s = C.rsi16_reftest_case(n, ecc)
+        ref = L.RSi16(n, ecc)
+        par = np.frombuffer(ref.encode(msgs.tobytes()), np.uint16).reshape(-1, ecc)
+        sca = L.RSi16(n, ecc, simd_x4=False, simd_x8=False, simd_x16=False)
+        par_s = np.frombuffer(sca.encode(msgs.tobytes()), np.uint16).reshape(-1, ecc)
+        cws = np.concatenate([msgs, par], 1)
+        W.rsi16_corrupt(cws, counts, s)
+        st, rep = ref_repair_rsi16(ref, cws, k)
+        t = f"{n}_{ecc}"
+        ri.update({f"in_sha_{t}": C.sha(msgs, counts), f"parity_crc_{t}": C.row_crc(par),
+                   f"parity_scalar_crc_{t}": C.row_crc(par_s), f"corrupt_sha_{t}": C.sha(cws),
+                   f"status_{t}": st, f"repaired_crc_{t}": C.row_crc(rep)})
+        print("rsi16 reftest", t, "statuses", np.bincount(st, minlength=5)[[0, 2, 4]], "255:", int((st == 255).sum()))
+    return {"rs256_reftest": rs, "rsi16_reftest": ri}
+
+
+def divergence_fixture(L):
+    """(h) RSi16 shapes whose mix has a constant 65536: messages (explicit) and the parity of
+    the SIMD words and of the scalar words, which differ where the message residue at that
+    index is 0 mod p (and its lazy sum non-negative)."""
+    d = {}
+    for n, ecc in C.RSI16_DIV_SHAPES:
+        msgs = C.rsi16_divergence_case(n, ecc)
+        pv = np.frombuffer(L.RSi16(n, ecc).encode(msgs.tobytes()), np.uint16).reshape(-1, ecc)
+        ps = np.frombuffer(L.RSi16(n, ecc, simd_x4=False, simd_x8=False, simd_x16=False)
+                           .encode(msgs.tobytes()), np.uint16).reshape(-1, ecc)
+        t = f"{n}_{ecc}"
+        d.update({f"msgs_{t}": msgs, f"parity_{t}": pv, f"parity_scalar_{t}": ps})
+        print("divergence", t, "rows whose words differ:", int((pv != ps).any(1).sum()), "of", len(msgs))
+    return {"rsi16_simd_div": d}
+
+
+def ilsimd_fixture(L):
+    """(i) interleaved repair without error positions with the SIMD words, W = 16, 8, 4
+    (simd_x16; simd_x16=False; simd_x16=simd_x8=False), where the reference repairs the columns
+    in coupled groups (pyrsi16.hpp:389-453): per trial the digest of the corrupted input, the
+    call's status (255: raises) and the CRCs of both buffers after the call."""
+    d = {}
+    flags = {16: {}, 8: dict(simd_x16=False), 4: dict(simd_x16=False, simd_x8=False)}
+    for n, ecc, il in C.RSI16_ILSIMD_CASES:
+        for w in C.RSI16_ILSIMD_WIDTHS:
+            ref = L.RSi16(n, ecc, il, **flags[w])
+            st = np.zeros(C.RSI16_ILSIMD_TRIALS, np.uint8)
+            sha = np.zeros((C.RSI16_ILSIMD_TRIALS, 32), np.uint8)
+            crc = np.zeros((C.RSI16_ILSIMD_TRIALS, 2), np.uint32)
+            for t in range(C.RSI16_ILSIMD_TRIALS):
+                msg, s = C.rsi16_ilsimd_messages(n, ecc, il, w, t)
+                par = np.frombuffer(ref.encode(msg.tobytes()), np.uint16)
+                m, e = C.rsi16_ilsimd_corrupt(n, ecc, il, msg, par, s, t)
+                sha[t] = C.sha(m, e)
+                mb, eb = bytearray(m.tobytes()), bytearray(e.tobytes())
+                try:
+                    st[t] = int(ref.repair(mb, eb))
+                except RuntimeError:
+                    st[t] = 255
+                crc[t] = [C.row_crc(np.frombuffer(mb, np.uint16)[None])[0], C.row_crc(np.frombuffer(eb, np.uint16)[None])[0]]
+            tag = f"{n}_{ecc}_{il}_w{w}"
+            d.update({f"sha_{tag}": sha, f"status_{tag}": st, f"crc_{tag}": crc})
+            # with error positions
+            kst = np.zeros(C.RSI16_ILSIMD_KNOWN_TRIALS, np.uint8)
+            kcrc = np.zeros((C.RSI16_ILSIMD_KNOWN_TRIALS, 2), np.uint32)
+            ksha = np.zeros((C.RSI16_ILSIMD_KNOWN_TRIALS, 32), np.uint8)
+            for t in range(C.RSI16_ILSIMD_KNOWN_TRIALS):
+                msg, s = C.rsi16_ilsimd_messages(n, ecc, il, w, 100 + t)
+                par = np.frombuffer(ref.encode(msg.tobytes()), np.uint16)
+                m, e, pos = C.rsi16_ilsimd_known(n, ecc, il, msg, par, s)
+                ksha[t] = C.sha(m, e, pos)
+                mb, eb = bytearray(m.tobytes()), bytearray(e.tobytes())
+                try:
+                    kst[t] = int(ref.repair(mb, eb, [int(x) for x in pos]))
+                except RuntimeError:
+                    kst[t] = 255
+                kcrc[t] = [C.row_crc(np.frombuffer(mb, np.uint16)[None])[0],
+                           C.row_crc(np.frombuffer(eb, np.uint16)[None])[0]]
+            d.update({f"ksha_{tag}": ksha, f"kstatus_{tag}": kst, f"kcrc_{tag}": kcrc})
+            print("ilsimd", tag, "statuses", st.tolist())
+    return {"rsi16_ilsimd": d}
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["interleave"]:
         for key, v in interleave_fixture(load_reference()).items():
             np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
     elif sys.argv[1:] == ["interleave_repair"]:
         for key, v in interleave_repair_fixture(load_reference()).items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    elif sys.argv[1:] == ["ilsimd"]:
+        for key, v in ilsimd_fixture(load_reference()).items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    elif sys.argv[1:] == ["reftest"]:
+        for key, v in reftest_fixture(load_reference()).items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    elif sys.argv[1:] == ["divergence"]:
+        for key, v in divergence_fixture(load_reference()).items():
             np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
     elif sys.argv[1:] == ["radius"]:
         from libffrs_amd import workloads as W  # noqa: F811

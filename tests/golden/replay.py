@@ -222,3 +222,90 @@ def check_rsi16_interleaved_repair(be):
             st = be.rsi16_repair_interleaved(n, ecc, il, m, e, pos if len(pos) else None)
             assert st == fx["status"][t], (n, ecc, il, t, st, fx["status"][t])
             assert (m == fx["msgs_out"][t]).all() and (e == fx["eccs_out"][t]).all(), (n, ecc, il, t)
+
+
+def check_rs256_reftest(be, block_len, ecc):
+    """One shape of the reference's own RS256 tests: parity, status and repaired bytes."""
+    fx = fixture("rs256_reftest")
+    t = f"{block_len}_{ecc}"
+    k = block_len - ecc
+    msgs, counts, s = C.rs256_reftest_case(block_len, ecc)
+    assert (C.sha(msgs, counts) == fx["in_sha_" + t]).all(), "input generator drift"
+    par = be.rs256_encode_batch(ecc, msgs)
+    crc = C.row_crc(par)
+    assert (crc == fx["parity_crc_" + t]).all(), "parity: " + _mismatch(crc, fx["parity_crc_" + t])
+    cws = np.concatenate([msgs, par], 1)
+    W.rs256_corrupt(cws, counts, s)
+    assert (C.sha(cws) == fx["corrupt_sha_" + t]).all()
+    m = np.ascontiguousarray(cws[:, :k])
+    p = np.ascontiguousarray(cws[:, k:])
+    st = be.rs256_repair_batch(ecc, m, p)
+    assert (st == fx["status_" + t]).all(), "status: " + _mismatch(st, fx["status_" + t])
+    rep = C.row_crc(np.concatenate([m, p], 1))
+    assert (rep == fx["repaired_crc_" + t]).all(), "repaired: " + _mismatch(rep, fx["repaired_crc_" + t])
+
+
+def check_rsi16_reftest(be, n, ecc):
+    """One shape of the reference's own RSi16 tests: parity in both words, status and
+    repaired symbols."""
+    fx = fixture("rsi16_reftest")
+    t = f"{n}_{ecc}"
+    k = n - ecc
+    msgs, counts, s = C.rsi16_reftest_case(n, ecc)
+    assert (C.sha(msgs, counts) == fx["in_sha_" + t]).all(), "input generator drift"
+    par = be.rsi16_encode_batch(n, ecc, msgs, True)
+    crc = C.row_crc(par)
+    assert (crc == fx["parity_crc_" + t]).all(), "parity: " + _mismatch(crc, fx["parity_crc_" + t])
+    crc = C.row_crc(be.rsi16_encode_batch(n, ecc, msgs, False))
+    assert (crc == fx["parity_scalar_crc_" + t]).all(), "scalar parity: " + _mismatch(
+        crc, fx["parity_scalar_crc_" + t])
+    cws = np.concatenate([msgs, par], 1)
+    W.rsi16_corrupt(cws, counts, s)
+    assert (C.sha(cws) == fx["corrupt_sha_" + t]).all()
+    m = np.ascontiguousarray(cws[:, :k])
+    p = np.ascontiguousarray(cws[:, k:])
+    st = be.rsi16_repair_batch(n, ecc, m, p)
+    assert (st == fx["status_" + t]).all(), "status: " + _mismatch(st, fx["status_" + t])
+    rep = C.row_crc(np.concatenate([m, p], 1))
+    ok = fx["status_" + t] != 255
+    assert (rep[ok] == fx["repaired_crc_" + t][ok]).all(), "repaired: " + _mismatch(
+        rep[ok], fx["repaired_crc_" + t][ok])
+
+
+def check_rsi16_divergence(be, n, ecc):
+    """Shapes whose mix holds 65536: SIMD-word and scalar-word parity, each as the reference
+    computes it (they differ on most of these messages)."""
+    fx = fixture("rsi16_simd_div")
+    t = f"{n}_{ecc}"
+    msgs = fx["msgs_" + t]
+    assert (msgs == C.rsi16_divergence_case(n, ecc)).all(), "input generator drift"
+    pv = be.rsi16_encode_batch(n, ecc, msgs, True)
+    assert (pv == fx["parity_" + t]).all(), "SIMD words: " + _mismatch((pv != fx["parity_" + t]).any(1), False)
+    ps = be.rsi16_encode_batch(n, ecc, msgs, False)
+    assert (ps == fx["parity_scalar_" + t]).all(), "scalar words: " + _mismatch(
+        (ps != fx["parity_scalar_" + t]).any(1), False)
+
+
+def check_rsi16_ilsimd(be, n, ecc, il, width):
+    """Interleaved repair with the SIMD words in coupled groups of `width` columns against the
+    reference: the call's status and both buffers after it (CRCs)."""
+    fx = fixture("rsi16_ilsimd")
+    tag = f"{n}_{ecc}_{il}_w{width}"
+    for t in range(C.RSI16_ILSIMD_TRIALS):
+        msg, s = C.rsi16_ilsimd_messages(n, ecc, il, width, t)
+        par = np.asarray(be.rsi16_encode_interleaved(n, ecc, il, msg[None], True), np.uint16).reshape(-1)
+        m, e = C.rsi16_ilsimd_corrupt(n, ecc, il, msg, par, s, t)
+        assert (C.sha(m, e) == fx["sha_" + tag][t]).all(), (tag, t, "input drift")
+        st = be.rsi16_repair_interleaved_simd(n, ecc, il, width, m, e)
+        assert st == fx["status_" + tag][t], (tag, t, st, fx["status_" + tag][t])
+        crc = [C.row_crc(m[None])[0], C.row_crc(e[None])[0]]
+        assert crc[0] == fx["crc_" + tag][t][0] and crc[1] == fx["crc_" + tag][t][1], (tag, t, "repaired symbols")
+    for t in range(C.RSI16_ILSIMD_KNOWN_TRIALS):
+        msg, s = C.rsi16_ilsimd_messages(n, ecc, il, width, 100 + t)
+        par = np.asarray(be.rsi16_encode_interleaved(n, ecc, il, msg[None], True), np.uint16).reshape(-1)
+        m, e, pos = C.rsi16_ilsimd_known(n, ecc, il, msg, par, s)
+        assert (C.sha(m, e, pos) == fx["ksha_" + tag][t]).all(), (tag, t, "input drift")
+        st = be.rsi16_repair_interleaved_simd(n, ecc, il, width, m, e, pos)
+        assert st == fx["kstatus_" + tag][t], (tag, "known", t, st, fx["kstatus_" + tag][t])
+        crc = [C.row_crc(m[None])[0], C.row_crc(e[None])[0]]
+        assert crc[0] == fx["kcrc_" + tag][t][0] and crc[1] == fx["kcrc_" + tag][t][1], (tag, "known", t)

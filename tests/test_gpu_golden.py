@@ -59,12 +59,23 @@ def test_rsi16_interleaved_repair(gpu):
     R.check_rsi16_interleaved_repair(gpu)
 
 
-def test_rsi16_interleaved_repair_simd_words_raises(gpu):
-    """With the SIMD words the reference repairs columns in coupled SIMD groups; not built."""
-    import numpy as np
-    ri = gpu.L.RSi16(64, 8, 3, simd_x4=True)
-    assert ri.message_len == 56 * 3 and ri.ecc_len == 8 * 3 and ri.interleave == 3
-    m = np.zeros(56 * 3, np.uint16)
-    e = np.zeros(8 * 3, np.uint16)
-    with pytest.raises(RuntimeError, match="interleave"):
-        ri.repair(m.view(np.uint8), e.view(np.uint8))
+@pytest.mark.parametrize("shape", C.RS256_REF_SHAPES, ids=lambda s: f"{s[0]}_{s[1]}")
+def test_rs256_reference_test_shapes(gpu_rs, shape):
+    R.check_rs256_reftest(gpu_rs, *shape)
+
+
+@pytest.mark.parametrize("shape", C.RSI16_REF_SHAPES, ids=lambda s: f"{s[0]}_{s[1]}")
+def test_rsi16_reference_test_shapes(gpu, shape):
+    R.check_rsi16_reftest(gpu, *shape)
+
+
+@pytest.mark.parametrize("shape", C.RSI16_DIV_SHAPES, ids=lambda s: f"{s[0]}_{s[1]}")
+def test_rsi16_simd_word_divergence(gpu, shape):
+    R.check_rsi16_divergence(gpu, *shape)
+
+
+@pytest.mark.parametrize("width", C.RSI16_ILSIMD_WIDTHS)
+@pytest.mark.parametrize("case", C.RSI16_ILSIMD_CASES, ids=lambda c: f"{c[0]}_{c[1]}_{c[2]}")
+def test_rsi16_interleaved_repair_simd_words(gpu, case, width):
+    """Coupled groups of 16, 8 or 4 columns, as the reference's vector instantiations repair them."""
+    R.check_rsi16_ilsimd(gpu, *case, width)

@@ -36,3 +36,19 @@ def test_rs256_shapes_as_plan(plan, shape):
 @pytest.mark.parametrize("name", C.RSI16_PLAIN_CASES)
 def test_rsi16_plain_as_plan(plan, name):
     R.check_rsi16_plain(plan, name)
+
+
+@pytest.mark.parametrize("shape", C.RS256_REF_SHAPES, ids=lambda s: f"{s[0]}_{s[1]}")
+def test_rs256_reference_test_shapes_as_plan(plan, shape):
+    R.check_rs256_reftest(plan, *shape)
+
+
+@pytest.mark.parametrize("shape", C.RSI16_REF_SHAPES, ids=lambda s: f"{s[0]}_{s[1]}")
+def test_rsi16_reference_test_shapes_as_plan(plan, shape):
+    """Includes block_len 32768 and 65536 and ecc_len 2048 (the LDS kernels) in plans."""
+    R.check_rsi16_reftest(plan, *shape)
+
+
+@pytest.mark.parametrize("shape", C.RSI16_DIV_SHAPES, ids=lambda s: f"{s[0]}_{s[1]}")
+def test_rsi16_simd_word_divergence_as_plan(plan, shape):
+    R.check_rsi16_divergence(plan, *shape)
