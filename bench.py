@@ -317,12 +317,14 @@ def run_plain(name, args):
     value = msg_bytes / (ms_step * 1e-3) / 1e9
     if rs256:
         enc_names = ["rs256_encode_kernel"]
-        # ecc 2, 4, 8, 16, 32 run BM over the product table, other ecc <= 32 from the log tables
-        rep_names = (["rs256_synd_kernel", "rs256_bm_mul_kernel", "rs256_chien16_kernel"] if ecc in (2, 4, 8, 16, 32)
+        # ecc 2, 4, 8, 16, 32 run BM inside the Chien kernel; other ecc <= 32 launch it on its own
+        rep_names = (["rs256_synd_kernel", "rs256_chien16_kernel"] if ecc in (2, 4, 8, 16, 32)
                      else ["rs256_synd_kernel", "rs256_bm_kernel", "rs256_chien16_kernel"] if ecc <= 32
                      else ["rs256_repair_kernel"])
     else:
-        enc_names = ["rsi16_encode_lane_kernel" if ecc <= 32 else
+        # the int8 MFMA contraction where it applies (these shapes have no mix constant of 65536)
+        enc_names = ["rsi16_encode_mfma_kernel" if ecc % 16 == 0 and ecc <= 256 and k <= 16384 else
+                     "rsi16_encode_lane_kernel" if ecc <= 32 else
                      ("rsi16_encode_reg_kernel" if ecc <= 256 else "rsi16_encode_kernel")]
         rep_names = ["rsi16_repair_reg_kernel" if ecc <= 256 else "rsi16_repair_kernel"]
     ke_ms = np.mean(np.array(kes), axis=0)

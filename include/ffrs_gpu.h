@@ -140,6 +140,13 @@ void ffrs_rsi16_destroy(ffrs_rsi16 *rs);
  * libffrs_amd.autotune picks them per (block_len, ecc_len) by timing. */
 int ffrs_rsi16_set_groups(ffrs_rsi16 *rs, unsigned g_encode, unsigned g_repair);
 int ffrs_rsi16_get_groups(const ffrs_rsi16 *rs, unsigned *g_encode, unsigned *g_repair);
+/* Encode kernels (no counterpart in the reference; results do not depend on them). The int8
+ * MFMA contraction parity = message . G mod p applies where it is the encode of the words asked
+ * for: interleave 1, ecc_len a multiple of 16 up to 256, message_len <= 16384, 16-byte aligned
+ * buffers, and with the SIMD words only where no mix constant of rsi16v_impl.hpp:68-86 is 65536.
+ * 0 automatic (the contraction where it applies and ecc_len <= 32, else the butterfly kernels),
+ * 1 always the butterfly kernels, 2 the contraction wherever it applies. */
+int ffrs_rsi16_set_encode_kernel(ffrs_rsi16 *rs, int kernel);
 uint32_t ffrs_rsi16_root(const ffrs_rsi16 *rs);
 size_t ffrs_rsi16_ntt_len(const ffrs_rsi16 *rs);
 

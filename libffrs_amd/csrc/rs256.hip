@@ -772,17 +772,16 @@ __global__ __launch_bounds__(256) void rs256_synds_kernel(const uint8_t *__restr
 
 // ======================================================================
 // Repair pipeline for ecc <= 32 (Lambda has at most 16 roots):
-//   rs256_synd_kernel   lane per codeword: remainder + XOR parity; zero -> status 1;
-//                       else syndromes from nibble tables ST[j][h][v] (E bytes each:
-//                       (v << 4h) alpha^(i (E-1-j)), i < E) and status PENDING
-//   rs256_bm_kernel     lane per codeword: Berlekamp-Massey; L > E/2 -> status 0; else
-//                       a record {L, Lambda_0..L, Omega_0..L-1} (Omega = S Lambda mod x^E
-//                       has degree < L once the LFSR of length L generates all E syndromes)
-//                       (ecc 2, 4, 8, 16, 32: the Chien kernel runs this BM itself per tile
-//                       and keeps the records in registers)
-//   rs256_chien_kernel  Chien search one codeword per wave (4 positions per lane, nibble
-//                       tables CT[i][h][v] of 256 position bytes addressed by wave-uniform
-//                       coefficients), then Forney one codeword per lane.
+//   rs256_synd_kernel    lane per codeword: remainder + XOR parity; zero -> status 1;
+//                        else syndromes from nibble tables ST[j][h][v] (E bytes each:
+//                        (v << 4h) alpha^(i (E-1-j)), i < E) and status PENDING
+//   rs256_bm_kernel      lane per codeword: Berlekamp-Massey; L > E/2 -> status 0; else
+//                        a record {L, Lambda_0..L, Omega_0..L-1} (Omega = S Lambda mod x^E
+//                        has degree < L once the LFSR of length L generates all E syndromes)
+//                        (ecc 2, 4, 8, 16, 32: the Chien kernel runs this BM itself per tile
+//                        and keeps the records in registers)
+//   rs256_chien16_kernel Chien search, sixteen lanes per codeword over nibble tables, then
+//                        Forney at the roots.
 // ======================================================================
 
 constexpr uint8_t RS_PENDING = 0xFE;
@@ -974,31 +973,14 @@ __global__ __launch_bounds__(WG) void rs256_bm_kernel(uint32_t count, uint32_t E
     }
 }
 
-// Log / exp tables for the register BM, two layouts:
-//  LogExp1K  EXP2 (1,024 bytes, zero above 509) and LOG2 (u16, log 0 = ZLOG = 511): a product
-//            of logs is one read at the sum; the reads land in 256 dwords of the LDS (up to
-//            four per bank), so lanes collide on banks.
-//  LogExp256 EXP (256 bytes, EXP[255] = 0) and LOG (u16, log 0 = 767): the index of a sum s is
-//            min(s, s - 255, 255), which reduces s mod 255 and sends any sum with a zero operand
-//            (>= 767) to EXP[255]; the 256 bytes are one dword per bank, so lanes never collide.
+// Log / exp tables of the register BM: EXP2 (1,024 bytes, zero above 509) and LOG2 (u16,
+// log 0 = ZLOG = 511), so that a product of logs is one read at the sum.
 struct LogExp1K {
     const uint8_t *ex;
     const uint16_t *lg;
     static constexpr uint32_t Z = ZLOG;
     __device__ inline uint32_t log(uint32_t a) const { return lg[a]; }
     __device__ inline uint32_t exp_sum(uint32_t s) const { return ex[s]; }
-};
-struct LogExp256 {
-    const uint8_t *ex;
-    const uint16_t *lg;
-    static constexpr uint32_t Z = 767;
-    __device__ inline uint32_t log(uint32_t a) const { return lg[a]; }
-    __device__ inline uint32_t exp_sum(uint32_t s) const
-    {
-        const uint32_t t = s - 255u;
-        const uint32_t m = s < t ? s : t;
-        return ex[m < 255u ? m : 255u];  // one v_min3_u32
-    }
 };
 
 // Berlekamp-Massey with every polynomial in registers, fully unrolled for a fixed E.
@@ -1105,369 +1087,6 @@ __device__ inline void xor_byte(uint8_t *p, uint32_t v)
     __hip_atomic_fetch_xor(w, v << (8 * (a & 3)), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Chien search one codeword per wave, then Forney one codeword per lane
-// (reed_solomon.hpp:516-538, 704-743).
-// Chien: lane l holds positions 4l..4l+3 as the bytes of one dword; with the nibble
-// tables CT[i][h][v] (256 position bytes (v << 4h) alpha^(-i p)) a wave evaluates
-// Lambda_even(X^-1) and Lambda_odd(X^-1), X = alpha^p, for all positions at once. Every
-// table address is wave-uniform, so the LDS reads are conflict-free. When the roots
-// number exactly L, each root is stored as (p, Lambda_odd(X^-1)) in the wave's LDS slots.
-// Forney: lane o takes codeword o of the tile. At a root, Lambda'(X^-1) = X Lambda_odd(X^-1)
-// (characteristic 2), so the reference's e = X div(Omega(X^-1), Lambda'(X^-1)) equals
-// div(Omega(X^-1), Lambda_odd(X^-1)), or X Omega(X^-1) when Lambda_odd = 0 (its
-// div(a, 0) = a). Omega(X^-1) = sum_k Omega_k alpha^(-p k) is summed from independent
-// log-domain terms.
-// EB > 0: the kernel first runs the degree-bounded BM (bm_fixed_one<EB>) of each pending codeword
-// of a tile, one codeword per lane, from its syndromes `synd`, and keeps the record in registers
-// (the lookup-bound BM and the issue-bound Chien loop of other waves then share each CU);
-// EB == 0: the records come from `rec`.
-template <int WG, int EB>
-__global__ __launch_bounds__(WG) void rs256_chien_kernel(uint8_t *__restrict__ msgs, uint8_t *__restrict__ eccs,
-                                                          uint32_t count, uint32_t k, uint32_t E,
-                                                          uint8_t *__restrict__ status, const uint8_t *__restrict__ rec,
-                                                          const uint32_t *__restrict__ synd,
-                                                          const uint32_t *__restrict__ ctab, Rs256Tables tab)
-{
-    // [CHIEN_LMAX][2][16][64] dwords at a known LDS address, so that the table offsets fold
-    // into the read instructions
-    __shared__ uint32_t s_ct[CHIEN_TABLE_BYTES / 4];
-    extern __shared__ uint32_t s_mem[];
-    uint16_t *s_roots = reinterpret_cast<uint16_t *>(s_mem);  // [WG/64][64][16]
-    uint8_t *s_exp2 = reinterpret_cast<uint8_t *>(s_roots + WG * CHIEN_LMAX);
-    uint16_t *s_log2 = reinterpret_cast<uint16_t *>(s_exp2 + EXP2_LEN);
-    for (uint32_t i = threadIdx.x; i < CHIEN_TABLE_BYTES / 16; i += blockDim.x)
-        reinterpret_cast<uint4 *>(s_ct)[i] = reinterpret_cast<const uint4 *>(ctab)[i];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += blockDim.x)
-        s_exp2[i] = tab.exp2[i];
-    for (uint32_t i = threadIdx.x; i < 256u; i += blockDim.x)
-        s_log2[i] = tab.log2[i];
-    __syncthreads();
-    const uint32_t lane = threadIdx.x & 63;
-    // byte offsets of this lane's column in the first and the second 64 KB of the table
-    const uint32_t lb_lo = lane * 4, lb_hi = lane * 4 + 0x10000u;
-    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    uint16_t *my_roots = s_roots + wave * 64 * CHIEN_LMAX;
-    const uint32_t n = k + E;
-    const uint32_t *rec32 = reinterpret_cast<const uint32_t *>(rec);
-    const uint32_t RSW = rec_stride(E) / 4;
-    bool in_cw[4];      // position 4 lane + q lies in the codeword
-    uint64_t valid[4];  // the same as lane masks
-    uint32_t pos[4];    // that position
-#pragma unroll
-    for (int q = 0; q < 4; ++q) {
-        pos[q] = lane * 4 + q;
-        in_cw[q] = pos[q] < n;
-        valid[q] = __builtin_amdgcn_ballot_w64(in_cw[q]);
-    }
-    for (uint32_t base = blockIdx.x * WG + wave * 64; base < count; base += gridDim.x * WG) {
-        uint32_t cw = base + lane;
-        bool pend = cw < count && status[cw] == RS_PENDING;
-        // the tile's records, one per lane; per codeword read back by readlane
-        uint32_t recw[10];
-        if constexpr (EB > 0) {
-            uint32_t words[12];
-#pragma unroll
-            for (int w = 0; w < 12; ++w)
-                words[w] = 0;
-            if (pend && !bm_fixed_one<EB>(synd + (size_t)cw * ((EB + 3) / 4), LogExp1K{s_exp2, s_log2}, words)) {
-                status[cw] = 0;
-                pend = false;
-            }
-#pragma unroll
-            for (int w = 0; w < 10; ++w)
-                recw[w] = words[w];
-        }
-        uint64_t pend_mask = __ballot(pend);
-        if (!pend_mask)
-            continue;
-        if constexpr (EB == 0) {
-            // coalesced record loads
-            const uint4 *r4 = reinterpret_cast<const uint4 *>(rec32 + (size_t)(cw < count ? cw : 0) * RSW);
-            uint4 a = r4[0], b = r4[1], c = r4[2];
-            recw[0] = a.x; recw[1] = a.y; recw[2] = a.z; recw[3] = a.w;
-            recw[4] = b.x; recw[5] = b.y; recw[6] = b.z; recw[7] = b.w;
-            recw[8] = c.x; recw[9] = c.y;
-        }
-        // Lambda_1..Lambda_16 of this lane's codeword as nibble bytes: dword m holds the low
-        // and high nibbles of Lambda_(2m+1), then of Lambda_(2m+2) (each the row of its table)
-        uint32_t nib[8];
-#pragma unroll
-        for (int g = 0; g < 4; ++g) {
-            const uint32_t q = alignbyte(recw[6 + g], recw[5 + g], 1);  // Lambda_(4g+1..4g+4)
-#pragma unroll
-            for (int hw = 0; hw < 2; ++hw) {
-                const uint32_t t = (q >> (16 * hw)) & 0xffffu;
-                const uint32_t b = (t & 0xffu) | ((t & 0xff00u) << 8);   // bytes a, 0, b, 0
-                nib[2 * g + hw] = (b & 0x000f000fu) | ((b & 0x00f000f0u) << 4);
-            }
-        }
-        uint64_t ok_mask = 0;
-        // Count the roots of codeword o; when they number exactly L, store them as
-        // (p, Lambda_odd(X^-1)) in the codeword's slots.
-        auto settle = [&](uint32_t o, uint32_t Lo, uint32_t v_even, uint32_t v_odd) {
-            const uint32_t val = v_even ^ v_odd;
-            bool root[4];  // this lane's own predicate, which is also the ballot's exec mask
-            uint64_t zb[4];
-            uint32_t cnt = 0;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                const bool z = (val & (0xffu << (8 * q))) == 0;
-                zb[q] = __builtin_amdgcn_ballot_w64(z) & valid[q];
-                root[q] = z && in_cw[q];
-                cnt += (uint32_t)__builtin_popcountll(zb[q]);
-            }
-            if (cnt != Lo)
-                return;
-            ok_mask |= 1ull << o;
-            uint32_t slot = o * CHIEN_LMAX;
-#pragma unroll
-            for (int q = 0; q < 4; ++q) {
-                if (root[q]) {
-                    const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(zb[q] >> 32),
-                                                                     __builtin_amdgcn_mbcnt_lo((uint32_t)zb[q], 0u));
-                    // position in byte 0, Lambda_odd(X^-1) (byte q of v_odd) in byte 1
-                    my_roots[slot + below] = (uint16_t)__builtin_amdgcn_perm(v_odd, pos[q], 0x0c0c0400u + (uint32_t)q * 0x100u);
-                }
-                slot += (uint32_t)__builtin_popcountll(zb[q]);
-            }
-        };
-        while (pend_mask) {
-            const uint32_t o = (uint32_t)__builtin_ctzll(pend_mask);
-            pend_mask &= pend_mask - 1;
-            const uint32_t Lo = __builtin_amdgcn_readlane(recw[0], o) & 0xffu;
-            // Lambda_i = 0 for L < i <= 16 (records are zero-filled) and row v = 0 of every
-            // table is zero, so terms past L add nothing; whole groups of four past L are skipped.
-            // Row addresses: one byte permute puts the nibble (wave-uniform) in byte 1 over this
-            // lane's column offset; the table and nibble parts are immediate offsets.
-            uint32_t v_even = 0x01010101u, v_odd = 0;
-            const char *ct = reinterpret_cast<const char *>(s_ct);
-#pragma unroll
-            for (int g = 0; g < 4; ++g) {
-                if (4 * g + 1 > (int)Lo)
-                    break;
-                const uint32_t nw[2] = {(uint32_t)__builtin_amdgcn_readlane(nib[2 * g], o),
-                                        (uint32_t)__builtin_amdgcn_readlane(nib[2 * g + 1], o)};
-#pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int i = 4 * g + j + 1;
-                    uint32_t t[2];
-#pragma unroll
-                    for (int h = 0; h < 2; ++h) {
-                        const int tbl = (i - 1) * 2 + h;                 // 4 KB table index
-                        const uint32_t sel = 0x0c020000u | ((4u + (uint32_t)(2 * (j & 1) + h)) << 8);
-                        const uint32_t a = __builtin_amdgcn_perm(nw[j >> 1], tbl >= 16 ? lb_hi : lb_lo, sel);
-                        t[h] = *reinterpret_cast<const uint32_t *>(ct + a + (tbl & 15) * 4096);
-                    }
-                    if (i & 1)
-                        v_odd = xor3(v_odd, t[0], t[1]);
-                    else
-                        v_even = xor3(v_even, t[0], t[1]);
-                }
-            }
-            settle(o, Lo, v_even, v_odd);
-        }
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        if (!pend)
-            continue;
-        const bool ok = (ok_mask >> lane) & 1ull;
-        status[cw] = ok ? 1 : 0;
-        if (!ok)
-            continue;
-        const uint32_t L = recw[0] & 0xffu;
-        uint32_t lom[CHIEN_LMAX];
-#pragma unroll
-        for (int w = 0; w < 4; ++w)
-#pragma unroll
-            for (int b = 0; b < 4; ++b)
-                lom[4 * w + b] = s_log2[(recw[1 + w] >> (8 * b)) & 0xffu];
-        uint8_t *mrow = msgs + (size_t)cw * k;
-        uint8_t *erow = eccs + (size_t)cw * E;
-        for (uint32_t j = 0; j < L; ++j) {
-            const uint32_t r = my_roots[lane * CHIEN_LMAX + j];
-            const uint32_t p = r & 0xffu, dod = r >> 8;
-            const uint32_t lxi = p ? 255 - p : 0;
-            uint32_t num = 0, e = 0;
-#pragma unroll
-            for (int kk = 0; kk < CHIEN_LMAX; ++kk) {
-                num ^= s_exp2[lom[kk] + e];
-                e += lxi;
-                e = e >= 255 ? e - 255 : e;
-            }
-            const uint32_t ln = s_log2[num], ld = s_log2[dod];
-            const uint32_t mag = num == 0 ? 0u : (dod ? (uint32_t)s_exp2[ln + 255 - ld] : (uint32_t)s_exp2[p + ln]);
-            const uint32_t at = n - 1 - p;
-            xor_byte(at < k ? mrow + at : erow + (at - k), mag);
-        }
-    }
-}
-
-
-// ======================================================================
-// Berlekamp-Massey over a product table (ecc 2, 4, 8, 16, 32), lane per codeword:
-// MUL[a][b] = a b (64 KB, a-major) and INV[a] (inv(0) = 1) in LDS, every polynomial in
-// registers and fully unrolled. A product is one table read at (a << 8) | b, so no
-// coefficient returns to the log domain after an update. Same iteration as bm_fixed_one<E>
-// (reed_solomon.hpp:655-702: degree-bounded, B kept as x^m B, L > E/2 fails) and the same
-// 48-byte record: L in byte 0, Omega_0..Omega_(E/2-1) from byte 4, Lambda_0..Lambda_(E/2)
-// from byte 20, zero-filled.
-// ======================================================================
-
-constexpr uint32_t MULT_BYTES = 65536 + 256;
-
-// Offset of MUL[a][b] from the low bytes of a and b (one byte permute; higher bits ignored).
-__device__ inline uint32_t mul_at(uint32_t a, uint32_t b) { return __builtin_amdgcn_perm(a, b, 0x0c0c0400u); }
-// (a ^ b) & 0xff in one instruction: written as plain XORs of table bytes, the compiler narrows
-// them to 16-bit operations and masks every result again before it goes into an address.
-__device__ inline uint32_t xr(uint32_t a, uint32_t b) { return __builtin_amdgcn_bitop3_b32(a, b, 0xffu, 0x28); }
-
-template <int E>
-__device__ inline bool bm_mul_one(const uint32_t *__restrict__ sw, const uint8_t *mul, const uint8_t *inv,
-                                  uint32_t (&words)[12])
-{
-    constexpr int EW = (E + 3) / 4;
-    constexpr int H = E / 2;
-    uint32_t S[E];
-#pragma unroll
-    for (int w = 0; w < EW; ++w) {
-        const uint32_t v = sw[w];
-#pragma unroll
-        for (int b = 0; b < 4; ++b)
-            if (w * 4 + b < E)
-                S[w * 4 + b] = (v >> (8 * b)) & 0xffu;
-    }
-    // While L <= E/2, deg Lambda <= L and every x^m B that enters an update has degree <= the
-    // new L, so coefficients above E/2 never reach a successful result.
-    uint32_t lam[H + 1], bp[H + 1];  // Lambda_0..Lambda_H; B' = x^m B, coefficients 0..H
-#pragma unroll
-    for (int i = 0; i <= H; ++i) {
-        lam[i] = i == 0 ? 1u : 0u;
-        bp[i] = i == 1 ? 1u : 0u;
-    }
-    uint32_t L = 0, binv = 1;  // b = 1
-#pragma unroll
-    for (int n = 0; n < E; ++n) {
-        uint32_t d = S[n];
-#pragma unroll
-        for (int i = 1; i <= (n < H ? n : H); ++i)
-            d = xr(d, mul[mul_at(lam[i], S[n - i])]);
-        const bool grow = d != 0 && 2 * L <= (uint32_t)n;
-        // c = d / b; d == 0 gives c = 0 and leaves Lambda as it is
-        const uint32_t c = mul[mul_at(d, binv)];
-        uint32_t old[H + 1];
-#pragma unroll
-        for (int i = 0; i <= H; ++i)
-            old[i] = lam[i];
-        const int hi = n + 1 < H ? n + 1 : H;
-#pragma unroll
-        for (int i = 1; i <= hi; ++i)
-            lam[i] = xr(lam[i], mul[mul_at(c, bp[i])]);
-        const int bh = n + 2 < H ? n + 2 : H;
-#pragma unroll
-        for (int i = bh; i >= 1; --i)
-            bp[i] = grow ? old[i - 1] : bp[i - 1];
-        bp[0] = 0;
-        if (grow) {
-            L = (uint32_t)n + 1 - L;
-            binv = inv[d & 0xffu];
-        }
-    }
-    if (L > (uint32_t)H)
-        return false;
-#pragma unroll
-    for (int w = 0; w < 12; ++w)
-        words[w] = 0;
-    words[0] = L;
-    const uint32_t LO = rec_lam_off(), OO = rec_om_off(E);
-#pragma unroll
-    for (int i = 0; i <= H; ++i)
-        words[(LO + i) >> 2] |= lam[i] << (8 * ((LO + i) & 3));
-    // Omega_k = sum_{i <= k} Lambda_i S_(k-i), k < L (reed_solomon.hpp:708-728)
-#pragma unroll
-    for (int kk = 0; kk < H; ++kk) {
-        uint32_t v = S[kk];
-#pragma unroll
-        for (int i = 1; i <= kk; ++i)
-            v = xr(v, mul[mul_at(lam[i], S[kk - i])]);
-        v = (uint32_t)kk < L ? v : 0u;
-        words[(OO + kk) >> 2] |= v << (8 * ((OO + kk) & 3));
-    }
-    return true;
-}
-
-template <int E, int WG>
-__global__ __launch_bounds__(WG) void rs256_bm_mul_kernel(uint32_t count, uint8_t *__restrict__ status,
-                                                           const uint32_t *__restrict__ synd,
-                                                           uint32_t *__restrict__ rec,
-                                                           const uint32_t *__restrict__ mult)
-{
-    __shared__ __align__(16) uint8_t s_mul[MULT_BYTES];
-    for (uint32_t i = threadIdx.x; i < MULT_BYTES / 16; i += WG)
-        reinterpret_cast<uint4 *>(s_mul)[i] = reinterpret_cast<const uint4 *>(mult)[i];
-    __syncthreads();
-    const uint8_t *mul = s_mul;
-    const uint8_t *inv = s_mul + 65536;
-    constexpr int EW = (E + 3) / 4;
-    for (uint32_t cw = blockIdx.x * WG + threadIdx.x; cw < count; cw += gridDim.x * WG) {
-        if (status[cw] != RS_PENDING)
-            continue;
-        uint32_t words[12];
-        if (!bm_mul_one<E>(synd + (size_t)cw * EW, mul, inv, words)) {
-            status[cw] = 0;
-            continue;
-        }
-        uint4 *r = reinterpret_cast<uint4 *>(rec + (size_t)cw * 12);
-        r[0] = make_uint4(words[0], words[1], words[2], words[3]);
-        r[1] = make_uint4(words[4], words[5], words[6], words[7]);
-        r[2] = make_uint4(words[8], words[9], words[10], words[11]);
-    }
-}
-
-
-// bm_fixed_one<E> (log / exp tables in LDS) as a kernel of its own: lane per pending codeword,
-// the 48-byte record to `rec`, status 0 where L > E/2.
-// LDS copies of the LogExp256 tables from the device's EXP2 / LOG2.
-__device__ inline void stage_logexp256(uint8_t *ex, uint16_t *lg, const Rs256Tables &tab, uint32_t tid, uint32_t nt)
-{
-    for (uint32_t i = tid; i < 256u; i += nt) {
-        ex[i] = i < 255 ? tab.exp2[i] : 0;
-        lg[i] = i ? tab.log2[i] : (uint16_t)LogExp256::Z;
-    }
-}
-
-template <int E, int WG, bool CF>
-__global__ __launch_bounds__(WG) void rs256_bm_log_kernel(uint32_t count, uint8_t *__restrict__ status,
-                                                           const uint32_t *__restrict__ synd,
-                                                           uint32_t *__restrict__ rec, Rs256Tables tab)
-{
-    __shared__ uint8_t s_exp2[EXP2_LEN];
-    __shared__ uint16_t s_log2[256];
-    __shared__ uint8_t s_ex256[256];
-    __shared__ uint16_t s_lg256[256];
-    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += WG)
-        s_exp2[i] = tab.exp2[i];
-    for (uint32_t i = threadIdx.x; i < 256u; i += WG)
-        s_log2[i] = tab.log2[i];
-    stage_logexp256(s_ex256, s_lg256, tab, threadIdx.x, WG);
-    __syncthreads();
-    using TB = typename std::conditional<CF, LogExp256, LogExp1K>::type;
-    const TB tb{CF ? s_ex256 : s_exp2, CF ? s_lg256 : s_log2};
-    for (uint32_t cw = blockIdx.x * WG + threadIdx.x; cw < count; cw += gridDim.x * WG) {
-        if (status[cw] != RS_PENDING)
-            continue;
-        uint32_t words[12];
-        if (!bm_fixed_one<E>(synd + (size_t)cw * ((E + 3) / 4), tb, words)) {
-            status[cw] = 0;
-            continue;
-        }
-        uint4 *r = reinterpret_cast<uint4 *>(rec + (size_t)cw * 12);
-        r[0] = make_uint4(words[0], words[1], words[2], words[3]);
-        r[1] = make_uint4(words[4], words[5], words[6], words[7]);
-        r[2] = make_uint4(words[8], words[9], words[10], words[11]);
-    }
-}
-
 // ======================================================================
 // Chien search and Forney, sixteen lanes per codeword (records of the BM kernels, L <= 16;
 // reed_solomon.hpp:516-538, 704-743). Lane j of a group holds positions 16j .. 16j+15 of its
@@ -1547,7 +1166,7 @@ __device__ inline uint32_t row16_sum(uint32_t x)
 // EB > 0: the BM records come from bm_fixed_one<EB> run per tile, one pending codeword per lane,
 // on the syndromes `synd` (its LDS lookups overlap the Chien loops of other waves); EB == 0:
 // from `rec`.
-template <int WG, int EB, bool CF = false>
+template <int WG, int EB>
 __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__ msgs, uint8_t *__restrict__ eccs,
                                                             uint32_t count, uint32_t k, uint32_t E,
                                                             uint8_t *__restrict__ status,
@@ -1560,19 +1179,14 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
     __shared__ uint4 s_ct4[CHIEN_TABLE_BYTES / 16];
     __shared__ uint8_t s_exp2[EXP2_LEN];
     __shared__ uint16_t s_log2[256];
-    __shared__ uint8_t s_ex256[CF ? 256 : 4];
-    __shared__ uint16_t s_lg256[CF ? 256 : 2];
     for (uint32_t i = threadIdx.x; i < CHIEN_TABLE_BYTES / 16; i += WG)
         s_ct4[i] = reinterpret_cast<const uint4 *>(ctab)[i];
     for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += WG)
         s_exp2[i] = tab.exp2[i];
     for (uint32_t i = threadIdx.x; i < 256u; i += WG)
         s_log2[i] = tab.log2[i];
-    if constexpr (CF)
-        stage_logexp256(s_ex256, s_lg256, tab, threadIdx.x, WG);
     __syncthreads();
-    using TB = typename std::conditional<CF, LogExp256, LogExp1K>::type;
-    const TB tb{CF ? s_ex256 : s_exp2, CF ? s_lg256 : s_log2};
+    const LogExp1K tb{s_exp2, s_log2};
     const char *ct = reinterpret_cast<const char *>(s_ct4);
     const uint32_t lane = threadIdx.x & 63, j = lane & 15, grp = lane >> 4;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -1838,7 +1452,6 @@ struct ffrs_rs256 {
     uint16_t *d_log2 = nullptr;
     uint32_t *d_stab = nullptr;  // syndrome nibble tables (ecc <= 32)
     uint32_t *d_ctab = nullptr;  // Chien nibble tables (ecc <= 32)
-    uint8_t *d_mult = nullptr;   // product table MUL[a][b] then INV[a] (ecc <= 32)
     void *d_work = nullptr;      // syndromes + BM records of the last repair
     size_t work_size = 0;
     int repair_mode = 0;         // 0 automatic, 1 pipeline (ecc <= 32), 2 one kernel, lane per codeword
@@ -2029,12 +1642,18 @@ int reserve_work(ffrs_rs256 *rs, size_t bytes)
     return FFRS_OK;
 }
 
+// Whether the Chien kernel runs Berlekamp-Massey itself (ecc 2, 4, 8, 16, 32), so that the
+// pipeline keeps no BM records.
+bool bm_in_chien(uint32_t E) { return E == 2 || E == 4 || E == 8 || E == 16 || E == 32; }
+
 // Workspace of the repair pipeline for `count` codewords of ecc E (word class EW): the
-// syndromes (EW dwords per codeword), then the 48-byte BM records from the next 256-byte
-// boundary.
+// syndromes (EW dwords per codeword), then, for ecc outside bm_in_chien, the 48-byte BM
+// records from the next 256-byte boundary.
 size_t pipeline_work_bytes(uint32_t E, uint32_t EW, size_t count)
 {
     const size_t synd_bytes = count * EW * 4;
+    if (bm_in_chien(E))
+        return synd_bytes;
     return ((synd_bytes + 255) & ~(size_t)255) + count * rec_stride(E);
 }
 
@@ -2063,103 +1682,34 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
 
-    if (rs->repair_mode == 3) {
-        // BM inside the wave-per-codeword Chien kernel: 15 waves (the tables and root slots just
-        // fit in 160 KB); 11 with the ecc-32 BM inside, whose registers (145) exceed the 128
-        // that four waves per SIMD leave
-        constexpr int WG3 = 960, WG3B = 704;
-        auto chien = [&](auto kern, int wg) -> int {
-            size_t lds3 = (size_t)wg * CHIEN_LMAX * 2 + EXP2_LEN + 512;  // plus the static table
-            int g3 = persistent_grid(ctx, 1, (count + wg - 1) / wg);
-            if (int r = allow_lds(kern, lds3))
-                return r;
-            hipLaunchKernelGGL(kern, dim3(g3), dim3(wg), lds3, ctx->stream, d_msg, d_ecc, (uint32_t)count, k, E,
-                               d_status, (const uint8_t *)d_rec, (const uint32_t *)d_synd, rs->d_ctab, tab);
-            FFRS_HIP(hipGetLastError());
-            mark_kernel(ctx);
-            return FFRS_OK;
-        };
-        switch (E) {
-        case 2: return chien(rs256_chien_kernel<WG3, 2>, WG3);
-        case 4: return chien(rs256_chien_kernel<WG3, 4>, WG3);
-        case 8: return chien(rs256_chien_kernel<WG3, 8>, WG3);
-        case 16: return chien(rs256_chien_kernel<WG3, 16>, WG3);
-        case 32: return chien(rs256_chien_kernel<WG3B, 32>, WG3B);
-        default: break;
-        }
-    }
-
-    // ecc 2, 4, 8, 16, 32: BM inside the sixteen-lane Chien kernel
+    // ecc 2, 4, 8, 16, 32: BM inside the sixteen-lane Chien kernel, no records (512 threads
+    // for ecc 32, whose BM would spill registers at 768)
     auto fused = [&](auto kern, int wg) -> int {
         int g = persistent_grid(ctx, 1, (count + wg - 1) / wg);
         hipLaunchKernelGGL(kern, dim3(g), dim3(wg), 0, ctx->stream, d_msg, d_ecc, (uint32_t)count, k, E, d_status,
-                           (const uint32_t *)d_rec, (const uint32_t *)d_synd, rs->d_ctab, tab);
+                           (const uint32_t *)nullptr, (const uint32_t *)d_synd, rs->d_ctab, tab);
         FFRS_HIP(hipGetLastError());
         mark_kernel(ctx);
         return FFRS_OK;
     };
-    if (rs->repair_mode <= 1 && E == 32)
-        return fused(rs256_chien16_kernel<768, 32>, 768);
-    if (rs->repair_mode == 5 && E == 32)
-        return fused(rs256_chien16_kernel<512, 32>, 512);
-    if (rs->repair_mode == 8 && E == 32)
-        return fused(rs256_chien16_kernel<512, 32, true>, 512);
-    if (rs->repair_mode <= 1 || rs->repair_mode == 5) {
-        switch (E) {
-        case 2: return fused(rs256_chien16_kernel<1024, 2>, 1024);
-        case 4: return fused(rs256_chien16_kernel<1024, 4>, 1024);
-        case 8: return fused(rs256_chien16_kernel<1024, 8>, 1024);
-        case 16: return fused(rs256_chien16_kernel<1024, 16>, 1024);
-        default: break;
-        }
-    }
-    if ((rs->repair_mode == 6 || rs->repair_mode == 7) && E == 32) {
-        int g = persistent_grid(ctx, 3, (count + 255) / 256);
-        auto *bk = rs->repair_mode == 7 ? rs256_bm_log_kernel<32, 256, true> : rs256_bm_log_kernel<32, 256, false>;
-        hipLaunchKernelGGL(bk, dim3(g), dim3(256), 0, ctx->stream, (uint32_t)count,
-                           d_status, (const uint32_t *)d_synd, (uint32_t *)d_rec, tab);
-        FFRS_HIP(hipGetLastError());
-        mark_kernel(ctx);
-        constexpr int WGC = 1024;
-        int gc = persistent_grid(ctx, 1, (count + WGC - 1) / WGC);
-        hipLaunchKernelGGL((rs256_chien16_kernel<WGC, 0>), dim3(gc), dim3(WGC), 0, ctx->stream, d_msg, d_ecc,
-                           (uint32_t)count, k, E, d_status, (const uint32_t *)d_rec, (const uint32_t *)d_synd,
-                           rs->d_ctab, tab);
-        FFRS_HIP(hipGetLastError());
-        mark_kernel(ctx);
-        return FFRS_OK;
-    }
-    // Berlekamp-Massey: over the product table for ecc 2, 4, 8, 16, 32, else from the log tables
-    constexpr int WGB = 512;
-    auto bm_mul = [&](auto kern) -> int {
-        int g = persistent_grid(ctx, 2, (count + WGB - 1) / WGB);  // 64 KB of LDS each: two per CU
-        hipLaunchKernelGGL(kern, dim3(g), dim3(WGB), 0, ctx->stream, (uint32_t)count, d_status,
-                           (const uint32_t *)d_synd, (uint32_t *)d_rec, (const uint32_t *)rs->d_mult);
-        FFRS_HIP(hipGetLastError());
-        mark_kernel(ctx);
-        return FFRS_OK;
-    };
-    int rb;
     switch (E) {
-    case 2: rb = bm_mul(rs256_bm_mul_kernel<2, WGB>); break;
-    case 4: rb = bm_mul(rs256_bm_mul_kernel<4, WGB>); break;
-    case 8: rb = bm_mul(rs256_bm_mul_kernel<8, WGB>); break;
-    case 16: rb = bm_mul(rs256_bm_mul_kernel<16, WGB>); break;
-    case 32: rb = bm_mul(rs256_bm_mul_kernel<32, WGB>); break;
-    default: {
-        size_t lds2 = EXP2_LEN + 512 + (size_t)256 * bm_row_stride(E);
-        if (int r = allow_lds(rs256_bm_kernel<256>, lds2))
-            return r;
-        int g2 = persistent_grid(ctx, (int)std::max<size_t>(1, (160 * 1024) / lds2), (count + 255) / 256);
-        hipLaunchKernelGGL((rs256_bm_kernel<256>), dim3(g2), dim3(256), lds2, ctx->stream, (uint32_t)count, E,
-                           (uint32_t)EW, d_status, (const uint32_t *)d_synd, d_rec, tab);
-        FFRS_HIP(hipGetLastError());
-        mark_kernel(ctx);
-        rb = FFRS_OK;
+    case 2: return fused(rs256_chien16_kernel<1024, 2>, 1024);
+    case 4: return fused(rs256_chien16_kernel<1024, 4>, 1024);
+    case 8: return fused(rs256_chien16_kernel<1024, 8>, 1024);
+    case 16: return fused(rs256_chien16_kernel<1024, 16>, 1024);
+    case 32: return fused(rs256_chien16_kernel<512, 32>, 512);
+    default: break;
     }
-    }
-    if (rb)
-        return rb;
+    // other ecc: Berlekamp-Massey from the log tables, one lane per codeword, records in the
+    // workspace
+    size_t lds2 = EXP2_LEN + 512 + (size_t)256 * bm_row_stride(E);
+    if (int r = allow_lds(rs256_bm_kernel<256>, lds2))
+        return r;
+    int g2 = persistent_grid(ctx, (int)std::max<size_t>(1, (160 * 1024) / lds2), (count + 255) / 256);
+    hipLaunchKernelGGL((rs256_bm_kernel<256>), dim3(g2), dim3(256), lds2, ctx->stream, (uint32_t)count, E,
+                       (uint32_t)EW, d_status, (const uint32_t *)d_synd, d_rec, tab);
+    FFRS_HIP(hipGetLastError());
+    mark_kernel(ctx);
     // Chien + Forney: the 128 KB table fills the CU's LDS, one workgroup of sixteen waves per CU
     constexpr int WGC = 1024;
     int gc = persistent_grid(ctx, 1, (count + WGC - 1) / WGC);
@@ -2330,22 +1880,9 @@ int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsig
     if (e == hipSuccess && rs->EW <= 8) {
         std::vector<uint32_t> st = build_synd_tables(*rs);
         std::vector<uint32_t> ct = build_chien_tables(*rs);
-        std::vector<uint8_t> mt(MULT_BYTES);
-        for (unsigned a = 0; a < 256; ++a)
-            for (unsigned b = 0; b < 256; ++b)
-                mt[a * 256 + b] = rs->gf.mul((uint8_t)a, (uint8_t)b);
-        mt[65536] = 1;  // inv(0) = 1 (galois.hpp:98-114)
-        for (unsigned a = 1; a < 256; ++a)
-            for (unsigned b = 1; b < 256; ++b)
-                if (mt[a * 256 + b] == 1)
-                    mt[65536 + a] = (uint8_t)b;
         e = hipMalloc(&rs->d_stab, st.size() * 4);
         if (e == hipSuccess)
             e = hipMalloc(&rs->d_ctab, ct.size() * 4);
-        if (e == hipSuccess)
-            e = hipMalloc(&rs->d_mult, mt.size());
-        if (e == hipSuccess)
-            e = hipMemcpy(rs->d_mult, mt.data(), mt.size(), hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(rs->d_stab, st.data(), st.size() * 4, hipMemcpyHostToDevice);
         if (e == hipSuccess)
@@ -2376,8 +1913,6 @@ void ffrs_rs256_destroy(ffrs_rs256 *rs)
         (void)hipFree(rs->d_stab);
     if (rs->d_ctab)
         (void)hipFree(rs->d_ctab);
-    if (rs->d_mult)
-        (void)hipFree(rs->d_mult);
     if (rs->d_work)
         (void)hipFree(rs->d_work);
     delete rs;
@@ -2387,7 +1922,7 @@ int ffrs_rs256_set_repair_mode(ffrs_rs256 *rs, int mode)
 {
     if (!rs)
         return fail(FFRS_ERR_INVALID, "codec is NULL");
-    if (mode < 0 || mode > 8)
+    if (mode < 0 || mode > 2)
         return fail(FFRS_ERR_INVALID, "repair mode %d (0 automatic, 1 pipeline, 2 lane per codeword)", mode);
     rs->repair_mode = mode;
     return FFRS_OK;

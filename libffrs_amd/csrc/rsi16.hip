@@ -1795,6 +1795,104 @@ __global__ __launch_bounds__(64) void rsi16_repair_known_kernel(uint16_t *__rest
         *status = FFRS_RSI16_REPAIR_OK;
 }
 
+// ---------------------------------------------------------------- encode as an int8 MFMA contraction
+
+// The parity of a message is a fixed linear map of it mod p: parity_j = sum_m v_m G[m][j] mod p,
+// then & 0xFFFF (pyrsi16.hpp:128-140), with G[m] the encode of the unit message at m (the residue
+// NTT, mix and GS butterfly of rsi16v_impl.hpp:93-119 are linear). That holds for the scalar
+// words always, and for the SIMD words wherever no mix constant is 65536 (there a zero residue
+// reduces to p, and p * 65536 wraps to 65536 in 32 bits). This kernel evaluates the map exactly
+// on the 16x16x64 int8 matrix cores:
+//  - symbol v = v0 + 256 v1; the limbs enter as int8 v_a - 128 (the byte XOR 0x80);
+//  - G entry g = g0 + 128 g1 + 16384 g2 (g0, g1 < 128, g2 <= 4), three non-negative int8 limbs;
+//  - P_ab = sum_m (v_a[m] - 128) g_b[m][j] accumulates in int32 (|P| <= 128 * 127 * k);
+//  - value_j = sum_ab 256^a 128^b P_ab + K_j mod p, where K_j = 128 * 257 * sum_b 128^b C_b[j]
+//    (C_b[j] = sum_m g_b[m][j]) puts the offsets back; with 2^16 = -1 mod p every partial sum
+//    is folded as lo16 - hi16 before it can overflow.
+// One wave takes 16 codewords: B operand lane l = codeword l % 16, symbols 16 (l / 16) .. +16 of a
+// 64-symbol chunk; A operand = the matching G limbs of 16 parity rows; the C tile holds parity
+// rows 4 (l / 16) .. +4 of codeword l % 16. Symbols past k meet zero rows of G.
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+
+// x - 65536 floor(x / 65536) - floor(x / 65536) = x mod p (up to a multiple of p), small enough
+// for the next step
+__device__ inline int32_t fold16(int32_t x) { return (x & 0xffff) - (x >> 16); }
+
+template <int CT>
+__global__ __launch_bounds__(256) void rsi16_encode_mfma_kernel(const uint16_t *__restrict__ msgs, uint32_t msg_bytes,
+                                                                uint32_t count, uint16_t *__restrict__ out,
+                                                                const uint4 *__restrict__ gpack,
+                                                                const uint32_t *__restrict__ kconst, uint32_t k,
+                                                                uint32_t E, uint32_t nq)
+{
+    const uint32_t lane = threadIdx.x & 63, c = lane & 15, h = lane >> 4;
+    const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), waves = gridDim.x * 4;
+    const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, msg_bytes);
+    const uint32_t T = E / 16;
+    for (uint32_t grp = wave; grp * 16 < count; grp += waves) {
+        const uint32_t cw = grp * 16 + c;
+        const bool valid = cw < count;
+        const uint32_t row = cw * k * 2;  // byte offset of the codeword's message (< 2^32: launches are split)
+        for (uint32_t t0 = 0; t0 < T; t0 += CT) {
+            v4i_t acc[CT][2][3];
+#pragma unroll
+            for (int tt = 0; tt < CT; ++tt)
+#pragma unroll
+                for (int a = 0; a < 2; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+                        acc[tt][a][b] = v4i_t{0, 0, 0, 0};
+            for (uint32_t q = 0; q < nq; ++q) {
+                uint4 x0 = make_uint4(0, 0, 0, 0), x1 = x0;
+                if (valid) {  // 32-byte aligned reads, wholly inside or wholly past the buffer
+                    const uint32_t off = row + (64 * q + 16 * h) * 2;
+                    x0 = buf_load_u32x4(rsrc, off);
+                    x1 = buf_load_u32x4(rsrc, off + 16);
+                }
+                const v4i_t lo = {(int)(__builtin_amdgcn_perm(x0.y, x0.x, 0x06040200u) ^ 0x80808080u),
+                                  (int)(__builtin_amdgcn_perm(x0.w, x0.z, 0x06040200u) ^ 0x80808080u),
+                                  (int)(__builtin_amdgcn_perm(x1.y, x1.x, 0x06040200u) ^ 0x80808080u),
+                                  (int)(__builtin_amdgcn_perm(x1.w, x1.z, 0x06040200u) ^ 0x80808080u)};
+                const v4i_t hi = {(int)(__builtin_amdgcn_perm(x0.y, x0.x, 0x07050301u) ^ 0x80808080u),
+                                  (int)(__builtin_amdgcn_perm(x0.w, x0.z, 0x07050301u) ^ 0x80808080u),
+                                  (int)(__builtin_amdgcn_perm(x1.y, x1.x, 0x07050301u) ^ 0x80808080u),
+                                  (int)(__builtin_amdgcn_perm(x1.w, x1.z, 0x07050301u) ^ 0x80808080u)};
+#pragma unroll
+                for (int tt = 0; tt < CT; ++tt) {
+                    const uint32_t t = t0 + tt;
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        const uint4 g = gpack[((size_t)(b * T + t) * nq + q) * 64 + lane];
+                        const v4i_t A = {(int)g.x, (int)g.y, (int)g.z, (int)g.w};
+                        acc[tt][0][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, lo, acc[tt][0][b], 0, 0, 0);
+                        acc[tt][1][b] = __builtin_amdgcn_mfma_i32_16x16x64_i8(A, hi, acc[tt][1][b], 0, 0, 0);
+                    }
+                }
+            }
+#pragma unroll
+            for (int tt = 0; tt < CT; ++tt) {
+                const uint32_t t = t0 + tt;
+                uint32_t v[4];
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const uint32_t j = 16 * t + 4 * h + i;
+                    int32_t z[3];
+#pragma unroll
+                    for (int b = 0; b < 3; ++b)
+                        z[b] = fold16(fold16(acc[tt][0][b][i]) + 256 * fold16(acc[tt][1][b][i]));
+                    int32_t f = fold16(z[0] + 128 * z[1] + 16384 * z[2] + (int32_t)kconst[j]);
+                    f = f < 0 ? f + (int32_t)P16 : f;
+                    f = f >= (int32_t)P16 ? f - (int32_t)P16 : f;
+                    v[i] = (uint32_t)f & 0xffffu;
+                }
+                if (valid)
+                    *reinterpret_cast<uint2 *>(out + (size_t)cw * E + 16 * t + 4 * h) =
+                        make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16));
+            }
+        }
+    }
+}
+
 // ---------------------------------------------------------------- interleaved repair, SIMD words
 
 // The reference's vector instantiations repair the columns of an interleaved block in groups of
@@ -2294,6 +2392,12 @@ struct ffrs_rsi16 {
     uint32_t g_enc = 0, g_rep = 0;  // lanes per codeword of the encode and repair kernels
     void *d_work = nullptr;          // workspace of the SIMD-word interleaved repair
     size_t work_size = 0;
+    // the encode as an int8 MFMA contraction (rsi16_encode_mfma_kernel), where it applies
+    uint4 *d_gpack = nullptr;        // G limbs in the A-operand lane layout
+    uint32_t *d_kconst = nullptr;    // K_j
+    uint32_t mfma_nq = 0;            // 64-symbol chunks of the message (0: no MFMA encode)
+    bool mix_minus_one = false;      // a mix constant is 65536: the SIMD words are not the linear map
+    int encode_mode = 0;             // 0 automatic, 1 butterflies, 2 MFMA wherever it applies
 };
 
 namespace {
@@ -2365,11 +2469,36 @@ int launch_encode_reg(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint1
     return FFRS_OK;
 }
 
+// The MFMA contraction over `count` codewords, in launches of < 2^31 message bytes.
+int launch_encode_mfma(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc)
+{
+    const size_t k = rs->k, E = rs->E;
+    const size_t step = std::max<size_t>(16, (((size_t)1 << 31) / (k * 2)) & ~(size_t)15);
+    auto *kern = E == 16 ? rsi16_encode_mfma_kernel<1> : E == 32 ? rsi16_encode_mfma_kernel<2>
+                                                                 : rsi16_encode_mfma_kernel<4>;
+    for (size_t b = 0; b < count; b += step) {
+        const size_t c = std::min(step, count - b);
+        const size_t groups = (c + 15) / 16;
+        const int grid = persistent_grid(rs->ctx, 8, (groups + 3) / 4);
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(256), 0, rs->ctx->stream, d_msg + b * k, (uint32_t)(c * k * 2),
+                           (uint32_t)c, d_ecc + b * E, (const uint4 *)rs->d_gpack, (const uint32_t *)rs->d_kconst,
+                           (uint32_t)k, (uint32_t)E, rs->mfma_nq);
+        FFRS_HIP(hipGetLastError());
+    }
+    mark_kernel(rs->ctx);
+    return FFRS_OK;
+}
+
 int launch_encode(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count, uint16_t *d_ecc, int vec, uint32_t I = 1)
 {
     marks_begin(rs->ctx);
     if (count == 0)
         return FFRS_OK;
+    // the linear map on the matrix cores where it is the encode of the words asked for (and,
+    // automatically, where it is measured faster than the butterflies: ecc_len <= 32)
+    if (rs->mfma_nq && I == 1 && rs->encode_mode != 1 && (rs->encode_mode == 2 || rs->E <= 32) &&
+        !(vec && rs->mix_minus_one) && (((uintptr_t)d_msg | (uintptr_t)d_ecc) & 15) == 0)
+        return launch_encode_mfma(rs, d_msg, count, d_ecc);
     // one lane per codeword (g = 1): the register kernel for ecc_len <= 32, plain layout
     const bool aligned = rs->E < 8 || (((uintptr_t)d_msg | (uintptr_t)d_ecc) & 15) == 0;
     if (rs->g_enc == 1 && rs->E <= 32 && I == 1 && aligned)
@@ -2461,6 +2590,80 @@ int launch_repair(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count
                        d_status, rs->dev, g);
     FFRS_HIP(hipGetLastError());
     mark_kernel(rs->ctx);
+    return FFRS_OK;
+}
+
+}  // namespace
+
+namespace {
+
+// NTT::gs_butterfly, ntt.hpp:280-302 (natural in, RBO out), scalar words
+void gs_host(const std::vector<uint32_t> &roots, uint32_t *x, uint32_t len)
+{
+    for (uint32_t st = len / 2, ef = 0; st > 0; st /= 2, ++ef)
+        for (uint32_t b = 0; b < len; b += 2 * st)
+            for (uint32_t i = b; i < b + st; ++i) {
+                const uint32_t a = x[i], c = x[i + st];
+                x[i] = S::add(a, c);
+                x[i + st] = i == b ? S::sub(a, c) : S::mul(S::sub(a, c), roots[(i - b) << ef]);
+            }
+}
+
+// The tables of rsi16_encode_mfma_kernel: G[m] = the parity of the unit message at m, i.e. the GS
+// butterfly of mix_i X_m^i with X_m = w^rbo(m) (the message residue of rsi16v_impl.hpp:93-97 is
+// sum_m v_m X_m^i, then :108-119), split into three 7-bit limbs and packed in the A-operand
+// lane order, and K_j. Shapes with ecc_len not a multiple of 16, above 256, or message_len above
+// 16384 keep the butterfly kernels.
+int build_mfma_encode(ffrs_rsi16 *rs, const std::vector<uint32_t> &mix, const std::vector<uint32_t> &roots_i_ecc,
+                      const std::vector<uint32_t> &roots_ntt)
+{
+    const uint32_t k = rs->k, E = rs->E;
+    if (E % 16 || E > 256 || k > 16384)
+        return FFRS_OK;
+    const uint32_t nq = (k + 63) / 64, T = E / 16, kp = nq * 64;
+    std::vector<uint8_t> limb((size_t)3 * kp * E, 0);  // [b][m][j]
+    std::vector<uint64_t> C((size_t)3 * E, 0);
+    std::vector<uint32_t> row(E);
+    for (uint32_t m = 0; m < k; ++m) {
+        const uint32_t x = roots_ntt[rbo_bits(m, rs->nbits)];
+        uint32_t pw = 1;
+        for (uint32_t i = 0; i < E; ++i) {
+            row[i] = S::mul(mix[i], pw);
+            pw = S::mul(pw, x);
+        }
+        gs_host(roots_i_ecc, row.data(), E);
+        for (uint32_t j = 0; j < E; ++j) {
+            const uint32_t g = row[j], gb[3] = {g & 127u, (g >> 7) & 127u, g >> 14};
+            for (int b = 0; b < 3; ++b) {
+                limb[((size_t)b * kp + m) * E + j] = (uint8_t)gb[b];
+                C[(size_t)b * E + j] += gb[b];
+            }
+        }
+    }
+    std::vector<uint8_t> pack((size_t)3 * T * nq * 64 * 16);
+    for (uint32_t b = 0; b < 3; ++b)
+        for (uint32_t t = 0; t < T; ++t)
+            for (uint32_t q = 0; q < nq; ++q)
+                for (uint32_t l = 0; l < 64; ++l)
+                    for (uint32_t bt = 0; bt < 16; ++bt) {
+                        const uint32_t m = 64 * q + 16 * (l / 16) + bt, j = 16 * t + l % 16;
+                        pack[(((size_t)(b * T + t) * nq + q) * 64 + l) * 16 + bt] = limb[((size_t)b * kp + m) * E + j];
+                    }
+    std::vector<uint32_t> K(E);
+    for (uint32_t j = 0; j < E; ++j) {
+        const uint64_t sum = (C[j] + 128 * C[E + j] + 16384 * C[2 * (size_t)E + j]) % P16;
+        K[j] = (uint32_t)(sum * 128 % P16 * 257 % P16);
+    }
+    hipError_t e = hipMalloc(&rs->d_gpack, pack.size());
+    if (e == hipSuccess)
+        e = hipMalloc(&rs->d_kconst, K.size() * 4);
+    if (e == hipSuccess)
+        e = hipMemcpy(rs->d_gpack, pack.data(), pack.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(rs->d_kconst, K.data(), K.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_DEVICE, "MFMA encode tables: %s", hipGetErrorString(e));
+    rs->mfma_nq = nq;
     return FFRS_OK;
 }
 
@@ -2593,7 +2796,23 @@ int ffrs_rsi16_create(ffrs_ctx *ctx, size_t block_len, size_t ecc_len, uint32_t 
     }
     rs->g_rep = group_size(rs->E);
     rs->g_enc = rs->E <= 32 ? 1u : group_size(rs->E);  // register kernel, one codeword per lane
+    rs->mix_minus_one = std::any_of(mix.begin(), mix.end(), [](uint32_t v) { return v == 65536; });
+    if (int r = build_mfma_encode(rs, mix, roots_i_ecc, roots_ntt)) {
+        ffrs_rsi16_destroy(rs);
+        return r;
+    }
     *out = rs;
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_set_encode_kernel(ffrs_rsi16 *rs, int kernel)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (kernel < 0 || kernel > 2)
+        return fail(FFRS_ERR_INVALID, "encode kernel %d (0 automatic, 1 butterflies, 2 MFMA where it applies)",
+                    kernel);
+    rs->encode_mode = kernel;
     return FFRS_OK;
 }
 
@@ -2606,6 +2825,10 @@ void ffrs_rsi16_destroy(ffrs_rsi16 *rs)
         (void)hipFree(rs->d_tables);
     if (rs->d_work)
         (void)hipFree(rs->d_work);
+    if (rs->d_gpack)
+        (void)hipFree(rs->d_gpack);
+    if (rs->d_kconst)
+        (void)hipFree(rs->d_kconst);
     delete rs;
 }
 

@@ -127,3 +127,45 @@ def test_encode_kernels_agree(n, ecc, vector):
     rows = np.r_[0:16, count - 16:count]
     ref = O.RSi16(n, ecc).encode(np.ascontiguousarray(msgs[rows]), vector_lanes=vector)
     assert (ref == outs[0][rows]).all()
+
+
+MFMA_SHAPES = [(64, 16), (256, 32), (208, 16), (512, 64), (1024, 128), (4096, 256), (128, 8), (256, 128)]
+
+
+@pytest.mark.parametrize("n,ecc", MFMA_SHAPES)
+def test_mfma_encode_matches_butterflies(n, ecc):
+    """The int8 MFMA encode (forced wherever it applies, and the automatic choice) against the
+    butterfly kernels on random, constant and extreme messages, for the SIMD and the scalar
+    words. (128, 8) has ecc 8 and (256, 128) a mix constant of 65536: there the contraction does
+    not apply to the SIMD words (or at all for ecc 8) and the butterflies run, which must still
+    agree."""
+    from libffrs_amd import capi
+    k = n - ecc
+    msgs, _, s = W.rsi16_plain(3000, n, ecc, 0, seed=40 + n + ecc)
+    edge = np.stack([np.zeros(k), np.full(k, 65535), np.full(k, 1), np.arange(k) % 2 * 65535,
+                     np.arange(k) * 977 % 65536]).astype(np.uint16)
+    msgs = np.concatenate([msgs, edge])
+    count = msgs.shape[0]
+    print(f"mfma encode ({n},{ecc}) on {count} messages", flush=True)
+    ctx = capi.Context(0)
+    codec = capi.RSi16Codec(ctx, n, ecc)
+    dm, dp = ctx.alloc(msgs.nbytes), ctx.alloc(count * ecc * 2)
+    try:
+        dm.upload(msgs)
+        for vec in (True, False):
+            outs = []
+            for kernel in (2, 1, 0):
+                codec.set_encode_kernel(kernel)
+                codec.encode_device(dm, count, dp, vec)
+                ctx.sync()
+                o = np.empty((count, ecc), np.uint16)
+                dp.download(o)
+                outs.append(o)
+            assert (outs[0] == outs[1]).all(), (n, ecc, vec, int((outs[0] != outs[1]).any(1).sum()))
+            assert (outs[2] == outs[1]).all()
+        codec.set_encode_kernel(0)
+    finally:
+        dm.free()
+        dp.free()
+        codec.close()
+        ctx.close()
