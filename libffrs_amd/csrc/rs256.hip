@@ -91,7 +91,12 @@ struct Rs256Tables {
 // row reads start at every 16-byte bank group of the LDS instead of every other one.
 __host__ __device__ constexpr int lut_row(int EW) { return (EW % 8 == 0) ? EW + 4 : EW; }
 
-// Replicated two-slice layout of the 32-byte-row table (S == LUT_REP, EW == 8): chunk h (16
+// Replicated two-slice layouts (S == LUT_REP), 64 KB per slice, slice s at byte s << 16.
+// Rows of 4, 8 or 16 bytes (EW = 1, 2, 4): row b of slice s is stored C = 64 / EW times, copy c
+// at byte offset (s << 16) | (b << 8) | (c * 4 EW), so that each row fills the 64 banks once.
+// The LDS serves 64 / EW lanes of a 4 EW-byte read together, and lane l reads copy l % C: the
+// lanes served together always sit in distinct banks, whatever rows the data selects.
+// Rows of 32 bytes (EW == 8): chunk h (16
 // bytes) of row b of slice s is stored LUT_COPIES times, copy c at byte offset
 // (s << 16) | (b << 8) | (h << 7) | (c << 4), i.e. in 16-byte bank group 8h + c of the 64
 // banks. The LDS serves a 16-byte read sixteen lanes at a time (one lane per bank group when
@@ -134,11 +139,45 @@ __device__ inline uint32_t *lfsr_table(uint32_t *dyn)
 // Dynamic LDS bytes that the table of layout (EW, S) takes.
 __host__ __device__ constexpr int lfsr_dyn_dwords(int EW, int S) { return S == LUT_REP ? 0 : lut_dwords(EW, S); }
 
+// One row of EW dwords of an LDS table, in one read.
+template <int EW>
+__device__ inline void lds_row(const char *p, uint32_t (&r)[EW])
+{
+    if constexpr (EW == 4) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(p);
+        r[0] = v.x; r[1] = v.y; r[2] = v.z; r[3] = v.w;
+    } else if constexpr (EW == 2) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p);
+        r[0] = v.x; r[1] = v.y;
+    } else {
+        static_assert(EW == 1, "rows of 4, 8 or 16 bytes");
+        r[0] = *reinterpret_cast<const uint32_t *>(p);
+    }
+}
+
 template <int EW, int S>
 __device__ inline void lut_step(uint32_t (&rem)[EW], uint32_t w, const uint32_t *__restrict__ s_lut)
 {
-    if constexpr (S == LUT_REP) {
-        static_assert(EW == 8, "replicated layout holds 32-byte rows");
+    if constexpr (S == LUT_REP && EW < 8) {
+        // two bytes per half-step: slice 1 row x.byte0, slice 0 row x.byte1 at this lane's copy
+        // (offsets built by byte permutes: copy in byte 0, row in byte 1, slice in byte 2)
+        constexpr uint32_t C = 64 / EW;
+        const uint32_t lane_off1 = ((threadIdx.x % C) * 4 * EW) | 0x10000u;
+        const char *lds0 = reinterpret_cast<const char *>(s_lut);
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const uint32_t x = rem[0] ^ (h ? w >> 16 : w);
+            uint32_t a1[EW], a0[EW];
+            lds_row<EW>(lds0 + __builtin_amdgcn_perm(x, lane_off1, 0x0c020400u), a1);
+            lds_row<EW>(lds0 + __builtin_amdgcn_perm(x, lane_off1, 0x0c0c0500u), a0);
+#pragma unroll
+            for (int d = 0; d < EW; ++d) {
+                const uint32_t nx = d + 1 < EW ? alignbyte(rem[(d + 1) % EW], rem[d], 2) : rem[d] >> 16;
+                rem[d] = xor3(nx, a1[d], a0[d]);
+            }
+        }
+    } else if constexpr (S == LUT_REP) {
+        static_assert(EW == 8, "replicated layouts hold rows of 4 to 32 bytes");
         // byte permutes build the offsets: this lane's copy and first chunk in byte 0, the row
         // in byte 1, the slice in byte 2; the second chunk is 128 bytes away
         const bool swapped = lut_swapped();
@@ -370,7 +409,107 @@ __device__ inline void message_remainder(__amdgpu_buffer_rsrc_t rsrc, uint32_t n
         lut_step<EW, S>(rem, w, s_lut);
         prev = c;
     }
-    if constexpr (S == LUT_REP)
+    if constexpr (S == LUT_REP && EW == 8)
+        rep_to_bytes(rem);
+}
+
+// Remainders of the 64 messages of a wave, cooperatively loaded (replicated layouts, k <= 254,
+// so that a message with the dword of the word before it spans at most 65 dwords). Lane 16 g + p
+// takes the message of row `base + 16 g + p`. Each message window (from the dword holding the
+// word before the first fed one, as message_remainder reads it) is read by the sixteen lanes of
+// its group, 16 bytes each, in one instruction: 256 contiguous bytes per group instead of 64
+// scattered 16-byte reads per instruction (on MI355X, 6.1 against 2.6 TB/s for 247-byte rows).
+// A 16 x 16 transpose of the 16-byte pieces within each group (DPP exchanges, no LDS) then gives
+// every lane its own window, and the LFSR runs on registers.
+__device__ inline uint32_t dpp_xor(uint32_t x, int b)
+{
+    if (b == 1)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0xB1, 0xF, 0xF, false);  // quad_perm [1,0,3,2]
+    if (b == 2)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x4E, 0xF, 0xF, false);  // quad_perm [2,3,0,1]
+    if (b == 8)
+        return (uint32_t)__builtin_amdgcn_mov_dpp((int)x, 0x128, 0xF, 0xF, false);  // row_ror:8 (lane ^ 8)
+    // lane ^ 4 = (lane ^ 3) ^ 7: quad_perm [3,2,1,0], then row_half_mirror
+    const int y = __builtin_amdgcn_mov_dpp((int)x, 0x1B, 0xF, 0xF, false);
+    return (uint32_t)__builtin_amdgcn_mov_dpp(y, 0x141, 0xF, 0xF, false);
+}
+
+// D[slot][lane] (slot = 16-byte piece index held by this lane, 4 dwords each) becomes D[lane][slot]
+// within each 16-lane row: for every bit b, the pieces whose slot bit b differs from the lane's
+// bit b trade places with the partner lane (lane ^ b, slot ^ b).
+__device__ inline void transpose16(uint32_t (&d)[16][4], uint32_t lane)
+{
+#pragma unroll
+    for (int b = 1; b < 16; b <<= 1) {
+        const bool hi = (lane & (uint32_t)b) != 0;
+#pragma unroll
+        for (int s0 = 0; s0 < 16; ++s0) {
+            if (s0 & b)
+                continue;
+            const int s1 = s0 | b;
+#pragma unroll
+            for (int w = 0; w < 4; ++w) {
+                const uint32_t send = hi ? d[s0][w] : d[s1][w];
+                const uint32_t recv = dpp_xor(send, b);
+                d[s1][w] = hi ? d[s1][w] : recv;
+                d[s0][w] = hi ? recv : d[s0][w];
+            }
+        }
+    }
+}
+
+// Issues the cooperative window reads of the 64 messages of a wave from row `base` into d
+// (lane 16 g + p reads 16-byte piece p of the windows of rows base + 16 g + 0..15); the caller
+// waits for them where it first uses d.
+__device__ inline void coop_issue(__amdgpu_buffer_rsrc_t rsrc, uint32_t base, uint32_t k, uint32_t (&d)[16][4])
+{
+    const uint32_t lane = threadIdx.x & 63, p = lane & 15, g0 = base + (lane & 48);
+    const uint32_t pad = ((k + 3) & ~3u) - k;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        // window of row r: from the dword at floor((r k - pad) / 4) (wraps below zero)
+        const uint32_t o = (uint32_t)(((int64_t)(g0 + i) * k - (int64_t)pad) >> 2) * 4u + 16 * p;
+        // The window of the first message starts a dword before the buffer when pad > 0; that
+        // dword only holds pad bytes, which the head mask clears. Such a piece is read from the
+        // buffer start (a read at the wrapped offset is out of range as a whole) and shifted.
+        const bool before = o >= 0xFFFFFFF0u;
+        const uint4 v = buf_load_u32x4(rsrc, before ? o + 4 : o);
+        d[i][0] = before ? 0u : v.x;
+        d[i][1] = before ? v.x : v.y;
+        d[i][2] = before ? v.y : v.z;
+        d[i][3] = before ? v.z : v.w;
+    }
+}
+
+// Remainder of this lane's message from the windows issued by coop_issue for row `base`
+// (transposed here, so that every lane holds its own window) and the replicated tables.
+template <int EW, int S>
+__device__ inline void coop_remainder(__amdgpu_buffer_rsrc_t rsrc, uint32_t base, uint32_t k, uint32_t (&d)[16][4],
+                                      const uint32_t *__restrict__ s_lut, uint32_t (&rem)[EW])
+{
+    static_assert(S == LUT_REP, "cooperative loads feed the replicated tables");
+    const uint32_t lane = threadIdx.x & 63;
+    const uint32_t P = (k + 3) & ~3u, pad = P - k, nw = P >> 2;
+    const int64_t start = (int64_t)(base + lane) * k - (int64_t)pad;
+    const uint32_t sh = (uint32_t)(start - (start >> 2) * 4);
+    const uint32_t off = (uint32_t)(start >> 2) * 4u;
+    const uint32_t extra = nw >= 64 ? buf_load_u32(rsrc, off + 256) : 0u;  // the 65th dword
+    transpose16(d, lane & 15);
+    const uint32_t headmask = pad ? ~((1u << (8 * pad)) - 1u) : 0xffffffffu;
+#pragma unroll
+    for (int j = 0; j < EW; ++j)
+        rem[j] = 0;
+#pragma unroll
+    for (int t = 0; t < 64; ++t) {
+        if ((uint32_t)t < nw) {
+            const uint32_t hi = t + 1 < 64 ? d[(t + 1) >> 2][(t + 1) & 3] : extra;
+            uint32_t w = alignbyte(hi, d[t >> 2][t & 3], sh);
+            if (t == 0)
+                w &= headmask;
+            lut_step<EW, S>(rem, w, s_lut);
+        }
+    }
+    if constexpr (EW == 8)
         rep_to_bytes(rem);
 }
 
@@ -384,7 +523,11 @@ __device__ inline uint8_t rem_byte(const uint32_t (&rem)[EW], uint32_t j)
     return (uint8_t)(v >> (8 * (j & 3)));
 }
 
-template <int EW, int S, int WG>
+// Message loads (LM): LOAD_LANE each lane its own message (message_remainder), LOAD_COOP sixteen
+// lanes per message window (coop_issue / coop_remainder).
+constexpr int LOAD_LANE = 0, LOAD_COOP = 1;
+
+template <int EW, int S, int WG, int LM = LOAD_LANE>
 __global__ __launch_bounds__(WG) void rs256_encode_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
                                                             uint32_t count, uint32_t k, uint8_t *__restrict__ out,
                                                             uint32_t ecc, const uint32_t *__restrict__ lut)
@@ -398,10 +541,8 @@ __global__ __launch_bounds__(WG) void rs256_encode_kernel(const uint8_t *__restr
 
     const uint32_t nrec = (msg_bytes + 3u) & ~3u;
     __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, nrec);
-    uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += stride) {
-        uint32_t rem[EW];
-        message_remainder<EW, S>(rsrc, nrec, (int64_t)cw * k, k, s_lut, rem);
+    const uint32_t stride = gridDim.x * blockDim.x;
+    auto store = [&](uint32_t cw, const uint32_t (&rem)[EW]) {
         uint8_t *o = out + (size_t)cw * ecc;
         if ((ecc & 3) == 0) {
             uint32_t *o4 = reinterpret_cast<uint32_t *>(o);
@@ -412,6 +553,23 @@ __global__ __launch_bounds__(WG) void rs256_encode_kernel(const uint8_t *__restr
         } else {
             for (uint32_t j = 0; j < ecc; ++j)
                 o[j] = rem_byte<EW>(rem, j);
+        }
+    };
+    if constexpr (LM == LOAD_COOP) {
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        for (uint32_t base = (blockIdx.x * (WG / 64) + wave) * 64; base < count; base += stride) {
+            uint32_t d[16][4], rem[EW];
+            coop_issue(rsrc, base, k, d);
+            coop_remainder<EW, S>(rsrc, base, k, d, s_lut, rem);
+            const uint32_t cw = base + (threadIdx.x & 63);
+            if (cw < count)
+                store(cw, rem);
+        }
+    } else {
+        for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += stride) {
+            uint32_t rem[EW];
+            message_remainder<EW, S>(rsrc, nrec, (int64_t)cw * k, k, s_lut, rem);
+            store(cw, rem);
         }
     }
 }
@@ -794,7 +952,8 @@ __host__ __device__ inline uint32_t rec_lam_off() { return 20; }
 __host__ __device__ inline uint32_t rec_om_off(uint32_t) { return 4; }
 __host__ __device__ inline uint32_t rec_stride(uint32_t) { return 48; }
 
-template <int EW, int S, int WG>
+// LM: message loads, as for rs256_encode_kernel.
+template <int EW, int S, int WG, int LM = LOAD_LANE>
 __global__ __launch_bounds__(WG) void rs256_synd_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
                                                           const uint8_t *__restrict__ eccs, uint32_t count,
                                                           uint32_t k, uint32_t E, uint8_t *__restrict__ status,
@@ -811,18 +970,15 @@ __global__ __launch_bounds__(WG) void rs256_synd_kernel(const uint8_t *__restric
     __syncthreads();
     const uint32_t nrec = (msg_bytes + 3u) & ~3u;
     __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, nrec);
-    for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += gridDim.x * blockDim.x) {
-        uint32_t rem[EW];
-        // whole-word parity rows are loaded before the message, so that their latency overlaps
-        // the LFSR
+    // whole-word parity rows, loaded before the message so that their latency overlaps the LFSR
+    auto load_parity = [&](uint32_t cw, uint32_t (&par)[EW]) {
         const uint8_t *pe = eccs + (size_t)cw * E;
-        uint32_t par[EW];
-        if ((E & 3) == 0) {
 #pragma unroll
-            for (int d = 0; d < EW; ++d)
-                par[d] = (uint32_t)d * 4 < E ? reinterpret_cast<const uint32_t *>(pe)[d] : 0u;
-        }
-        message_remainder<EW, S>(rsrc, nrec, (int64_t)cw * k, k, s_lut, rem);
+        for (int d = 0; d < EW; ++d)
+            par[d] = (E & 3) == 0 && (uint32_t)d * 4 < E ? reinterpret_cast<const uint32_t *>(pe)[d] : 0u;
+    };
+    // remainder ^ parity: zero -> status 1; else the syndromes from the nibble tables, PENDING
+    auto finish = [&](uint32_t cw, uint32_t (&rem)[EW], const uint32_t (&par)[EW]) {
         uint32_t nz = 0;
         if ((E & 3) == 0) {
 #pragma unroll
@@ -831,11 +987,11 @@ __global__ __launch_bounds__(WG) void rs256_synd_kernel(const uint8_t *__restric
                 nz |= rem[d];
             }
         } else {
-            nz = xor_parity<EW>(rem, pe, E);
+            nz = xor_parity<EW>(rem, eccs + (size_t)cw * E, E);
         }
         if (nz == 0) {
             status[cw] = 1;
-            continue;
+            return;
         }
         uint32_t acc[EW];
 #pragma unroll
@@ -869,6 +1025,26 @@ __global__ __launch_bounds__(WG) void rs256_synd_kernel(const uint8_t *__restric
         for (int d = 0; d < EW; ++d)
             o[d] = acc[d];
         status[cw] = RS_PENDING;
+    };
+    const uint32_t stride = gridDim.x * blockDim.x;
+    if constexpr (LM == LOAD_COOP) {
+        const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+        for (uint32_t base = (blockIdx.x * (WG / 64) + wave) * 64; base < count; base += stride) {
+            const uint32_t cw = base + (threadIdx.x & 63);
+            uint32_t d[16][4], par[EW], rem[EW];
+            load_parity(cw < count ? cw : base, par);
+            coop_issue(rsrc, base, k, d);
+            coop_remainder<EW, S>(rsrc, base, k, d, s_lut, rem);
+            if (cw < count)
+                finish(cw, rem, par);
+        }
+    } else {
+        for (uint32_t cw = blockIdx.x * blockDim.x + threadIdx.x; cw < count; cw += stride) {
+            uint32_t par[EW], rem[EW];
+            load_parity(cw, par);
+            message_remainder<EW, S>(rsrc, nrec, (int64_t)cw * k, k, s_lut, rem);
+            finish(cw, rem, par);
+        }
     }
 }
 
@@ -1447,7 +1623,7 @@ struct ffrs_rs256 {
     uint8_t gen[256] = {0};
     uint8_t roots[256] = {0};
     uint32_t *d_lut = nullptr;
-    uint32_t *d_lutr = nullptr;  // the same table in the replicated two-slice layout (EW == 8)
+    uint32_t *d_lutr = nullptr;  // the same table in the replicated two-slice layout (EW <= 8)
     uint8_t *d_exp2 = nullptr;
     uint16_t *d_log2 = nullptr;
     uint32_t *d_stab = nullptr;  // syndrome nibble tables (ecc <= 32)
@@ -1501,25 +1677,34 @@ std::vector<uint32_t> build_lut(const ffrs_rs256 &rs)
     return lut;
 }
 
-// The replicated two-slice layout of lut_dwords(8, LUT_REP) dwords (ecc 29..32).
+// The replicated two-slice layout of lut_dwords(EW, LUT_REP) dwords (ecc_len <= 32).
 std::vector<uint32_t> build_lut_rep(const ffrs_rs256 &rs)
 {
-    unsigned E = rs.ecc;
-    std::vector<uint32_t> t((size_t)lut_dwords(8, LUT_REP), 0);
+    const unsigned E = rs.ecc, EW = rs.EW;
+    std::vector<uint32_t> t((size_t)lut_dwords((int)EW, LUT_REP), 0);
     for (unsigned b = 0; b < 256; ++b) {
         uint8_t r[256] = {0};
         r[0] = (uint8_t)b;
         lfsr_zero_step(rs, r);
         for (unsigned s = 0; s < 2; ++s) {
-            // halfword-paired: dword j = symbols pair j | pair (j + 8) << 16
-            uint32_t row[8] = {0};
-            for (unsigned j = 0; j < E; ++j) {
-                const unsigned pair = j / 2;
-                row[pair % 8] |= (uint32_t)r[j] << (8 * (j % 2) + 16 * (pair / 8));
+            if (EW == 8) {
+                // halfword-paired: dword j = symbols pair j | pair (j + 8) << 16
+                uint32_t row[8] = {0};
+                for (unsigned j = 0; j < E; ++j) {
+                    const unsigned pair = j / 2;
+                    row[pair % 8] |= (uint32_t)r[j] << (8 * (j % 2) + 16 * (pair / 8));
+                }
+                for (unsigned h = 0; h < 2; ++h)
+                    for (unsigned c = 0; c < (unsigned)LUT_COPIES; ++c)
+                        memcpy(&t[((((size_t)s * 256 + b) * 2 + h) * LUT_COPIES + c) * 4], row + 4 * h, 16);
+            } else {
+                // byte order, 64 / EW copies side by side
+                uint32_t row[4] = {0};
+                for (unsigned j = 0; j < E; ++j)
+                    row[j / 4] |= (uint32_t)r[j] << (8 * (j % 4));
+                for (unsigned c = 0; c < 64 / EW; ++c)
+                    memcpy(&t[(((size_t)s * 256 + b) * 64) + c * EW], row, 4 * EW);
             }
-            for (unsigned h = 0; h < 2; ++h)
-                for (unsigned c = 0; c < (unsigned)LUT_COPIES; ++c)
-                    memcpy(&t[((((size_t)s * 256 + b) * 2 + h) * LUT_COPIES + c) * 4], row + 4 * h, 16);
             lfsr_zero_step(rs, r);
         }
     }
@@ -1538,19 +1723,33 @@ template <int S>
 const uint32_t *lut_of(const ffrs_rs256 *rs) { return S == LUT_REP ? rs->d_lutr : rs->d_lut; }
 constexpr int synd_wg(int) { return 256; }
 
+// Message loads of the LFSR kernels: cooperative windows where k <= 254 over the replicated
+// tables, except for the syndromes of 32-byte rows (whose registers then spill: there one lane
+// per message is faster, 131 against 148 us at ecc 32 on MI355X), else one lane per message.
+template <int EW, int S>
+int lfsr_load_mode(uint32_t k, bool synd)
+{
+    return S == LUT_REP && k <= 254 && !(synd && EW == 8) ? LOAD_COOP : LOAD_LANE;
+}
+
 template <int EW, int S>
 int launch_encode(ffrs_rs256 *rs, const uint8_t *d_msg, size_t count, uint32_t k, uint8_t *d_ecc)
 {
-    constexpr int WG = lfsr_wg(EW, S);
     ffrs_ctx *ctx = rs->ctx;
+    const bool coop = lfsr_load_mode<EW, S>(k, false) == LOAD_COOP;
+    constexpr int WG = lfsr_wg(EW, S), WGC = 1024;
+    constexpr bool rep = S == LUT_REP;
+    const int wg = coop ? WGC : WG;
     size_t lds = (size_t)lfsr_dyn_dwords(EW, S) * 4;
-    size_t tiles = (count + WG - 1) / WG;
-    int wgs_per_cu = (int)std::min<size_t>(2048 / WG, std::max<size_t>(1, (160 * 1024) / (lut_dwords(EW, S) * 4)));
+    size_t tiles = (count + wg - 1) / wg;
+    int wgs_per_cu = (int)std::min<size_t>(2048 / wg, std::max<size_t>(1, (160 * 1024) / (lut_dwords(EW, S) * 4)));
     int grid = persistent_grid(ctx, wgs_per_cu, tiles);
-    if (int r = allow_lds(rs256_encode_kernel<EW, S, WG>, lds))
+    auto *kern = coop ? rs256_encode_kernel<EW, S, rep ? WGC : WG, rep ? LOAD_COOP : LOAD_LANE>
+                      : rs256_encode_kernel<EW, S, WG, LOAD_LANE>;
+    if (int r = allow_lds(kern, lds))
         return r;
-    hipLaunchKernelGGL((rs256_encode_kernel<EW, S, WG>), dim3(grid), dim3(WG), lds, ctx->stream, d_msg,
-                       (uint32_t)(count * k), (uint32_t)count, k, d_ecc, rs->ecc, lut_of<S>(rs));
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(wg), lds, ctx->stream, d_msg, (uint32_t)(count * k), (uint32_t)count, k,
+                       d_ecc, rs->ecc, lut_of<S>(rs));
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
     return FFRS_OK;
@@ -1671,14 +1870,19 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     uint8_t *d_rec = (uint8_t *)work + ((synd_bytes + 255) & ~(size_t)255);
     Rs256Tables tab{rs->d_lut, rs->d_exp2, rs->d_log2};
 
-    constexpr int WG1 = S == LUT_REP ? lfsr_wg(EW, S) : synd_wg(EW);
+    const bool coop = lfsr_load_mode<EW, S>(k, true) == LOAD_COOP;
+    constexpr int WG1 = S == LUT_REP ? lfsr_wg(EW, S) : synd_wg(EW), WG1C = 1024;
+    constexpr bool rep = S == LUT_REP;
+    const int wg1 = coop ? WG1C : WG1;
     size_t lds1 = (size_t)lfsr_dyn_dwords(EW, S) * 4 + (size_t)E * 32 * EW * 4;
-    if (int r = allow_lds(rs256_synd_kernel<EW, S, WG1>, lds1))
+    auto *k1 = coop ? rs256_synd_kernel<EW, S, rep ? WG1C : WG1, rep ? LOAD_COOP : LOAD_LANE>
+                    : rs256_synd_kernel<EW, S, WG1, LOAD_LANE>;
+    if (int r = allow_lds(k1, lds1))
         return r;
-    int g1 = persistent_grid(ctx, (int)std::min<size_t>(2048 / WG1, std::max<size_t>(1, (160 * 1024) / (lds1 + (lut_dwords(EW, S) - lfsr_dyn_dwords(EW, S)) * 4))),
-                             (count + WG1 - 1) / WG1);
-    hipLaunchKernelGGL((rs256_synd_kernel<EW, S, WG1>), dim3(g1), dim3(WG1), lds1, ctx->stream, d_msg,
-                       (uint32_t)(count * k), d_ecc, (uint32_t)count, k, E, d_status, d_synd, lut_of<S>(rs), rs->d_stab);
+    int g1 = persistent_grid(ctx, (int)std::min<size_t>(2048 / wg1, std::max<size_t>(1, (160 * 1024) / (lds1 + (lut_dwords(EW, S) - lfsr_dyn_dwords(EW, S)) * 4))),
+                             (count + wg1 - 1) / wg1);
+    hipLaunchKernelGGL(k1, dim3(g1), dim3(wg1), lds1, ctx->stream, d_msg, (uint32_t)(count * k), d_ecc,
+                       (uint32_t)count, k, E, d_status, d_synd, lut_of<S>(rs), rs->d_stab);
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
 
@@ -1737,9 +1941,9 @@ int encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *d_msg, size
         uint8_t *e = d_ecc + b * rs->ecc;
         int r;
         switch (rs->EW) {
-        case 1: r = launch_encode<1, 4>(rs, m, c, k, e); break;
-        case 2: r = launch_encode<2, 4>(rs, m, c, k, e); break;
-        case 4: r = launch_encode<4, 4>(rs, m, c, k, e); break;
+        case 1: r = rs->d_lutr ? launch_encode<1, LUT_REP>(rs, m, c, k, e) : launch_encode<1, 4>(rs, m, c, k, e); break;
+        case 2: r = rs->d_lutr ? launch_encode<2, LUT_REP>(rs, m, c, k, e) : launch_encode<2, 4>(rs, m, c, k, e); break;
+        case 4: r = rs->d_lutr ? launch_encode<4, LUT_REP>(rs, m, c, k, e) : launch_encode<4, 4>(rs, m, c, k, e); break;
         case 8: r = rs->d_lutr ? launch_encode<8, LUT_REP>(rs, m, c, k, e) : launch_encode<8, 4>(rs, m, c, k, e); break;
         case 16: r = launch_encode<16, 2>(rs, m, c, k, e); break;
         case 32: r = launch_encode<32, 1>(rs, m, c, k, e); break;
@@ -1780,9 +1984,9 @@ int repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d
         uint8_t *st = d_status + b;
         int r;
         switch (rs->EW) {
-        case 1: r = pipe ? launch_repair_pipeline<1, 4>(rs, m, e, c, k, st, work) : launch_repair<1, 4, 256>(rs, m, e, c, k, st); break;
-        case 2: r = pipe ? launch_repair_pipeline<2, 4>(rs, m, e, c, k, st, work) : launch_repair<2, 4, 256>(rs, m, e, c, k, st); break;
-        case 4: r = pipe ? launch_repair_pipeline<4, 4>(rs, m, e, c, k, st, work) : launch_repair<4, 4, 256>(rs, m, e, c, k, st); break;
+        case 1: r = pipe ? launch_repair_pipeline<1, LUT_REP>(rs, m, e, c, k, st, work) : launch_repair<1, 4, 256>(rs, m, e, c, k, st); break;
+        case 2: r = pipe ? launch_repair_pipeline<2, LUT_REP>(rs, m, e, c, k, st, work) : launch_repair<2, 4, 256>(rs, m, e, c, k, st); break;
+        case 4: r = pipe ? launch_repair_pipeline<4, LUT_REP>(rs, m, e, c, k, st, work) : launch_repair<4, 4, 256>(rs, m, e, c, k, st); break;
         case 8:
             r = !pipe ? launch_repair<8, 4, 128>(rs, m, e, c, k, st)
                 : rs->d_lutr ? launch_repair_pipeline<8, LUT_REP>(rs, m, e, c, k, st, work)
@@ -1867,12 +2071,13 @@ int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsig
         e = hipMalloc(&rs->d_log2, 512);
     if (e == hipSuccess)
         e = hipMemcpy(rs->d_lut, lut.data(), lut.size() * 4, hipMemcpyHostToDevice);
-    if (e == hipSuccess && rs->EW == 8) {
+    if (e == hipSuccess && rs->EW <= 8) {
         std::vector<uint32_t> lr = build_lut_rep(*rs);
         e = hipMalloc(&rs->d_lutr, lr.size() * 4);
         if (e == hipSuccess)
             e = hipMemcpy(rs->d_lutr, lr.data(), lr.size() * 4, hipMemcpyHostToDevice);
     }
+
     if (e == hipSuccess)
         e = hipMemcpy(rs->d_exp2, exp2, EXP2_LEN, hipMemcpyHostToDevice);
     if (e == hipSuccess)
@@ -1905,6 +2110,7 @@ void ffrs_rs256_destroy(ffrs_rs256 *rs)
         (void)hipFree(rs->d_lut);
     if (rs->d_lutr)
         (void)hipFree(rs->d_lutr);
+
     if (rs->d_exp2)
         (void)hipFree(rs->d_exp2);
     if (rs->d_log2)
