@@ -125,6 +125,20 @@ int ffrs_rs256_encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *
                              uint8_t *d_ecc);
 int ffrs_rs256_repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d_ecc, size_t count,
                              uint8_t *d_status);
+/* Allocation-free form of ffrs_rs256_repair_device (no counterpart in the reference, whose
+ * repair has no device workspace): for callers that cannot allocate device memory, such as
+ * code under HIP graph capture or a serving loop that forbids hipMalloc. Same arguments and
+ * results; it calls no hipMalloc/hipFree or allocating library routine. The syndrome / BM
+ * workspace of the repair pipeline lives in one static device array of
+ * FFRS_RS256_NOALLOC_SCRATCH_BYTES per GPU (held until the process ends), which limits a call:
+ * ecc_len 2, 4, 8, 16 and 32 take 4, 4, 8, 16 and 32 bytes per codeword (ecc_len 32: at most
+ * 1,179,648 codewords per call), other ecc_len <= 32 take 4-32 bytes plus a 48-byte record,
+ * ecc_len > 32 needs none. Larger calls fail with FFRS_ERR_SIZE and name the limit. Every call
+ * on one GPU shares that array: calls that overlap in time on one GPU (other streams or host
+ * threads) race and give wrong results; calls in order on one stream are safe. */
+#define FFRS_RS256_NOALLOC_SCRATCH_BYTES (36u << 20)
+int ffrs_rs256_repair_device_noalloc(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d_ecc,
+                                     size_t count, uint8_t *d_status);
 
 /* ---- RSi16 over GF(65537) ---------------------------------------------------------- */
 /* Codec (block_len, ecc_len, primitive) with interleave 1: field, NTT and mix tables

@@ -57,6 +57,7 @@ SIGNATURES = [
     ("ffrs_rs256_synds", c_int, [c_vp, c_sz, c_vp, c_vp, c_sz, c_vp]),
     ("ffrs_rs256_encode_device", c_int, [c_vp, c_sz, c_vp, c_sz, c_vp]),
     ("ffrs_rs256_repair_device", c_int, [c_vp, c_sz, c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_rs256_repair_device_noalloc", c_int, [c_vp, c_sz, c_vp, c_vp, c_sz, c_vp]),
     ("ffrs_rsi16_create", c_int, [c_vp, c_sz, c_sz, c_u32, ctypes.POINTER(c_vp)]),
     ("ffrs_rsi16_destroy", None, [c_vp]),
     ("ffrs_rsi16_set_groups", c_int, [c_vp, c_uint, c_uint]),
@@ -251,6 +252,14 @@ class RS256Codec:
         check(lib().ffrs_rs256_repair_device(self.h, message_len, d_msg.ptr, d_ecc.ptr, count, d_status.ptr),
               "rs256 repair")
 
+    def repair_device_noalloc(self, message_len, d_msg, d_ecc, count, d_status):
+        """ffrs_rs256_repair_device_noalloc: the same repair over the library's static device
+        workspace (no device allocation)."""
+        check(lib().ffrs_rs256_repair_device_noalloc(self.h, message_len, d_msg.ptr, d_ecc.ptr, count,
+                                                     d_status.ptr), "rs256 repair (allocation-free)")
+
+
+class RSi16Codec:
     def __init__(self, ctx, block_len, ecc_len, primitive=3):
         h = c_vp()
         check(lib().ffrs_rsi16_create(ctx.h, block_len, ecc_len, primitive, ctypes.byref(h)), "ffrs_rsi16_create")

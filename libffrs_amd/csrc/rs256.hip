@@ -1614,6 +1614,44 @@ __global__ __launch_bounds__(WG) void rs256_plan_kernel(uint8_t *__restrict__ ms
 
 using namespace ffrs;
 
+// ---------------------------------------------------------------- allocation-free repair
+
+// Workspace of ffrs_rs256_repair_device_noalloc: a file-scope device array, one copy per GPU,
+// held from the library's load until the process ends. Host code never hands this name to a
+// kernel (on the host it is only a stand-in for the symbol); it uses the device address that
+// hipGetSymbolAddress returns with that GPU current, looked up once per device when the
+// first codec is created there.
+__device__ __align__(256) uint8_t scratch[FFRS_RS256_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void *g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: the device address of this GPU's copy of `scratch`.
+int resolve_scratch(int device)
+{
+    if (device < 0 || device >= kMaxDevices)
+        return fail(FFRS_ERR_DEVICE, "device %d beyond the %d this build addresses", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device])
+        return FFRS_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return fail(FFRS_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                    hipGetErrorString(e));
+    g_scratch_ptr[device] = ptr;
+    return FFRS_OK;
+}
+
+void *scratch_ptr(int device)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    return device >= 0 && device < kMaxDevices ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
+
 // ---------------------------------------------------------------- C ABI
 
 struct ffrs_rs256 {
@@ -2060,6 +2098,8 @@ int ffrs_rs256_create(ffrs_ctx *ctx, unsigned ecc_len, unsigned primitive, unsig
         log2[a] = rs->gf.log_t[a] % 255;
 
     int r = ctx_set_device(ctx);
+    if (!r)
+        r = resolve_scratch(ctx->device);
     if (r) {
         delete rs;
         return r;
@@ -2166,6 +2206,42 @@ int ffrs_rs256_repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg,
     if (int r = ctx_set_device(rs->ctx))
         return r;
     return repair_device(rs, message_len, d_msg, d_ecc, count, d_status);
+}
+
+int ffrs_rs256_repair_device_noalloc(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d_ecc,
+                                     size_t count, uint8_t *d_status)
+{
+    if (int r = check_codec(rs, message_len))
+        return r;
+    if (count && (!d_msg || !d_ecc || !d_status))
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
+    if (int r = ctx_set_device(rs->ctx))
+        return r;
+    void *work = nullptr;
+    if (count && uses_pipeline(rs)) {
+        const size_t per_launch = std::min(max_batch(message_len), count);
+        const size_t need = pipeline_work_bytes(rs->ecc, rs->EW, per_launch);
+        if (need > (size_t)FFRS_RS256_NOALLOC_SCRATCH_BYTES) {
+            size_t most = 0;  // the largest count whose workspace fits
+            for (size_t lo = 0, hi = per_launch; lo <= hi;) {
+                const size_t mid = lo + (hi - lo) / 2;
+                if (pipeline_work_bytes(rs->ecc, rs->EW, mid) <= (size_t)FFRS_RS256_NOALLOC_SCRATCH_BYTES) {
+                    most = mid;
+                    lo = mid + 1;
+                } else {
+                    hi = mid - 1;
+                }
+            }
+            return fail(FFRS_ERR_SIZE,
+                        "allocation-free repair: %zu codewords of ecc_len %u need %zu bytes of workspace, the "
+                        "static workspace holds %u (at most %zu codewords per call)",
+                        count, rs->ecc, need, (unsigned)FFRS_RS256_NOALLOC_SCRATCH_BYTES, most);
+        }
+        work = scratch_ptr(rs->ctx->device);
+        if (!work)
+            return fail(FFRS_ERR_DEVICE, "no workspace address for device %d", rs->ctx->device);
+    }
+    return repair_device(rs, message_len, d_msg, d_ecc, count, d_status, work);
 }
 
 int ffrs_rs256_encode(ffrs_rs256 *rs, size_t message_len, const uint8_t *buf, size_t size, uint8_t *out,
