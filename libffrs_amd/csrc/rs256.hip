@@ -900,6 +900,240 @@ __global__ __launch_bounds__(WG) void rs256_repair_kernel(uint8_t *__restrict__ 
                                        s_lut, gf, row);
 }
 
+// ======================================================================
+// Wave-cooperative repair, one codeword per wave, for ecc_len 33..252 (the same steps and
+// results as repair_one, reed_solomon.hpp:559-595, spread over the 64 lanes instead of run by
+// one):
+//   syndromes  S_i = sum_j c_j alpha^(i (n-1-j)), syndrome i by lane i % 64 (direct sums over
+//              the codeword c = message then parity, whose logs sit in LDS); all zero -> 1
+//   BM         lane l holds Lambda_i and x^m B (logs) for i = l + 64 c, c < NC; each step's
+//              discrepancy is a wave XOR reduction; terms of Lambda above E/2 and of x^m B
+//              above E/2 + 1 are dropped as in bm_fixed_one; L > E/2 -> 0
+//   Omega      Omega_k = sum_{i <= k} Lambda_i S_(k-i), k < L, by lane k
+//   Chien      positions p = lane + 64 q < n: Lambda(alpha^-p) == 0; roots != L -> 0
+//   Forney     at each root e = X div(Omega(X^-1), Lambda_odd(X^-1)), X = alpha^p, XORed
+//              into the message or parity byte n - 1 - p
+// ======================================================================
+
+// XOR of x over the 64 lanes, in every lane (row butterflies, then the four rows).
+__device__ inline uint32_t wave_xor(uint32_t x)
+{
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    x ^= (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false);  // row_mirror
+    return (uint32_t)(__builtin_amdgcn_readlane((int)x, 0) ^ __builtin_amdgcn_readlane((int)x, 16) ^
+                      __builtin_amdgcn_readlane((int)x, 32) ^ __builtin_amdgcn_readlane((int)x, 48));
+}
+
+// Per-wave LDS of repair_wave.
+struct WaveRows {
+    uint16_t lc[256];   // logs of the codeword bytes (ZLOG for 0)
+    uint8_t cw[256];    // codeword bytes
+    uint8_t sy[256];    // syndromes
+    uint16_t ls[256];   // their logs
+    uint16_t ll[128];   // logs of Lambda_0..L
+    uint16_t lo[128];   // logs of Omega_0..L-1
+};
+
+// NC Lambda coefficients and NS syndromes per lane (E/2 + 1 <= 64 NC, E <= 64 NS).
+template <int NC, int NS>
+__device__ inline uint8_t repair_wave(uint8_t *pm, uint8_t *pe, uint32_t k, uint32_t E, const GfLds &gf, WaveRows &w)
+{
+    const uint32_t lane = threadIdx.x & 63, n = k + E, H = E / 2;
+    // ---- codeword into LDS, with its logs
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t j = lane + 64 * q;
+        const uint32_t c = j < k ? pm[j] : j < n ? pe[j - k] : 0u;
+        w.cw[j] = (uint8_t)c;
+        w.lc[j] = (uint16_t)gf.log2[c];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ---- syndromes: lane l sums S_(l + 64 q) over the positions
+    uint32_t sy[NS], e[NS], st[NS];
+#pragma unroll
+    for (int q = 0; q < NS; ++q) {
+        const uint32_t i = lane + 64 * q;
+        sy[q] = 0;
+        st[q] = i % 255;
+        e[q] = (i * (n - 1)) % 255;  // exponent of alpha^i at position j = 0
+    }
+    for (uint32_t j = 0; j < n; ++j) {
+        const uint32_t l = w.lc[j];
+#pragma unroll
+        for (int q = 0; q < NS; ++q) {
+            sy[q] ^= gf.exp2[l + e[q]];
+            e[q] = e[q] >= st[q] ? e[q] - st[q] : e[q] + 255 - st[q];
+        }
+    }
+    uint32_t any = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t i = lane + 64 * q;
+        const uint32_t v = q < NS && i < E ? sy[q < NS ? q : 0] : 0u;
+        any |= v;
+        w.sy[i] = (uint8_t)v;
+        w.ls[i] = (uint16_t)gf.log2[v];
+    }
+    if (__ballot(any != 0) == 0)
+        return 1;  // all syndromes zero (reed_solomon.hpp:565-566)
+    __builtin_amdgcn_wave_barrier();
+
+    // ---- Berlekamp-Massey over lanes (logs of x^m B pre-shifted, as bm_fixed_one)
+    uint32_t lam[NC], llam[NC], lB[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const uint32_t i = lane + 64 * c;
+        lam[c] = i == 0 ? 1u : 0u;
+        llam[c] = i == 0 ? 0u : ZLOG;
+        lB[c] = i == 1 ? 0u : ZLOG;
+    }
+    uint32_t L = 0, lb = 0;
+    for (uint32_t r = 0; r < E; ++r) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const uint32_t i = lane + 64 * c;
+            const uint32_t ls = (i >= 1 && i <= r && i <= H) ? (uint32_t)w.ls[r - i] : (uint32_t)ZLOG;
+            t ^= gf.exp2[llam[c] + ls];
+        }
+        const uint32_t d = w.sy[r] ^ wave_xor(t);
+        uint32_t nlB[NC];
+        if (d == 0) {
+#pragma unroll
+            for (int c = 0; c < NC; ++c)
+                nlB[c] = lB[c];
+        } else {
+            const bool grow = 2 * L <= r;
+            if (grow && r + 1 - L > H)
+                return 0;  // L exceeds E / 2 from here on
+            const uint32_t ld = gf.log2[d];
+            uint32_t lc = ld + 255 - lb;
+            lc = lc >= 255 ? lc - 255 : lc;
+#pragma unroll
+            for (int c = 0; c < NC; ++c) {
+                const uint32_t i = lane + 64 * c;
+                const uint32_t old = llam[c];
+                if (i >= 1 && i <= H) {
+                    lam[c] ^= gf.exp2[lc + lB[c]];
+                    llam[c] = gf.log2[lam[c]];
+                }
+                nlB[c] = grow ? old : lB[c];
+            }
+            if (grow) {
+                L = r + 1 - L;
+                lb = ld;
+            }
+        }
+        // x^m B <- x (x^m B), or x Lambda_old on growth: one lane up, index 0 empty, terms
+        // above min(E/2 + 1, E - 1) dropped
+        const uint32_t top = H + 1 < E - 1 ? H + 1 : E - 1;
+#pragma unroll
+        for (int c = NC - 1; c >= 0; --c) {
+            uint32_t v = (uint32_t)__builtin_amdgcn_update_dpp((int)ZLOG, (int)nlB[c], 0x138, 0xF, 0xF, false);  // wave_shr:1
+            if (c > 0) {
+                const uint32_t carry = (uint32_t)__builtin_amdgcn_readlane((int)nlB[c - 1], 63);
+                v = lane == 0 ? carry : v;
+            } else {
+                v = lane == 0 ? (uint32_t)ZLOG : v;
+            }
+            const uint32_t i = lane + 64 * c;
+            lB[c] = i <= top ? v : (uint32_t)ZLOG;
+        }
+    }
+    if (L > H)
+        return 0;
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const uint32_t i = lane + 64 * c;
+        if (i <= L)
+            w.ll[i] = (uint16_t)llam[c];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ---- Omega_k, k < L (reed_solomon.hpp:708-728), logs into LDS
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const uint32_t kk = lane + 64 * c;
+        uint32_t v = 0;
+        for (uint32_t i = 0; i < L; ++i)
+            if (i <= kk)
+                v ^= gf.exp2[(uint32_t)w.ll[i] + w.ls[kk - i]];
+        if (kk < L)
+            w.lo[kk] = (uint16_t)gf.log2[v];
+    }
+    __builtin_amdgcn_wave_barrier();
+    // ---- Chien over p = lane + 64 q < n (reed_solomon.hpp:516-538)
+    uint32_t zero = 0;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t p = lane + 64 * q;
+        const uint32_t step = p ? 255 - p % 255 : 0;
+        uint32_t acc = 0, ex = 0;
+        for (uint32_t i = 0; i <= L; ++i) {
+            acc ^= gf.exp2[(uint32_t)w.ll[i] + ex];
+            ex += step;
+            ex = ex >= 255 ? ex - 255 : ex;
+        }
+        zero |= (acc == 0 && p < n) ? 1u << q : 0u;
+    }
+    const uint32_t roots = (uint32_t)__builtin_popcountll(__ballot(zero & 1)) +
+                           (uint32_t)__builtin_popcountll(__ballot(zero & 2)) +
+                           (uint32_t)__builtin_popcountll(__ballot(zero & 4)) +
+                           (uint32_t)__builtin_popcountll(__ballot(zero & 8));
+    if (roots != L)
+        return 0;
+    // ---- Forney at the roots (reed_solomon.hpp:730-743)
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (zero & (1u << q)) {
+            const uint32_t p = lane + 64 * q;
+            const uint32_t pinv = p ? 255 - p : 0;
+            uint32_t num = 0, den = 0, ex = 0;
+            for (uint32_t kk = 0; kk < L; ++kk) {
+                num ^= gf.exp2[(uint32_t)w.lo[kk] + ex];
+                if ((kk & 1) == 0)
+                    den ^= gf.exp2[(uint32_t)w.ll[kk + 1] + ex];
+                ex += pinv;
+                ex = ex >= 255 ? ex - 255 : ex;
+            }
+            const uint32_t mag = gf.mul(gf.exp2[p], gf.div(num, den));
+            const uint32_t at = n - 1 - p;
+            const uint32_t v = (uint32_t)w.cw[at] ^ mag;
+            if (at < k)
+                pm[at] = (uint8_t)v;
+            else
+                pe[at - k] = (uint8_t)v;
+        }
+    }
+    return 1;
+}
+
+// One codeword per wave over `count` codewords of message length k (rows of k and E bytes).
+template <int NC, int NS, int WG>
+__global__ __launch_bounds__(WG) void rs256_repair_wave_kernel(uint8_t *__restrict__ msgs, uint8_t *__restrict__ eccs,
+                                                                uint32_t count, uint32_t k, uint32_t E,
+                                                                uint8_t *__restrict__ status, Rs256Tables tab)
+{
+    __shared__ uint8_t s_exp2[EXP2_LEN];
+    __shared__ uint16_t s_log2[256];
+    __shared__ WaveRows s_rows[WG / 64];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += WG)
+        s_exp2[i] = tab.exp2[i];
+    for (uint32_t i = threadIdx.x; i < 256u; i += WG)
+        s_log2[i] = tab.log2[i];
+    __syncthreads();
+    const GfLds gf{s_exp2, s_log2};
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    WaveRows &w = s_rows[wave];
+    for (uint32_t cw = blockIdx.x * (WG / 64) + wave; cw < count; cw += gridDim.x * (WG / 64)) {
+        const uint8_t st = repair_wave<NC, NS>(msgs + (size_t)cw * k, eccs + (size_t)cw * E, k, E, gf, w);
+        if ((threadIdx.x & 63) == 0)
+            status[cw] = st;
+        __builtin_amdgcn_wave_barrier();
+    }
+}
+
 template <int EW, int S>
 __global__ __launch_bounds__(256) void rs256_synds_kernel(const uint8_t *__restrict__ msgs, uint32_t msg_bytes,
                                                            const uint8_t *__restrict__ eccs, uint32_t count,
@@ -1553,7 +1787,8 @@ __global__ __launch_bounds__(WG) void rs256_plan_kernel(uint8_t *__restrict__ ms
                                                          const PlanItem *__restrict__ items,
                                                          const PlanChunk *__restrict__ chunks, uint32_t nchunks,
                                                          const PlanClass *__restrict__ classes, uint32_t *queue,
-                                                         uint32_t lut_words, uint32_t row_stride, Rs256Tables tab)
+                                                         uint32_t chunk0, uint32_t lut_words, uint32_t row_stride,
+                                                         Rs256Tables tab)
 {
     extern __shared__ uint32_t s_mem[];
     __shared__ uint32_t s_chunk;
@@ -1574,7 +1809,7 @@ __global__ __launch_bounds__(WG) void rs256_plan_kernel(uint8_t *__restrict__ ms
     uint32_t loaded = 0xffffffffu;
     for (;;) {
         if (threadIdx.x == 0)
-            s_chunk = atomicAdd(queue, 1u);
+            s_chunk = chunk0 + atomicAdd(queue, 1u);
         __syncthreads();
         const uint32_t c = s_chunk;
         if (c >= nchunks)
@@ -1607,6 +1842,39 @@ __global__ __launch_bounds__(WG) void rs256_plan_kernel(uint8_t *__restrict__ ms
             }
         }
         __syncthreads();
+    }
+}
+
+// Repair of the plan items [0, nitems) (ecc_len > 32), one per wave (repair_wave).
+template <int WG>
+__global__ __launch_bounds__(WG) void rs256_plan_wave_kernel(uint8_t *__restrict__ msgs, uint8_t *__restrict__ eccs,
+                                                              uint8_t *__restrict__ status,
+                                                              const PlanItem *__restrict__ items, uint32_t nitems,
+                                                              Rs256Tables tab)
+{
+    __shared__ uint8_t s_exp2[EXP2_LEN];
+    __shared__ uint16_t s_log2[256];
+    __shared__ WaveRows s_rows[WG / 64];
+    for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += WG)
+        s_exp2[i] = tab.exp2[i];
+    for (uint32_t i = threadIdx.x; i < 256u; i += WG)
+        s_log2[i] = tab.log2[i];
+    __syncthreads();
+    const GfLds gf{s_exp2, s_log2};
+    const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    WaveRows &w = s_rows[wave];
+    for (uint32_t j = blockIdx.x * (WG / 64) + wave; j < nitems; j += gridDim.x * (WG / 64)) {
+        const PlanItem it = items[j];
+        const uint32_t E = it.E, k = it.k;
+        uint8_t *pm = msgs + it.msg_off, *pe = eccs + it.ecc_off;
+        const uint8_t st = E <= 64    ? repair_wave<1, 1>(pm, pe, k, E, gf, w)
+                           : E <= 124 ? repair_wave<1, 2>(pm, pe, k, E, gf, w)
+                           : E <= 128 ? repair_wave<2, 2>(pm, pe, k, E, gf, w)
+                           : E <= 192 ? repair_wave<2, 3>(pm, pe, k, E, gf, w)
+                                      : repair_wave<2, 4>(pm, pe, k, E, gf, w);
+        if ((threadIdx.x & 63) == 0)
+            status[it.idx] = st;
+        __builtin_amdgcn_wave_barrier();
     }
 }
 
@@ -1812,6 +2080,32 @@ int launch_repair(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_t count, 
     return FFRS_OK;
 }
 
+// The wave-cooperative repair (ecc_len 33..254).
+int launch_repair_wave(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_t count, uint32_t k, uint8_t *d_status)
+{
+    ffrs_ctx *ctx = rs->ctx;
+    const uint32_t E = rs->ecc;
+    constexpr int WG = 256;
+    Rs256Tables tab{rs->d_lut, rs->d_exp2, rs->d_log2};
+    auto go = [&](auto kern) -> int {
+        const int grid = persistent_grid(ctx, 8, (count + WG / 64 - 1) / (WG / 64));
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), 0, ctx->stream, d_msg, d_ecc, (uint32_t)count, k, E, d_status,
+                           tab);
+        FFRS_HIP(hipGetLastError());
+        mark_kernel(ctx);
+        return FFRS_OK;
+    };
+    if (E <= 64)
+        return go(rs256_repair_wave_kernel<1, 1, WG>);
+    if (E <= 124)
+        return go(rs256_repair_wave_kernel<1, 2, WG>);
+    if (E <= 128)
+        return go(rs256_repair_wave_kernel<2, 2, WG>);
+    if (E <= 192)
+        return go(rs256_repair_wave_kernel<2, 3, WG>);
+    return go(rs256_repair_wave_kernel<2, 4, WG>);
+}
+
 template <int EW, int S>
 int launch_synds(ffrs_rs256 *rs, const uint8_t *d_msg, const uint8_t *d_ecc, size_t count, uint32_t k,
                  uint8_t *d_out)
@@ -1963,6 +2257,10 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     return FFRS_OK;
 }
 
+// The replicated tables (128 KB per workgroup to load) for launches of at least 8,192
+// codewords; smaller ones take the slicing-by-S tables.
+bool use_rep(const ffrs_rs256 *rs, size_t count) { return rs->d_lutr && count >= 8192; }
+
 // Splits launches so that every launch addresses < 2 GiB of messages with 32-bit offsets.
 size_t max_batch(size_t k) { return std::max<size_t>(1, ((size_t)1 << 31) / std::max<size_t>(k, 1)) & ~(size_t)255; }
 
@@ -1978,11 +2276,12 @@ int encode_device(ffrs_rs256 *rs, size_t message_len, const uint8_t *d_msg, size
         const uint8_t *m = d_msg + b * message_len;
         uint8_t *e = d_ecc + b * rs->ecc;
         int r;
+        const bool rep = use_rep(rs, c);
         switch (rs->EW) {
-        case 1: r = rs->d_lutr ? launch_encode<1, LUT_REP>(rs, m, c, k, e) : launch_encode<1, 4>(rs, m, c, k, e); break;
-        case 2: r = rs->d_lutr ? launch_encode<2, LUT_REP>(rs, m, c, k, e) : launch_encode<2, 4>(rs, m, c, k, e); break;
-        case 4: r = rs->d_lutr ? launch_encode<4, LUT_REP>(rs, m, c, k, e) : launch_encode<4, 4>(rs, m, c, k, e); break;
-        case 8: r = rs->d_lutr ? launch_encode<8, LUT_REP>(rs, m, c, k, e) : launch_encode<8, 4>(rs, m, c, k, e); break;
+        case 1: r = rep ? launch_encode<1, LUT_REP>(rs, m, c, k, e) : launch_encode<1, 4>(rs, m, c, k, e); break;
+        case 2: r = rep ? launch_encode<2, LUT_REP>(rs, m, c, k, e) : launch_encode<2, 4>(rs, m, c, k, e); break;
+        case 4: r = rep ? launch_encode<4, LUT_REP>(rs, m, c, k, e) : launch_encode<4, 4>(rs, m, c, k, e); break;
+        case 8: r = rep ? launch_encode<8, LUT_REP>(rs, m, c, k, e) : launch_encode<8, 4>(rs, m, c, k, e); break;
         case 16: r = launch_encode<16, 2>(rs, m, c, k, e); break;
         case 32: r = launch_encode<32, 1>(rs, m, c, k, e); break;
         default: r = launch_encode<64, 1>(rs, m, c, k, e); break;
@@ -2022,17 +2321,29 @@ int repair_device(ffrs_rs256 *rs, size_t message_len, uint8_t *d_msg, uint8_t *d
         uint8_t *st = d_status + b;
         int r;
         switch (rs->EW) {
-        case 1: r = pipe ? launch_repair_pipeline<1, LUT_REP>(rs, m, e, c, k, st, work) : launch_repair<1, 4, 256>(rs, m, e, c, k, st); break;
-        case 2: r = pipe ? launch_repair_pipeline<2, LUT_REP>(rs, m, e, c, k, st, work) : launch_repair<2, 4, 256>(rs, m, e, c, k, st); break;
-        case 4: r = pipe ? launch_repair_pipeline<4, LUT_REP>(rs, m, e, c, k, st, work) : launch_repair<4, 4, 256>(rs, m, e, c, k, st); break;
+        case 1:
+            r = !pipe ? launch_repair<1, 4, 256>(rs, m, e, c, k, st)
+                : use_rep(rs, c) ? launch_repair_pipeline<1, LUT_REP>(rs, m, e, c, k, st, work)
+                : launch_repair_pipeline<1, 4>(rs, m, e, c, k, st, work);
+            break;
+        case 2:
+            r = !pipe ? launch_repair<2, 4, 256>(rs, m, e, c, k, st)
+                : use_rep(rs, c) ? launch_repair_pipeline<2, LUT_REP>(rs, m, e, c, k, st, work)
+                : launch_repair_pipeline<2, 4>(rs, m, e, c, k, st, work);
+            break;
+        case 4:
+            r = !pipe ? launch_repair<4, 4, 256>(rs, m, e, c, k, st)
+                : use_rep(rs, c) ? launch_repair_pipeline<4, LUT_REP>(rs, m, e, c, k, st, work)
+                : launch_repair_pipeline<4, 4>(rs, m, e, c, k, st, work);
+            break;
         case 8:
             r = !pipe ? launch_repair<8, 4, 128>(rs, m, e, c, k, st)
-                : rs->d_lutr ? launch_repair_pipeline<8, LUT_REP>(rs, m, e, c, k, st, work)
+                : use_rep(rs, c) ? launch_repair_pipeline<8, LUT_REP>(rs, m, e, c, k, st, work)
                 : launch_repair_pipeline<8, 4>(rs, m, e, c, k, st, work);
             break;
-        case 16: r = launch_repair<16, 2, 128>(rs, m, e, c, k, st); break;
-        case 32: r = launch_repair<32, 1, 64>(rs, m, e, c, k, st); break;
-        default: r = launch_repair<64, 1, 64>(rs, m, e, c, k, st); break;
+        case 16: r = rs->repair_mode == 2 ? launch_repair<16, 2, 128>(rs, m, e, c, k, st) : launch_repair_wave(rs, m, e, c, k, st); break;
+        case 32: r = rs->repair_mode == 2 ? launch_repair<32, 1, 64>(rs, m, e, c, k, st) : launch_repair_wave(rs, m, e, c, k, st); break;
+        default: r = rs->repair_mode == 2 ? launch_repair<64, 1, 64>(rs, m, e, c, k, st) : launch_repair_wave(rs, m, e, c, k, st); break;
         }
         if (r)
             return r;
@@ -2367,6 +2678,8 @@ struct ffrs_rs256_plan {
     PlanClass *d_classes = nullptr;
     uint32_t *d_queue = nullptr;
     uint32_t count = 0, nchunks = 0, max_ew = 1, max_e = 1, wg = 256, lut_words = 0;
+    // repair: items [0, n_wave) (ecc_len > 32) one per wave, chunks from chunk0 one per lane
+    uint32_t n_wave = 0, chunk0 = 0, lane_max_ew = 1, lane_max_e = 1, lane_lut_words = 0;
     uint64_t msg_end = 0, ecc_end = 0;
     bool use_graph = false;
     struct Graph {
@@ -2399,28 +2712,46 @@ int plan_launch(ffrs_rs256_plan *p, uint8_t *msgs, size_t msg_bytes, uint8_t *ec
     ffrs_ctx *ctx = p->ctx;
     const ffrs_rs256 *c0 = p->codecs[0];
     Rs256Tables tab{nullptr, c0->d_exp2, c0->d_log2};
-    uint32_t row_stride = REPAIR ? repair_row_stride(p->max_e) : 0;
-    size_t lds = (size_t)p->lut_words * 4 + (REPAIR ? EXP2_LEN + 512 + (size_t)WG * row_stride : 0);
+    // repair runs the lane kernel over the chunks of ecc_len <= 32 only
+    const uint32_t chunk0 = REPAIR ? p->chunk0 : 0;
+    const uint32_t max_e = REPAIR ? p->lane_max_e : p->max_e;
+    const uint32_t lut_words = REPAIR ? p->lane_lut_words : p->lut_words;
+    if (chunk0 >= p->nchunks)
+        return FFRS_OK;
+    uint32_t row_stride = REPAIR ? repair_row_stride(max_e) : 0;
+    size_t lds = (size_t)lut_words * 4 + (REPAIR ? EXP2_LEN + 512 + (size_t)WG * row_stride : 0);
     if (lds > 160 * 1024)
         return fail(FFRS_ERR_INVALID, "plan workspace of %zu bytes exceeds LDS", lds);
     auto kern = rs256_plan_kernel<MAXEW, WG, REPAIR>;
     if (int r = allow_lds(kern, lds))
         return r;
     int per_cu = (int)std::max<size_t>(1, (160 * 1024) / std::max<size_t>(lds, 1));
-    int grid = persistent_grid(ctx, std::min(per_cu, 2048 / WG), p->nchunks);
+    int grid = persistent_grid(ctx, std::min(per_cu, 2048 / WG), p->nchunks - chunk0);
     hipLaunchKernelGGL(plan_reset_kernel, dim3(1), dim3(1), 0, ctx->stream, p->d_queue);
     hipLaunchKernelGGL(kern, dim3(grid), dim3(WG), lds, ctx->stream, msgs, (uint32_t)msg_bytes, eccs, status,
-                       p->d_items, p->d_chunks, p->nchunks, p->d_classes, p->d_queue, p->lut_words, row_stride, tab);
+                       p->d_items, p->d_chunks, p->nchunks, p->d_classes, p->d_queue, chunk0, lut_words, row_stride,
+                       tab);
     FFRS_HIP(hipGetLastError());
     return FFRS_OK;
 }
 
 int plan_enqueue(ffrs_rs256_plan *p, int mode, uint8_t *msgs, size_t msg_bytes, uint8_t *eccs, uint8_t *status)
 {
-    bool small = p->max_ew <= 16;
-    if (mode == 0)
+    if (mode == 0) {
+        bool small = p->max_ew <= 16;
         return small ? plan_launch<16, 256, false>(p, msgs, msg_bytes, eccs, status)
                      : plan_launch<64, 64, false>(p, msgs, msg_bytes, eccs, status);
+    }
+    if (p->n_wave) {
+        constexpr int WG = 256;
+        const ffrs_rs256 *c0 = p->codecs[0];
+        Rs256Tables tab{nullptr, c0->d_exp2, c0->d_log2};
+        int grid = persistent_grid(p->ctx, 8, (p->n_wave + WG / 64 - 1) / (WG / 64));
+        hipLaunchKernelGGL((rs256_plan_wave_kernel<WG>), dim3(grid), dim3(WG), 0, p->ctx->stream, msgs, eccs, status,
+                           p->d_items, p->n_wave, tab);
+        FFRS_HIP(hipGetLastError());
+    }
+    bool small = p->lane_max_ew <= 16;
     return small ? plan_launch<16, 256, true>(p, msgs, msg_bytes, eccs, status)
                  : plan_launch<64, 64, true>(p, msgs, msg_bytes, eccs, status);
 }
@@ -2507,6 +2838,11 @@ int ffrs_rs256_plan_create(ffrs_ctx *ctx, size_t count, const uint8_t *ecc_len, 
         p->max_ew = std::max(p->max_ew, c->EW);
         p->max_e = std::max(p->max_e, e);
         p->lut_words = std::max(p->lut_words, c->S * 256u * (uint32_t)lut_row((int)c->EW));
+        if (e <= 32) {
+            p->lane_max_ew = std::max(p->lane_max_ew, c->EW);
+            p->lane_max_e = std::max(p->lane_max_e, e);
+            p->lane_lut_words = std::max(p->lane_lut_words, c->S * 256u * (uint32_t)lut_row((int)c->EW));
+        }
     }
     p->wg = p->max_ew <= 16 ? 256 : 64;
     std::vector<uint32_t> order(count);
@@ -2531,6 +2867,11 @@ int ffrs_rs256_plan_create(ffrs_ctx *ctx, size_t count, const uint8_t *ecc_len, 
         chunks.back().count++;
     }
     p->nchunks = (uint32_t)chunks.size();
+    // the items of ecc_len > 32 lead the sorted list (descending ecc_len), in chunks of their own
+    while (p->n_wave < count && items[p->n_wave].E > 32)
+        p->n_wave++;
+    while (p->chunk0 < p->nchunks && chunks[p->chunk0].first < p->n_wave)
+        p->chunk0++;
     hipError_t e = hipSuccess;
     auto up = [&](auto **dst, const auto &v) {
         size_t bytes = std::max<size_t>(1, v.size() * sizeof(v[0]));
