@@ -926,8 +926,8 @@ __device__ inline uint32_t wave_xor(uint32_t x)
                       __builtin_amdgcn_readlane((int)x, 32) ^ __builtin_amdgcn_readlane((int)x, 48));
 }
 
-// Per-wave LDS of repair_wave.
-struct WaveRows {
+// Per-wave LDS of repair_wave (16-byte aligned: the logs are read eight and four at a time).
+struct alignas(16) WaveRows {
     uint16_t lc[256];   // logs of the codeword bytes (ZLOG for 0)
     uint8_t cw[256];    // codeword bytes
     uint8_t sy[256];    // syndromes
@@ -959,12 +959,22 @@ __device__ inline uint8_t repair_wave(uint8_t *pm, uint8_t *pe, uint32_t k, uint
         st[q] = i % 255;
         e[q] = (i * (n - 1)) % 255;  // exponent of alpha^i at position j = 0
     }
-    for (uint32_t j = 0; j < n; ++j) {
-        const uint32_t l = w.lc[j];
+    // eight positions per pass: their logs in one 16-byte read (positions past n hold ZLOG
+    // logs of zero bytes, which add nothing), then the eight lookups of each syndrome together
+    for (uint32_t j0 = 0; j0 < n; j0 += 8) {
+        const uint4 l8 = *reinterpret_cast<const uint4 *>(&w.lc[j0]);
+        const uint32_t lw[4] = {l8.x, l8.y, l8.z, l8.w};
 #pragma unroll
         for (int q = 0; q < NS; ++q) {
-            sy[q] ^= gf.exp2[l + e[q]];
-            e[q] = e[q] >= st[q] ? e[q] - st[q] : e[q] + 255 - st[q];
+            uint32_t t[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                t[u] = gf.exp2[((lw[u >> 1] >> (16 * (u & 1))) & 0xffffu) + e[q]];
+                e[q] = e[q] >= st[q] ? e[q] - st[q] : e[q] + 255 - st[q];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                sy[q] ^= t[u];
         }
     }
     uint32_t any = 0;
@@ -990,13 +1000,26 @@ __device__ inline uint8_t repair_wave(uint8_t *pm, uint8_t *pe, uint32_t k, uint
         lB[c] = i == 1 ? 0u : ZLOG;
     }
     uint32_t L = 0, lb = 0;
+    // lsr[c] = log S_(r - i) for i = lane + 64 c (ZLOG for i > r): one lane up per step
+    uint32_t lsr[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c)
+        lsr[c] = lane == 0 && c == 0 ? (uint32_t)w.ls[0] : (uint32_t)ZLOG;
     for (uint32_t r = 0; r < E; ++r) {
+        if (r > 0) {
+            const uint32_t head = w.ls[r];
+#pragma unroll
+            for (int c = NC - 1; c >= 0; --c) {
+                uint32_t v = (uint32_t)__builtin_amdgcn_update_dpp((int)ZLOG, (int)lsr[c], 0x138, 0xF, 0xF, false);
+                const uint32_t in = c > 0 ? (uint32_t)__builtin_amdgcn_readlane((int)lsr[c - 1], 63) : head;
+                lsr[c] = lane == 0 ? in : v;
+            }
+        }
         uint32_t t = 0;
 #pragma unroll
         for (int c = 0; c < NC; ++c) {
             const uint32_t i = lane + 64 * c;
-            const uint32_t ls = (i >= 1 && i <= r && i <= H) ? (uint32_t)w.ls[r - i] : (uint32_t)ZLOG;
-            t ^= gf.exp2[llam[c] + ls];
+            t ^= gf.exp2[llam[c] + ((i >= 1 && i <= H) ? lsr[c] : (uint32_t)ZLOG)];
         }
         const uint32_t d = w.sy[r] ^ wave_xor(t);
         uint32_t nlB[NC];
@@ -1064,19 +1087,35 @@ __device__ inline uint8_t repair_wave(uint8_t *pm, uint8_t *pe, uint32_t k, uint
     }
     __builtin_amdgcn_wave_barrier();
     // ---- Chien over p = lane + 64 q < n (reed_solomon.hpp:516-538)
-    uint32_t zero = 0;
+    // Lambda_(L+1) .. Lambda_(L+3) read as zero (ZLOG) so that terms go four at a time
+#pragma unroll
+    for (int c = 0; c < NC; ++c) {
+        const uint32_t i = lane + 64 * c;
+        if (i > L && i <= L + 3 && i < 128)
+            w.ll[i] = (uint16_t)ZLOG;
+    }
+    __builtin_amdgcn_wave_barrier();
+    uint32_t zero = 0, acc[4] = {0, 0, 0, 0}, ex[4] = {0, 0, 0, 0}, step[4];
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
         const uint32_t p = lane + 64 * q;
-        const uint32_t step = p ? 255 - p % 255 : 0;
-        uint32_t acc = 0, ex = 0;
-        for (uint32_t i = 0; i <= L; ++i) {
-            acc ^= gf.exp2[(uint32_t)w.ll[i] + ex];
-            ex += step;
-            ex = ex >= 255 ? ex - 255 : ex;
-        }
-        zero |= (acc == 0 && p < n) ? 1u << q : 0u;
+        step[q] = p ? 255 - p % 255 : 0;
     }
+    for (uint32_t i0 = 0; i0 <= L; i0 += 4) {
+        const uint2 l4 = *reinterpret_cast<const uint2 *>(&w.ll[i0]);
+        const uint32_t li[4] = {l4.x & 0xffffu, l4.x >> 16, l4.y & 0xffffu, l4.y >> 16};
+#pragma unroll
+        for (int t = 0; t < 4; ++t)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                acc[q] ^= gf.exp2[li[t] + ex[q]];
+                ex[q] += step[q];
+                ex[q] = ex[q] >= 255 ? ex[q] - 255 : ex[q];
+            }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q)
+        zero |= (acc[q] == 0 && lane + 64 * q < n) ? 1u << q : 0u;
     const uint32_t roots = (uint32_t)__builtin_popcountll(__ballot(zero & 1)) +
                            (uint32_t)__builtin_popcountll(__ballot(zero & 2)) +
                            (uint32_t)__builtin_popcountll(__ballot(zero & 4)) +
