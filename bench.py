@@ -425,6 +425,14 @@ def run_mixed(args):
     ctx = capi.Context(0)
     plan = capi.RS256Plan(ctx, eccs, lens, moff, eoff, graph=not args.no_graph)
     plan_i = capi.RSi16Plan(ctx, ns_, eccs_i, moff_i, eoff_i, graph=not args.no_graph)
+    tuned = {}
+    if not args.no_tune:
+        # per-class kernel shapes from the autotuner (its cache, tuning the classes it lacks)
+        from libffrs_amd import autotune
+        t1 = time.perf_counter()
+        tuned = {"rs256": {str(e): r["mode"] for e, r in autotune.apply_plan(plan).items()},
+                 "rsi16": {f"{n},{e}": [r["l_encode"], r["g_repair"]] for (n, e), r in autotune.apply_plan(plan_i).items()}}
+        log(f"plan shapes from the autotuner in {time.perf_counter() - t1:.1f} s: {tuned}")
     d_m, d_p = ctx.alloc(hm.nbytes), ctx.alloc(int(eccs.sum()))
     d_mi, d_pi = ctx.alloc(hm_i.nbytes), ctx.alloc(int(eccs_i.sum()) * 2)
     d_m.upload(hm)
@@ -611,6 +619,7 @@ def run_mixed(args):
         "repair_speedup_vs_plain_graph": round(pgr[1] / fr[1], 2) if not args.no_graph else None,
         "encode_speedup_vs_plain_graph": round(pge[1] / fe[1], 2) if not args.no_graph else None,
         "graphs": not args.no_graph,
+        "tuned_shapes": tuned or None,
         "identical_results": same,
         "status_counts": {"rs256": {int(v): int(c) for v, c in zip(*np.unique(st_f, return_counts=True))},
                           "rsi16": {int(v): int(c) for v, c in zip(*np.unique(sti_f, return_counts=True))}},
@@ -779,6 +788,7 @@ def main():
     ap.add_argument("--mixed", action="store_true", help="run only the mixed-size batch and print its result")
     ap.add_argument("--no-mixed", action="store_true", help="leave the mixed-size batch out of the bench line")
     ap.add_argument("--no-graph", action="store_true", help="mixed batch without HIP graphs (for tracing tools)")
+    ap.add_argument("--no-tune", action="store_true", help="mixed batch with the plans' default shapes")
     ap.add_argument("--serving", action="store_true", help="run only the serving loop and print its result")
     ap.add_argument("--no-serving", action="store_true", help="leave the serving loop out of the bench line")
     ap.add_argument("--serve-requests", type=int, default=300)

@@ -229,6 +229,11 @@ int ffrs_rs256_plan_create(ffrs_ctx *ctx, size_t count, const uint8_t *ecc_len, 
                            const uint64_t *msg_off, const uint64_t *ecc_off, ffrs_rs256_plan **out);
 void ffrs_rs256_plan_destroy(ffrs_rs256_plan *plan);
 int ffrs_rs256_plan_use_graph(ffrs_rs256_plan *plan, int enable);
+/* Repair kernel of the plan's codewords of one ecc_len (no counterpart in the reference; results
+ * do not depend on it): 0 automatic (one codeword per wave above ecc_len 32, else one per lane),
+ * 1 one codeword per lane, 2 one codeword per wave. Lays the plan out again (graphs are
+ * recaptured on the next run). libffrs_amd.autotune picks it by timing. */
+int ffrs_rs256_plan_set_class_mode(ffrs_rs256_plan *plan, unsigned ecc_len, int mode);
 int ffrs_rs256_plan_encode(ffrs_rs256_plan *plan, const uint8_t *d_msgs, size_t msg_bytes, uint8_t *d_eccs,
                            size_t ecc_bytes);
 int ffrs_rs256_plan_repair(ffrs_rs256_plan *plan, uint8_t *d_msgs, size_t msg_bytes, uint8_t *d_eccs,
@@ -242,6 +247,13 @@ int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint32_t *block_le
                            ffrs_rsi16_plan **out);
 void ffrs_rsi16_plan_destroy(ffrs_rsi16_plan *plan);
 int ffrs_rsi16_plan_use_graph(ffrs_rsi16_plan *plan, int enable);
+/* Shape of the plan's RSi16(block_len, ecc_len) codewords (no counterpart in the reference;
+ * results do not depend on it): l_encode lanes per codeword for the encode of ecc_len <= 32 (0:
+ * unchanged), g_repair lanes per codeword for the repair (0 or min(64, ecc_len/2): the register
+ * code; other powers of two: the LDS code). Lays the plan out again. libffrs_amd.autotune picks
+ * them by timing. */
+int ffrs_rsi16_plan_set_class_groups(ffrs_rsi16_plan *plan, uint32_t block_len, uint32_t ecc_len, unsigned l_encode,
+                                     unsigned g_repair);
 int ffrs_rsi16_plan_encode(ffrs_rsi16_plan *plan, const uint16_t *d_msgs, size_t msg_elems, uint16_t *d_eccs,
                            size_t ecc_elems, int vector_lanes);
 int ffrs_rsi16_plan_repair(ffrs_rsi16_plan *plan, uint16_t *d_msgs, size_t msg_elems, uint16_t *d_eccs,

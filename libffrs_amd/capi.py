@@ -79,11 +79,13 @@ SIGNATURES = [
     ("ffrs_rs256_plan_create", c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
     ("ffrs_rs256_plan_destroy", None, [c_vp]),
     ("ffrs_rs256_plan_use_graph", c_int, [c_vp, c_int]),
+    ("ffrs_rs256_plan_set_class_mode", c_int, [c_vp, ctypes.c_uint, c_int]),
     ("ffrs_rs256_plan_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz]),
     ("ffrs_rs256_plan_repair", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_vp]),
     ("ffrs_rsi16_plan_create", c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint32, ctypes.POINTER(c_vp)]),
     ("ffrs_rsi16_plan_destroy", None, [c_vp]),
     ("ffrs_rsi16_plan_use_graph", c_int, [c_vp, c_int]),
+    ("ffrs_rsi16_plan_set_class_groups", c_int, [c_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint, ctypes.c_uint]),
     ("ffrs_rsi16_plan_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_int]),
     ("ffrs_rsi16_plan_repair", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_vp]),
 ]
@@ -323,6 +325,21 @@ class RS256Plan:
         check(lib().ffrs_rs256_plan_repair(self.h, d_msgs.ptr, d_msgs.nbytes, d_eccs.ptr, d_eccs.nbytes,
                                            d_status.ptr), "rs256 plan repair")
 
+    def classes(self):
+        """The distinct ecc_len of the plan's codewords."""
+        import numpy as np
+        return sorted(int(e) for e in np.unique(self._keep[0]))
+
+    def class_sizes(self):
+        """{ecc_len: number of codewords}."""
+        import numpy as np
+        e, c = np.unique(self._keep[0], return_counts=True)
+        return {int(a): int(b) for a, b in zip(e, c)}
+
+    def set_class_mode(self, ecc_len, mode):
+        """Repair kernel of the codewords of one ecc_len: 0 automatic, 1 one per lane, 2 one per wave."""
+        check(lib().ffrs_rs256_plan_set_class_mode(self.h, int(ecc_len), int(mode)), "set_class_mode")
+
 
 class RSi16Plan:
     """A mixed-size RSi16 work list (per-codeword block_len, ecc_len and u16 offsets)."""
@@ -350,3 +367,18 @@ class RSi16Plan:
     def repair(self, d_msgs, d_eccs, d_status):
         check(lib().ffrs_rsi16_plan_repair(self.h, d_msgs.ptr, d_msgs.nbytes // 2, d_eccs.ptr, d_eccs.nbytes // 2,
                                            d_status.ptr), "rsi16 plan repair")
+
+    def classes(self):
+        """The distinct (block_len, ecc_len) of the plan's codewords."""
+        return sorted(self.class_sizes())
+
+    def class_sizes(self):
+        """{(block_len, ecc_len): number of codewords}."""
+        import collections
+        return dict(collections.Counter(zip((int(x) for x in self._keep[0]), (int(x) for x in self._keep[1]))))
+
+    def set_class_groups(self, block_len, ecc_len, l_encode=0, g_repair=0):
+        """Lanes per codeword of one class: l_encode for the encode (ecc_len <= 32; 0 keeps it),
+        g_repair for the repair (0: the register code)."""
+        check(lib().ffrs_rsi16_plan_set_class_groups(self.h, int(block_len), int(ecc_len), int(l_encode),
+                                                     int(g_repair)), "set_class_groups")

@@ -2717,9 +2717,14 @@ struct ffrs_rs256_plan {
     PlanClass *d_classes = nullptr;
     uint32_t *d_queue = nullptr;
     uint32_t count = 0, nchunks = 0, max_ew = 1, max_e = 1, wg = 256, lut_words = 0;
-    // repair: items [0, n_wave) (ecc_len > 32) one per wave, chunks from chunk0 one per lane
+    // repair: items [0, n_wave) one per wave, chunks from chunk0 one per lane
     uint32_t n_wave = 0, chunk0 = 0, lane_max_ew = 1, lane_max_e = 1, lane_lut_words = 0;
     uint64_t msg_end = 0, ecc_end = 0;
+    // the caller's list (in its order) and the repair kernel of each class: 0 automatic (one
+    // codeword per wave for ecc_len > 32, else per lane), 1 per lane, 2 per wave
+    std::vector<PlanItem> list;
+    int cls_of[256];
+    int mode_of[256] = {0};
     bool use_graph = false;
     struct Graph {
         int mode;
@@ -2793,6 +2798,64 @@ int plan_enqueue(ffrs_rs256_plan *p, int mode, uint8_t *msgs, size_t msg_bytes, 
     bool small = p->lane_max_ew <= 16;
     return small ? plan_launch<16, 256, true>(p, msgs, msg_bytes, eccs, status)
                  : plan_launch<64, 64, true>(p, msgs, msg_bytes, eccs, status);
+}
+
+// Whether the items of class ecc_len E are repaired one per wave.
+bool plan_wave(const ffrs_rs256_plan *p, unsigned E)
+{
+    const int m = p->mode_of[E];
+    return m == 2 || (m == 0 && E > 32);
+}
+
+// Sorts the work list (per-wave classes first, then per-lane ones; costliest first within each:
+// largest ecc, then longest message, so that the longest chunks do not start last), cuts it into
+// chunks of one class and uploads it. Drops the captured graphs, which hold the old layout.
+int plan_layout(ffrs_rs256_plan *p)
+{
+    const uint32_t count = p->count;
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; ++i)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const PlanItem &x = p->list[a], &y = p->list[b];
+        const bool wx = plan_wave(p, x.E), wy = plan_wave(p, y.E);
+        if (wx != wy)
+            return wx;
+        if (x.E != y.E)
+            return x.E > y.E;
+        return x.k > y.k;
+    });
+    std::vector<PlanItem> items(count);
+    std::vector<PlanChunk> chunks;
+    p->lane_max_ew = p->lane_max_e = 1;
+    p->lane_lut_words = 0;
+    p->n_wave = 0;
+    for (uint32_t j = 0; j < count; ++j) {
+        items[j] = p->list[order[j]];
+        if (chunks.empty() || chunks.back().cls != items[j].cls || chunks.back().count == p->wg)
+            chunks.push_back({j, 0, (uint16_t)items[j].cls});
+        chunks.back().count++;
+        if (plan_wave(p, items[j].E)) {
+            p->n_wave = j + 1;
+        } else {
+            const ffrs_rs256 *c = p->codecs[items[j].cls];
+            p->lane_max_ew = std::max(p->lane_max_ew, c->EW);
+            p->lane_max_e = std::max(p->lane_max_e, c->ecc);
+            p->lane_lut_words = std::max(p->lane_lut_words, c->S * 256u * (uint32_t)lut_row((int)c->EW));
+        }
+    }
+    p->nchunks = (uint32_t)chunks.size();
+    p->chunk0 = 0;
+    while (p->chunk0 < p->nchunks && chunks[p->chunk0].first < p->n_wave)
+        p->chunk0++;
+    for (auto &g : p->graphs)
+        (void)hipGraphExecDestroy(g.exec);
+    p->graphs.clear();
+    if (count) {
+        FFRS_HIP(hipMemcpy(p->d_items, items.data(), items.size() * sizeof(items[0]), hipMemcpyHostToDevice));
+        FFRS_HIP(hipMemcpy(p->d_chunks, chunks.data(), chunks.size() * sizeof(chunks[0]), hipMemcpyHostToDevice));
+    }
+    return FFRS_OK;
 }
 
 int plan_run(ffrs_rs256_plan *p, int mode, uint8_t *msgs, size_t msg_bytes, uint8_t *eccs, uint8_t *status)
@@ -2877,56 +2940,34 @@ int ffrs_rs256_plan_create(ffrs_ctx *ctx, size_t count, const uint8_t *ecc_len, 
         p->max_ew = std::max(p->max_ew, c->EW);
         p->max_e = std::max(p->max_e, e);
         p->lut_words = std::max(p->lut_words, c->S * 256u * (uint32_t)lut_row((int)c->EW));
-        if (e <= 32) {
-            p->lane_max_ew = std::max(p->lane_max_ew, c->EW);
-            p->lane_max_e = std::max(p->lane_max_e, e);
-            p->lane_lut_words = std::max(p->lane_lut_words, c->S * 256u * (uint32_t)lut_row((int)c->EW));
-        }
     }
     p->wg = p->max_ew <= 16 ? 256 : 64;
-    std::vector<uint32_t> order(count);
-    for (size_t i = 0; i < count; ++i)
-        order[i] = (uint32_t)i;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        // costliest first (largest ecc, then longest message), so the longest chunks
-        // do not start last
-        if (ecc_len[a] != ecc_len[b])
-            return ecc_len[a] > ecc_len[b];
-        return message_len[a] > message_len[b];
-    });
-    std::vector<PlanItem> items(count);
-    std::vector<PlanChunk> chunks;
-    for (size_t j = 0; j < count; ++j) {
-        uint32_t i = order[j];
-        items[j] = {msg_off[i], ecc_off[i], i, message_len[i], ecc_len[i], (uint8_t)cls_of[ecc_len[i]]};
+    std::copy(cls_of, cls_of + 256, p->cls_of);
+    p->list.resize(count);
+    for (size_t i = 0; i < count; ++i) {
+        p->list[i] = {msg_off[i], ecc_off[i], (uint32_t)i, message_len[i], ecc_len[i], (uint8_t)cls_of[ecc_len[i]]};
         p->msg_end = std::max<uint64_t>(p->msg_end, msg_off[i] + message_len[i]);
         p->ecc_end = std::max<uint64_t>(p->ecc_end, ecc_off[i] + ecc_len[i]);
-        if (chunks.empty() || chunks.back().cls != items[j].cls || chunks.back().count == p->wg)
-            chunks.push_back({(uint32_t)j, 0, (uint16_t)items[j].cls});
-        chunks.back().count++;
     }
-    p->nchunks = (uint32_t)chunks.size();
-    // the items of ecc_len > 32 lead the sorted list (descending ecc_len), in chunks of their own
-    while (p->n_wave < count && items[p->n_wave].E > 32)
-        p->n_wave++;
-    while (p->chunk0 < p->nchunks && chunks[p->chunk0].first < p->n_wave)
-        p->chunk0++;
     hipError_t e = hipSuccess;
-    auto up = [&](auto **dst, const auto &v) {
-        size_t bytes = std::max<size_t>(1, v.size() * sizeof(v[0]));
+    auto up = [&](auto **dst, size_t bytes, const void *src) {
         if (e == hipSuccess)
-            e = hipMalloc((void **)dst, bytes);
-        if (e == hipSuccess && !v.empty())
-            e = hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
+            e = hipMalloc((void **)dst, std::max<size_t>(1, bytes));
+        if (e == hipSuccess && src && bytes)
+            e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
     };
-    up(&p->d_items, items);
-    up(&p->d_chunks, chunks);
-    up(&p->d_classes, classes);
+    up(&p->d_items, count * sizeof(PlanItem), nullptr);
+    up(&p->d_chunks, count * sizeof(PlanChunk), nullptr);  // at most one chunk per item
+    up(&p->d_classes, classes.size() * sizeof(PlanClass), classes.data());
     if (e == hipSuccess)
         e = hipMalloc((void **)&p->d_queue, 4);
     if (e != hipSuccess) {
         plan_free(p);
         return fail(FFRS_ERR_NOMEM, "plan upload failed: %s", hipGetErrorString(e));
+    }
+    if (int r = plan_layout(p)) {
+        plan_free(p);
+        return r;
     }
     *out = p;
     return FFRS_OK;
@@ -2936,6 +2977,21 @@ void ffrs_rs256_plan_destroy(ffrs_rs256_plan *plan)
 {
     if (plan)
         plan_free(plan);
+}
+
+int ffrs_rs256_plan_set_class_mode(ffrs_rs256_plan *plan, unsigned ecc_len, int mode)
+{
+    if (!plan)
+        return fail(FFRS_ERR_INVALID, "plan is NULL");
+    if (ecc_len > 255 || plan->cls_of[ecc_len] < 0)
+        return fail(FFRS_ERR_INVALID, "the plan has no codewords of ecc_len %u", ecc_len);
+    if (mode < 0 || mode > 2)
+        return fail(FFRS_ERR_INVALID, "class mode %d (0 automatic, 1 one codeword per lane, 2 per wave)", mode);
+    std::lock_guard<std::mutex> lk(plan->ctx->mu);
+    if (int r = ctx_set_device(plan->ctx))
+        return r;
+    plan->mode_of[ecc_len] = mode;
+    return plan_layout(plan);
 }
 
 int ffrs_rs256_plan_use_graph(ffrs_rs256_plan *plan, int enable)

@@ -3235,6 +3235,11 @@ struct ffrs_rsi16_plan {
     std::vector<Range> rep_ranges, enc_ranges;
     uint32_t count = 0, nchunks = 0;
     uint64_t msg_end = 0, ecc_end = 0;
+    // the caller's list (in its order, class per item) and the classes, kept to lay the plan
+    // out again when a class's shape changes
+    std::vector<Rsi16PlanItem> list;
+    std::vector<Rsi16PlanClass> classes;
+    std::vector<std::pair<uint32_t, uint32_t>> keys;  // (block_len, ecc_len) per class
     bool use_graph = false;
     struct Graph {
         int mode;
@@ -3257,6 +3262,69 @@ void rsi16_plan_free(ffrs_rsi16_plan *p)
     for (auto *c : p->codecs)
         ffrs_rsi16_destroy(c);
     delete p;
+}
+
+// Sorts the work list by class (costliest first: largest ecc_len, then largest block_len),
+// cuts it into chunks of one class holding a wave's worth for the kernel that runs them
+// (consecutive chunks with the same kernel form one range, one launch) and uploads the list,
+// the chunks and the classes. Drops the captured graphs, which hold the old layout.
+int rsi16_plan_layout(ffrs_rsi16_plan *p)
+{
+    const uint32_t count = p->count;
+    std::vector<uint32_t> order(count);
+    for (uint32_t i = 0; i < count; ++i)
+        order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
+        const uint32_t ca = p->list[a].cls, cb = p->list[b].cls;
+        const auto &ka = p->keys[ca], &kb = p->keys[cb];
+        if (ka.second != kb.second)
+            return ka.second > kb.second;
+        if (ka.first != kb.first)
+            return ka.first > kb.first;
+        return ca < cb;
+    });
+    std::vector<Rsi16PlanItem> items(count);
+    for (uint32_t j = 0; j < count; ++j)
+        items[j] = p->list[order[j]];
+    std::vector<Rsi16PlanChunk> chunks;
+    p->rep_ranges.clear();
+    p->enc_ranges.clear();
+    auto build = [&](bool repair, std::vector<ffrs_rsi16_plan::Range> &ranges) {
+        for (uint32_t j = 0; j < count; ++j) {
+            const uint32_t c = items[j].cls;
+            const ffrs_rsi16 *codec = p->codecs[c];
+            const uint32_t E = codec->E, g = p->classes[c].g;
+            const uint32_t G = reg_group(E);
+            const bool dflt = E >= 2 && E <= 256 && g == group_size(E);
+            const bool reg = repair ? dflt : (E >= 2 && E <= 32) || dflt;
+            const uint32_t key = reg ? E : 0u;
+            const uint32_t cpw = !reg ? 64 / g : repair ? 64 / G : (E <= 32 ? 64 / p->classes[c].l_enc : 64 / G);
+            const uint32_t lds = repair ? (reg ? (64 / G) * REG_WORDS * E * 4 : (64 / g) * REPAIR_WORDS * E * 4)
+                                        : (reg ? 0u : (64 / g) * 2 * E * 4);
+            const bool new_range = ranges.empty() || ranges.back().key != key;
+            if (new_range)
+                ranges.push_back({key, (uint32_t)chunks.size(), 0, 0});
+            if (new_range || chunks.back().cls != c || chunks.back().count == cpw) {
+                chunks.push_back({j, 0, (uint16_t)c});
+                ranges.back().n++;
+            }
+            chunks.back().count++;
+            ranges.back().lds = std::max(ranges.back().lds, lds);
+        }
+    };
+    build(true, p->rep_ranges);
+    build(false, p->enc_ranges);
+    p->nchunks = (uint32_t)chunks.size();
+    for (auto &g : p->graphs)
+        (void)hipGraphExecDestroy(g.exec);
+    p->graphs.clear();
+    if (count) {
+        FFRS_HIP(hipMemcpy(p->d_items, items.data(), items.size() * sizeof(items[0]), hipMemcpyHostToDevice));
+        FFRS_HIP(hipMemcpy(p->d_chunks, chunks.data(), chunks.size() * sizeof(chunks[0]), hipMemcpyHostToDevice));
+    }
+    FFRS_HIP(hipMemcpy(p->d_classes, p->classes.data(), p->classes.size() * sizeof(p->classes[0]),
+                       hipMemcpyHostToDevice));
+    return FFRS_OK;
 }
 
 template <int MODE, int E>
@@ -3406,75 +3474,61 @@ int ffrs_rsi16_plan_create(ffrs_ctx *ctx, size_t count, const uint32_t *block_le
         }
         cls[i] = (uint32_t)c;
     }
-    std::vector<uint32_t> order(count);
-    for (size_t i = 0; i < count; ++i)
-        order[i] = (uint32_t)i;
-    // costliest classes first (largest ecc_len, then largest block_len)
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) {
-        const auto &ka = keys[cls[a]], &kb = keys[cls[b]];
-        if (ka.second != kb.second)
-            return ka.second > kb.second;
-        if (ka.first != kb.first)
-            return ka.first > kb.first;
-        return cls[a] < cls[b];
-    });
-    std::vector<Rsi16PlanItem> items(count);
-    for (size_t j = 0; j < count; ++j) {
-        uint32_t i = order[j], c = cls[i];
-        const ffrs_rsi16 *codec = p->codecs[c];
-        items[j] = {msg_off[i], ecc_off[i], i, c};
+    p->list.resize(count);
+    for (size_t i = 0; i < count; ++i) {
+        const ffrs_rsi16 *codec = p->codecs[cls[i]];
+        p->list[i] = {msg_off[i], ecc_off[i], (uint32_t)i, cls[i]};
         p->msg_end = std::max<uint64_t>(p->msg_end, msg_off[i] + codec->k);
         p->ecc_end = std::max<uint64_t>(p->ecc_end, ecc_off[i] + codec->E);
     }
-    // Chunks hold codewords of one class: a wave's worth for the kernel that runs them.
-    // Consecutive chunks with the same kernel form one range (one launch).
-    std::vector<Rsi16PlanChunk> chunks;
-    auto build = [&](bool repair, std::vector<ffrs_rsi16_plan::Range> &ranges) {
-        for (size_t j = 0; j < count; ++j) {
-            const uint32_t c = items[j].cls;
-            const ffrs_rsi16 *codec = p->codecs[c];
-            const uint32_t E = codec->E, g = classes[c].g;
-            const uint32_t G = reg_group(E);
-            const bool dflt = E >= 2 && E <= 256 && g == group_size(E);
-            const bool reg = repair ? dflt : (E >= 2 && E <= 32) || dflt;
-            const uint32_t key = reg ? E : 0u;
-            const uint32_t cpw = !reg ? 64 / g : repair ? 64 / G : (E <= 32 ? 64 / classes[c].l_enc : 64 / G);
-            const uint32_t lds = repair ? (reg ? (64 / G) * REG_WORDS * E * 4 : (64 / g) * REPAIR_WORDS * E * 4)
-                                        : (reg ? 0u : (64 / g) * 2 * E * 4);
-            const bool new_range = ranges.empty() || ranges.back().key != key;
-            if (new_range)
-                ranges.push_back({key, (uint32_t)chunks.size(), 0, 0});
-            if (new_range || chunks.back().cls != c || chunks.back().count == cpw) {
-                chunks.push_back({(uint32_t)j, 0, (uint16_t)c});
-                ranges.back().n++;
-            }
-            chunks.back().count++;
-            ranges.back().lds = std::max(ranges.back().lds, lds);
-        }
-    };
-    build(true, p->rep_ranges);
-    build(false, p->enc_ranges);
-    p->nchunks = (uint32_t)chunks.size();
+    p->classes = classes;
+    p->keys = keys;
     std::lock_guard<std::mutex> lk(ctx->mu);
     hipError_t e = hipSetDevice(ctx->device);
-    auto up = [&](auto **dst, const auto &v) {
-        size_t bytes = std::max<size_t>(1, v.size() * sizeof(v[0]));
+    auto alloc = [&](auto **dst, size_t bytes) {
         if (e == hipSuccess)
-            e = hipMalloc((void **)dst, bytes);
-        if (e == hipSuccess && !v.empty())
-            e = hipMemcpy(*dst, v.data(), v.size() * sizeof(v[0]), hipMemcpyHostToDevice);
+            e = hipMalloc((void **)dst, std::max<size_t>(1, bytes));
     };
-    up(&p->d_items, items);
-    up(&p->d_chunks, chunks);
-    up(&p->d_classes, classes);
-    if (e == hipSuccess)
-        e = hipMalloc((void **)&p->d_queue, 4 * std::max<size_t>(1, p->rep_ranges.size() + p->enc_ranges.size()));
+    alloc(&p->d_items, count * sizeof(Rsi16PlanItem));
+    alloc(&p->d_chunks, 2 * count * sizeof(Rsi16PlanChunk));  // repair and encode chunks, at most one per item each
+    alloc(&p->d_classes, classes.size() * sizeof(Rsi16PlanClass));
+    // one counter per range: at most one repair and one encode range per class
+    alloc(&p->d_queue, 4 * (2 * classes.size() + 2));
     if (e != hipSuccess) {
         rsi16_plan_free(p);
         return fail(FFRS_ERR_NOMEM, "plan upload failed: %s", hipGetErrorString(e));
     }
+    if (int r = rsi16_plan_layout(p)) {
+        rsi16_plan_free(p);
+        return r;
+    }
     *out = p;
     return FFRS_OK;
+}
+
+int ffrs_rsi16_plan_set_class_groups(ffrs_rsi16_plan *plan, uint32_t block_len, uint32_t ecc_len, unsigned l_encode,
+                                     unsigned g_repair)
+{
+    if (!plan)
+        return fail(FFRS_ERR_INVALID, "plan is NULL");
+    size_t c = std::find(plan->keys.begin(), plan->keys.end(), std::make_pair(block_len, ecc_len)) - plan->keys.begin();
+    if (c == plan->keys.size())
+        return fail(FFRS_ERR_INVALID, "the plan has no codewords of RSi16(%u, %u)", block_len, ecc_len);
+    const ffrs_rsi16 *codec = plan->codecs[c];
+    const uint32_t E = codec->E;
+    auto pow2 = [](unsigned x) { return x && (x & (x - 1)) == 0; };
+    if (l_encode && (!pow2(l_encode) || l_encode > 64))
+        return fail(FFRS_ERR_INVALID, "encode lanes per codeword %u (a power of two up to 64)", l_encode);
+    if (g_repair && (!pow2(g_repair) || g_repair > std::min<uint32_t>(64, E) ||
+                     (size_t)(64 / g_repair) * REPAIR_WORDS * E * 4 > 160 * 1024))
+        return fail(FFRS_ERR_INVALID, "repair lanes per codeword %u for ecc_len %u", g_repair, E);
+    std::lock_guard<std::mutex> lk(plan->ctx->mu);
+    if (int r = ctx_set_device(plan->ctx))
+        return r;
+    if (l_encode)
+        plan->classes[c].l_enc = l_encode;
+    plan->classes[c].g = g_repair ? g_repair : group_size(E);
+    return rsi16_plan_layout(plan);
 }
 
 void ffrs_rsi16_plan_destroy(ffrs_rsi16_plan *plan)

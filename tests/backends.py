@@ -280,12 +280,18 @@ class PlanBackend(GpuBackend):
     eagerly and then three times from a captured HIP graph (inputs restored in between);
     every replay must equal the eager result."""
 
-    def __init__(self, graph_replays=3):
+    def __init__(self, graph_replays=3, shape=None):
         super().__init__()
         from libffrs_amd import capi
         self.capi = capi
         self.ctx = capi.Context(0)
         self.replays = graph_replays
+        self.shape = shape  # called with every new plan: per-class kernel shapes
+
+    def _shaped(self, plan):
+        if self.shape:
+            self.shape(plan)
+        return plan
 
     def _layout(self, sizes, seed):
         rng = np.random.default_rng(seed)
@@ -342,12 +348,12 @@ class PlanBackend(GpuBackend):
         return outs_m, outs_e, st
 
     def rs256_mixed_encode(self, eccs, lens, msgs):
-        mk = lambda mo, eo, g: self.capi.RS256Plan(self.ctx, eccs, lens, mo, eo, graph=g)  # noqa: E731
+        mk = lambda mo, eo, g: self._shaped(self.capi.RS256Plan(self.ctx, eccs, lens, mo, eo, graph=g))  # noqa: E731
         _, pars, _ = self._run_mixed(mk, 1, np.uint8, lambda i: int(lens[i]), lambda i: int(eccs[i]), msgs)
         return pars
 
     def rs256_mixed_repair(self, eccs, lens, cws):
-        mk = lambda mo, eo, g: self.capi.RS256Plan(self.ctx, eccs, lens, mo, eo, graph=g)  # noqa: E731
+        mk = lambda mo, eo, g: self._shaped(self.capi.RS256Plan(self.ctx, eccs, lens, mo, eo, graph=g))  # noqa: E731
         ms = [c[:int(l)] for c, l in zip(cws, lens)]
         ps = [c[int(l):] for c, l in zip(cws, lens)]
         om, oe, st = self._run_mixed(mk, 1, np.uint8, lambda i: int(lens[i]), lambda i: int(eccs[i]), ms, ps,
@@ -358,12 +364,12 @@ class PlanBackend(GpuBackend):
         return st
 
     def rsi16_mixed_encode(self, ns, eccs, msgs):
-        mk = lambda mo, eo, g: self.capi.RSi16Plan(self.ctx, ns, eccs, mo, eo, graph=g)  # noqa: E731
+        mk = lambda mo, eo, g: self._shaped(self.capi.RSi16Plan(self.ctx, ns, eccs, mo, eo, graph=g))  # noqa: E731
         _, pars, _ = self._run_mixed(mk, 2, np.uint16, lambda i: int(ns[i] - eccs[i]), lambda i: int(eccs[i]), msgs)
         return pars
 
     def rsi16_mixed_repair(self, ns, eccs, cws):
-        mk = lambda mo, eo, g: self.capi.RSi16Plan(self.ctx, ns, eccs, mo, eo, graph=g)  # noqa: E731
+        mk = lambda mo, eo, g: self._shaped(self.capi.RSi16Plan(self.ctx, ns, eccs, mo, eo, graph=g))  # noqa: E731
         ks = [int(n - e) for n, e in zip(ns, eccs)]
         ms = [c[:k] for c, k in zip(cws, ks)]
         ps = [c[k:] for c, k in zip(cws, ks)]
