@@ -205,6 +205,18 @@ int ffrs_rsi16_synds(ffrs_rsi16 *rs, const uint16_t *msg, const uint16_t *ecc, s
  * reference raises. */
 int ffrs_rsi16_sugiyama(ffrs_rsi16 *rs, const uint32_t *synds, size_t count, uint32_t *locator,
                         uint32_t *evaluator, uint32_t *length);
+/* The reference's NTT class methods (libffrs/pyntt.hpp:28-84, rsi16v_impl.hpp:670-795) and
+ * RSi16._sugiyama<W> (pyrsi16.hpp:653-674) on `width` = 1, 4, 8 or 16 vectors of this codec's
+ * NTT (width 1: the scalar words; 4-16: one SIMD lane per vector, coupled where the reference's
+ * vector code couples its lanes). Vectors are consecutive in a, b and out (width x len values,
+ * len = ntt_len for ops 0-3, ecc_len for 4-7). op: 0 ntt (RBO in, natural out), 1 intt,
+ * 2 nttr (first ecc_len values), 3 inttr, 4 poly_mul(a, b), 5 poly_div(a, b), 6 poly_inv(a, n)
+ * (n <= ecc_len), 7 Sugiyama of the syndromes a (out: evaluator, out2: locator). result[l]:
+ * 1 where the reference raises (ops 5, 6), the locator length or 0xFFFFFFFF (op 7), else 0.
+ * Values are field elements (0..65536); the Python methods return them truncated to u16 as the
+ * reference does. */
+int ffrs_rsi16_ntt_op(ffrs_rsi16 *rs, int op, unsigned width, const uint32_t *a, const uint32_t *b, uint32_t n,
+                      uint32_t *out, uint32_t *out2, uint32_t *result);
 /* py_roots (libffrs/pyrsi16.hpp:676-691): is_root[m] = 1 where the partial inverse NTT
  * of the ecc_len coefficients vanishes, m < block_len. */
 int ffrs_rsi16_roots(ffrs_rsi16 *rs, const uint32_t *poly, uint8_t *is_root);

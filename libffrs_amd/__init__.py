@@ -11,6 +11,7 @@ the import fails (build it with ``python libffrs_amd/build.py``).
 from .libffrs import (  # noqa: F401
     GF256,
     GFi16,
+    NTT,
     RS256,
     RSi16,
     RepairStatus,
@@ -23,4 +24,4 @@ from .libffrs import (  # noqa: F401
     set_logger,
 )
 
-__all__ = ["GF256", "GFi16", "RS256", "RSi16", "RepairStatus", "create_buffer", "set_logger", "__version__"]
+__all__ = ["GF256", "GFi16", "NTT", "RS256", "RSi16", "RepairStatus", "create_buffer", "set_logger", "__version__"]

@@ -85,6 +85,7 @@ SIGNATURES = [
     ("ffrs_rsi16_plan_create", c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, ctypes.c_uint32, ctypes.POINTER(c_vp)]),
     ("ffrs_rsi16_plan_destroy", None, [c_vp]),
     ("ffrs_rsi16_plan_use_graph", c_int, [c_vp, c_int]),
+    ("ffrs_rsi16_ntt_op", c_int, [c_vp, c_int, ctypes.c_uint, c_vp, c_vp, ctypes.c_uint32, c_vp, c_vp, c_vp]),
     ("ffrs_rsi16_plan_set_class_groups", c_int, [c_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint, ctypes.c_uint]),
     ("ffrs_rsi16_plan_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_int]),
     ("ffrs_rsi16_plan_repair", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_vp]),

@@ -320,12 +320,76 @@ public:
 
 enum class RepairStatus : uint32_t { NoErrors, NoErrorsZero, RepairOk, RepairFail, ErrorLocationFail };
 
+// The reference's NTT class (pyntt.hpp:28-84): the transforms and polynomial operations of an
+// RSi16(block_len, ecc_len) code's NTT, on 1, 4, 8 or 16 vectors per call (the numbered
+// variants run the reference's SIMD instantiations: one vector per SIMD lane). It holds its own
+// codec (its device tables); RSi16.ntt returns one for the codec's shape.
+class PyNTT {
+public:
+    size_t block_len, ecc_len, ntt_len;
+    ffrs_rsi16 *codec = nullptr;
+
+    PyNTT(const GFi16 &gf, size_t block_len_, size_t ecc_len_) : block_len(block_len_), ecc_len(ecc_len_)
+    {
+        int r = ffrs_rsi16_create(default_ctx(), block_len, ecc_len, gf.primitive, &codec);
+        check(r, "NTT");
+        ntt_len = ffrs_rsi16_ntt_len(codec);
+    }
+    ~PyNTT()
+    {
+        if (codec)
+            ffrs_rsi16_destroy(codec);
+    }
+
+    // ntt.hpp:125-128: reverse the low log2(ntt_len) bits
+    uint16_t rbo(uint16_t b) const
+    {
+        unsigned bits = 0;
+        while ((1u << bits) < ntt_len)
+            ++bits;
+        uint32_t r = 0;
+        for (unsigned i = 0; i < bits; ++i)
+            r |= ((b >> i) & 1u) << (bits - 1 - i);
+        return (uint16_t)r;
+    }
+
+    py::bytearray run(int op, unsigned W, const std::vector<py::buffer> &in, uint32_t n = 0)
+    {
+        const size_t len = op <= 3 ? ntt_len : ecc_len;
+        std::vector<std::vector<uint32_t>> v;
+        for (const auto &b : in) {
+            Buf<const uint16_t> x(b);
+            PY_ASSERT(x.size == W * len, std::to_string(x.size) + " != " + std::to_string(W * len));
+            v.emplace_back(x.data, x.data + x.size);
+        }
+        std::vector<uint32_t> out(W * len), res(W);
+        int r;
+        {
+            py::gil_scoped_release nogil;
+            r = ffrs_rsi16_ntt_op(codec, op, W, v[0].data(), v.size() > 1 ? v[1].data() : nullptr, n, out.data(), nullptr,
+                                  res.data());
+        }
+        check(r, "NTT");
+        for (uint32_t x : res)
+            if (x)
+                throw std::runtime_error(op == 5 ? "Assertion failed: !any((g_len == 0) && (f_len > 0))"
+                                                 : "Assertion failed: g_len + l - 2 <= ecc_len (aliasing)");
+        py::bytearray o;
+        PyByteArray_Resize(o.ptr(), (Py_ssize_t)(out.size() * 2));
+        uint16_t *d = reinterpret_cast<uint16_t *>(PyByteArray_AsString(o.ptr()));
+        for (size_t i = 0; i < out.size(); ++i)
+            d[i] = (uint16_t)(out[i] & 0xffffu);  // the reference copies GFT words into u16
+        return o;
+    }
+};
+
 class PyRSi16 {
 public:
     size_t block_len, message_len, ecc_len, interleave;
     bool simd_x4, simd_x8, simd_x16;
     std::unique_ptr<GFi16> gf;
     ffrs_rsi16 *codec = nullptr;
+    std::unique_ptr<PyNTT> ntt_obj;  // RSi16.ntt, made on first use
 
     PyRSi16(size_t block_len_, uint16_t ecc_len_, size_t interleave_, uint32_t primitive, std::optional<bool> x4,
             std::optional<bool> x8, std::optional<bool> x16)
@@ -523,6 +587,20 @@ public:
         return py::make_tuple(loc, ev);
     }
 
+    // py_sugiyama<W> (pyrsi16.hpp:653-674): W syndrome vectors of ecc_len values, one after
+    // another; the coupled vector Sugiyama of W lanes
+    py::tuple sugiyama_w(std::vector<uint32_t> synd, unsigned W)
+    {
+        PY_ASSERT(synd.size() >= (size_t)W * ecc_len, "synd must hold " + std::to_string(W * ecc_len) + " values");
+        std::vector<uint32_t> loc(W * ecc_len), ev(W * ecc_len), len(W);
+        check(ffrs_rsi16_ntt_op(codec, 7, W, synd.data(), nullptr, 0, ev.data(), loc.data(), len.data()),
+              ("RSi16._sugiyama" + std::to_string(W)).c_str());
+        for (uint32_t l : len)
+            if (l == 0xFFFFFFFFu)
+                throw std::runtime_error("Assertion failed: g_len > 0 (the reference raises on this input)");
+        return py::make_tuple(loc, ev);
+    }
+
     std::vector<size_t> roots(std::vector<uint32_t> poly)
     {
         PY_ASSERT(poly.size() == ecc_len, std::to_string(poly.size()));
@@ -633,6 +711,43 @@ PYBIND11_MODULE(libffrs, m)
         .def("log", [](GFi16 &s, uint32_t a) { return s.log_t.at(a); }, "value"_a)
         .def("pow", &GFi16::pow, "base"_a, "exponent"_a);
 
+    {  // pyntt.hpp:28-84
+        auto cls = py::class_<PyNTT>(m, "NTT", "NTT transforms and polynomial operations of an RSi16 code")
+            .def(py::init<const GFi16 &, size_t, size_t>(), "gf"_a, "block_len"_a, "ecc_len"_a)
+            .def_property_readonly("ntt_len", [](PyNTT &s) { return s.ntt_len; })
+            .def_property_readonly("ntt_size", [](PyNTT &s) { return s.ntt_len * 2; })
+            .def_property_readonly("ntt4_size", [](PyNTT &s) { return s.ntt_len * 8; })
+            .def_property_readonly("ntt8_size", [](PyNTT &s) { return s.ntt_len * 16; })
+            .def_property_readonly("ntt16_size", [](PyNTT &s) { return s.ntt_len * 32; })
+            .def_property_readonly("block_len", [](PyNTT &s) { return s.block_len; })
+            .def_property_readonly("ecc_len", [](PyNTT &s) { return s.ecc_len; })
+            .def_property_readonly("ecc_size", [](PyNTT &s) { return s.ecc_len * 2; })
+            .def("rbo", &PyNTT::rbo, "Reverse Bit Order for log2(ntt_len) bits", "i"_a);
+        static const char *const names[] = {"ntt", "intt", "nttr", "inttr"};
+        static const char *const docs[] = {"RBO input, normal order output", "Normal order input, RBO output",
+                                           "Normal order input, RBO output", "RBO input, Normal order output"};
+        static const char *const sfx[] = {"", "4", "8", "16"};
+        static const unsigned widths[] = {1, 4, 8, 16};
+        for (int w = 0; w < 4; ++w) {
+            const unsigned W = widths[w];
+            for (int op = 0; op < 4; ++op)
+                cls.def((std::string(names[op]) + sfx[w]).c_str(),
+                        [op, W](PyNTT &s, py::buffer in) { return s.run(op, W, {in}); }, docs[op], "input"_a);
+            cls.def((std::string("poly_mul") + sfx[w]).c_str(),
+                    [W](PyNTT &s, py::buffer a, py::buffer b) { return s.run(4, W, {a, b}); }, "Element wise multiplication",
+                    "a"_a, "b"_a);
+            cls.def((std::string("poly_div") + sfx[w]).c_str(),
+                    [W](PyNTT &s, py::buffer a, py::buffer b) { return s.run(5, W, {a, b}); }, "Polynomial division",
+                    "a"_a, "b"_a);
+            cls.def((std::string("poly_inv") + sfx[w]).c_str(),
+                    [W](PyNTT &s, py::buffer a, size_t n) {
+                        PY_ASSERT(n <= s.ecc_len, "n must not exceed ecc_len: " + std::to_string(n));
+                        return s.run(6, W, {a}, (uint32_t)n);
+                    },
+                    "Polynomial inverse", "a"_a, "n"_a);
+        }
+    }
+
     py::class_<PyRS256>(m, "RS256", "Reed-Solomon coding over GF(2^8)")
         .def(py::init<std::optional<uint8_t>, std::optional<uint8_t>, std::optional<uint8_t>, uint8_t, uint16_t>(),
              "block_len"_a = py::none(), "message_len"_a = py::none(), "ecc_len"_a = py::none(), "primitive"_a = 2,
@@ -699,6 +814,17 @@ PYBIND11_MODULE(libffrs, m)
         .def("_synd", &PyRSi16::synd, "Calculate syndromes for the given message and ecc buffers", "message"_a,
              "ecc"_a)
         .def("_sugiyama", &PyRSi16::sugiyama, "Compute error locator and evaluator polynomials", "synd"_a)
+        .def("_sugiyama4", [](PyRSi16 &s, std::vector<uint32_t> v) { return s.sugiyama_w(std::move(v), 4); },
+             "Compute error locator and evaluator polynomials", "synd"_a)
+        .def("_sugiyama8", [](PyRSi16 &s, std::vector<uint32_t> v) { return s.sugiyama_w(std::move(v), 8); },
+             "Compute error locator and evaluator polynomials", "synd"_a)
+        .def("_sugiyama16", [](PyRSi16 &s, std::vector<uint32_t> v) { return s.sugiyama_w(std::move(v), 16); },
+             "Compute error locator and evaluator polynomials", "synd"_a)
+        .def_property_readonly("ntt", [](PyRSi16 &s) {  // one per codec, as the reference's member
+            if (!s.ntt_obj)
+                s.ntt_obj = std::make_unique<PyNTT>(*s.gf, s.block_len, s.ecc_len);
+            return s.ntt_obj.get();
+        }, py::return_value_policy::reference_internal, ":class:`NTT` instance")
         .def("message_offset", &PyRSi16::message_offset, "row"_a, "col"_a)
         .def("ecc_offset", &PyRSi16::ecc_offset, "row"_a, "col"_a)
         .def("_roots", &PyRSi16::roots, "Find locator polynomial roots", "synd"_a)

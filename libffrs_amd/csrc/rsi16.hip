@@ -2035,15 +2035,26 @@ __device__ inline bool lv_poly_inv(const Rsi16Dev &T, LaneVec g, uint32_t g_len,
     return true;
 }
 
-// NTT::poly_div_ntt, ntt.hpp:471-564, vector instantiation: q <- f // g (transforms);
-// false where it raises
+// NTT::poly_div_ntt, ntt.hpp:471-564: q <- f // g (transforms); false where it raises. W = 1
+// is the reference's integral instantiation (a zero divisor always raises, f_len < g_len
+// gives q = 0), W = 4, 8, 16 its vector one.
 template <int W>
 __device__ inline bool lv_poly_div(const Rsi16Dev &T, LaneVec f, uint32_t f_len, LaneVec g, uint32_t g_len, uint32_t f0,
                                    uint32_t g0, LaneVec t, LaneVec q, uint32_t &q_len)
 {
     const uint32_t E = T.E;
-    if (cl_any<W>(g_len == 0 && f_len > 0))  // py_assert(!any((g_len == 0) && (f_len > 0)))
+    if constexpr (W == 1) {
+        if (g_len == 0)  // py_assert(g_len > 0)
+            return false;
+        if (f_len < g_len) {
+            for (uint32_t i = 0; i < E; ++i)
+                q[i] = 0;
+            q_len = 0;
+            return true;
+        }
+    } else if (cl_any<W>(g_len == 0 && f_len > 0)) {  // py_assert(!any((g_len == 0) && (f_len > 0)))
         return false;
+    }
     q_len = (g_len != 0 && f_len >= g_len) ? f_len - g_len + 1 : 0u;
     for (uint32_t i = 0; i < E; ++i)
         t[i] = g[i];
@@ -2157,6 +2168,89 @@ __device__ inline uint32_t lv_eval(LaneVec r, uint32_t len, uint32_t x)
     for (uint32_t i = len - 1; i < len; --i)
         s = S::add(S::mul(s, x), r[i]);
     return s;
+}
+
+// The reference's NTT methods and RSi16._sugiyama<W> (pyntt.hpp:28-84, rsi16v_impl.hpp:670-795,
+// pyrsi16.hpp:653-674) on W vectors, one lane per vector as one SIMD lane of the reference's
+// vector words (W = 1: its scalar words): vector l's elements sit at ws[i * W + l] in regions of
+// `len` elements. Division, inverse and Sugiyama take the lanes' group decisions as the
+// reference's vector code does (the same lane functions as the coupled interleaved repair).
+// result[l]: 1 where the operation raises in the reference (division, inverse), or the locator
+// length (Sugiyama; 0xFFFFFFFF where it raises).
+enum : int { NTTOP_NTT, NTTOP_INTT, NTTOP_NTTR, NTTOP_INTTR, NTTOP_MUL, NTTOP_DIV, NTTOP_INV, NTTOP_SUGIYAMA };
+
+template <int W>
+__global__ __launch_bounds__(64) void rsi16_ntt_util_kernel(Rsi16Dev T, int op, uint32_t *__restrict__ ws, uint32_t len,
+                                                            const uint32_t *__restrict__ roots_i_ntt, uint32_t ntt_len_i,
+                                                            uint32_t n_arg, uint32_t *__restrict__ result)
+{
+    const uint32_t lane = threadIdx.x, E = T.E, N = T.N;
+    auto vec = [&](int r) { return LaneVec{ws + (size_t)r * len * W + lane, (uint32_t)W}; };
+    LaneVec a = vec(0), b = vec(1);
+    uint32_t res = 0;
+    switch (op) {
+    case NTTOP_NTT:  // ntt.hpp:596-599
+        lv_ct(a, T.roots_ntt, N);
+        break;
+    case NTTOP_INTT:  // ntt.hpp:601-606
+        lv_gs(a, roots_i_ntt, N);
+        for (uint32_t i = 0; i < N; ++i)
+            a[i] = S::mul(a[i], ntt_len_i);
+        break;
+    case NTTOP_NTTR:
+        lv_nttr(T, a);
+        break;
+    case NTTOP_INTTR:
+        lv_inttr(T, a);
+        break;
+    case NTTOP_MUL:  // py_poly_mul: nttr both, poly_mul_ntt (pointwise), inttr
+        lv_nttr(T, a);
+        lv_nttr(T, b);
+        for (uint32_t i = 0; i < E; ++i)
+            a[i] = S::mul(a[i], b[i]);
+        lv_inttr(T, a);
+        break;
+    case NTTOP_DIV: {  // py_poly_div: lengths and end coefficients first, then poly_div_ntt
+        LaneVec t = vec(2), q = vec(3);
+        const uint32_t a_len = lv_len(a, E, E), b_len = lv_len(b, E, E);
+        const uint32_t al = a[0], bh = b[(b_len - 1) & (E - 1)];
+        lv_nttr(T, a);
+        lv_nttr(T, b);
+        uint32_t q_len = 0;
+        if (!lv_poly_div<W>(T, a, a_len, b, b_len, al, bh, t, q, q_len)) {
+            res = 1;
+            break;
+        }
+        lv_inttr(T, q);
+        for (uint32_t i = 0; i < E; ++i)
+            a[i] = q[i];
+        break;
+    }
+    case NTTOP_INV: {  // py_poly_inv
+        LaneVec r = vec(2);
+        const uint32_t g0 = a[0], g_len = lv_len(a, E, E);
+        lv_nttr(T, a);
+        if (!lv_poly_inv<W>(T, a, g_len, g0, n_arg, r)) {
+            res = 1;
+            break;
+        }
+        lv_inttr(T, r);
+        for (uint32_t i = 0, m = n_arg - 1; i < n_arg / 2; ++i, --m) {  // std::reverse(&res[0], &res[n])
+            const uint32_t x = r[i];
+            r[i] = r[m];
+            r[m] = x;
+        }
+        for (uint32_t i = 0; i < E; ++i)
+            a[i] = r[i];
+        break;
+    }
+    default: {  // NTTOP_SUGIYAMA: syndromes in a, locator out in b, evaluator out in a
+        SimdLane L{vec(8), a, b, vec(2), vec(3), vec(4), vec(5), vec(6), vec(7)};
+        res = lv_sugiyama<W>(T, L);
+        break;
+    }
+    }
+    result[lane] = res;
 }
 
 // One lane per SIMD lane: group `g0 + lane / W` of an interleaved block (columns
@@ -3187,6 +3281,72 @@ int ffrs_rsi16_sugiyama(ffrs_rsi16 *rs, const uint32_t *synds, size_t count, uin
     FFRS_HIP(hipMemcpyAsync(evaluator, eo, sb, hipMemcpyDeviceToHost, ctx->stream));
     FFRS_HIP(hipMemcpyAsync(length, d_l, count * 4, hipMemcpyDeviceToHost, ctx->stream));
     FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_ntt_op(ffrs_rsi16 *rs, int op, unsigned width, const uint32_t *a, const uint32_t *b, uint32_t n,
+                      uint32_t *out, uint32_t *out2, uint32_t *result)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (width != 1 && width != 4 && width != 8 && width != 16)
+        return fail(FFRS_ERR_INVALID, "width %u (1, 4, 8 or 16)", width);
+    if (op < NTTOP_NTT || op > NTTOP_SUGIYAMA)
+        return fail(FFRS_ERR_INVALID, "operation %d", op);
+    const bool two = op == NTTOP_MUL || op == NTTOP_DIV;
+    if (!a || !out || !result || (two && !b) || (op == NTTOP_SUGIYAMA && !out2))
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
+    const uint32_t W = width, E = rs->E, N = rs->N;
+    if (op == NTTOP_INV && n > E)
+        return fail(FFRS_ERR_PARAM, "n = %u (0 .. ecc_len)", n);
+    const uint32_t len = op <= NTTOP_INTTR ? N : E;
+    const size_t region = (size_t)len * W * 4, wbytes = 9 * region + (size_t)N * 4 + 64;
+    ffrs_ctx *ctx = rs->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    if (rs->work_size < wbytes) {
+        if (rs->d_work)
+            FFRS_HIP(hipFree(rs->d_work));
+        rs->d_work = nullptr;
+        rs->work_size = 0;
+        hipError_t e = hipMalloc(&rs->d_work, wbytes);
+        if (e != hipSuccess)
+            return fail(FFRS_ERR_NOMEM, "workspace of %zu bytes: %s", wbytes, hipGetErrorString(e));
+        rs->work_size = wbytes;
+    }
+    // inputs vector-major [W][len] -> element-major [len][W]
+    std::vector<uint32_t> h(2 * (size_t)len * W, 0);
+    for (uint32_t l = 0; l < W; ++l)
+        for (uint32_t i = 0; i < len; ++i) {
+            h[(size_t)i * W + l] = a[(size_t)l * len + i];
+            if (two)
+                h[(size_t)len * W + (size_t)i * W + l] = b[(size_t)l * len + i];
+        }
+    Gfi16Host gfh(rs->primitive);
+    std::vector<uint32_t> ri(N);  // powers of the inverse root (ntt.hpp:74-79)
+    const uint64_t rinv = gfh.inv(rs->root);
+    ri[0] = 1;
+    for (uint32_t i = 1; i < N; ++i)
+        ri[i] = (uint32_t)(ri[i - 1] * rinv % P16);
+    const uint32_t ntt_len_i = gfh.inv(N % P16);
+    uint32_t *ws = (uint32_t *)rs->d_work, *d_ri = ws + 9 * (size_t)len * W, *d_res = d_ri + N;
+    FFRS_HIP(hipMemcpyAsync(ws, h.data(), h.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(d_ri, ri.data(), ri.size() * 4, hipMemcpyHostToDevice, ctx->stream));
+    auto *kern = W == 1 ? rsi16_ntt_util_kernel<1> : W == 4 ? rsi16_ntt_util_kernel<4>
+                 : W == 8 ? rsi16_ntt_util_kernel<8> : rsi16_ntt_util_kernel<16>;
+    hipLaunchKernelGGL(kern, dim3(1), dim3(W), 0, ctx->stream, rs->dev, op, ws, len, (const uint32_t *)d_ri, ntt_len_i,
+                       n, d_res);
+    FFRS_HIP(hipGetLastError());
+    FFRS_HIP(hipMemcpyAsync(h.data(), ws, 2 * region, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipMemcpyAsync(result, d_res, W * 4, hipMemcpyDeviceToHost, ctx->stream));
+    FFRS_HIP(hipStreamSynchronize(ctx->stream));
+    for (uint32_t l = 0; l < W; ++l)
+        for (uint32_t i = 0; i < len; ++i) {
+            out[(size_t)l * len + i] = h[(size_t)i * W + l];
+            if (op == NTTOP_SUGIYAMA)
+                out2[(size_t)l * len + i] = h[(size_t)len * W + (size_t)i * W + l];
+        }
     return FFRS_OK;
 }
 

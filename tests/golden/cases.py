@@ -305,3 +305,44 @@ def rsi16_ilsimd_known(n, ecc, il, msg, par, s):
         cws[c, where] = v
     cws = cws.astype(np.uint16)
     return (np.ascontiguousarray(cws[:, :k].T).reshape(-1), np.ascontiguousarray(cws[:, k:].T).reshape(-1), pos)
+
+
+# The NTT class and RSi16._sugiyama<W>: (block_len, ecc_len, primitive) and the vector counts
+NTT_SHAPES = ((4, 2, 3), (8, 4, 3), (16, 8, 3), (32, 16, 3), (256, 32, 3), (1024, 64, 3), (16, 8, 5))
+NTT_WIDTHS = (1, 4, 8, 16)
+SUGIYAMA_SHAPES = ((16, 8), (64, 16), (256, 32), (1024, 64))
+
+
+def ntt_inv_mods(E):
+    """The n of the poly_inv trials: every n for E <= 16, else the powers of two, their
+    neighbours and E."""
+    if E <= 16:
+        return list(range(0, E + 1))
+    out = {0, 1, E}
+    p = 2
+    while p <= E:
+        out.update({p - 1, p, min(p + 1, E)})
+        p *= 2
+    return sorted(out)
+
+
+def poly_lengths(s, W, E, kind):
+    """Per-vector polynomial lengths of a trial: 'half' (E/2, as the reference's own tests),
+    'full', 'rand' (0..E) or 'pos' (1..E)."""
+    if kind == "half":
+        return np.full(W, E // 2, np.int64)
+    if kind == "full":
+        return np.full(W, E, np.int64)
+    if kind == "rand":
+        return s.ints(0, E, W)
+    return s.ints(1, E, W)
+
+
+def poly_vectors(s, W, E, lens):
+    """W polynomials of E coefficients (u16, nonzero leading term), vector v of length lens[v]."""
+    x = s.u16(W * E).reshape(W, E)
+    for v in range(W):
+        x[v, lens[v]:] = 0
+        if lens[v] > 0 and x[v, lens[v] - 1] == 0:
+            x[v, lens[v] - 1] = 1
+    return x.reshape(-1)

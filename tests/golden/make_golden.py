@@ -235,6 +235,7 @@ def main():
     out.update(reftest_fixture(L))
     out.update(divergence_fixture(L))
     out.update(ilsimd_fixture(L))
+    out.update(ntt_fixture(L))
 
     for key, d in out.items():
         np.savez_compressed(os.path.join(HERE, key + ".npz"), **d)
@@ -438,6 +439,105 @@ def ilsimd_fixture(L):
     return {"rsi16_ilsimd": d}
 
 
+def _raises(fn, *args):
+    try:
+        return fn(*args), 0
+    except RuntimeError:
+        return None, 1
+
+
+def ntt_fixture(L):
+    """(j) the NTT class (pyntt.hpp:28-84) and RSi16._sugiyama<W> (pyrsi16.hpp:653-674) with
+    explicit inputs and the reference's outputs (u16 as the reference returns them; Sugiyama's
+    u32 field elements); raises[t] = 1 where the call raises RuntimeError (outputs zero)."""
+    d = {}
+    s = W.Stream(47)
+    for bl, ecc, prim in C.NTT_SHAPES:
+        gf = L.GFi16(prim)  # the reference's NTT keeps a reference to gf: it must stay alive
+        ntt = L.NTT(gf, bl, ecc)
+        N, E = ntt.ntt_len, ntt.ecc_len
+        for w in C.NTT_WIDTHS:
+            sfx = "" if w == 1 else str(w)
+            tag = f"{bl}_{ecc}_{prim}_w{w}"
+            if N <= 256 or w <= 4:  # the transforms of ntt_len 1024 on 1 and 4 vectors only
+                x = s.u16(w * N)
+                d[f"x_{tag}"] = x
+                for op in ("ntt", "intt", "nttr", "inttr"):
+                    d[f"{op}_{tag}"] = np.frombuffer(getattr(ntt, op + sfx)(x.tobytes()), np.uint16)
+            # poly_mul: a and b of the given length kinds
+            kinds = [("half", "half"), ("full", "full"), ("rand", "rand"), ("rand", "half")]
+            A, B, O = [], [], []
+            for ka, kb in kinds:
+                a = C.poly_vectors(s, w, E, C.poly_lengths(s, w, E, ka))
+                b = C.poly_vectors(s, w, E, C.poly_lengths(s, w, E, kb))
+                A.append(a), B.append(b)
+                O.append(np.frombuffer(getattr(ntt, "poly_mul" + sfx)(a.tobytes(), b.tobytes()), np.uint16))
+            d[f"mul_a_{tag}"], d[f"mul_b_{tag}"], d[f"mul_out_{tag}"] = np.stack(A), np.stack(B), np.stack(O)
+            # poly_div: random lengths, a zero divisor, f shorter than g, b with its top two zero
+            A, B, O, R = [], [], [], []
+            for t in range(10):
+                la = C.poly_lengths(s, w, E, "rand")
+                lb = C.poly_lengths(s, w, E, "pos")
+                if t == 1:
+                    lb[s.ints(0, w - 1, 1)[0]] = 0          # one zero divisor
+                elif t == 2:
+                    la[:], lb[:] = 0, 0                      # 0 // 0
+                elif t == 3:
+                    la[:] = np.minimum(lb, E) - 1            # deg f < deg g
+                    la[la < 0] = 0
+                elif t == 4:
+                    lb[:] = E - 2                            # the reference's test_poly_div_n
+                    la[:] = E
+                elif t == 5:
+                    lb[:] = 1                                # constant divisors
+                a = C.poly_vectors(s, w, E, la)
+                b = C.poly_vectors(s, w, E, lb)
+                o, r = _raises(getattr(ntt, "poly_div" + sfx), a.tobytes(), b.tobytes())
+                A.append(a), B.append(b), R.append(r)
+                O.append(np.zeros(w * E, np.uint16) if r else np.frombuffer(o, np.uint16))
+            d[f"div_a_{tag}"], d[f"div_b_{tag}"] = np.stack(A), np.stack(B)
+            d[f"div_out_{tag}"], d[f"div_raises_{tag}"] = np.stack(O), np.array(R, np.uint8)
+            # poly_inv: every n of the list, a of half and random lengths (length >= 1)
+            A, NN, O, R = [], [], [], []
+            for n in C.ntt_inv_mods(E):
+                for kind in ("half", "pos") if E <= 16 else ("pos",):
+                    a = C.poly_vectors(s, w, E, C.poly_lengths(s, w, E, kind))
+                    o, r = _raises(getattr(ntt, "poly_inv" + sfx), a.tobytes(), n)
+                    A.append(a), NN.append(n), R.append(r)
+                    O.append(np.zeros(w * E, np.uint16) if r else np.frombuffer(o, np.uint16))
+            d[f"inv_a_{tag}"], d[f"inv_n_{tag}"] = np.stack(A), np.array(NN, np.uint32)
+            d[f"inv_out_{tag}"], d[f"inv_raises_{tag}"] = np.stack(O), np.array(R, np.uint8)
+            print("ntt", tag, "div raises", sum(d[f"div_raises_{tag}"]), "inv raises", sum(d[f"inv_raises_{tag}"]))
+
+    # _sugiyama<W>: syndromes of codewords with 0 .. ecc/2 + 2 errors per vector, and random ones
+    for n, ecc in C.SUGIYAMA_SHAPES:
+        ref = L.RSi16(n, ecc)
+        k = n - ecc
+        for w in C.NTT_WIDTHS:
+            tag = f"{n}_{ecc}_w{w}"
+            S, LOC, EV, R = [], [], [], []
+            for t in range(6):
+                if t < 5:
+                    msgs = s.u16(w * k).reshape(w, k)
+                    par = np.frombuffer(ref.encode(msgs.tobytes()), np.uint16).reshape(w, ecc)
+                    cws = np.concatenate([msgs, par], 1)
+                    counts = s.ints(0, ecc // 2 + 2, w) if t else np.full(w, ecc // 2, np.int64)
+                    W.rsi16_corrupt(cws, counts, s)
+                    syn = np.concatenate([np.array(ref._synd(cw[:k].tobytes(), cw[k:].tobytes()), np.uint32)
+                                          for cw in cws])
+                else:
+                    syn = s.ints(0, 65536, w * ecc).astype(np.uint32)
+                fn = ref._sugiyama if w == 1 else getattr(ref, f"_sugiyama{w}")
+                o, r = _raises(fn, [int(v) for v in syn])
+                S.append(syn), R.append(r)
+                LOC.append(np.zeros(w * ecc, np.uint32) if r else np.array(o[0], np.uint32))
+                EV.append(np.zeros(w * ecc, np.uint32) if r else np.array(o[1], np.uint32))
+            d[f"sug_synd_{tag}"], d[f"sug_loc_{tag}"] = np.stack(S), np.stack(LOC)
+            d[f"sug_ev_{tag}"], d[f"sug_raises_{tag}"] = np.stack(EV), np.array(R, np.uint8)
+            print("sugiyama", tag, "raises", R)
+    return {"rsi16_ntt": d}
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["interleave"]:
         for key, v in interleave_fixture(load_reference()).items():
@@ -453,6 +553,9 @@ if __name__ == "__main__":
             np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
     elif sys.argv[1:] == ["divergence"]:
         for key, v in divergence_fixture(load_reference()).items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    elif sys.argv[1:] == ["ntt"]:
+        for key, v in ntt_fixture(load_reference()).items():
             np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
     elif sys.argv[1:] == ["radius"]:
         from libffrs_amd import workloads as W  # noqa: F811
