@@ -225,6 +225,37 @@ int ffrs_rsi16_encode_device(ffrs_rsi16 *rs, const uint16_t *d_msg, size_t count
 int ffrs_rsi16_repair_device(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t count,
                              uint8_t *d_status);
 
+/* ---- CIRC16 ------------------------------------------------------------------------------
+ * PyCIRC16 (libffrs/pycirc16.hpp:33-431): a cross-interleaved code over two RSi16 codecs of one
+ * context and primitive, inner RSi16(inner_block_len, inner_ecc_len) and outer
+ * RSi16(outer_block_len, outer_ecc_len) whose interleave is inner_message_len * interleave
+ * (pycirc16.hpp:45-64). simd_width: the lanes of the reference's SIMD words that the simd_x*
+ * flags select (16, 8, 4; 1 for the scalar words). The codecs must outlive the handle.
+ * Message: outer_message_len rows of inner_message_len * interleave symbols; ecc: outer parity
+ * (outer_ecc_len rows of the same width), inner parity of every message row block, inner parity
+ * of every outer parity row block (message_offset / rso_ecc_offset / rsi_ecc_offset,
+ * pycirc16.hpp:423-441). */
+typedef struct ffrs_circ16 ffrs_circ16;
+int ffrs_circ16_create(ffrs_rsi16 *inner, ffrs_rsi16 *outer, size_t interleave, unsigned simd_width,
+                       ffrs_circ16 **out);
+void ffrs_circ16_destroy(ffrs_circ16 *c);
+size_t ffrs_circ16_message_len(const ffrs_circ16 *c);
+size_t ffrs_circ16_ecc_len(const ffrs_circ16 *c);
+/* py_encode / encode_block (pycirc16.hpp:356-381): `blocks` messages of message_len symbols to
+ * blocks * ecc_len ecc symbols; the inner parity of the outer parity rows is computed from the
+ * outer parity's field elements (a parity symbol of 65536 is stored as 0 but encoded as 65536). */
+int ffrs_circ16_encode(ffrs_circ16 *c, const uint16_t *msg, size_t blocks, uint16_t *ecc);
+/* py_repair (pycirc16.hpp:383-403) of one block, in place. *raised = 1 where the reference's
+ * repair raises (a py_assert of the outer repair without positions): msg then holds what the
+ * reference leaves (the interleave blocks before the raising one repaired, that one up to its
+ * raising SIMD group) and ecc is unchanged. */
+int ffrs_circ16_repair(ffrs_circ16 *c, uint16_t *msg, uint16_t *ecc, int *raised);
+/* py_find_outer_error_locations (pycirc16.hpp:405-421): the outer rows of interleave block
+ * `interleave` whose inner syndromes are not all zero after the zero repairs, ascending, into
+ * rows[0 .. *count) (rows holds outer_block_len entries). */
+int ffrs_circ16_find_outer_error_locations(ffrs_circ16 *c, const uint16_t *msg, const uint16_t *ecc,
+                                           size_t interleave, uint32_t *rows, size_t *count);
+
 /* ---- mixed-size batches --------------------------------------------------------------
  * No counterpart in the reference, whose codec objects serve one (block_len, ecc_len) each
  * (pyrs256.hpp:58-97, pyrsi16.hpp:41-125): a plan lists `count` codewords, each with its own

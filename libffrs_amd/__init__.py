@@ -9,6 +9,7 @@ package loads the compiled pybind11 module ``libffrs_amd.libffrs``; if it is mis
 the import fails (build it with ``python libffrs_amd/build.py``).
 """
 from .libffrs import (  # noqa: F401
+    CIRC16,
     GF256,
     GFi16,
     NTT,
@@ -24,4 +25,4 @@ from .libffrs import (  # noqa: F401
     set_logger,
 )
 
-__all__ = ["GF256", "GFi16", "NTT", "RS256", "RSi16", "RepairStatus", "create_buffer", "set_logger", "__version__"]
+__all__ = ["CIRC16", "GF256", "GFi16", "NTT", "RS256", "RSi16", "RepairStatus", "create_buffer", "set_logger", "__version__"]

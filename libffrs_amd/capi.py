@@ -88,6 +88,13 @@ SIGNATURES = [
     ("ffrs_rsi16_ntt_op", c_int, [c_vp, c_int, ctypes.c_uint, c_vp, c_vp, ctypes.c_uint32, c_vp, c_vp, c_vp]),
     ("ffrs_rsi16_plan_set_class_groups", c_int, [c_vp, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_uint, ctypes.c_uint]),
     ("ffrs_rsi16_plan_encode", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_int]),
+    ("ffrs_circ16_create", c_int, [c_vp, c_vp, c_sz, ctypes.c_uint, ctypes.POINTER(c_vp)]),
+    ("ffrs_circ16_destroy", None, [c_vp]),
+    ("ffrs_circ16_message_len", c_sz, [c_vp]),
+    ("ffrs_circ16_ecc_len", c_sz, [c_vp]),
+    ("ffrs_circ16_encode", c_int, [c_vp, c_vp, c_sz, c_vp]),
+    ("ffrs_circ16_repair", c_int, [c_vp, c_vp, c_vp, ctypes.POINTER(c_int)]),
+    ("ffrs_circ16_find_outer_error_locations", c_int, [c_vp, c_vp, c_vp, c_sz, c_vp, ctypes.POINTER(c_sz)]),
     ("ffrs_rsi16_plan_repair", c_int, [c_vp, c_vp, c_sz, c_vp, c_sz, c_vp]),
 ]
 

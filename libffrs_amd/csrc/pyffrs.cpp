@@ -628,6 +628,125 @@ public:
     }
 };
 
+// ---------------------------------------------------------------- CIRC16 (pycirc16.hpp:33-431)
+
+// The reference's CIRC16: inner RSi16(inner_block_len, inner_ecc_len) and outer
+// RSi16(outer_block_len, outer_ecc_len, inner_message_len * interleave) with the same primitive
+// and SIMD flags (pycirc16.hpp:45-64); the cross-interleaved encode and repair run on the GPU
+// through ffrs_circ16_*.
+class PyCIRC16 {
+public:
+    py::object rsi_obj, rso_obj;
+    PyRSi16 *rsi = nullptr, *rso = nullptr;
+    size_t interleave;
+    ffrs_circ16 *c = nullptr;
+
+    PyCIRC16(size_t ibl, size_t iecc, size_t obl, size_t oecc, size_t interleave_, uint32_t primitive,
+             std::optional<bool> x4, std::optional<bool> x8, std::optional<bool> x16)
+        : interleave(interleave_)
+    {
+        PY_ASSERT(iecc <= 0xffff && oecc <= 0xffff, "ecc_len must fit 16 bits");
+        auto ri = std::make_unique<PyRSi16>(ibl, (uint16_t)iecc, 1, primitive, x4, x8, x16);
+        auto ro = std::make_unique<PyRSi16>(obl, (uint16_t)oecc, ri->message_len * interleave, primitive, x4, x8, x16);
+        rsi = ri.get();
+        rso = ro.get();
+        rsi_obj = py::cast(std::move(ri));
+        rso_obj = py::cast(std::move(ro));
+        check(ffrs_circ16_create(rsi->codec, rso->codec, interleave, rsi->simd_width(), &c), "CIRC16");
+    }
+    ~PyCIRC16()
+    {
+        if (c)
+            ffrs_circ16_destroy(c);
+    }
+
+    size_t block_len() const { return rsi->block_len * rso->block_len * interleave; }
+    size_t message_len() const { return rso->message_len * rso->interleave; }
+    size_t rsi_interleaved_ecc_len() const { return rsi->ecc_len * rso->message_len * interleave; }
+    size_t rsio_ecc_len() const { return rsi->ecc_len * rso->ecc_len * interleave; }
+    size_t ecc_len() const { return rso->ecc_len * rso->interleave + rsi_interleaved_ecc_len() + rsio_ecc_len(); }
+
+    py::bytearray encode(py::buffer buffer)
+    {
+        Buf<const uint16_t> buf(buffer);
+        const size_t ml = message_len();
+        PY_ASSERT(buf.size % ml == 0, std::to_string(buf.size));
+        const size_t blocks = buf.size / ml;
+        py::bytearray out;
+        PyByteArray_Resize(out.ptr(), (Py_ssize_t)(blocks * ecc_len() * 2));
+        if (blocks) {
+            uint16_t *o = reinterpret_cast<uint16_t *>(PyByteArray_AsString(out.ptr()));
+            int r;
+            {
+                py::gil_scoped_release nogil;
+                r = ffrs_circ16_encode(c, buf.data, blocks, o);
+            }
+            check(r, "CIRC16.encode");
+        }
+        return out;
+    }
+
+    void check_block(const Buf<uint16_t> &m, const Buf<uint16_t> &e)
+    {
+        PY_ASSERT(m.size == message_len(), std::to_string(m.size) + " != " + std::to_string(message_len()));
+        PY_ASSERT(e.size == ecc_len(), std::to_string(e.size));
+    }
+
+    bool repair(py::buffer message, py::buffer ecc)
+    {
+        Buf<uint16_t> m(message), e(ecc);
+        check_block(m, e);
+        int raised = 0, r;
+        {
+            py::gil_scoped_release nogil;
+            r = ffrs_circ16_repair(c, m.data, e.data, &raised);
+        }
+        check(r, "CIRC16.repair");
+        if (raised)
+            throw std::runtime_error("Assertion failed: g_len > 0 (the reference raises on this input)");
+        return false;  // py_repair returns false (pycirc16.hpp:402)
+    }
+
+    std::vector<size_t> find_outer_error_locations(py::buffer message, py::buffer ecc, size_t il)
+    {
+        Buf<uint16_t> m(message), e(ecc);
+        check_block(m, e);
+        PY_ASSERT(il < interleave, "interleave < this->interleave");
+        std::vector<uint32_t> rows(rso->block_len);
+        size_t n = 0;
+        int r;
+        {
+            py::gil_scoped_release nogil;
+            r = ffrs_circ16_find_outer_error_locations(c, m.data, e.data, il, rows.data(), &n);
+        }
+        check(r, "CIRC16._find_outer_error_locations");
+        return std::vector<size_t>(rows.begin(), rows.begin() + n);
+    }
+
+    // pycirc16.hpp:423-445
+    size_t message_offset(size_t il, size_t row, size_t col) const
+    {
+        PY_ASSERT(il < interleave, "interleave < this->interleave");
+        PY_ASSERT(row < rso->message_len, "row < rso.message_len");
+        PY_ASSERT(col < rsi->message_len, "col < rsi.message_len");
+        return rso->interleave * row + il * rsi->message_len + col;
+    }
+    size_t rso_ecc_offset(size_t il, size_t row, size_t col) const
+    {
+        PY_ASSERT(il < interleave, "interleave < this->interleave");
+        PY_ASSERT(row < rso->ecc_len, "row < rso.ecc_len");
+        PY_ASSERT(col < rsi->message_len, "col < rsi.message_len");
+        return rso->interleave * row + il * rsi->message_len + col;
+    }
+    size_t rsi_ecc_offset(size_t il, size_t row, size_t col) const
+    {
+        PY_ASSERT(il < interleave, "interleave < this->interleave");
+        PY_ASSERT(row < rso->block_len, "row < rso.block_len");
+        PY_ASSERT(col < rsi->ecc_len, "col < rsi.ecc_len");
+        return rso->ecc_len * rso->interleave + (row * interleave + il) * rsi->ecc_len + col;
+    }
+};
+
 // ---------------------------------------------------------------- create_buffer (pyffrs.cpp:41-98)
 
 py::memoryview create_buffer(size_t requested)
@@ -830,4 +949,42 @@ PYBIND11_MODULE(libffrs, m)
         .def("_roots", &PyRSi16::roots, "Find locator polynomial roots", "synd"_a)
         .def("_set_groups", [](PyRSi16 &s, unsigned ge, unsigned gr) { check(ffrs_rsi16_set_groups(s.codec, ge, gr), "RSi16"); },
              "Lanes per codeword of the encode and repair kernels (0 keeps)", "g_encode"_a, "g_repair"_a);
+
+    py::class_<PyCIRC16>(m, "CIRC16", "Cross-interleaved Reed-Solomon coding over GF(65537)")
+        .def(py::init<size_t, size_t, size_t, size_t, size_t, uint32_t, std::optional<bool>, std::optional<bool>,
+                      std::optional<bool>>(),
+             "inner_block_len"_a, "inner_ecc_len"_a, "outer_block_len"_a, "outer_ecc_len"_a, "interleave"_a = 1,
+             py::kw_only(), "primitive"_a = 3, "simd_x4"_a = py::none(), "simd_x8"_a = py::none(),
+             "simd_x16"_a = py::none())
+        .def_property_readonly("rsi", [](PyCIRC16 &s) { return s.rsi_obj; }, "Inner codec (RSi16)")
+        .def_property_readonly("rso", [](PyCIRC16 &s) { return s.rso_obj; }, "Outer codec (RSi16)")
+        .def_property_readonly("block_len", [](PyCIRC16 &s) { return s.block_len(); })
+        .def_property_readonly("block_size", [](PyCIRC16 &s) { return s.block_len() * 2; })
+        .def_property_readonly("message_len", [](PyCIRC16 &s) { return s.message_len(); })
+        .def_property_readonly("message_size", [](PyCIRC16 &s) { return s.message_len() * 2; })
+        .def_property_readonly("ecc_len", [](PyCIRC16 &s) { return s.ecc_len(); })
+        .def_property_readonly("ecc_size", [](PyCIRC16 &s) { return s.ecc_len() * 2; })
+        .def_property_readonly("inner_block_len", [](PyCIRC16 &s) { return s.rsi->block_len; })
+        .def_property_readonly("inner_block_size", [](PyCIRC16 &s) { return s.rsi->block_len * 2; })
+        .def_property_readonly("inner_message_len", [](PyCIRC16 &s) { return s.rsi->message_len; })
+        .def_property_readonly("inner_message_size", [](PyCIRC16 &s) { return s.rsi->message_len * 2; })
+        .def_property_readonly("inner_ecc_len", [](PyCIRC16 &s) { return s.rsi->ecc_len; })
+        .def_property_readonly("inner_ecc_size", [](PyCIRC16 &s) { return s.rsi->ecc_len * 2; })
+        .def_property_readonly("outer_block_len", [](PyCIRC16 &s) { return s.rso->block_len; })
+        .def_property_readonly("outer_message_len", [](PyCIRC16 &s) { return s.rso->message_len; })
+        .def_property_readonly("outer_ecc_len", [](PyCIRC16 &s) { return s.rso->ecc_len; })
+        .def_property_readonly("outer_interleave", [](PyCIRC16 &s) { return s.rso->interleave; })
+        .def_property_readonly("interleave", [](PyCIRC16 &s) { return s.interleave; })
+        .def_property_readonly("primitive", [](PyCIRC16 &s) { return s.rsi->gf->primitive; })
+        .def("__sizeof_cpp__", [](PyCIRC16 &) { return sizeof(PyCIRC16); })
+        .def_property_readonly("simd_x4", [](PyCIRC16 &s) { return s.rsi->simd_x4; })
+        .def_property_readonly("simd_x8", [](PyCIRC16 &s) { return s.rsi->simd_x8; })
+        .def_property_readonly("simd_x16", [](PyCIRC16 &s) { return s.rsi->simd_x16; })
+        .def("encode", &PyCIRC16::encode, "Encode data", "buffer"_a)
+        .def("repair", &PyCIRC16::repair, "Repair data", "message"_a, "ecc"_a)
+        .def("_find_outer_error_locations", &PyCIRC16::find_outer_error_locations,
+             "Find outer codec error locations for a given interleaved block", "message"_a, "ecc"_a, "interleave"_a)
+        .def("message_offset", &PyCIRC16::message_offset, "interleave"_a, "row"_a, "col"_a)
+        .def("rso_ecc_offset", &PyCIRC16::rso_ecc_offset, "interleave"_a, "row"_a, "col"_a)
+        .def("rsi_ecc_offset", &PyCIRC16::rsi_ecc_offset, "interleave"_a, "row"_a, "col"_a);
 }

@@ -265,9 +265,11 @@ __device__ inline uint32_t group_len(const uint32_t *r, uint32_t max, Grp G)
 // ---------------------------------------------------------------- encode
 
 // One codeword per lane group: pntt_message_residue + _mix_ecc_residue
-// (rsi16v_impl.hpp:93-119), values written as & 0xFFFF. buf holds 2 E words.
-template <class O>
-__device__ inline void encode_group(const Rsi16Dev &T, const uint16_t *m, uint16_t *o, uint32_t *buf, Grp G,
+// (rsi16v_impl.hpp:93-119). buf holds 2 E words. Symbols are u16, or u32 field elements (the
+// reference encodes GFT sources as they are); u16 outputs are written as & 0xFFFF, u32 outputs
+// keep the word.
+template <class O, class TI = uint16_t, class TO = uint16_t>
+__device__ inline void encode_group(const Rsi16Dev &T, const TI *m, TO *o, uint32_t *buf, Grp G,
                                     uint32_t stride = 1)
 {
     const uint32_t E = T.E;
@@ -288,15 +290,19 @@ __device__ inline void encode_group(const Rsi16Dev &T, const uint16_t *m, uint16
         acc[j] = O::mul(O::mod_p(acc[j]), T.mix[j]);
     wave_sync();
     gs_full<O>(acc, E, T.roots_i_ecc, G);
-    for (uint32_t j = G.gl; j < E; j += G.g)
-        o[(size_t)j * stride] = (uint16_t)(acc[j] & 0xffff);
+    for (uint32_t j = G.gl; j < E; j += G.g) {
+        if constexpr (sizeof(TO) == 2)
+            o[(size_t)j * stride] = (uint16_t)(acc[j] & 0xffff);
+        else
+            o[(size_t)j * stride] = acc[j];
+    }
 }
 
 // Codeword cw = b * I + c is column c of interleaved block b: symbol i of it sits at
 // i * I + c of the block (pyrsi16.hpp:222-232, 324-361); I = 1 is the plain layout.
-template <class O>
-__global__ __launch_bounds__(64) void rsi16_encode_kernel(const uint16_t *__restrict__ msgs, uint32_t count,
-                                                           uint16_t *__restrict__ out, Rsi16Dev T, uint32_t g,
+template <class O, class TI = uint16_t, class TO = uint16_t>
+__global__ __launch_bounds__(64) void rsi16_encode_kernel(const TI *__restrict__ msgs, uint32_t count,
+                                                           TO *__restrict__ out, Rsi16Dev T, uint32_t g,
                                                            uint32_t I)
 {
     extern __shared__ uint32_t s_mem[];
@@ -308,7 +314,7 @@ __global__ __launch_bounds__(64) void rsi16_encode_kernel(const uint16_t *__rest
     if (cw >= count)
         return;
     const size_t b = cw / I, c = cw % I;
-    encode_group<O>(T, msgs + b * T.k * I + c, out + b * E * I + c, s_mem + grp * 2 * E, G, I);
+    encode_group<O, TI, TO>(T, msgs + b * T.k * I + c, out + b * E * I + c, s_mem + grp * 2 * E, G, I);
 }
 
 // ---- one codeword per lane, transforms in registers (ecc_len <= 32)
@@ -494,15 +500,16 @@ __global__ __launch_bounds__(256) void rsi16_encode_lane_kernel(const uint16_t *
 // ---------------------------------------------------------------- syndromes
 
 // NTT::pntt (ntt.hpp:133-152) of the codeword msg || ecc into syn[0..E); buf is E scratch.
-__device__ inline void pntt_codeword(const Rsi16Dev &T, const uint16_t *m, const uint16_t *e, uint32_t *buf,
-                                     uint32_t *syn, Grp G)
+template <class TM = uint16_t, class TE = uint16_t>
+__device__ inline void pntt_codeword(const Rsi16Dev &T, const TM *m, const TE *e, uint32_t *buf, uint32_t *syn, Grp G,
+                                     size_t ms = 1, size_t es = 1)
 {
     const uint32_t E = T.E;
     uint32_t nb = T.n / E;
     for (uint32_t c = 0; c < nb; ++c) {
         for (uint32_t j = G.gl; j < E; j += G.g) {
             uint32_t idx = c * E + j;
-            buf[j] = idx < T.k ? (uint32_t)m[idx] : (uint32_t)e[idx - T.k];
+            buf[j] = idx < T.k ? (uint32_t)m[idx * ms] : (uint32_t)e[(idx - T.k) * es];
         }
         wave_sync();
         ct_full(buf, E, T.roots_ecc, G);
@@ -1728,26 +1735,14 @@ __global__ __launch_bounds__(64) void rsi16_roots_kernel(const uint32_t *__restr
     }
 }
 
-// Known-position repair, rsi16v_impl.hpp:171-231 (one codeword, one group).
-__global__ __launch_bounds__(64) void rsi16_repair_known_kernel(uint16_t *__restrict__ m, uint16_t *__restrict__ e,
-                                                                 const uint32_t *__restrict__ pos, uint32_t npos,
-                                                                 uint8_t *__restrict__ status, Rsi16Dev T, uint32_t g)
+// _repair with error positions (rsi16v_impl.hpp:171-231) past its all-zero syndrome check: from
+// the syndromes in ev[0..E), apply(r, pos[r], e_r) for every position with its error value e_r.
+// LDS words: ev 2E, loc E, tmp 2E, der E.
+template <class F>
+__device__ inline void known_errors(const Rsi16Dev &T, const uint32_t *pos, uint32_t npos, uint32_t *ev,
+                                    uint32_t *loc, uint32_t *tmp, uint32_t *der, Grp G, F apply)
 {
-    extern __shared__ uint32_t s_mem[];
     const uint32_t E = T.E;
-    if (threadIdx.x >= g)
-        return;
-    Grp G{threadIdx.x, g};
-    uint32_t *ev = s_mem, *loc = s_mem + 2 * E, *tmp = s_mem + 3 * E, *der = s_mem + 5 * E, *chunk = s_mem + 6 * E;
-    pntt_codeword(T, m, e, chunk, ev, G);
-    bool any = false;
-    for (uint32_t j = 0; j < E; ++j)
-        any |= ev[j] != 0;
-    if (!any) {
-        if (G.gl == 0)
-            *status = FFRS_RSI16_NO_ERRORS;
-        return;
-    }
     // _error_locator (rsi16v_impl.hpp:532-542): coefficients c_1..c_npos of prod (1 - x_j z)
     if (G.gl == 0) {
         for (uint32_t i = 0; i < E; ++i)
@@ -1787,10 +1782,34 @@ __global__ __launch_bounds__(64) void rsi16_repair_known_kernel(uint16_t *__rest
         uint32_t xi = T.roots_ntt[(T.N - ri) & (T.N - 1)];
         uint32_t num = eval_poly(ev, E, xi);
         uint32_t den = eval_poly(der, dlen, xi);
-        uint32_t err = S::mul(g_div(num, den), x);
+        apply(r, p, S::mul(g_div(num, den), x));
+    }
+}
+
+// Known-position repair, rsi16v_impl.hpp:171-231 (one codeword, one group).
+__global__ __launch_bounds__(64) void rsi16_repair_known_kernel(uint16_t *__restrict__ m, uint16_t *__restrict__ e,
+                                                                 const uint32_t *__restrict__ pos, uint32_t npos,
+                                                                 uint8_t *__restrict__ status, Rsi16Dev T, uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E;
+    if (threadIdx.x >= g)
+        return;
+    Grp G{threadIdx.x, g};
+    uint32_t *ev = s_mem, *loc = s_mem + 2 * E, *tmp = s_mem + 3 * E, *der = s_mem + 5 * E, *chunk = s_mem + 6 * E;
+    pntt_codeword(T, m, e, chunk, ev, G);
+    bool any = false;
+    for (uint32_t j = 0; j < E; ++j)
+        any |= ev[j] != 0;
+    if (!any) {
+        if (G.gl == 0)
+            *status = FFRS_RSI16_NO_ERRORS;
+        return;
+    }
+    known_errors(T, pos, npos, ev, loc, tmp, der, G, [&](uint32_t, uint32_t p, uint32_t err) {
         uint16_t *at = p < T.k ? &m[p] : &e[p - T.k];
         *at = (uint16_t)(S::add((uint32_t)*at, err) & 0xffff);
-    }
+    });
     if (G.gl == 0)
         *status = FFRS_RSI16_REPAIR_OK;
 }
@@ -2253,22 +2272,22 @@ __global__ __launch_bounds__(64) void rsi16_ntt_util_kernel(Rsi16Dev T, int op, 
     result[lane] = res;
 }
 
-// One lane per SIMD lane: group `g0 + lane / W` of an interleaved block (columns
-// (g0 + lane / W) * W + lane % W, those past the interleave zero), its codeword loaded into the
-// workspace, repaired there; gstatus[group] = the group's status (the maximum over its lanes)
-// or FFRS_RSI16_RAISES.
-template <int W>
+// One lane per SIMD lane: group `g0 + lane / W` of the ncols columns from col0 of an interleaved
+// block (columns col0 + (g0 + lane / W) * W + lane % W, those past ncols zero), its codeword
+// loaded into the workspace, repaired there; gstatus[group] = the group's status (the maximum
+// over its lanes) or FFRS_RSI16_RAISES. The ecc rows hold u16 symbols or u32 field elements.
+template <int W, class TE = uint16_t>
 __global__ __launch_bounds__(64) void rsi16_repair_simd_kernel(Rsi16Dev T, const uint16_t *__restrict__ msg,
-                                                               const uint16_t *__restrict__ ecc, uint32_t il, uint32_t g0,
-                                                               uint32_t lanes_total, uint32_t *__restrict__ work,
-                                                               uint8_t *__restrict__ gstatus)
+                                                               const TE *__restrict__ ecc, uint32_t il, uint32_t col0,
+                                                               uint32_t ncols, uint32_t g0, uint32_t lanes_total,
+                                                               uint32_t *__restrict__ work, uint8_t *__restrict__ gstatus)
 {
     const uint32_t lane = blockIdx.x * 64 + threadIdx.x;
     if (lane >= lanes_total)  // lanes_total is a multiple of W: clusters are whole
         return;
     const uint32_t n = T.n, k = T.k, E = T.E;
-    const uint32_t grp = g0 + lane / W, col = grp * W + lane % W;
-    const bool valid = col < il;
+    const uint32_t grp = g0 + lane / W, cc = grp * W + lane % W, col = col0 + cc;
+    const bool valid = cc < ncols;
     auto row = [&](uint32_t r) { return LaneVec{work + (size_t)r * lanes_total + lane, lanes_total}; };
     SimdLane L{row(0), row(n), row(n + E), row(n + 2 * E), row(n + 3 * E), row(n + 4 * E), row(n + 5 * E),
                row(n + 6 * E), row(n + 7 * E)};
@@ -2326,27 +2345,230 @@ __global__ __launch_bounds__(64) void rsi16_repair_simd_kernel(Rsi16Dev T, const
         gstatus[grp] = st;
 }
 
-// Writes the repaired columns of groups [g0, g_end) back (values & 0xFFFF), from the workspace
-// of the launch that repaired groups g0.. (lanes_total lanes).
-template <int W>
+// Writes the repaired columns of groups [g0, g_end) back (message values & 0xFFFF, ecc values
+// likewise or whole for u32 rows), from the workspace of the launch that repaired groups g0..
+// (lanes_total lanes).
+template <int W, class TE = uint16_t>
 __global__ __launch_bounds__(256) void rsi16_simd_writeback_kernel(const Rsi16Dev T, uint16_t *__restrict__ msg,
-                                                                   uint16_t *__restrict__ ecc, uint32_t il, uint32_t g0,
-                                                                   uint32_t g_end, uint32_t lanes_total,
+                                                                   TE *__restrict__ ecc, uint32_t il, uint32_t col0,
+                                                                   uint32_t ncols, uint32_t g0, uint32_t g_end,
+                                                                   uint32_t lanes_total,
                                                                    const uint32_t *__restrict__ work)
 {
     const uint32_t lane = blockIdx.x * 64 + (threadIdx.x & 63);
     if (lane >= lanes_total)
         return;
-    const uint32_t grp = g0 + lane / W, col = grp * W + lane % W;
-    if (grp >= g_end || col >= il)
+    const uint32_t grp = g0 + lane / W, cc = grp * W + lane % W, col = col0 + cc;
+    if (grp >= g_end || cc >= ncols)
         return;
     const uint32_t n = T.n, k = T.k;
     for (uint32_t i = threadIdx.x >> 6; i < n; i += 4) {
-        const uint16_t v = (uint16_t)work[(size_t)i * lanes_total + lane];
+        const uint32_t v = work[(size_t)i * lanes_total + lane];
         if (i < k)
-            msg[(size_t)i * il + col] = v;
+            msg[(size_t)i * il + col] = (uint16_t)v;
         else
-            ecc[(size_t)(i - k) * il + col] = v;
+            ecc[(size_t)(i - k) * il + col] = (TE)(sizeof(TE) == 2 ? (v & 0xffffu) : v);
+    }
+}
+
+// ---------------------------------------------------------------- CIRC16 (pycirc16.hpp)
+
+// One CIRC16 block (pycirc16.hpp:45-64, 423-441): message row r (r < outer k) holds the inner
+// messages of the I interleave blocks side by side, inner k symbols each, so inner codeword
+// q = r I + kk has its message at q * inner_k; the outer parity has outer E rows of the same
+// shape; the inner parity of codeword q sits at q * inner_E, message rows first, then the
+// rows of the outer parity. The repair keeps the outer parity and the inner parity as u32
+// field elements (CircRepair's GFT buffers), so values of 65536 survive between the steps.
+
+// load_ecc (pycirc16.hpp:158-162): the u16 ecc into the u32 state
+__global__ __launch_bounds__(256) void circ16_expand_kernel(const uint16_t *__restrict__ ecc, uint32_t *__restrict__ rso32,
+                                                             uint32_t rso_len, uint32_t *__restrict__ rsi32,
+                                                             uint32_t rsi_len)
+{
+    for (uint32_t i = blockIdx.x * 256 + threadIdx.x; i < rso_len + rsi_len; i += gridDim.x * 256) {
+        if (i < rso_len)
+            rso32[i] = ecc[i];
+        else
+            rsi32[i - rso_len] = ecc[i];
+    }
+}
+
+// compute_rsi_synd (pycirc16.hpp:164-180): NTT::pntt syndromes of `rows` inner codewords, one
+// lane group each; message q at m + q k (u16 message rows or u32 outer-parity rows), parity at
+// e + q E.
+template <class TM>
+__global__ __launch_bounds__(64) void circ16_synd_kernel(const Rsi16Dev T, const TM *__restrict__ m,
+                                                          const uint32_t *__restrict__ e, uint32_t rows,
+                                                          uint32_t *__restrict__ syn, uint32_t g)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t E = T.E, cpw = 64 / g, grp = threadIdx.x / g;
+    const uint32_t q = blockIdx.x * cpw + grp;
+    if (q >= rows)
+        return;
+    Grp G{threadIdx.x % g, g};
+    uint32_t *chunk = s_mem + grp * 2 * E, *ev = chunk + E;
+    pntt_codeword<TM, uint32_t>(T, m + (size_t)q * T.k, e + (size_t)q * E, chunk, ev, G);
+    for (uint32_t j = G.gl; j < E; j += G.g)
+        syn[(size_t)q * E + j] = ev[j];
+}
+
+// repair_outer_zeros (pycirc16.hpp:182-244), one wave per inner codeword q of the outer parity
+// rows (message at rso32 + q k, parity at rsio32 + q E): where its syndromes are not all zero
+// and it holds 1 .. E zero symbols, the known-position repair at those positions; where every
+// repaired value is 0 mod 0x10000 the values are written back and the syndromes recomputed.
+__global__ __launch_bounds__(64) void circ16_outer_zeros_kernel(const Rsi16Dev T, uint32_t *__restrict__ rso32,
+                                                                 uint32_t *__restrict__ rsio32, uint32_t rows,
+                                                                 uint32_t *__restrict__ syn)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t q = blockIdx.x, lane = threadIdx.x, E = T.E, k = T.k;
+    if (q >= rows)
+        return;
+    uint32_t *m = rso32 + (size_t)q * k, *e = rsio32 + (size_t)q * E, *sy = syn + (size_t)q * E;
+    uint32_t *ev = s_mem, *loc = s_mem + 2 * E, *tmp = s_mem + 3 * E, *der = s_mem + 5 * E, *chunk = s_mem + 6 * E;
+    uint32_t *pos = s_mem + 7 * E, *err = s_mem + 8 * E;
+    bool nz = false;
+    for (uint32_t j = lane; j < E; j += 64)
+        nz |= sy[j] != 0;
+    if (!__ballot(nz))
+        return;
+    uint32_t cnt = 0;
+    for (uint32_t j = lane; j < k + E; j += 64)
+        cnt += (j < k ? m[j] : e[j - k]) == 0 ? 1u : 0u;
+    for (int o = 32; o > 0; o >>= 1)
+        cnt += (uint32_t)__shfl_xor((int)cnt, o);
+    if (cnt == 0 || cnt > E)  // none, or too many zeros: left as they are
+        return;
+    // the zero positions in ascending order (message part, then parity part)
+    uint32_t base = 0;
+    for (uint32_t j0 = 0; j0 < k + E; j0 += 64) {
+        const uint32_t j = j0 + lane;
+        const bool z = j < k + E && (j < k ? m[j] : e[j - k]) == 0;
+        const uint64_t mk = __ballot(z);
+        if (z)
+            pos[base + __builtin_amdgcn_mbcnt_hi((uint32_t)(mk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)mk, 0u))] = j;
+        base += (uint32_t)__builtin_popcountll(mk);
+    }
+    for (uint32_t j = lane; j < E; j += 64)
+        ev[j] = sy[j];
+    wave_sync();
+    Grp G{lane, 64};
+    known_errors(T, pos, cnt, ev, loc, tmp, der, G, [&](uint32_t r, uint32_t, uint32_t v) { err[r] = v; });
+    wave_sync();
+    bool bad = false;
+    for (uint32_t r = lane; r < cnt; r += 64)
+        bad |= (err[r] & 0xffffu) != 0;  // the symbol was 0: the repaired value is the error
+    if (__ballot(bad))
+        return;
+    for (uint32_t r = lane; r < cnt; r += 64) {
+        const uint32_t p = pos[r];
+        if (p < k)
+            m[p] = err[r];
+        else
+            e[p - k] = err[r];
+    }
+    __syncthreads();  // the stores above are read back by other lanes below
+    pntt_codeword<uint32_t, uint32_t>(T, m, e, chunk, ev, G);  // synd_block (pyrsi16.hpp:186-189)
+    for (uint32_t j = lane; j < E; j += 64)
+        sy[j] = ev[j];
+}
+
+// repair_inner_zeros (pycirc16.hpp:246-289) and find_error_locations (:291-295), one wave per
+// inner codeword q: where the syndromes are not all zero and its parity holds a zero, mix_ecc of
+// the syndromes (pyrsi16.hpp:218-220); where that is 0 mod 0x10000 everywhere it becomes the
+// parity and the syndromes are cleared. flag[q] = the syndromes are not all zero afterwards.
+__global__ __launch_bounds__(64) void circ16_inner_zeros_kernel(const Rsi16Dev T, uint32_t *__restrict__ rsi32,
+                                                                 uint32_t rows, uint32_t *__restrict__ syn,
+                                                                 uint8_t *__restrict__ flag)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t q = blockIdx.x, lane = threadIdx.x, E = T.E;
+    if (q >= rows)
+        return;
+    uint32_t *e = rsi32 + (size_t)q * E, *sy = syn + (size_t)q * E;
+    bool nz = false, zero = false;
+    for (uint32_t j = lane; j < E; j += 64) {
+        nz |= sy[j] != 0;
+        zero |= e[j] == 0;
+    }
+    bool any = __ballot(nz) != 0;
+    if (any && __ballot(zero)) {
+        for (uint32_t j = lane; j < E; j += 64)
+            s_mem[j] = S::mul(sy[j], T.mix[j]);
+        wave_sync();
+        gs_full<S>(s_mem, E, T.roots_i_ecc, Grp{lane, 64});
+        bool bad = false;
+        for (uint32_t j = lane; j < E; j += 64)
+            bad |= (s_mem[j] & 0xffffu) != 0;
+        if (!__ballot(bad)) {
+            for (uint32_t j = lane; j < E; j += 64) {
+                e[j] = s_mem[j];
+                sy[j] = 0;
+            }
+            any = false;
+        }
+    }
+    if (lane == 0)
+        flag[q] = any ? 1 : 0;
+}
+
+// repair_outer_errors with error locations (pycirc16.hpp:297-313 -> pyrsi16.hpp:458-522): the
+// known-position repair of outer codeword `col` (symbols at msg + i il, parity rows at
+// rso32 + (i - k) il) at the rows pos[kk * E ..] of its interleave block kk = col / inner_k,
+// where that block has 1 .. E error rows (cnt[kk]). With positions the SIMD words give each
+// column the scalar result. One wave per column.
+__global__ __launch_bounds__(64) void circ16_outer_known_kernel(const Rsi16Dev T, uint16_t *__restrict__ msg,
+                                                                 uint32_t *__restrict__ rso32, uint32_t il, uint32_t ik,
+                                                                 const uint32_t *__restrict__ pos,
+                                                                 const uint32_t *__restrict__ cnt)
+{
+    extern __shared__ uint32_t s_mem[];
+    const uint32_t col = blockIdx.x, lane = threadIdx.x, E = T.E, k = T.k;
+    if (col >= il)
+        return;
+    const uint32_t kk = col / ik, npos = cnt[kk];
+    if (npos == 0 || npos > E)
+        return;
+    uint32_t *ev = s_mem, *loc = s_mem + 2 * E, *tmp = s_mem + 3 * E, *der = s_mem + 5 * E, *chunk = s_mem + 6 * E;
+    Grp G{lane, 64};
+    uint16_t *m = msg + col;
+    uint32_t *e = rso32 + col;
+    pntt_codeword<uint16_t, uint32_t>(T, m, e, chunk, ev, G, il, il);
+    bool nz = false;
+    for (uint32_t j = lane; j < E; j += 64)
+        nz |= ev[j] != 0;
+    if (!__ballot(nz))
+        return;
+    known_errors(T, pos + (size_t)kk * E, npos, ev, loc, tmp, der, G, [&](uint32_t, uint32_t p, uint32_t v) {
+        if (p < k)
+            m[(size_t)p * il] = (uint16_t)(S::add((uint32_t)m[(size_t)p * il], v) & 0xffffu);
+        else
+            e[(size_t)(p - k) * il] = S::add(e[(size_t)(p - k) * il], v);
+    });
+}
+
+// dump_ecc (pycirc16.hpp:323-327) of `blocks` blocks: out block b = [outer parity (u32 state)
+// & 0xFFFF | inner parity of the message rows | inner parity of the outer parity rows]
+__global__ __launch_bounds__(256) void circ16_assemble_kernel(const uint32_t *__restrict__ rso32,
+                                                               const uint16_t *__restrict__ rsi16,
+                                                               const uint16_t *__restrict__ rsio16, uint32_t rso_len,
+                                                               uint32_t rsi_len, uint32_t rsio_len, uint32_t blocks,
+                                                               uint16_t *__restrict__ out)
+{
+    const uint32_t el = rso_len + rsi_len + rsio_len;
+    const size_t total = (size_t)el * blocks;
+    for (size_t t = (size_t)blockIdx.x * 256 + threadIdx.x; t < total; t += (size_t)gridDim.x * 256) {
+        const size_t b = t / el;
+        const uint32_t x = (uint32_t)(t - b * el);
+        uint16_t v;
+        if (x < rso_len)
+            v = (uint16_t)(rso32[b * rso_len + x] & 0xffffu);
+        else if (x < rso_len + rsi_len)
+            v = rsi16[b * rsi_len + (x - rso_len)];
+        else
+            v = rsio16[b * rsio_len + (x - rso_len - rsi_len)];
+        out[t] = v;
     }
 }
 
@@ -3108,7 +3330,7 @@ int ffrs_rsi16_repair_interleaved_simd(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *
         auto *rk = W == 4 ? rsi16_repair_simd_kernel<4> : W == 8 ? rsi16_repair_simd_kernel<8>
                                                                   : rsi16_repair_simd_kernel<16>;
         hipLaunchKernelGGL(rk, grid, dim3(64), 0, ctx->stream, rs->dev, (const uint16_t *)d_m,
-                           (const uint16_t *)d_e, il, g0, lanes, (uint32_t *)rs->d_work, (uint8_t *)d_s);
+                           (const uint16_t *)d_e, il, 0u, il, g0, lanes, (uint32_t *)rs->d_work, (uint8_t *)d_s);
         FFRS_HIP(hipGetLastError());
         FFRS_HIP(hipMemcpyAsync(gst.data() + g0, (uint8_t *)d_s + g0, ng, hipMemcpyDeviceToHost, ctx->stream));
         FFRS_HIP(hipStreamSynchronize(ctx->stream));
@@ -3120,8 +3342,8 @@ int ffrs_rsi16_repair_interleaved_simd(ffrs_rsi16 *rs, uint16_t *msg, uint16_t *
         if (end > g0) {
             auto *wk = W == 4 ? rsi16_simd_writeback_kernel<4> : W == 8 ? rsi16_simd_writeback_kernel<8>
                                                                         : rsi16_simd_writeback_kernel<16>;
-            hipLaunchKernelGGL(wk, grid, dim3(256), 0, ctx->stream, rs->dev, (uint16_t *)d_m, (uint16_t *)d_e, il, g0,
-                               end, lanes, (const uint32_t *)rs->d_work);
+            hipLaunchKernelGGL(wk, grid, dim3(256), 0, ctx->stream, rs->dev, (uint16_t *)d_m, (uint16_t *)d_e, il, 0u,
+                               il, g0, end, lanes, (const uint32_t *)rs->d_work);
             FFRS_HIP(hipGetLastError());
         }
     }
@@ -3720,6 +3942,376 @@ int ffrs_rsi16_plan_repair(ffrs_rsi16_plan *plan, uint16_t *d_msgs, size_t msg_e
     if (!plan || (plan->count && (!d_msgs || !d_eccs || !d_status)))
         return fail(FFRS_ERR_INVALID, "NULL argument");
     return rsi16_plan_run(plan, 2, d_msgs, msg_elems, d_eccs, ecc_elems, d_status);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- CIRC16 (host)
+
+struct ffrs_circ16 {
+    ffrs_rsi16 *ri = nullptr, *ro = nullptr;  // inner and outer codecs (not owned)
+    uint32_t I = 1, W = 1;                     // interleave blocks; lanes of the SIMD words (1: scalar)
+    uint32_t ik = 0, iE = 0, ok = 0, oE = 0, on = 0, rI = 0;
+    size_t msg_len = 0, rso_len = 0, rsi_len = 0, rsio_len = 0, ecc_len = 0;
+    void *d_buf = nullptr;
+    size_t buf_size = 0;
+};
+
+namespace {
+
+int circ_reserve(ffrs_circ16 *c, size_t bytes)
+{
+    if (c->buf_size >= bytes)
+        return FFRS_OK;
+    if (c->d_buf)
+        FFRS_HIP(hipFree(c->d_buf));
+    c->d_buf = nullptr;
+    c->buf_size = 0;
+    hipError_t e = hipMalloc(&c->d_buf, bytes);
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_NOMEM, "CIRC16 workspace of %zu bytes: %s", bytes, hipGetErrorString(e));
+    c->buf_size = bytes;
+    return FFRS_OK;
+}
+
+size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+// The generic lane-group encode of `count` codewords (interleave stride I), u16 or u32 symbols in
+// and out, in the SIMD or scalar words.
+template <class TI, class TO>
+int circ_encode(ffrs_rsi16 *rs, const TI *d_in, size_t count, TO *d_out, bool vec, uint32_t I)
+{
+    if (count == 0)
+        return FFRS_OK;
+    const uint32_t g = rs->g_enc == 1 && rs->E <= 32 ? group_size(rs->E) : rs->g_enc, cpw = 64 / g;
+    const size_t lds = (size_t)cpw * 2 * rs->E * 4;
+    auto *kv = rsi16_encode_kernel<VectorOps, TI, TO>;
+    auto *ks = rsi16_encode_kernel<ScalarOps, TI, TO>;
+    auto *kern = vec ? kv : ks;
+    if (int r = allow_lds(kern, lds))
+        return r;
+    hipLaunchKernelGGL(kern, dim3((unsigned)((count + cpw - 1) / cpw)), dim3(64), lds, rs->ctx->stream, d_in,
+                       (uint32_t)count, d_out, rs->dev, g, I);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+// Repair workspace of one block: u16 message and ecc, u32 outer parity, u32 inner parity,
+// u32 syndromes, row flags, error rows per interleave block and their counts, group statuses.
+struct CircWs {
+    uint16_t *msg, *ecc;
+    uint32_t *rso32, *rsi32, *syn, *pos, *cnt;
+    uint8_t *flag, *gst;
+    uint32_t *work;
+    size_t work_lanes;
+};
+
+size_t circ_repair_layout(const ffrs_circ16 *c, CircWs *w, char *base)
+{
+    const size_t rows = (size_t)c->on * c->I, ngr = (c->ik + c->W - 1) / c->W;
+    const size_t wrows = (size_t)c->ro->n + 8 * (size_t)c->oE;  // per-lane words of the SIMD repair
+    size_t lanes = ngr * c->W;
+    const size_t cap = std::max<size_t>(c->W, ((size_t)256 << 20) / (wrows * 4) / c->W * c->W);
+    lanes = std::min(lanes, cap);
+    size_t o = 0;
+    auto take = [&](size_t bytes) {
+        char *p = base ? base + o : nullptr;
+        o += al256(bytes);
+        return p;
+    };
+    w->msg = (uint16_t *)take(c->msg_len * 2);
+    w->ecc = (uint16_t *)take(c->ecc_len * 2);
+    w->rso32 = (uint32_t *)take(c->rso_len * 4);
+    w->rsi32 = (uint32_t *)take(rows * c->iE * 4);
+    w->syn = (uint32_t *)take(rows * c->iE * 4);
+    w->flag = (uint8_t *)take(rows);
+    w->pos = (uint32_t *)take((size_t)c->I * c->oE * 4);
+    w->cnt = (uint32_t *)take((size_t)c->I * 4);
+    w->gst = (uint8_t *)take(ngr);
+    w->work = (uint32_t *)take(lanes * wrows * 4);
+    w->work_lanes = lanes;
+    return o;
+}
+
+// load_ecc, compute_rsi_synd, repair_outer_zeros, repair_inner_zeros and the row flags of
+// find_error_locations, for every interleave block at once (they touch disjoint rows).
+int circ_prepare(ffrs_circ16 *c, const CircWs &w)
+{
+    hipStream_t st = c->ri->ctx->stream;
+    const uint32_t rows = c->on * c->I, mrows = c->ok * c->I, erows = c->oE * c->I, iE = c->iE;
+    const size_t rsi_all = (size_t)rows * iE;
+    hipLaunchKernelGGL(circ16_expand_kernel, dim3(256), dim3(256), 0, st, (const uint16_t *)w.ecc, w.rso32,
+                       (uint32_t)c->rso_len, w.rsi32, (uint32_t)rsi_all);
+    FFRS_HIP(hipGetLastError());
+    const uint32_t g = group_size(iE), cpw = 64 / g;
+    const size_t lds = (size_t)cpw * 2 * iE * 4;
+    if (int r = allow_lds(circ16_synd_kernel<uint16_t>, lds))
+        return r;
+    if (int r = allow_lds(circ16_synd_kernel<uint32_t>, lds))
+        return r;
+    hipLaunchKernelGGL(circ16_synd_kernel<uint16_t>, dim3((mrows + cpw - 1) / cpw), dim3(64), lds, st, c->ri->dev,
+                       (const uint16_t *)w.msg, (const uint32_t *)w.rsi32, mrows, w.syn, g);
+    FFRS_HIP(hipGetLastError());
+    hipLaunchKernelGGL(circ16_synd_kernel<uint32_t>, dim3((erows + cpw - 1) / cpw), dim3(64), lds, st, c->ri->dev,
+                       (const uint32_t *)w.rso32, (const uint32_t *)(w.rsi32 + (size_t)mrows * iE), erows,
+                       w.syn + (size_t)mrows * iE, g);
+    FFRS_HIP(hipGetLastError());
+    const size_t lz = (size_t)9 * iE * 4;
+    if (int r = allow_lds(circ16_outer_zeros_kernel, lz))
+        return r;
+    hipLaunchKernelGGL(circ16_outer_zeros_kernel, dim3(erows), dim3(64), lz, st, c->ri->dev, w.rso32,
+                       w.rsi32 + (size_t)mrows * iE, erows, w.syn + (size_t)mrows * iE);
+    FFRS_HIP(hipGetLastError());
+    if (int r = allow_lds(circ16_inner_zeros_kernel, (size_t)iE * 4))
+        return r;
+    hipLaunchKernelGGL(circ16_inner_zeros_kernel, dim3(rows), dim3(64), (size_t)iE * 4, st, c->ri->dev, w.rsi32,
+                       rows, w.syn, w.flag);
+    FFRS_HIP(hipGetLastError());
+    return FFRS_OK;
+}
+
+// The error rows of interleave block kk from the row flags (find_error_locations: rows i with
+// non-zero inner syndromes of codeword i I + kk, ascending).
+std::vector<uint32_t> circ_rows(const ffrs_circ16 *c, const std::vector<uint8_t> &flag, uint32_t kk)
+{
+    std::vector<uint32_t> rows;
+    for (uint32_t i = 0; i < c->on; ++i)
+        if (flag[(size_t)i * c->I + kk])
+            rows.push_back(i);
+    return rows;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffrs_circ16_create(ffrs_rsi16 *inner, ffrs_rsi16 *outer, size_t interleave, unsigned simd_width, ffrs_circ16 **out)
+{
+    if (!inner || !outer || !out)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    if (inner->ctx != outer->ctx)
+        return fail(FFRS_ERR_INVALID, "inner and outer codecs belong to different contexts");
+    if (inner->primitive != outer->primitive)
+        return fail(FFRS_ERR_INVALID, "inner and outer codecs use different primitives");
+    if (simd_width != 1 && simd_width != 4 && simd_width != 8 && simd_width != 16)
+        return fail(FFRS_ERR_INVALID, "SIMD width must be 1, 4, 8 or 16, got %u", simd_width);
+    if (interleave == 0 || (size_t)inner->k * interleave > 0xffffu)
+        return fail(FFRS_ERR_INVALID, "inner message_len * interleave must be in [1, 65535], got %zu",
+                    (size_t)inner->k * interleave);
+    auto *c = new ffrs_circ16;
+    c->ri = inner;
+    c->ro = outer;
+    c->I = (uint32_t)interleave;
+    c->W = simd_width;
+    c->ik = inner->k;
+    c->iE = inner->E;
+    c->ok = outer->k;
+    c->oE = outer->E;
+    c->on = outer->n;
+    c->rI = c->ik * c->I;
+    c->msg_len = (size_t)c->ok * c->rI;
+    c->rso_len = (size_t)c->oE * c->rI;
+    c->rsi_len = (size_t)c->iE * c->ok * c->I;
+    c->rsio_len = (size_t)c->iE * c->oE * c->I;
+    c->ecc_len = c->rso_len + c->rsi_len + c->rsio_len;
+    *out = c;
+    return FFRS_OK;
+}
+
+void ffrs_circ16_destroy(ffrs_circ16 *c)
+{
+    if (!c)
+        return;
+    if (c->d_buf) {
+        (void)hipSetDevice(c->ri->ctx->device);
+        (void)hipFree(c->d_buf);
+    }
+    delete c;
+}
+
+size_t ffrs_circ16_message_len(const ffrs_circ16 *c) { return c ? c->msg_len : 0; }
+size_t ffrs_circ16_ecc_len(const ffrs_circ16 *c) { return c ? c->ecc_len : 0; }
+
+int ffrs_circ16_encode(ffrs_circ16 *c, const uint16_t *msg, size_t blocks, uint16_t *ecc)
+{
+    if (!c)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (blocks == 0)
+        return FFRS_OK;
+    if (!msg || !ecc)
+        return fail(FFRS_ERR_INVALID, "NULL buffer");
+    if (int r = check_count(blocks * c->rI))
+        return r;
+    ffrs_ctx *ctx = c->ri->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    const size_t mb = al256(blocks * c->msg_len * 2), ob = al256(blocks * c->rso_len * 4),
+                 ib = al256(blocks * c->rsi_len * 2), iob = al256(blocks * c->rsio_len * 2),
+                 eb = al256(blocks * c->ecc_len * 2);
+    if (int r = circ_reserve(c, mb + ob + ib + iob + eb))
+        return r;
+    char *base = (char *)c->d_buf;
+    uint16_t *d_msg = (uint16_t *)base, *d_rsi = (uint16_t *)(base + mb + ob), *d_rsio = (uint16_t *)(base + mb + ob + ib),
+             *d_out = (uint16_t *)(base + mb + ob + ib + iob);
+    uint32_t *d_rso = (uint32_t *)(base + mb);
+    hipStream_t st = ctx->stream;
+    const bool vec = c->W > 1;
+    FFRS_HIP(hipMemcpyAsync(d_msg, msg, blocks * c->msg_len * 2, hipMemcpyHostToDevice, st));
+    // encode_block (pycirc16.hpp:371-381): the outer parity as field elements, then the inner
+    // parity of the message rows and of the outer parity rows
+    if (int r = circ_encode<uint16_t, uint32_t>(c->ro, d_msg, blocks * c->rI, d_rso, vec, c->rI))
+        return r;
+    if (int r = launch_encode(c->ri, d_msg, blocks * c->ok * c->I, d_rsi, vec ? 1 : 0))
+        return r;
+    if (int r = circ_encode<uint32_t, uint16_t>(c->ri, d_rso, blocks * c->oE * c->I, d_rsio, vec, 1))
+        return r;
+    hipLaunchKernelGGL(circ16_assemble_kernel, dim3(1024), dim3(256), 0, st, (const uint32_t *)d_rso,
+                       (const uint16_t *)d_rsi, (const uint16_t *)d_rsio, (uint32_t)c->rso_len, (uint32_t)c->rsi_len,
+                       (uint32_t)c->rsio_len, (uint32_t)blocks, d_out);
+    FFRS_HIP(hipGetLastError());
+    FFRS_HIP(hipMemcpyAsync(ecc, d_out, blocks * c->ecc_len * 2, hipMemcpyDeviceToHost, st));
+    FFRS_HIP(hipStreamSynchronize(st));
+    return FFRS_OK;
+}
+
+int ffrs_circ16_find_outer_error_locations(ffrs_circ16 *c, const uint16_t *msg, const uint16_t *ecc,
+                                           size_t interleave, uint32_t *rows, size_t *count)
+{
+    if (!c || !msg || !ecc || !rows || !count)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    if (interleave >= c->I)
+        return fail(FFRS_ERR_PARAM, "interleave %zu >= %u", interleave, c->I);
+    ffrs_ctx *ctx = c->ri->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    CircWs w;
+    if (int r = circ_reserve(c, circ_repair_layout(c, &w, nullptr)))
+        return r;
+    circ_repair_layout(c, &w, (char *)c->d_buf);
+    hipStream_t st = ctx->stream;
+    FFRS_HIP(hipMemcpyAsync(w.msg, msg, c->msg_len * 2, hipMemcpyHostToDevice, st));
+    FFRS_HIP(hipMemcpyAsync(w.ecc, ecc, c->ecc_len * 2, hipMemcpyHostToDevice, st));
+    if (int r = circ_prepare(c, w))
+        return r;
+    std::vector<uint8_t> flag((size_t)c->on * c->I);
+    FFRS_HIP(hipMemcpyAsync(flag.data(), w.flag, flag.size(), hipMemcpyDeviceToHost, st));
+    FFRS_HIP(hipStreamSynchronize(st));
+    auto v = circ_rows(c, flag, (uint32_t)interleave);
+    std::copy(v.begin(), v.end(), rows);
+    *count = v.size();
+    return FFRS_OK;
+}
+
+int ffrs_circ16_repair(ffrs_circ16 *c, uint16_t *msg, uint16_t *ecc, int *raised)
+{
+    if (!c || !msg || !ecc || !raised)
+        return fail(FFRS_ERR_INVALID, "NULL argument");
+    *raised = 0;
+    ffrs_ctx *ctx = c->ri->ctx;
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    if (int r = ctx_set_device(ctx))
+        return r;
+    CircWs w;
+    if (int r = circ_reserve(c, circ_repair_layout(c, &w, nullptr)))
+        return r;
+    circ_repair_layout(c, &w, (char *)c->d_buf);
+    hipStream_t st = ctx->stream;
+    FFRS_HIP(hipMemcpyAsync(w.msg, msg, c->msg_len * 2, hipMemcpyHostToDevice, st));
+    FFRS_HIP(hipMemcpyAsync(w.ecc, ecc, c->ecc_len * 2, hipMemcpyHostToDevice, st));
+    if (int r = circ_prepare(c, w))
+        return r;
+    std::vector<uint8_t> flag((size_t)c->on * c->I);
+    FFRS_HIP(hipMemcpyAsync(flag.data(), w.flag, flag.size(), hipMemcpyDeviceToHost, st));
+    FFRS_HIP(hipStreamSynchronize(st));
+    // repair_outer_errors (pycirc16.hpp:297-313) per interleave block: with the error rows as
+    // positions where there are at most outer E of them, else the repair without positions
+    std::vector<uint32_t> pos((size_t)c->I * c->oE, 0), cnt(c->I, 0);
+    std::vector<uint32_t> blind;  // blocks repaired without positions, ascending
+    for (uint32_t kk = 0; kk < c->I; ++kk) {
+        auto rv = circ_rows(c, flag, kk);
+        if (rv.empty())
+            continue;
+        if (rv.size() <= c->oE) {
+            cnt[kk] = (uint32_t)rv.size();
+            std::copy(rv.begin(), rv.end(), pos.begin() + (size_t)kk * c->oE);
+        } else {
+            blind.push_back(kk);
+        }
+    }
+    if (std::any_of(cnt.begin(), cnt.end(), [](uint32_t v) { return v != 0; })) {
+        FFRS_HIP(hipMemcpyAsync(w.pos, pos.data(), pos.size() * 4, hipMemcpyHostToDevice, st));
+        FFRS_HIP(hipMemcpyAsync(w.cnt, cnt.data(), cnt.size() * 4, hipMemcpyHostToDevice, st));
+        const size_t lds = (size_t)7 * c->oE * 4;
+        if (int r = allow_lds(circ16_outer_known_kernel, lds))
+            return r;
+        hipLaunchKernelGGL(circ16_outer_known_kernel, dim3(c->rI), dim3(64), lds, st, c->ro->dev, w.msg, w.rso32,
+                           c->rI, c->ik, (const uint32_t *)w.pos, (const uint32_t *)w.cnt);
+        FFRS_HIP(hipGetLastError());
+    }
+    // without positions: the reference's SIMD groups of W columns from the block's first column
+    // (pyrsi16.hpp:391-456), in block order; a group that raises stops the repair there
+    const uint32_t W = c->W, ngr = (c->ik + W - 1) / W;
+    const uint32_t gpb = (uint32_t)std::max<size_t>(1, w.work_lanes / W);
+    uint32_t raise_kk = c->I;
+    std::vector<uint8_t> gst(ngr);
+    for (uint32_t kk : blind) {
+        bool stop = false;
+        for (uint32_t g0 = 0; g0 < ngr && !stop; g0 += gpb) {
+            const uint32_t ng = std::min(gpb, ngr - g0), lanes = ng * W;
+            const dim3 grid((lanes + 63) / 64);
+            auto *rk = W == 1 ? rsi16_repair_simd_kernel<1, uint32_t> : W == 4 ? rsi16_repair_simd_kernel<4, uint32_t>
+                       : W == 8 ? rsi16_repair_simd_kernel<8, uint32_t> : rsi16_repair_simd_kernel<16, uint32_t>;
+            hipLaunchKernelGGL(rk, grid, dim3(64), 0, st, c->ro->dev, (const uint16_t *)w.msg,
+                               (const uint32_t *)w.rso32, c->rI, kk * c->ik, c->ik, g0, lanes, w.work, w.gst);
+            FFRS_HIP(hipGetLastError());
+            FFRS_HIP(hipMemcpyAsync(gst.data() + g0, w.gst + g0, ng, hipMemcpyDeviceToHost, st));
+            FFRS_HIP(hipStreamSynchronize(st));
+            uint32_t end = g0;
+            while (end < g0 + ng && gst[end] != FFRS_RSI16_RAISES)
+                ++end;
+            stop = end < g0 + ng;
+            if (end > g0) {
+                auto *wk = W == 1 ? rsi16_simd_writeback_kernel<1, uint32_t>
+                           : W == 4 ? rsi16_simd_writeback_kernel<4, uint32_t>
+                           : W == 8 ? rsi16_simd_writeback_kernel<8, uint32_t> : rsi16_simd_writeback_kernel<16, uint32_t>;
+                hipLaunchKernelGGL(wk, grid, dim3(256), 0, st, c->ro->dev, w.msg, w.rso32, c->rI, kk * c->ik, c->ik, g0,
+                                   end, lanes, (const uint32_t *)w.work);
+                FFRS_HIP(hipGetLastError());
+            }
+        }
+        if (stop) {
+            raise_kk = kk;
+            break;
+        }
+    }
+    if (raise_kk < c->I) {
+        // the reference raises inside repair_all: the blocks before raise_kk are repaired, block
+        // raise_kk up to its raising group, the others untouched; ecc is not written back
+        std::vector<uint16_t> rep(c->msg_len);
+        FFRS_HIP(hipMemcpyAsync(rep.data(), w.msg, c->msg_len * 2, hipMemcpyDeviceToHost, st));
+        FFRS_HIP(hipStreamSynchronize(st));
+        const size_t lim = (size_t)(raise_kk + 1) * c->ik;
+        for (size_t r = 0; r < c->ok; ++r)
+            std::copy(rep.begin() + r * c->rI, rep.begin() + r * c->rI + lim, msg + r * c->rI);
+        *raised = 1;
+        return FFRS_OK;
+    }
+    // recompute_inner_ecc (pycirc16.hpp:317-321) and dump_ecc
+    const bool vec = W > 1;
+    uint16_t *d_rsi = w.ecc + c->rso_len, *d_rsio = d_rsi + c->rsi_len;
+    if (int r = launch_encode(c->ri, w.msg, (size_t)c->ok * c->I, d_rsi, vec ? 1 : 0))
+        return r;
+    if (int r = circ_encode<uint32_t, uint16_t>(c->ri, w.rso32, (size_t)c->oE * c->I, d_rsio, vec, 1))
+        return r;
+    hipLaunchKernelGGL(circ16_assemble_kernel, dim3(64), dim3(256), 0, st, (const uint32_t *)w.rso32,
+                       (const uint16_t *)d_rsi, (const uint16_t *)d_rsio, (uint32_t)c->rso_len, 0u, 0u, 1u, w.ecc);
+    FFRS_HIP(hipGetLastError());
+    FFRS_HIP(hipMemcpyAsync(msg, w.msg, c->msg_len * 2, hipMemcpyDeviceToHost, st));
+    FFRS_HIP(hipMemcpyAsync(ecc, w.ecc, c->ecc_len * 2, hipMemcpyDeviceToHost, st));
+    FFRS_HIP(hipStreamSynchronize(st));
+    return FFRS_OK;
 }
 
 }  // extern "C"

@@ -346,3 +346,147 @@ def poly_vectors(s, W, E, lens):
         if lens[v] > 0 and x[v, lens[v] - 1] == 0:
             x[v, lens[v] - 1] = 1
     return x.reshape(-1)
+
+
+# CIRC16: (inner_block_len, inner_ecc_len, outer_block_len, outer_ecc_len, interleave, SIMD width)
+# -- the reference's own test shapes (tests/test_lib_circ16.py:253-300) with their flag variants
+CIRC16_CASES = (
+    (8, 2, 8, 2, 1, 16), (16, 4, 16, 4, 1, 16), (32, 16, 4, 2, 1, 16), (128, 8, 128, 8, 1, 16),
+    (512, 4, 512, 32, 1, 16), (32768, 4, 16, 4, 1, 16), (16, 4, 32768, 4, 1, 16),
+    (4, 2, 4, 2, 1, 8), (16, 4, 16, 4, 1, 8), (128, 8, 128, 8, 1, 8), (512, 4, 512, 32, 1, 8),
+    (4, 2, 4, 2, 1, 4), (32, 16, 4, 2, 1, 4), (128, 8, 128, 8, 1, 4),
+    (4, 2, 4, 2, 1, 1), (16, 4, 16, 4, 1, 1), (32, 16, 4, 2, 1, 1), (128, 8, 128, 8, 1, 1),
+    (128, 2, 32, 2, 32, 16), (512, 4, 128, 8, 8, 16), (128, 8, 128, 8, 3, 16), (128, 8, 128, 8, 4, 16),
+    (128, 8, 128, 8, 4, 8), (128, 8, 128, 8, 4, 4), (128, 8, 128, 8, 4, 1),
+    (12, 2, 24, 2, 1, 16), (100, 2, 100, 2, 1, 16), (52, 2, 192, 64, 1, 16), (100, 2, 100, 2, 3, 16),
+    (52, 2, 192, 64, 7, 16),
+)
+
+
+def circ16_flags(w):
+    return {16: {}, 8: dict(simd_x16=False), 4: dict(simd_x16=False, simd_x8=False),
+            1: dict(simd_x16=False, simd_x8=False, simd_x4=False)}[w]
+
+
+def circ16_tag(case):
+    return "_".join(str(x) for x in case)
+
+
+class CircLayout:
+    """Offsets of pycirc16.hpp:423-445 for one block."""
+
+    def __init__(self, ibl, iecc, obl, oecc, il):
+        self.ik, self.iE, self.ok, self.oE, self.on, self.I = ibl - iecc, iecc, obl - oecc, oecc, obl, il
+        self.rI = self.ik * il
+        self.msg_len = self.ok * self.rI
+        self.rso_len = self.oE * self.rI
+        self.ecc_len = self.rso_len + self.iE * self.on * il
+
+    def message_offset(self, il, row, col):
+        return self.rI * row + il * self.ik + col
+
+    def rso_ecc_offset(self, il, row, col):
+        return self.rI * row + il * self.ik + col
+
+    def rsi_ecc_offset(self, il, row, col):
+        return self.rso_len + (row * self.I + il) * self.iE + col
+
+
+def circ16_xor(s, n):
+    """n u16 masks with both bytes non-zero (the reference's tests XOR each byte with 1..255)."""
+    lo = s.ints(1, 255, n)
+    hi = s.ints(1, 255, n)
+    return (lo | (hi << 8)).astype(np.uint16)
+
+
+def circ16_corrupt_rows(lay, msg, ecc, rows, ils, s):
+    """tests/test_lib_circ16.py:68-96: every symbol of the given outer rows (message or outer
+    parity, and their inner parity) of the given interleave blocks XOR a mask."""
+    for row in rows:
+        for il in ils:
+            if row < lay.ok:
+                idx = [lay.message_offset(il, row, c) for c in range(lay.ik)]
+                msg[idx] ^= circ16_xor(s, len(idx))
+            else:
+                idx = [lay.rso_ecc_offset(il, row - lay.ok, c) for c in range(lay.ik)]
+                ecc[idx] ^= circ16_xor(s, len(idx))
+            idx = [lay.rsi_ecc_offset(il, row, c) for c in range(lay.iE)]
+            ecc[idx] ^= circ16_xor(s, len(idx))
+
+
+def circ16_rows(s, lay, n):
+    """n distinct outer rows."""
+    return sorted(int(x) for x in s.distinct_positions([n], lay.on)[0])
+
+
+def circ16_message(i, case):
+    lay = CircLayout(*case[:5])
+    return lay, W.Stream(1600 + i).u16(lay.msg_len)
+
+
+def circ16_scenarios(i, case, lay, msg, ecc):
+    """(name, message, ecc) repair inputs of case i: clean; the outer rows of block 0 corrupted
+    (outer_ecc_len rows, as the reference's test_circ_repair); the same rows in every block; every
+    inner parity of the message rows replaced and half as many rows corrupted
+    (test_repair_fallback); more corrupted rows than outer_ecc_len (the repair without positions)."""
+    s = W.Stream(1700 + i)
+    out = [("none", msg.copy(), ecc.copy())]
+    m, e = msg.copy(), ecc.copy()
+    circ16_corrupt_rows(lay, m, e, circ16_rows(s, lay, lay.oE), [0], s)
+    out.append(("rows", m, e))
+    if lay.I > 1:
+        m, e = msg.copy(), ecc.copy()
+        circ16_corrupt_rows(lay, m, e, circ16_rows(s, lay, lay.oE), list(range(lay.I)), s)
+        out.append(("rowsall", m, e))
+    m, e = msg.copy(), ecc.copy()
+    n_rsi = lay.ok * lay.I * lay.iE
+    e[lay.rso_len:lay.rso_len + n_rsi] = s.u16(n_rsi)
+    circ16_corrupt_rows(lay, m, e, circ16_rows(s, lay, max(1, lay.oE // 2)), [0], s)
+    out.append(("fallback", m, e))
+    m, e = msg.copy(), ecc.copy()
+    circ16_corrupt_rows(lay, m, e, circ16_rows(s, lay, min(lay.on, lay.oE + 1 + lay.oE // 2)), [0], s)
+    out.append(("many", m, e))
+    return out
+
+
+def circ16_find_inputs(i, case, lay, msg, ecc):
+    """(interleave block, message) inputs of _find_outer_error_locations: the diagonal of block il
+    corrupted (test_find_errors_diagonal) and one message row of it (test_find_errors_row)."""
+    s = W.Stream(1800 + i)
+    out = []
+    for il in range(min(lay.I, 3)):
+        m = msg.copy()
+        for r in range(min(lay.ik, lay.ok)):
+            m[lay.message_offset(il, r, r)] ^= circ16_xor(s, 1)[0]
+        out.append((il, m))
+        m = msg.copy()
+        row = int(s.ints(0, lay.ok - 1, 1)[0])
+        cols = s.distinct_positions([min(5, lay.ik)], lay.ik)[0]
+        for c in cols:
+            m[lay.message_offset(il, row, int(c))] ^= circ16_xor(s, 1)[0]
+        out.append((il, m))
+    return out
+
+
+CIRC16_PLANT_CASES = ((8, 2, 8, 2, 1, 16), (16, 4, 16, 4, 1, 16), (16, 4, 16, 4, 1, 1), (4, 2, 4, 2, 1, 4),
+                      (128, 8, 128, 8, 1, 16), (128, 8, 128, 8, 4, 8), (100, 2, 100, 2, 3, 16), (32, 16, 4, 2, 1, 1))
+
+
+def circ16_planted_scenarios(i, kind, lay, msg, ecc):
+    """Repair inputs of a planted block: clean, and outer_ecc_len rows of block 0 corrupted."""
+    s = W.Stream(1950 + 2 * i + (kind == "inner"))
+    m, e = msg.copy(), ecc.copy()
+    circ16_corrupt_rows(lay, m, e, circ16_rows(s, lay, lay.oE), [0], s)
+    return [("none", msg.copy(), ecc.copy()), ("rows", m, e)]
+
+
+def circ16_planted_message(i, kind, lay, at, values):
+    """A seeded message with `values` planted: outer column `at` (kind 'outer') or inner message
+    row block `at` (kind 'inner')."""
+    m = W.Stream(2000 + 2 * i + (kind == "inner")).u16(lay.msg_len)
+    v = np.asarray(values, np.uint16)
+    if kind == "outer":
+        m[[lay.rI * r + at for r in range(lay.ok)]] = v
+    else:
+        m[at * lay.ik:(at + 1) * lay.ik] = v
+    return m

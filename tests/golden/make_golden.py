@@ -236,6 +236,8 @@ def main():
     out.update(divergence_fixture(L))
     out.update(ilsimd_fixture(L))
     out.update(ntt_fixture(L))
+    out.update(circ16_fixture(L))
+    out.update(circ16_planted_fixture(L))
 
     for key, d in out.items():
         np.savez_compressed(os.path.join(HERE, key + ".npz"), **d)
@@ -538,6 +540,132 @@ def ntt_fixture(L):
     return {"rsi16_ntt": d}
 
 
+def circ16_fixture(L):
+    """(k) CIRC16 (pycirc16.hpp:33-431) on the reference's own test shapes and flags: per case the
+    encode of a seeded message (ecc explicit where small, else its CRC), the repair of each
+    scenario of cases.circ16_scenarios (input digests, output CRCs, whether it raises) and the
+    outer error rows of cases.circ16_find_inputs."""
+    d = {}
+    for i, case in enumerate(C.CIRC16_CASES):
+        tag = C.circ16_tag(case)
+        ref = L.CIRC16(*case[:5], **C.circ16_flags(case[5]))
+        lay, msg = C.circ16_message(i, case)
+        assert ref.message_len == lay.msg_len and ref.ecc_len == lay.ecc_len
+        ecc = np.frombuffer(ref.encode(msg.tobytes()), np.uint16).copy()
+        d[f"{tag}_msg_sha"] = C.sha(msg)
+        d[f"{tag}_ecc_crc"] = C.row_crc(ecc[None])
+        if ecc.size <= 8192:
+            d[f"{tag}_ecc"] = ecc
+        names, st = [], []
+        for name, m, e in C.circ16_scenarios(i, case, lay, msg, ecc):
+            mb, eb = bytearray(m.tobytes()), bytearray(e.tobytes())
+            d[f"{tag}_{name}_in_sha"] = C.sha(m, e)
+            try:
+                ref.repair(mb, eb)
+                raised = 0
+            except RuntimeError:
+                raised = 1
+            d[f"{tag}_{name}_raised"] = np.array([raised], np.uint8)
+            d[f"{tag}_{name}_out_crc"] = C.row_crc(np.frombuffer(bytes(mb), np.uint16)[None]).tolist() + \
+                C.row_crc(np.frombuffer(bytes(eb), np.uint16)[None]).tolist()
+            d[f"{tag}_{name}_out_crc"] = np.array(d[f"{tag}_{name}_out_crc"], np.uint32)
+            names.append(name)
+            st.append(raised)
+        finds = []
+        for il, m in C.circ16_find_inputs(i, case, lay, msg, ecc):
+            rows = ref._find_outer_error_locations(bytearray(m.tobytes()), bytearray(ecc.tobytes()), il)
+            finds.append(list(rows))
+        d[f"{tag}_find"] = np.array([r + [-1] * (lay.on + 1 - len(r)) for r in finds], np.int32)
+        print("circ16", tag, dict(zip(names, st)), [len(f) for f in finds])
+    return {"circ16": d}
+
+
+def _generator_rows(L, n, ecc):
+    """G (k x ecc, canonical) of RSi16(n, ecc) in the scalar words: row m = the parity of the
+    unit message at m (a 0 that is really 65536 recovered from the parity of 2 e_m)."""
+    rs = L.RSi16(n, ecc, simd_x4=False, simd_x8=False, simd_x16=False)
+    k = n - ecc
+    eye = np.eye(k, dtype=np.uint16)
+    g1 = np.frombuffer(rs.encode(eye.tobytes()), np.uint16).reshape(k, ecc).astype(np.int64)
+    g2 = np.frombuffer(rs.encode((2 * eye).astype(np.uint16).tobytes()), np.uint16).reshape(k, ecc).astype(np.int64)
+    return np.where((g1 == 0) & (g2 == 65535), 65536, g1)
+
+
+def _parity65536(G, msgs):
+    """rows of msgs (u16) whose linear parity holds a 65536 somewhere"""
+    v = msgs.astype(np.int64)
+    par = np.zeros((len(msgs), G.shape[1]), np.int64)
+    for j in range(0, G.shape[0], 64):  # exact in int64: 64 * 65535 * 65536 < 2^63
+        par = (par + v[:, j:j + 64] @ G[j:j + 64]) % 65537
+    return np.nonzero((par == 65536).any(1))[0]
+
+
+def circ16_planted_fixture(L):
+    """(l) CIRC16 blocks whose outer parity holds a 65536 (stored 0, its inner parity encodes
+    65536: the repair's outer-zero step) and blocks whose inner parity holds one (the inner-zero
+    step), found with the linear map and confirmed by the reference; explicit messages, the
+    reference's ecc, and the repairs of the clean block and of corrupted rows."""
+    d = {}
+    for i, case in enumerate(C.CIRC16_PLANT_CASES):
+        tag = C.circ16_tag(case)
+        ibl, iecc, obl, oecc, il, w = case
+        ref = L.CIRC16(*case[:5], **C.circ16_flags(w))
+        lay = C.CircLayout(*case[:5])
+        s = W.Stream(1900 + i)
+        msgs = {}
+        # outer: a column whose outer parity holds 65536
+        Go = _generator_rows(L, obl, oecc)
+        for _ in range(64):
+            cols = s.u16(65536 * lay.ok).reshape(65536, lay.ok)
+            hit = _parity65536(Go, cols)
+            if hit.size:
+                break
+        c0 = int(s.ints(0, lay.rI - 1, 1)[0])
+        msgs["outer"] = C.circ16_planted_message(i, "outer", lay, c0, cols[hit[0]])
+        d[f"{tag}_outer_plant"] = np.concatenate([[c0], cols[hit[0]]]).astype(np.uint32)
+        # inner: a message row whose inner parity holds 65536
+        Gi = _generator_rows(L, ibl, iecc)
+        for _ in range(64):
+            rows = s.u16(65536 * lay.ik).reshape(65536, lay.ik)
+            hit = _parity65536(Gi, rows)
+            if hit.size:
+                break
+        q = int(s.ints(0, lay.ok * lay.I - 1, 1)[0])
+        msgs["inner"] = C.circ16_planted_message(i, "inner", lay, q, rows[hit[0]])
+        d[f"{tag}_inner_plant"] = np.concatenate([[q], rows[hit[0]]]).astype(np.uint32)
+        for kind, m in msgs.items():
+            ecc = np.frombuffer(ref.encode(m.tobytes()), np.uint16).copy()
+            pre = f"{tag}_{kind}"
+            d[f"{pre}_msg_sha"], d[f"{pre}_ecc"] = C.sha(m), ecc
+            if kind == "outer":  # confirmed: the inner parity of the outer parity rows differs
+                rso = ecc[:lay.rso_len]
+                rsio = ecc[lay.ecc_len - lay.iE * lay.oE * lay.I:]
+                assert bytes(ref.rsi.encode(rso.tobytes())) != rsio.tobytes(), tag
+            else:
+                k0 = q * lay.ik
+                e0 = lay.rso_len + q * lay.iE
+                sy = ref.rsi._synd(m[k0:k0 + lay.ik].tobytes(), ecc[e0:e0 + lay.iE].tobytes())
+                assert any(sy), tag
+            restored = {}
+            for name, mm, ee in C.circ16_planted_scenarios(i, kind, lay, m, ecc):
+                mb, eb = bytearray(mm.tobytes()), bytearray(ee.tobytes())
+                d[f"{pre}_{name}_in_sha"] = C.sha(mm, ee)
+                try:
+                    ref.repair(mb, eb)
+                    raised = 0
+                except RuntimeError:
+                    raised = 1
+                d[f"{pre}_{name}_raised"] = np.array([raised], np.uint8)
+                d[f"{pre}_{name}_out_crc"] = np.array(
+                    [C.row_crc(np.frombuffer(bytes(mb), np.uint16)[None])[0],
+                     C.row_crc(np.frombuffer(bytes(eb), np.uint16)[None])[0]], np.uint32)
+                restored[name] = bytes(mb) == m.tobytes() and bytes(eb) == ecc.tobytes()
+            rows = ref._find_outer_error_locations(bytearray(m.tobytes()), bytearray(ecc.tobytes()), 0)
+            d[f"{pre}_find0"] = np.array(list(rows) + [-1], np.int32)
+            print("circ16 planted", pre, "find0", list(rows), "restored", restored)
+    return {"circ16_planted": d}
+
+
 if __name__ == "__main__":
     if sys.argv[1:] == ["interleave"]:
         for key, v in interleave_fixture(load_reference()).items():
@@ -553,6 +681,12 @@ if __name__ == "__main__":
             np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
     elif sys.argv[1:] == ["divergence"]:
         for key, v in divergence_fixture(load_reference()).items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    elif sys.argv[1:] == ["circ16"]:
+        for key, v in circ16_fixture(load_reference()).items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    elif sys.argv[1:] == ["circ16_planted"]:
+        for key, v in circ16_planted_fixture(load_reference()).items():
             np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
     elif sys.argv[1:] == ["ntt"]:
         for key, v in ntt_fixture(load_reference()).items():
