@@ -326,7 +326,9 @@ def run_plain(name, args):
         enc_names = ["rsi16_encode_mfma_kernel" if ecc % 16 == 0 and ecc <= 256 and k <= 16384 else
                      "rsi16_encode_lane_kernel" if ecc <= 32 else
                      ("rsi16_encode_reg_kernel" if ecc <= 256 else "rsi16_encode_kernel")]
-        rep_names = ["rsi16_repair_reg_kernel" if ecc <= 256 else "rsi16_repair_kernel"]
+        # syndromes as an int8 MFMA contraction ahead of the register repair where its tables exist
+        rep_names = ((["rsi16_synd_mfma_kernel"] if ecc in (16, 32, 64) and k <= 16384 else []) +
+                     ["rsi16_repair_reg_kernel" if ecc <= 256 else "rsi16_repair_kernel"])
     ke_ms = np.mean(np.array(kes), axis=0)
     kr_ms = np.mean(np.array(krs), axis=0)
     if len(ke_ms) != len(enc_names) or len(kr_ms) != len(rep_names):

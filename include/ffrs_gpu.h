@@ -161,6 +161,11 @@ int ffrs_rsi16_get_groups(const ffrs_rsi16 *rs, unsigned *g_encode, unsigned *g_
  * 0 automatic (the contraction where it applies and ecc_len <= 32, else the butterfly kernels),
  * 1 always the butterfly kernels, 2 the contraction wherever it applies. */
 int ffrs_rsi16_set_encode_kernel(ffrs_rsi16 *rs, int kernel);
+/* Where the repair computes its syndromes (no counterpart in the reference; results do not
+ * depend on it): 0 automatic, 1 inside the repair kernel, 2 as an int8 MFMA contraction of the
+ * codewords with the matrix w^(j rbo(m)) ahead of it, where its tables exist (ecc_len 16, 32 or
+ * 64, message_len <= 16384; automatic uses it there). */
+int ffrs_rsi16_set_repair_syndromes(ffrs_rsi16 *rs, int mode);
 uint32_t ffrs_rsi16_root(const ffrs_rsi16 *rs);
 size_t ffrs_rsi16_ntt_len(const ffrs_rsi16 *rs);
 

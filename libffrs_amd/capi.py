@@ -76,6 +76,7 @@ SIGNATURES = [
     ("ffrs_rsi16_repair_device", c_int, [c_vp, c_vp, c_vp, c_sz, c_vp]),
     ("ffrs_rsi16_repair_interleaved_simd", c_int, [c_vp, c_vp, c_vp, c_sz, ctypes.c_uint, c_vp]),
     ("ffrs_rsi16_set_encode_kernel", c_int, [c_vp, c_int]),
+    ("ffrs_rsi16_set_repair_syndromes", c_int, [c_vp, c_int]),
     ("ffrs_rs256_plan_create", c_int, [c_vp, c_sz, c_vp, c_vp, c_vp, c_vp, ctypes.POINTER(c_vp)]),
     ("ffrs_rs256_plan_destroy", None, [c_vp]),
     ("ffrs_rs256_plan_use_graph", c_int, [c_vp, c_int]),
@@ -275,6 +276,9 @@ class RSi16Codec:
         check(lib().ffrs_rsi16_create(ctx.h, block_len, ecc_len, primitive, ctypes.byref(h)), "ffrs_rsi16_create")
         self.h, self.ctx = h, ctx
         self.block_len, self.ecc_len, self.message_len = block_len, ecc_len, block_len - ecc_len
+
+    def set_repair_syndromes(self, mode):
+        check(lib().ffrs_rsi16_set_repair_syndromes(self.h, int(mode)), "set_repair_syndromes")
 
     def set_encode_kernel(self, kernel):
         """0 automatic, 1 the butterfly kernels, 2 the int8 MFMA contraction wherever it applies."""

@@ -1427,9 +1427,9 @@ __device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, 
 // and derivative copies for Forney, the root list and its count).
 constexpr uint32_t REG_WORDS = 4;
 
-template <int E>
+template <int E, bool SIN = false>
 __device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t *e, uint32_t *base, const LaneR<E> &L,
-                                        uint8_t *st)
+                                        uint8_t *st, const uint32_t *syn_in = nullptr)
 {
     constexpr uint32_t G = RegShape<E>::G;
     constexpr int C = RegShape<E>::C;
@@ -1438,7 +1438,12 @@ __device__ inline void repair_group_reg(const Rsi16Dev &T, uint16_t *m, uint16_t
     constexpr uint32_t PF = C >= 4 ? 1 : 4;
     PR<E> syn;
     const uint32_t nb = T.n / E;
-    for (uint32_t c0 = 0; c0 < nb; c0 += PF) {
+    if constexpr (SIN) {  // syndromes computed ahead (rsi16_synd_mfma_kernel)
+#pragma unroll
+        for (int h = 0; h < C; ++h)
+            syn.v[h] = syn_in[L.gl + (uint32_t)h * G];
+    }
+    for (uint32_t c0 = 0; !SIN && c0 < nb; c0 += PF) {
         PR<E> x[PF], sh[PF];
 #pragma unroll
         for (uint32_t u = 0; u < PF; ++u)
@@ -1638,10 +1643,11 @@ __global__ __launch_bounds__(64) void rsi16_encode_reg_kernel(const uint16_t *__
     encode_group_reg<O, E>(T, msgs + (size_t)cw * T.k, out + (size_t)cw * E, L);
 }
 
-template <int E>
+template <int E, bool SIN = false>
 __global__ __launch_bounds__(64) void rsi16_repair_reg_kernel(uint16_t *__restrict__ msgs,
                                                                uint16_t *__restrict__ eccs, uint32_t count,
-                                                               uint8_t *__restrict__ status, Rsi16Dev T)
+                                                               uint8_t *__restrict__ status, Rsi16Dev T,
+                                                               const uint32_t *__restrict__ syn = nullptr)
 {
     extern __shared__ uint32_t s_mem[];
     constexpr uint32_t G = RegShape<E>::G, cpw = 64 / G;
@@ -1651,8 +1657,8 @@ __global__ __launch_bounds__(64) void rsi16_repair_reg_kernel(uint16_t *__restri
         return;
     LaneR<E> L;
     L.init(T, threadIdx.x);
-    repair_group_reg<E>(T, msgs + (size_t)cw * T.k, eccs + (size_t)cw * E, s_mem + grp * REG_WORDS * E, L,
-                        status + cw);
+    repair_group_reg<E, SIN>(T, msgs + (size_t)cw * T.k, eccs + (size_t)cw * E, s_mem + grp * REG_WORDS * E, L,
+                             status + cw, SIN ? syn + (size_t)cw * E : nullptr);
 }
 
 __global__ __launch_bounds__(64) void rsi16_repair_kernel(uint16_t *__restrict__ msgs, uint16_t *__restrict__ eccs,
@@ -1837,21 +1843,28 @@ typedef int v4i_t __attribute__((ext_vector_type(4)));
 // for the next step
 __device__ inline int32_t fold16(int32_t x) { return (x & 0xffff) - (x >> 16); }
 
-template <int CT>
-__global__ __launch_bounds__(256) void rsi16_encode_mfma_kernel(const uint16_t *__restrict__ msgs, uint32_t msg_bytes,
-                                                                uint32_t count, uint16_t *__restrict__ out,
-                                                                const uint4 *__restrict__ gpack,
-                                                                const uint32_t *__restrict__ kconst, uint32_t k,
-                                                                uint32_t E, uint32_t nq)
+//
+// SYN: the same contraction gives the syndromes of NTT::pntt (ntt.hpp:133-152), S_j = sum over the
+// n symbols of c_m w^(j rbo(m)) mod p, with the chunks of the message (msgs, k per codeword)
+// followed by those of the parity (eccs, E per codeword, ecc_bytes in the buffer) against the
+// matching rows of that matrix (nq_m message chunks); the canonical values go to syn (u32).
+template <int CT, bool SYN>
+__device__ inline void mfma_contract(const uint16_t *__restrict__ msgs, uint32_t msg_bytes, uint32_t count,
+                                     uint16_t *__restrict__ out, const uint4 *__restrict__ gpack,
+                                     const uint32_t *__restrict__ kconst, uint32_t k, uint32_t E, uint32_t nq,
+                                     const uint16_t *__restrict__ eccs, uint32_t ecc_bytes, uint32_t nq_m,
+                                     uint32_t *__restrict__ syn)
 {
     const uint32_t lane = threadIdx.x & 63, c = lane & 15, h = lane >> 4;
     const uint32_t wave = blockIdx.x * 4 + (threadIdx.x >> 6), waves = gridDim.x * 4;
     const __amdgpu_buffer_rsrc_t rsrc = make_rsrc(msgs, msg_bytes);
+    const __amdgpu_buffer_rsrc_t rsrc_e = make_rsrc(SYN ? eccs : msgs, SYN ? ecc_bytes : msg_bytes);
     const uint32_t T = E / 16;
     for (uint32_t grp = wave; grp * 16 < count; grp += waves) {
         const uint32_t cw = grp * 16 + c;
         const bool valid = cw < count;
         const uint32_t row = cw * k * 2;  // byte offset of the codeword's message (< 2^32: launches are split)
+        const uint32_t erow = cw * E * 2;
         for (uint32_t t0 = 0; t0 < T; t0 += CT) {
             v4i_t acc[CT][2][3];
 #pragma unroll
@@ -1864,9 +1877,15 @@ __global__ __launch_bounds__(256) void rsi16_encode_mfma_kernel(const uint16_t *
             for (uint32_t q = 0; q < nq; ++q) {
                 uint4 x0 = make_uint4(0, 0, 0, 0), x1 = x0;
                 if (valid) {  // 32-byte aligned reads, wholly inside or wholly past the buffer
-                    const uint32_t off = row + (64 * q + 16 * h) * 2;
-                    x0 = buf_load_u32x4(rsrc, off);
-                    x1 = buf_load_u32x4(rsrc, off + 16);
+                    if (!SYN || q < nq_m) {
+                        const uint32_t off = row + (64 * q + 16 * h) * 2;
+                        x0 = buf_load_u32x4(rsrc, off);
+                        x1 = buf_load_u32x4(rsrc, off + 16);
+                    } else {
+                        const uint32_t off = erow + (64 * (q - nq_m) + 16 * h) * 2;
+                        x0 = buf_load_u32x4(rsrc_e, off);
+                        x1 = buf_load_u32x4(rsrc_e, off + 16);
+                    }
                 }
                 const v4i_t lo = {(int)(__builtin_amdgcn_perm(x0.y, x0.x, 0x06040200u) ^ 0x80808080u),
                                   (int)(__builtin_amdgcn_perm(x0.w, x0.z, 0x06040200u) ^ 0x80808080u),
@@ -1902,14 +1921,41 @@ __global__ __launch_bounds__(256) void rsi16_encode_mfma_kernel(const uint16_t *
                     int32_t f = fold16(z[0] + 128 * z[1] + 16384 * z[2] + (int32_t)kconst[j]);
                     f = f < 0 ? f + (int32_t)P16 : f;
                     f = f >= (int32_t)P16 ? f - (int32_t)P16 : f;
-                    v[i] = (uint32_t)f & 0xffffu;
+                    v[i] = SYN ? (uint32_t)f : (uint32_t)f & 0xffffu;
                 }
-                if (valid)
-                    *reinterpret_cast<uint2 *>(out + (size_t)cw * E + 16 * t + 4 * h) =
-                        make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16));
+                if (valid) {
+                    if constexpr (SYN)
+                        *reinterpret_cast<uint4 *>(syn + (size_t)cw * E + 16 * t + 4 * h) =
+                            make_uint4(v[0], v[1], v[2], v[3]);
+                    else
+                        *reinterpret_cast<uint2 *>(out + (size_t)cw * E + 16 * t + 4 * h) =
+                            make_uint2(v[0] | (v[1] << 16), v[2] | (v[3] << 16));
+                }
             }
         }
     }
+}
+
+template <int CT>
+__global__ __launch_bounds__(256) void rsi16_encode_mfma_kernel(const uint16_t *__restrict__ msgs, uint32_t msg_bytes,
+                                                                uint32_t count, uint16_t *__restrict__ out,
+                                                                const uint4 *__restrict__ gpack,
+                                                                const uint32_t *__restrict__ kconst, uint32_t k,
+                                                                uint32_t E, uint32_t nq)
+{
+    mfma_contract<CT, false>(msgs, msg_bytes, count, out, gpack, kconst, k, E, nq, nullptr, 0, 0, nullptr);
+}
+
+// The syndromes of NTT::pntt (ntt.hpp:133-152) as the same contraction (mfma_contract, SYN).
+template <int CT>
+__global__ __launch_bounds__(256) void rsi16_synd_mfma_kernel(const uint16_t *__restrict__ msgs, uint32_t msg_bytes,
+                                                              const uint16_t *__restrict__ eccs, uint32_t ecc_bytes,
+                                                              uint32_t count, const uint4 *__restrict__ vpack,
+                                                              const uint32_t *__restrict__ vconst, uint32_t k,
+                                                              uint32_t E, uint32_t nq, uint32_t nq_m,
+                                                              uint32_t *__restrict__ syn)
+{
+    mfma_contract<CT, true>(msgs, msg_bytes, count, nullptr, vpack, vconst, k, E, nq, eccs, ecc_bytes, nq_m, syn);
 }
 
 // ---------------------------------------------------------------- interleaved repair, SIMD words
@@ -2714,6 +2760,13 @@ struct ffrs_rsi16 {
     uint32_t mfma_nq = 0;            // 64-symbol chunks of the message (0: no MFMA encode)
     bool mix_minus_one = false;      // a mix constant is 65536: the SIMD words are not the linear map
     int encode_mode = 0;             // 0 automatic, 1 butterflies, 2 MFMA wherever it applies
+    // the syndromes as the same contraction (rsi16_synd_mfma_kernel) ahead of the repair
+    uint4 *d_vpack = nullptr;        // limbs of w^(j rbo(m)), m < n, in the A-operand lane layout
+    uint32_t *d_vconst = nullptr;    // their offset constants
+    uint32_t synd_nq = 0, synd_nqm = 0;  // chunks in all, message chunks (0: not built)
+    uint32_t *d_syn = nullptr;       // syndromes of the last repair launch
+    size_t syn_size = 0;
+    int repair_synd = 0;             // 0 automatic, 1 in the repair kernel, 2 MFMA ahead of it
 };
 
 namespace {
@@ -2863,6 +2916,43 @@ int launch_repair_reg(ffrs_rsi16 *rs, uint16_t *d_msg, uint16_t *d_ecc, size_t c
 {
     constexpr uint32_t cpw = 64 / RegShape<E>::G;
     const size_t lds = (size_t)cpw * REG_WORDS * E * 4;
+    // the syndromes on the matrix cores first, where their tables exist (ecc 16, 32, 64)
+    const bool msyn = rs->synd_nq && rs->repair_synd != 1 && (((uintptr_t)d_msg | (uintptr_t)d_ecc) & 15) == 0;
+    if (msyn) {
+        const size_t step = std::max<size_t>(16, (((size_t)1 << 31) / (rs->k * 2)) & ~(size_t)15);
+        const size_t need = std::min(count, step) * E * 4;
+        if (rs->syn_size < need) {
+            if (rs->d_syn)
+                FFRS_HIP(hipFree(rs->d_syn));
+            rs->d_syn = nullptr;
+            rs->syn_size = 0;
+            hipError_t e = hipMalloc(&rs->d_syn, need);
+            if (e != hipSuccess)
+                return fail(FFRS_ERR_NOMEM, "syndrome workspace of %zu bytes: %s", need, hipGetErrorString(e));
+            rs->syn_size = need;
+        }
+        constexpr int CT = E >= 16 && E <= 64 ? E / 16 : 1;  // every tile in one pass (tables: ecc 16 .. 64)
+        auto *ks = rsi16_synd_mfma_kernel<CT>;
+        if (int r = allow_lds(rsi16_repair_reg_kernel<E, true>, lds))
+            return r;
+        for (size_t b = 0; b < count; b += step) {
+            const size_t c = std::min(step, count - b);
+            const size_t groups = (c + 15) / 16;
+            hipLaunchKernelGGL(ks, dim3(persistent_grid(rs->ctx, 8, (groups + 3) / 4)), dim3(256), 0, rs->ctx->stream,
+                               (const uint16_t *)(d_msg + b * rs->k), (uint32_t)(c * rs->k * 2),
+                               (const uint16_t *)(d_ecc + b * E), (uint32_t)(c * E * 2), (uint32_t)c,
+                               (const uint4 *)rs->d_vpack, (const uint32_t *)rs->d_vconst, rs->k, (uint32_t)E,
+                               rs->synd_nq, rs->synd_nqm, rs->d_syn);
+            FFRS_HIP(hipGetLastError());
+            mark_kernel(rs->ctx);
+            hipLaunchKernelGGL((rsi16_repair_reg_kernel<E, true>), dim3((unsigned)((c + cpw - 1) / cpw)), dim3(64), lds,
+                               rs->ctx->stream, d_msg + b * rs->k, d_ecc + b * E, (uint32_t)c, d_status + b, rs->dev,
+                               (const uint32_t *)rs->d_syn);
+            FFRS_HIP(hipGetLastError());
+            mark_kernel(rs->ctx);
+        }
+        return FFRS_OK;
+    }
     if (int r = allow_lds(rsi16_repair_reg_kernel<E>, lds))
         return r;
     hipLaunchKernelGGL(rsi16_repair_reg_kernel<E>, dim3((unsigned)((count + cpw - 1) / cpw)), dim3(64), lds,
@@ -2980,6 +3070,58 @@ int build_mfma_encode(ffrs_rsi16 *rs, const std::vector<uint32_t> &mix, const st
     if (e != hipSuccess)
         return fail(FFRS_ERR_DEVICE, "MFMA encode tables: %s", hipGetErrorString(e));
     rs->mfma_nq = nq;
+    return FFRS_OK;
+}
+
+// The syndrome matrix V[m][j] = w^(j rbo(m)) (NTT::pntt, ntt.hpp:133-152) for the MFMA
+// contraction: rows m < k in ceil(k / 64) message chunks, rows k + i (i < E) in the parity
+// chunks after them, packed and offset as build_mfma_encode does.
+int build_mfma_synd(ffrs_rsi16 *rs, const std::vector<uint32_t> &roots_ntt)
+{
+    const uint32_t k = rs->k, E = rs->E;
+    if (E % 16 || E > 64 || k > 16384)
+        return FFRS_OK;
+    const uint32_t nqm = (k + 63) / 64, nqe = (E + 63) / 64, nq = nqm + nqe, T = E / 16, kp = nq * 64;
+    std::vector<uint8_t> limb((size_t)3 * kp * E, 0);  // [b][row][j], rows in chunk order
+    std::vector<uint64_t> C((size_t)3 * E, 0);
+    for (uint32_t m = 0; m < k + E; ++m) {
+        const uint32_t r = m < k ? m : nqm * 64 + (m - k);  // row of the packed matrix
+        const uint32_t x = roots_ntt[rbo_bits(m, rs->nbits)];
+        uint32_t pw = 1;
+        for (uint32_t j = 0; j < E; ++j) {
+            const uint32_t g = pw, gb[3] = {g & 127u, (g >> 7) & 127u, g >> 14};
+            for (int b = 0; b < 3; ++b) {
+                limb[((size_t)b * kp + r) * E + j] = (uint8_t)gb[b];
+                C[(size_t)b * E + j] += gb[b];
+            }
+            pw = S::mul(pw, x);
+        }
+    }
+    std::vector<uint8_t> pack((size_t)3 * T * nq * 64 * 16);
+    for (uint32_t b = 0; b < 3; ++b)
+        for (uint32_t t = 0; t < T; ++t)
+            for (uint32_t q = 0; q < nq; ++q)
+                for (uint32_t l = 0; l < 64; ++l)
+                    for (uint32_t bt = 0; bt < 16; ++bt) {
+                        const uint32_t m = 64 * q + 16 * (l / 16) + bt, j = 16 * t + l % 16;
+                        pack[(((size_t)(b * T + t) * nq + q) * 64 + l) * 16 + bt] = limb[((size_t)b * kp + m) * E + j];
+                    }
+    std::vector<uint32_t> K(E);
+    for (uint32_t j = 0; j < E; ++j) {
+        const uint64_t sum = (C[j] + 128 * C[E + j] + 16384 * C[2 * (size_t)E + j]) % P16;
+        K[j] = (uint32_t)(sum * 128 % P16 * 257 % P16);
+    }
+    hipError_t e = hipMalloc(&rs->d_vpack, pack.size());
+    if (e == hipSuccess)
+        e = hipMalloc(&rs->d_vconst, K.size() * 4);
+    if (e == hipSuccess)
+        e = hipMemcpy(rs->d_vpack, pack.data(), pack.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess)
+        e = hipMemcpy(rs->d_vconst, K.data(), K.size() * 4, hipMemcpyHostToDevice);
+    if (e != hipSuccess)
+        return fail(FFRS_ERR_DEVICE, "MFMA syndrome tables: %s", hipGetErrorString(e));
+    rs->synd_nq = nq;
+    rs->synd_nqm = nqm;
     return FFRS_OK;
 }
 
@@ -3117,7 +3259,22 @@ int ffrs_rsi16_create(ffrs_ctx *ctx, size_t block_len, size_t ecc_len, uint32_t 
         ffrs_rsi16_destroy(rs);
         return r;
     }
+    if (int r = build_mfma_synd(rs, roots_ntt)) {
+        ffrs_rsi16_destroy(rs);
+        return r;
+    }
     *out = rs;
+    return FFRS_OK;
+}
+
+int ffrs_rsi16_set_repair_syndromes(ffrs_rsi16 *rs, int mode)
+{
+    if (!rs)
+        return fail(FFRS_ERR_INVALID, "codec is NULL");
+    if (mode < 0 || mode > 2)
+        return fail(FFRS_ERR_INVALID, "syndrome mode %d (0 automatic, 1 in the repair kernel, 2 MFMA where it applies)",
+                    mode);
+    rs->repair_synd = mode;
     return FFRS_OK;
 }
 
@@ -3145,6 +3302,12 @@ void ffrs_rsi16_destroy(ffrs_rsi16 *rs)
         (void)hipFree(rs->d_gpack);
     if (rs->d_kconst)
         (void)hipFree(rs->d_kconst);
+    if (rs->d_vpack)
+        (void)hipFree(rs->d_vpack);
+    if (rs->d_vconst)
+        (void)hipFree(rs->d_vconst);
+    if (rs->d_syn)
+        (void)hipFree(rs->d_syn);
     delete rs;
 }
 

@@ -169,3 +169,52 @@ def test_mfma_encode_matches_butterflies(n, ecc):
         dp.free()
         codec.close()
         ctx.close()
+
+
+@pytest.mark.parametrize("n,ecc", [(64, 16), (256, 16), (256, 32), (1024, 32), (512, 64), (4096, 64), (1024, 128)])
+def test_mfma_syndromes_repair_matches_in_kernel(n, ecc):
+    """The repair with its syndromes from the int8 MFMA contraction (ecc 16, 32, 64) against the
+    repair computing them in the kernel: codewords with 0 .. ecc errors (beyond the radius too),
+    clean ones and ones holding 0 / 65535 symbols; messages, parity and statuses identical.
+    (1024, 128) has no contraction tables: both modes run the in-kernel syndromes."""
+    from libffrs_amd import capi
+    k = n - ecc
+    count = 4000
+    msgs, counts, s = W.rsi16_plain(count, n, ecc, ecc, seed=60 + n + ecc)
+    msgs[:8] = 0
+    msgs[8:16] = 65535
+    print(f"mfma syndromes ({n},{ecc}) on {count} codewords", flush=True)
+    ctx = capi.Context(0)
+    codec = capi.RSi16Codec(ctx, n, ecc)
+    dm, dp, ds = ctx.alloc(count * k * 2), ctx.alloc(count * ecc * 2), ctx.alloc(count)
+    try:
+        dm.upload(msgs)
+        codec.encode_device(dm, count, dp, True)
+        ctx.sync()
+        par = np.empty((count, ecc), np.uint16)
+        dp.download(par)
+        cws = np.concatenate([msgs, par], 1)
+        counts[:16] = 0
+        W.rsi16_corrupt(cws, counts, s)
+        outs = []
+        for mode in (1, 2, 0):
+            codec.set_repair_syndromes(mode)
+            dm.upload(np.ascontiguousarray(cws[:, :k]))
+            dp.upload(np.ascontiguousarray(cws[:, k:]))
+            codec.repair_device(dm, dp, count, ds)
+            ctx.sync()
+            m, e, st = np.empty((count, k), np.uint16), np.empty((count, ecc), np.uint16), np.empty(count, np.uint8)
+            dm.download(m)
+            dp.download(e)
+            ds.download(st)
+            outs.append((m, e, st))
+        for o in outs[1:]:
+            assert (o[2] == outs[0][2]).all() and (o[0] == outs[0][0]).all() and (o[1] == outs[0][1]).all()
+        assert (outs[0][2] == 2).sum() > count // 4  # most codewords were repaired
+        codec.set_repair_syndromes(0)
+    finally:
+        dm.free()
+        dp.free()
+        ds.free()
+        codec.close()
+        ctx.close()
