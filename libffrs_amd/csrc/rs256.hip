@@ -1641,6 +1641,10 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t n = k + E;
     const uint32_t lo_off = 16 * j, hi_off = 16 * j + 0x10000u;
+    // with the BM in the kernel (EB > 0) L <= EB / 2: only the nibble words of coefficients
+    // 1 .. EB / 2 travel between lanes and only that many degrees are ranked
+    constexpr int LMX = EB > 0 ? EB / 2 : CHIEN_LMAX;
+    constexpr int NW = EB > 0 ? (EB >= 8 ? EB / 4 : 1) : 8;
     for (uint32_t base = (blockIdx.x * (WG / 64) + wave) * 64; base < count; base += gridDim.x * WG) {
         const uint32_t cw = base + lane;
         bool pend = cw < count && status[cw] == RS_PENDING;
@@ -1684,12 +1688,14 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
         record_nibbles<20>(r, lnib);  // Lambda_1..Lambda_16 (Lambda_0 = 1 at byte 20)
         r[5] &= 0xffffff00u;          // Omega_16 does not exist (byte 20 holds Lambda_0)
         record_nibbles<4>(r, onib);   // Omega_1..Omega_15
-        onib[7] |= ((r[1] & 0xffu) << 16) | (L << 24);  // Omega_0 and L in the unused bytes
+        // Omega_0 and L in the unused bytes of the last word that travels (those of Omega_(LMX),
+        // which does not exist)
+        onib[NW - 1] |= ((r[1] & 0xffu) << 16) | (L << 24);
         // rank of each pending codeword in ascending (L, lane) order; others after them
         const uint32_t npend = (uint32_t)__builtin_popcountll(pm);
         uint32_t rank = 0, acc = 0;
 #pragma unroll
-        for (uint32_t v = 1; v <= (uint32_t)CHIEN_LMAX; ++v) {
+        for (uint32_t v = 1; v <= (uint32_t)LMX; ++v) {
             const bool hit = pend && L == v;
             const uint64_t m = __ballot(hit);
             const uint32_t below = __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
@@ -1708,15 +1714,15 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
             uint32_t ln[8], on[8];
 #pragma unroll
             for (int m = 0; m < 8; ++m) {
-                ln[m] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)lnib[m]);
-                on[m] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)onib[m]);
+                ln[m] = m < NW ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)lnib[m]) : 0u;
+                on[m] = m < NW ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)onib[m]) : 0u;
             }
-            const uint32_t Lg = on[7] >> 24;
+            const uint32_t Lg = on[NW - 1] >> 24;
             const uint32_t last = (npend - 1 < 4 * ps + 3 ? npend - 1 : 4 * ps + 3) - 4 * ps;
             const uint32_t Lmax = (uint32_t)__builtin_amdgcn_readlane((int)Lg, (int)(16 * last));
             uint32_t ev[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};  // Lambda_0 = 1
             uint32_t od[4] = {0, 0, 0, 0};
-            chien16_eval<true, CHIEN_LMAX>(ct, ln, lo_off, hi_off, Lmax, ev, od);
+            chien16_eval<true, LMX>(ct, ln, lo_off, hi_off, Lmax, ev, od);
             uint32_t z[4], cnt = 0;
 #pragma unroll
             for (int d = 0; d < 4; ++d) {
@@ -1734,9 +1740,9 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
                 continue;
             // Omega(X^-1) at the lane's positions: Omega_0 + coefficients 1 .. Lmax - 1
             uint32_t w[4];
-            const uint32_t om0 = ((on[7] >> 16) & 0xffu) * 0x01010101u;
+            const uint32_t om0 = ((on[NW - 1] >> 16) & 0xffu) * 0x01010101u;
             w[0] = w[1] = w[2] = w[3] = om0;
-            chien16_eval<false, CHIEN_LMAX - 1>(ct, on, lo_off, hi_off, Lmax - 1, w, w);
+            chien16_eval<false, LMX - 1>(ct, on, lo_off, hi_off, Lmax - 1, w, w);
             // roots: bit 8q + d of rm <-> position 16 j + 4 d + q
             uint32_t rm = ok ? ((z[0] >> 7) | (z[1] >> 6) | (z[2] >> 5) | (z[3] >> 4)) : 0u;
             uint8_t *mrow = msgs + (size_t)cwo * k;
