@@ -266,6 +266,29 @@ def radius_fixture(L):
     return {"rsi16_edge_radius": dict(cws=cws[rows], counts=counts[rows], status=st, repaired=rep)}
 
 
+def radius1024_fixture(L):
+    """(e) RSi16(1024, 128) blocks with at most 60 additive errors (ecc/2 = 64, so still within
+    the radius where a parity symbol of 65536 was stored as 0) that the reference's repair
+    leaves unrepaired, or reports repaired with other contents than were encoded, found by the
+    reference itself over 160,000 blocks (all of the latter, 24 of the former), plus 8 ordinary ones; the reference decides every
+    expected output."""
+    n, ecc, k = 1024, 128, 896
+    ref = L.RSi16(n, ecc)
+    msgs, counts, s = W.rsi16_plain(160000, n, ecc, 60, seed=1024)
+    par = np.frombuffer(ref.encode(msgs.tobytes()), np.uint16).reshape(-1, ecc)
+    cws = np.concatenate([msgs, par], 1)
+    W.rsi16_corrupt(cws, counts, s)
+    st, rep = ref_repair_rsi16(ref, cws, k)
+    fail = np.nonzero(~np.isin(st, (0, 2)))[0]
+    wrong = np.nonzero(np.isin(st, (0, 2)) & (rep != np.concatenate([msgs, par], 1)).any(axis=1))[0]
+    rows = np.array(sorted(set(fail[:24].tolist()) | set(wrong.tolist()) | set(range(0, 8 * 9973, 9973))), np.int64)
+    print("radius 1024 rows:", len(fail), "unrepaired,", len(wrong), "repaired to other contents, of", len(st),
+          "; reference statuses", np.bincount(st[rows], minlength=5))
+    return {"rsi16_edge_radius_1024": dict(cws=cws[rows], counts=counts[rows], status=st[rows],
+                                           repaired_crc=C.row_crc(rep[rows]),
+                                           encoded_crc=C.row_crc(np.concatenate([msgs, par], 1)[rows]))}
+
+
 INTERLEAVE_CASES = ((64, 8, 3), (256, 32, 16), (256, 32, 5), (1024, 128, 2), (64, 16, 33))
 
 
@@ -690,6 +713,9 @@ if __name__ == "__main__":
             np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
     elif sys.argv[1:] == ["ntt"]:
         for key, v in ntt_fixture(load_reference()).items():
+            np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
+    elif sys.argv[1:] == ["radius1024"]:
+        for key, v in radius1024_fixture(load_reference()).items():
             np.savez_compressed(os.path.join(HERE, key + ".npz"), **v)
     elif sys.argv[1:] == ["radius"]:
         from libffrs_amd import workloads as W  # noqa: F811

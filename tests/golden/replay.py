@@ -168,6 +168,17 @@ def check_rsi16_edges(be):
     assert (np.concatenate([m, p], 1) == fx["repaired"]).all()
     assert (fx["counts"] <= ecc // 2).all() and (~np.isin(fx["status"], (0, 2))).sum() >= 3
 
+    # the same at RSi16(1024, 128): blocks with at most 60 errors that the reference leaves
+    # unrepaired, and one it reports repaired with other contents than were encoded
+    fx = fixture("rsi16_edge_radius_1024")
+    m = np.ascontiguousarray(fx["cws"][:, :896])
+    p = np.ascontiguousarray(fx["cws"][:, 896:])
+    st = be.rsi16_repair_batch(1024, 128, m, p)
+    assert (st == fx["status"]).all(), "status: " + _mismatch(st, fx["status"])
+    assert (C.row_crc(np.concatenate([m, p], 1)) == fx["repaired_crc"]).all()
+    assert (fx["counts"] <= 60).all() and (~np.isin(fx["status"], (0, 2))).sum() >= 10
+    assert (np.isin(fx["status"], (0, 2)) & (fx["repaired_crc"] != fx["encoded_crc"])).sum() >= 1
+
     fx = fixture("rsi16_edge_const")
     assert (be.rsi16_encode_batch(n, ecc, fx["msgs"], True) == fx["parity"]).all()
     assert (be.rsi16_encode_batch(n, ecc, fx["msgs"], False) == fx["parity_scalar"]).all()
