@@ -1545,10 +1545,11 @@ __device__ inline void xor_byte(uint8_t *p, uint32_t v)
 // per pass, two reads and four XORs per coefficient, with no value read across lanes. A
 // 64-codeword tile is taken in ascending L (the four codewords of a pass have nearly the same
 // degree and the pass stops at the largest). Where the zeros among positions [0, n) number
-// exactly L, Omega(X^-1) at the lane's positions comes from the same tables and each root's
-// magnitude follows: with Lambda'(X^-1) = X Lambda_odd(X^-1) (characteristic 2), the
-// reference's e = X div(Omega(X^-1), Lambda'(X^-1)) is div(Omega, Lambda_odd), or X Omega
-// where Lambda_odd = 0 (its div(a, 0) = a).
+// exactly L, the pass's roots (at most 64) are listed one per lane in LDS, and each lane
+// evaluates Omega at its root as a sum of log / exp reads (the logs of Omega's coefficients
+// taken once per codeword) and applies the magnitude: with Lambda'(X^-1) = X Lambda_odd(X^-1)
+// (characteristic 2), the reference's e = X div(Omega(X^-1), Lambda'(X^-1)) is
+// div(Omega, Lambda_odd), or X Omega where Lambda_odd = 0 (its div(a, 0) = a).
 // ======================================================================
 
 // Nibble bytes of coefficients 1..16 of a polynomial whose coefficient c sits at byte
@@ -1602,6 +1603,13 @@ __device__ inline void chien16_eval(const char *ct, const uint32_t (&nib)[8], ui
     }
 }
 
+// Orders this wave's LDS writes before its later reads of them by other lanes.
+__device__ inline void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+}
+
 // Sum over the sixteen lanes of this lane's row (DPP butterflies), in every lane of the row.
 __device__ inline uint32_t row16_sum(uint32_t x)
 {
@@ -1628,6 +1636,7 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
     __shared__ uint4 s_ct4[CHIEN_TABLE_BYTES / 16];
     __shared__ uint8_t s_exp2[EXP2_LEN];
     __shared__ uint16_t s_log2[256];
+    __shared__ uint32_t s_roots[WG];  // 64 root slots per wave
     for (uint32_t i = threadIdx.x; i < CHIEN_TABLE_BYTES / 16; i += WG)
         s_ct4[i] = reinterpret_cast<const uint4 *>(ctab)[i];
     for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += WG)
@@ -1641,6 +1650,7 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t n = k + E;
     const uint32_t lo_off = 16 * j, hi_off = 16 * j + 0x10000u;
+    uint32_t *s_rt = s_roots + 64 * wave;
     // with the BM in the kernel (EB > 0) L <= EB / 2: only the nibble words of coefficients
     // 1 .. EB / 2 travel between lanes and only that many degrees are ranked
     constexpr int LMX = EB > 0 ? EB / 2 : CHIEN_LMAX;
@@ -1684,13 +1694,23 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
             for (int q = 0; q < 4; ++q)
                 vmask[d] |= (16 * j + 4 * d + q < n) ? (0x80u << (8 * q)) : 0u;
         }
-        uint32_t lnib[8], onib[8];
+        uint32_t lnib[8];
         record_nibbles<20>(r, lnib);  // Lambda_1..Lambda_16 (Lambda_0 = 1 at byte 20)
-        r[5] &= 0xffffff00u;          // Omega_16 does not exist (byte 20 holds Lambda_0)
-        record_nibbles<4>(r, onib);   // Omega_1..Omega_15
-        // Omega_0 and L in the unused bytes of the last word that travels (those of Omega_(LMX),
-        // which does not exist)
-        onib[NW - 1] |= ((r[1] & 0xffu) << 16) | (L << 24);
+        // logs of Omega_0 .. Omega_(LMX-1) (ZLOG for zero; Omega_k = 0 for k >= L), two per
+        // word, for the magnitudes at the roots
+        constexpr int NWO = (LMX + 1) / 2;
+        uint32_t olog[NWO];
+#pragma unroll
+        for (int m = 0; m < NWO; ++m) {
+            uint32_t lw = 0;
+#pragma unroll
+            for (int hh = 0; hh < 2; ++hh) {
+                const int kk = 2 * m + hh;
+                const uint32_t b = kk < LMX ? (r[(4 + kk) >> 2] >> (8 * ((4 + kk) & 3))) & 0xffu : 0u;
+                lw |= (uint32_t)s_log2[b] << (16 * hh);
+            }
+            olog[m] = lw;
+        }
         // rank of each pending codeword in ascending (L, lane) order; others after them
         const uint32_t npend = (uint32_t)__builtin_popcountll(pm);
         uint32_t rank = 0, acc = 0;
@@ -1711,13 +1731,11 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
             const uint32_t rr = 4 * ps + grp;
             const bool active = rr < npend;
             const uint32_t o = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((rr & 63) * 4), (int)src) & 63u;
-            uint32_t ln[8], on[8];
+            uint32_t ln[8];
 #pragma unroll
-            for (int m = 0; m < 8; ++m) {
+            for (int m = 0; m < 8; ++m)
                 ln[m] = m < NW ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)lnib[m]) : 0u;
-                on[m] = m < NW ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)onib[m]) : 0u;
-            }
-            const uint32_t Lg = on[NW - 1] >> 24;
+            const uint32_t Lg = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)L);
             const uint32_t last = (npend - 1 < 4 * ps + 3 ? npend - 1 : 4 * ps + 3) - 4 * ps;
             const uint32_t Lmax = (uint32_t)__builtin_amdgcn_readlane((int)Lg, (int)(16 * last));
             uint32_t ev[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};  // Lambda_0 = 1
@@ -1733,40 +1751,69 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
             }
             cnt = row16_sum(cnt);
             const bool ok = active && cnt == Lg;
-            const uint32_t cwo = base + o;
             if (j == 0 && active)
-                status[cwo] = ok ? 1 : 0;
+                status[base + o] = ok ? 1 : 0;
             if (!__ballot(ok))
                 continue;
-            // Omega(X^-1) at the lane's positions: Omega_0 + coefficients 1 .. Lmax - 1
-            uint32_t w[4];
-            const uint32_t om0 = ((on[NW - 1] >> 16) & 0xffu) * 0x01010101u;
-            w[0] = w[1] = w[2] = w[3] = om0;
-            chien16_eval<false, LMX - 1>(ct, on, lo_off, hi_off, Lmax - 1, w, w);
-            // roots: bit 8q + d of rm <-> position 16 j + 4 d + q
+            // the pass's roots (at most 4 x 16) into this wave's slots, in lane order: bit
+            // 8q + d of rm <-> position 16 j + 4 d + q; a slot holds the codeword's lane, the
+            // position and Lambda_odd(X^-1) there
             uint32_t rm = ok ? ((z[0] >> 7) | (z[1] >> 6) | (z[2] >> 5) | (z[3] >> 4)) : 0u;
-            uint8_t *mrow = msgs + (size_t)cwo * k;
-            uint8_t *erow = eccs + (size_t)cwo * E;
+            const uint32_t nr = (uint32_t)__builtin_popcount(rm);
+            uint32_t slot = 0, total = 0;  // exclusive prefix sum of nr (<= 16) by bit planes
+#pragma unroll
+            for (int b = 0; b < 5; ++b) {
+                const uint64_t m = __ballot((nr >> b) & 1u);
+                slot += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
+                total += (uint32_t)__builtin_popcountll(m) << b;
+            }
             while (rm) {
                 const uint32_t t = (uint32_t)__builtin_ctz(rm);
                 rm &= rm - 1;
                 const uint32_t q = t >> 3, d = t & 7u;
-                const uint32_t p = 16 * j + 4 * d + q;
-                uint32_t wv = w[0], ov = od[0];
-                wv = d == 1 ? w[1] : wv;
+                uint32_t ov = od[0];
                 ov = d == 1 ? od[1] : ov;
-                wv = d == 2 ? w[2] : wv;
                 ov = d == 2 ? od[2] : ov;
-                wv = d == 3 ? w[3] : wv;
                 ov = d == 3 ? od[3] : ov;
-                const uint32_t num = (wv >> (8 * q)) & 0xffu, dod = (ov >> (8 * q)) & 0xffu;
+                s_rt[slot++] = o | ((16 * j + 4 * d + q) << 8) | (((ov >> (8 * q)) & 0xffu) << 16);
+            }
+            wave_lds_sync();
+            // one root per lane: Omega(X^-1) = sum_k Omega_k alpha^(-k p) from the logs held by
+            // the codeword's lane, then Forney's magnitude (as the reference's e = X Omega(X^-1) /
+            // Lambda'(X^-1), with Lambda' = X Lambda_odd in characteristic 2)
+            const uint32_t e = lane < total ? s_rt[lane] : 0u;
+            const uint32_t oc = e & 63u;
+            uint32_t ol[NWO];
+#pragma unroll
+            for (int m = 0; m < NWO; ++m)
+                ol[m] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(oc * 4), (int)olog[m]);
+            if (lane < total) {
+                const uint32_t p = (e >> 8) & 0xffu, dod = e >> 16;
+                // four terms at a time (their reads in flight together), up to the pass's
+                // largest degree; ek = -k p mod 255
+                uint32_t num = 0, ek = 0;
+#pragma unroll
+                for (int g4 = 0; g4 < (LMX + 3) / 4; ++g4) {
+                    if ((uint32_t)(4 * g4) >= Lmax)
+                        break;
+                    uint32_t x[4];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int kk = 4 * g4 + u;
+                        x[u] = kk < LMX ? s_exp2[((ol[kk >> 1] >> (16 * (kk & 1))) & 0xffffu) + ek] : 0u;
+                        ek = ek >= p ? ek - p : ek + 255 - p;
+                    }
+                    num ^= x[0] ^ x[1] ^ x[2] ^ x[3];
+                }
                 if (num) {
                     const uint32_t lnm = s_log2[num], ld = s_log2[dod];
                     const uint32_t mag = dod ? (uint32_t)s_exp2[lnm + 255 - ld] : (uint32_t)s_exp2[p + lnm];
                     const uint32_t at = n - 1 - p;
-                    xor_byte(at < k ? mrow + at : erow + (at - k), mag);
+                    const size_t cwo = base + oc;
+                    xor_byte(at < k ? msgs + cwo * k + at : eccs + cwo * E + (at - k), mag);
                 }
             }
+            wave_lds_sync();  // the next pass rewrites the slots
         }
     }
 }
