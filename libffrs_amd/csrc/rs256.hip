@@ -1211,7 +1211,7 @@ __global__ __launch_bounds__(256) void rs256_synds_kernel(const uint8_t *__restr
 //                        has degree < L once the LFSR of length L generates all E syndromes)
 //                        (ecc 2, 4, 8, 16, 32: the Chien kernel runs this BM itself per tile
 //                        and keeps the records in registers)
-//   rs256_chien16_kernel Chien search, sixteen lanes per codeword over nibble tables, then
+//   rs256_chien16_kernel Chien search, eight lanes per codeword over nibble tables, then
 //                        Forney at the roots.
 // ======================================================================
 
@@ -1537,15 +1537,16 @@ __device__ inline void xor_byte(uint8_t *p, uint32_t v)
 }
 
 // ======================================================================
-// Chien search and Forney, sixteen lanes per codeword (records of the BM kernels, L <= 16;
-// reed_solomon.hpp:516-538, 704-743). Lane j of a group holds positions 16j .. 16j+15 of its
-// codeword as the sixteen bytes of four dwords. For a coefficient Lambda_i (i = 1..16) with
-// nibbles v_0, v_1, one 16-byte read of row CT[i][h][v_h] per nibble gives
-// (v_h << 4h) alpha^(-ip) at the lane's sixteen positions, so a wave evaluates four codewords
-// per pass, two reads and four XORs per coefficient, with no value read across lanes. A
-// 64-codeword tile is taken in ascending L (the four codewords of a pass have nearly the same
-// degree and the pass stops at the largest). Where the zeros among positions [0, n) number
-// exactly L, the pass's roots (at most 64) are listed one per lane in LDS, and each lane
+// Chien search and Forney, G = 16 or 8 lanes per codeword (records of the BM kernels, L <= 16;
+// reed_solomon.hpp:516-538, 704-743). With G = 16, lane j of a group holds positions
+// 16j .. 16j+15 of its codeword as the sixteen bytes of four dwords; with G = 8, 32 positions
+// in eight dwords (16j .. 16j+15 and 128+16j .. 128+16j+15). For a coefficient Lambda_i
+// (i = 1..16) with nibbles v_0, v_1, a 16-byte read of row CT[i][h][v_h] per nibble and piece
+// gives (v_h << 4h) alpha^(-ip) at sixteen of the lane's positions, so a wave evaluates 64 / G
+// codewords per pass with no value read across lanes. A 64-codeword tile is taken in ascending
+// L (the codewords of a pass have nearly the same degree and the pass stops at the largest).
+// Where the zeros among positions [0, n) number exactly L, the pass's roots are listed one per
+// slot in LDS, and each lane
 // evaluates Omega at its root as a sum of log / exp reads (the logs of Omega's coefficients
 // taken once per codeword) and applies the magnitude: with Lambda'(X^-1) = X Lambda_odd(X^-1)
 // (characteristic 2), the reference's e = X div(Omega(X^-1), Lambda'(X^-1)) is
@@ -1571,11 +1572,14 @@ __device__ inline void record_nibbles(const uint32_t (&r)[12], uint32_t (&nib)[8
 }
 
 // acc ^= sum over coefficients i = 1..top (top <= IMAX, wave-uniform) of Lambda_i alpha^(-ip) at
-// the lane's sixteen positions, odd coefficients into acc_odd when SPLIT.
-template <bool SPLIT, int IMAX>
-__device__ inline void chien16_eval(const char *ct, const uint32_t (&nib)[8], uint32_t lo_off, uint32_t hi_off,
-                                    uint32_t top, uint32_t (&acc)[4], uint32_t (&acc_odd)[4])
+// the lane's positions, odd coefficients into acc_odd when SPLIT. D = 4: sixteen positions, one
+// 16-byte read per nibble at `off`; D = 8: thirty-two, a second read at off ^ 128 (the other
+// half of the 256-byte row).
+template <bool SPLIT, int IMAX, int D>
+__device__ inline void chien_eval(const char *ct, const uint32_t (&nib)[8], uint32_t lo_off, uint32_t hi_off,
+                                  uint32_t top, uint32_t (&acc)[D], uint32_t (&acc_odd)[D])
 {
+    static_assert(D == 4 || D == 8, "16 or 32 positions per lane");
 #pragma unroll
     for (int gq = 0; gq < 4; ++gq) {
         if (4 * gq + 1 > (int)top)
@@ -1586,21 +1590,51 @@ __device__ inline void chien16_eval(const char *ct, const uint32_t (&nib)[8], ui
             if (i > IMAX)
                 break;
             const uint32_t word = nib[2 * gq + (jj >> 1)];
-            uint4 t[2];
+            uint4 t[2][D / 4];
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int tbl = (i - 1) * 2 + h;  // 4 KB table index
                 const uint32_t sel = 0x0c020000u | ((4u + (uint32_t)(2 * (jj & 1) + h)) << 8);
                 const uint32_t a = __builtin_amdgcn_perm(word, tbl >= 16 ? hi_off : lo_off, sel);
-                t[h] = *reinterpret_cast<const uint4 *>(ct + a + (tbl & 15) * 4096);
+                t[h][0] = *reinterpret_cast<const uint4 *>(ct + a + (tbl & 15) * 4096);
+                if constexpr (D == 8)
+                    t[h][1] = *reinterpret_cast<const uint4 *>(ct + (a ^ 128u) + (tbl & 15) * 4096);
             }
-            uint32_t(&d)[4] = (SPLIT && (i & 1)) ? acc_odd : acc;
-            d[0] = xor3(d[0], t[0].x, t[1].x);
-            d[1] = xor3(d[1], t[0].y, t[1].y);
-            d[2] = xor3(d[2], t[0].z, t[1].z);
-            d[3] = xor3(d[3], t[0].w, t[1].w);
+            uint32_t(&d)[D] = (SPLIT && (i & 1)) ? acc_odd : acc;
+#pragma unroll
+            for (int r = 0; r < D / 4; ++r) {
+                d[4 * r + 0] = xor3(d[4 * r + 0], t[0][r].x, t[1][r].x);
+                d[4 * r + 1] = xor3(d[4 * r + 1], t[0][r].y, t[1][r].y);
+                d[4 * r + 2] = xor3(d[4 * r + 2], t[0][r].z, t[1][r].z);
+                d[4 * r + 3] = xor3(d[4 * r + 3], t[0][r].w, t[1][r].w);
+            }
         }
     }
+}
+
+// Position of byte q of position dword d of lane j in codeword group grp: G = 16 lanes hold 16
+// consecutive positions each; G = 8 lanes hold two runs of 16, one in each half of the
+// codeword, the half of dwords 0-3 alternating with the group so that one read instruction of
+// the eight groups covers all 64 LDS banks.
+template <int G>
+__device__ inline uint32_t chien_pos(uint32_t j, uint32_t grp, int d, int q)
+{
+    if constexpr (G == 16)
+        return 16 * j + 4 * (uint32_t)d + (uint32_t)q;
+    else
+        return 128 * (((uint32_t)d >> 2) ^ (grp & 1u)) + 16 * j + 4 * ((uint32_t)d & 3u) + (uint32_t)q;
+}
+
+// Sum over the G lanes of this lane's codeword group (DPP butterflies), in every lane of it.
+template <int G>
+__device__ inline uint32_t group_sum(uint32_t x)
+{
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);  // row_half_mirror
+    if constexpr (G == 16)
+        x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false);  // row_mirror
+    return x;
 }
 
 // Orders this wave's LDS writes before its later reads of them by other lanes.
@@ -1610,20 +1644,13 @@ __device__ inline void wave_lds_sync()
     __builtin_amdgcn_wave_barrier();
 }
 
-// Sum over the sixteen lanes of this lane's row (DPP butterflies), in every lane of the row.
-__device__ inline uint32_t row16_sum(uint32_t x)
-{
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0xB1, 0xF, 0xF, false);   // quad_perm [1,0,3,2]
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x4E, 0xF, 0xF, false);   // quad_perm [2,3,0,1]
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x141, 0xF, 0xF, false);  // row_half_mirror
-    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, 0x140, 0xF, 0xF, false);  // row_mirror
-    return x;
-}
 
 // EB > 0: the BM records come from bm_fixed_one<EB> run per tile, one pending codeword per lane,
 // on the syndromes `synd` (its LDS lookups overlap the Chien loops of other waves); EB == 0:
-// from `rec`.
-template <int WG, int EB>
+// from `rec`. G lanes per codeword in the Chien passes: 16 (sixteen positions each, four
+// codewords per pass) or 8 (thirty-two each, eight per pass: half the per-pass work per
+// codeword, for small ecc where that work dominates).
+template <int WG, int EB, int G = 16>
 __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__ msgs, uint8_t *__restrict__ eccs,
                                                             uint32_t count, uint32_t k, uint32_t E,
                                                             uint8_t *__restrict__ status,
@@ -1636,7 +1663,14 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
     __shared__ uint4 s_ct4[CHIEN_TABLE_BYTES / 16];
     __shared__ uint8_t s_exp2[EXP2_LEN];
     __shared__ uint16_t s_log2[256];
-    __shared__ uint32_t s_roots[WG];  // 64 root slots per wave
+    static_assert(G == 16 || G == 8, "16 or 8 lanes per codeword");
+    // with the BM in the kernel (EB > 0) L <= EB / 2: only the nibble words of coefficients
+    // 1 .. EB / 2 travel between lanes and only that many degrees are ranked
+    constexpr int LMX = EB > 0 ? EB / 2 : CHIEN_LMAX;
+    constexpr int NW = EB > 0 ? (EB >= 8 ? EB / 4 : 1) : 8;
+    constexpr int D = 64 / G;               // position dwords per lane, and codewords per pass
+    constexpr int RS = D * LMX > 64 ? D * LMX : 64;  // root slots per wave (a pass has <= D LMX roots)
+    __shared__ uint32_t s_roots[WG / 64 * RS];
     for (uint32_t i = threadIdx.x; i < CHIEN_TABLE_BYTES / 16; i += WG)
         s_ct4[i] = reinterpret_cast<const uint4 *>(ctab)[i];
     for (uint32_t i = threadIdx.x; i < (uint32_t)EXP2_LEN; i += WG)
@@ -1646,15 +1680,12 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
     __syncthreads();
     const LogExp1K tb{s_exp2, s_log2};
     const char *ct = reinterpret_cast<const char *>(s_ct4);
-    const uint32_t lane = threadIdx.x & 63, j = lane & 15, grp = lane >> 4;
+    const uint32_t lane = threadIdx.x & 63, j = lane % G, grp = lane / G;
     const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const uint32_t n = k + E;
-    const uint32_t lo_off = 16 * j, hi_off = 16 * j + 0x10000u;
-    uint32_t *s_rt = s_roots + 64 * wave;
-    // with the BM in the kernel (EB > 0) L <= EB / 2: only the nibble words of coefficients
-    // 1 .. EB / 2 travel between lanes and only that many degrees are ranked
-    constexpr int LMX = EB > 0 ? EB / 2 : CHIEN_LMAX;
-    constexpr int NW = EB > 0 ? (EB >= 8 ? EB / 4 : 1) : 8;
+    // byte offset of the lane's first 16-byte piece within a table row
+    const uint32_t lo_off = 16 * j + (G == 8 ? 128 * (grp & 1u) : 0u), hi_off = lo_off + 0x10000u;
+    uint32_t *s_rt = s_roots + RS * wave;
     for (uint32_t base = (blockIdx.x * (WG / 64) + wave) * 64; base < count; base += gridDim.x * WG) {
         const uint32_t cw = base + lane;
         bool pend = cw < count && status[cw] == RS_PENDING;
@@ -1684,15 +1715,15 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
                 r[w] = 0;
         }
         const uint32_t L = r[0] & 0xffu;
-        // bit 7 of byte q of dword d: position 16 j + 4 d + q lies in the codeword (set up after
-        // BM, whose registers it would otherwise add to)
-        uint32_t vmask[4];
+        // bit 7 of byte q of dword d: position chien_pos(d, q) lies in the codeword (set up
+        // after BM, whose registers it would otherwise add to)
+        uint32_t vmask[D];
 #pragma unroll
-        for (int d = 0; d < 4; ++d) {
+        for (int d = 0; d < D; ++d) {
             vmask[d] = 0;
 #pragma unroll
             for (int q = 0; q < 4; ++q)
-                vmask[d] |= (16 * j + 4 * d + q < n) ? (0x80u << (8 * q)) : 0u;
+                vmask[d] |= (chien_pos<G>(j, grp, d, q) < n) ? (0x80u << (8 * q)) : 0u;
         }
         uint32_t lnib[8];
         record_nibbles<20>(r, lnib);  // Lambda_1..Lambda_16 (Lambda_0 = 1 at byte 20)
@@ -1726,9 +1757,9 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
             rank = npend + __builtin_amdgcn_mbcnt_hi((uint32_t)(~pm >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)~pm, 0u));
         // lane r receives the lane of the codeword ranked r
         const uint32_t src = (uint32_t)__builtin_amdgcn_ds_permute((int)(rank * 4), (int)lane);
-        const uint32_t npass = (npend + 3) / 4;
+        const uint32_t npass = (npend + D - 1) / D;
         for (uint32_t ps = 0; ps < npass; ++ps) {
-            const uint32_t rr = 4 * ps + grp;
+            const uint32_t rr = D * ps + grp;
             const bool active = rr < npend;
             const uint32_t o = (uint32_t)__builtin_amdgcn_ds_bpermute((int)((rr & 63) * 4), (int)src) & 63u;
             uint32_t ln[8];
@@ -1736,33 +1767,41 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
             for (int m = 0; m < 8; ++m)
                 ln[m] = m < NW ? (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)lnib[m]) : 0u;
             const uint32_t Lg = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(o * 4), (int)L);
-            const uint32_t last = (npend - 1 < 4 * ps + 3 ? npend - 1 : 4 * ps + 3) - 4 * ps;
-            const uint32_t Lmax = (uint32_t)__builtin_amdgcn_readlane((int)Lg, (int)(16 * last));
-            uint32_t ev[4] = {0x01010101u, 0x01010101u, 0x01010101u, 0x01010101u};  // Lambda_0 = 1
-            uint32_t od[4] = {0, 0, 0, 0};
-            chien16_eval<true, LMX>(ct, ln, lo_off, hi_off, Lmax, ev, od);
-            uint32_t z[4], cnt = 0;
+            const uint32_t last = (npend - 1 < D * ps + D - 1 ? npend - 1 : D * ps + D - 1) - D * ps;
+            const uint32_t Lmax = (uint32_t)__builtin_amdgcn_readlane((int)Lg, (int)(G * last));
+            uint32_t ev[D], od[D];
 #pragma unroll
-            for (int d = 0; d < 4; ++d) {
+            for (int d = 0; d < D; ++d) {
+                ev[d] = 0x01010101u;  // Lambda_0 = 1
+                od[d] = 0;
+            }
+            chien_eval<true, LMX, D>(ct, ln, lo_off, hi_off, Lmax, ev, od);
+            uint32_t z[D], cnt = 0;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
                 const uint32_t v = ev[d] ^ od[d];
                 const uint32_t t = (v & 0x7f7f7f7fu) + 0x7f7f7f7fu;
                 z[d] = __builtin_amdgcn_bitop3_b32(t, v, vmask[d], 0x02);  // ~t & ~v & mask: zero bytes
                 cnt += (uint32_t)__builtin_popcount(z[d]);
             }
-            cnt = row16_sum(cnt);
+            cnt = group_sum<G>(cnt);
             const bool ok = active && cnt == Lg;
             if (j == 0 && active)
                 status[base + o] = ok ? 1 : 0;
             if (!__ballot(ok))
                 continue;
-            // the pass's roots (at most 4 x 16) into this wave's slots, in lane order: bit
-            // 8q + d of rm <-> position 16 j + 4 d + q; a slot holds the codeword's lane, the
+            // the pass's roots (at most D LMX) into this wave's slots, in lane order: bit 8q + d
+            // of rm <-> position chien_pos(d, q); a slot holds the codeword's lane, the
             // position and Lambda_odd(X^-1) there
-            uint32_t rm = ok ? ((z[0] >> 7) | (z[1] >> 6) | (z[2] >> 5) | (z[3] >> 4)) : 0u;
-            const uint32_t nr = (uint32_t)__builtin_popcount(rm);
-            uint32_t slot = 0, total = 0;  // exclusive prefix sum of nr (<= 16) by bit planes
+            uint32_t rm = 0;
 #pragma unroll
-            for (int b = 0; b < 5; ++b) {
+            for (int d = 0; d < D; ++d)
+                rm |= z[d] >> (7 - d);
+            rm = ok ? rm : 0u;
+            const uint32_t nr = (uint32_t)__builtin_popcount(rm);
+            uint32_t slot = 0, total = 0;  // exclusive prefix sum of nr (<= 4 D) by bit planes
+#pragma unroll
+            for (int b = 0; b < (D == 8 ? 6 : 5); ++b) {
                 const uint64_t m = __ballot((nr >> b) & 1u);
                 slot += __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u)) << b;
                 total += (uint32_t)__builtin_popcountll(m) << b;
@@ -1771,46 +1810,50 @@ __global__ __launch_bounds__(WG) void rs256_chien16_kernel(uint8_t *__restrict__
                 const uint32_t t = (uint32_t)__builtin_ctz(rm);
                 rm &= rm - 1;
                 const uint32_t q = t >> 3, d = t & 7u;
-                uint32_t ov = od[0];
-                ov = d == 1 ? od[1] : ov;
-                ov = d == 2 ? od[2] : ov;
-                ov = d == 3 ? od[3] : ov;
-                s_rt[slot++] = o | ((16 * j + 4 * d + q) << 8) | (((ov >> (8 * q)) & 0xffu) << 16);
+                uint32_t ov = od[0], p = chien_pos<G>(j, grp, 0, 0) + q;
+#pragma unroll
+                for (int dd = 1; dd < D; ++dd) {
+                    ov = d == (uint32_t)dd ? od[dd] : ov;
+                    p = d == (uint32_t)dd ? chien_pos<G>(j, grp, dd, 0) + q : p;
+                }
+                s_rt[slot++] = o | (p << 8) | (((ov >> (8 * q)) & 0xffu) << 16);
             }
             wave_lds_sync();
             // one root per lane: Omega(X^-1) = sum_k Omega_k alpha^(-k p) from the logs held by
             // the codeword's lane, then Forney's magnitude (as the reference's e = X Omega(X^-1) /
             // Lambda'(X^-1), with Lambda' = X Lambda_odd in characteristic 2)
-            const uint32_t e = lane < total ? s_rt[lane] : 0u;
-            const uint32_t oc = e & 63u;
-            uint32_t ol[NWO];
+            for (uint32_t r0 = 0; r0 < total; r0 += 64) {
+                const uint32_t e = r0 + lane < total ? s_rt[r0 + lane] : 0u;
+                const uint32_t oc = e & 63u;
+                uint32_t ol[NWO];
 #pragma unroll
-            for (int m = 0; m < NWO; ++m)
-                ol[m] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(oc * 4), (int)olog[m]);
-            if (lane < total) {
-                const uint32_t p = (e >> 8) & 0xffu, dod = e >> 16;
-                // four terms at a time (their reads in flight together), up to the pass's
-                // largest degree; ek = -k p mod 255
-                uint32_t num = 0, ek = 0;
+                for (int m = 0; m < NWO; ++m)
+                    ol[m] = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(oc * 4), (int)olog[m]);
+                if (r0 + lane < total) {
+                    const uint32_t p = (e >> 8) & 0xffu, dod = e >> 16;
+                    // four terms at a time (their reads in flight together), up to the pass's
+                    // largest degree; ek = -k p mod 255
+                    uint32_t num = 0, ek = 0;
 #pragma unroll
-                for (int g4 = 0; g4 < (LMX + 3) / 4; ++g4) {
-                    if ((uint32_t)(4 * g4) >= Lmax)
-                        break;
-                    uint32_t x[4];
+                    for (int g4 = 0; g4 < (LMX + 3) / 4; ++g4) {
+                        if ((uint32_t)(4 * g4) >= Lmax)
+                            break;
+                        uint32_t x[4];
 #pragma unroll
-                    for (int u = 0; u < 4; ++u) {
-                        const int kk = 4 * g4 + u;
-                        x[u] = kk < LMX ? s_exp2[((ol[kk >> 1] >> (16 * (kk & 1))) & 0xffffu) + ek] : 0u;
-                        ek = ek >= p ? ek - p : ek + 255 - p;
+                        for (int u = 0; u < 4; ++u) {
+                            const int kk = 4 * g4 + u;
+                            x[u] = kk < LMX ? s_exp2[((ol[kk >> 1] >> (16 * (kk & 1))) & 0xffffu) + ek] : 0u;
+                            ek = ek >= p ? ek - p : ek + 255 - p;
+                        }
+                        num ^= x[0] ^ x[1] ^ x[2] ^ x[3];
                     }
-                    num ^= x[0] ^ x[1] ^ x[2] ^ x[3];
-                }
-                if (num) {
-                    const uint32_t lnm = s_log2[num], ld = s_log2[dod];
-                    const uint32_t mag = dod ? (uint32_t)s_exp2[lnm + 255 - ld] : (uint32_t)s_exp2[p + lnm];
-                    const uint32_t at = n - 1 - p;
-                    const size_t cwo = base + oc;
-                    xor_byte(at < k ? msgs + cwo * k + at : eccs + cwo * E + (at - k), mag);
+                    if (num) {
+                        const uint32_t lnm = s_log2[num], ld = s_log2[dod];
+                        const uint32_t mag = dod ? (uint32_t)s_exp2[lnm + 255 - ld] : (uint32_t)s_exp2[p + lnm];
+                        const uint32_t at = n - 1 - p;
+                        const size_t cwo = base + oc;
+                        xor_byte(at < k ? msgs + cwo * k + at : eccs + cwo * E + (at - k), mag);
+                    }
                 }
             }
             wave_lds_sync();  // the next pass rewrites the slots
@@ -2310,8 +2353,10 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     FFRS_HIP(hipGetLastError());
     mark_kernel(ctx);
 
-    // ecc 2, 4, 8, 16, 32: BM inside the sixteen-lane Chien kernel, no records (512 threads
-    // for ecc 32, whose BM would spill registers at 768)
+    // ecc 2, 4, 8, 16, 32: BM inside the Chien kernel, no records (512 threads for ecc 32,
+    // whose BM would spill registers at 768); eight lanes per codeword in its passes, faster
+    // than sixteen at every ecc measured (1 M codewords of ecc 32: 402 vs 416 us; 2 M of ecc
+    // 16: 317 vs 348 us; 4 M of ecc 8: 384 vs 412 us)
     auto fused = [&](auto kern, int wg) -> int {
         int g = persistent_grid(ctx, 1, (count + wg - 1) / wg);
         hipLaunchKernelGGL(kern, dim3(g), dim3(wg), 0, ctx->stream, d_msg, d_ecc, (uint32_t)count, k, E, d_status,
@@ -2321,11 +2366,11 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
         return FFRS_OK;
     };
     switch (E) {
-    case 2: return fused(rs256_chien16_kernel<1024, 2>, 1024);
-    case 4: return fused(rs256_chien16_kernel<1024, 4>, 1024);
-    case 8: return fused(rs256_chien16_kernel<1024, 8>, 1024);
-    case 16: return fused(rs256_chien16_kernel<1024, 16>, 1024);
-    case 32: return fused(rs256_chien16_kernel<512, 32>, 512);
+    case 2: return fused(rs256_chien16_kernel<1024, 2, 8>, 1024);
+    case 4: return fused(rs256_chien16_kernel<1024, 4, 8>, 1024);
+    case 8: return fused(rs256_chien16_kernel<1024, 8, 8>, 1024);
+    case 16: return fused(rs256_chien16_kernel<1024, 16, 8>, 1024);
+    case 32: return fused(rs256_chien16_kernel<512, 32, 8>, 512);
     default: break;
     }
     // other ecc: Berlekamp-Massey from the log tables, one lane per codeword, records in the
@@ -2341,7 +2386,7 @@ int launch_repair_pipeline(ffrs_rs256 *rs, uint8_t *d_msg, uint8_t *d_ecc, size_
     // Chien + Forney: the 128 KB table fills the CU's LDS, one workgroup of sixteen waves per CU
     constexpr int WGC = 1024;
     int gc = persistent_grid(ctx, 1, (count + WGC - 1) / WGC);
-    hipLaunchKernelGGL((rs256_chien16_kernel<WGC, 0>), dim3(gc), dim3(WGC), 0, ctx->stream, d_msg, d_ecc,
+    hipLaunchKernelGGL((rs256_chien16_kernel<WGC, 0, 8>), dim3(gc), dim3(WGC), 0, ctx->stream, d_msg, d_ecc,
                        (uint32_t)count, k, E, d_status, (const uint32_t *)d_rec, (const uint32_t *)d_synd, rs->d_ctab,
                        tab);
     FFRS_HIP(hipGetLastError());
