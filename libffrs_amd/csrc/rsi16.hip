@@ -721,7 +721,7 @@ __device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1_out, uint32_
                 const uint32_t a0 = T.inv_tab[r1h];
                 const uint32_t aa = S::mul(a0, a0);
                 const uint32_t c0 = S::sub(0, S::mul(aa, r1[(g_len - 2) & mask]));
-                const uint32_t c1 = S::sub(S::mul(2, a0), S::mul(aa, r1h));
+                const uint32_t c1 = a0;  // 2 a0 - a0^2 r1h, with a0 r1h = 1 exactly
                 // quotient coefficients t[idx] of t = (r2 - f0) * (c0 + c1 x) mod x^E - 1
                 const uint32_t right_n = E - g_len < 2 ? E - g_len : 2u;
                 const uint32_t idx0 = right_n >= 1 ? g_len : 0u;
@@ -1347,7 +1347,7 @@ __device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, 
                 const uint32_t aa = F::mul(a0, a0);
                 const uint32_t r1s = pr_get<E>(r1, (g_len - 2) & mask, L);
                 const uint32_t c0 = F::sub(0, F::mul(aa, r1s));
-                const uint32_t c1 = F::sub(F::mul(2, a0), F::mul(aa, r1h));
+                const uint32_t c1 = a0;  // 2 a0 - a0^2 r1h, with a0 r1h = 1 exactly
                 // quotient coefficients t[idx] of t = (r2 - f0) * (c0 + c1 x) mod x^E - 1
                 const uint32_t right_n = (uint32_t)E - g_len < 2 ? (uint32_t)E - g_len : 2u;
                 const uint32_t idx0 = right_n >= 1 ? g_len : 0u;
