@@ -718,10 +718,10 @@ __device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1_out, uint32_
             const uint32_t g_len = r1_len, f_len = r2_len;
             if (f_len == g_len + 1) {  // q_len == 2 (g_len >= E / 2 >= 1 here)
                 q_len = 2;
+                // the reference's quotient factors c0 = -a0^2 r1s and c1 = 2 a0 - a0^2 r1h = a0
+                // (a0 r1h = 1 exactly), so q = c0 v + c1 v' = a0 (v' - w v) with w = a0 r1s
                 const uint32_t a0 = T.inv_tab[r1h];
-                const uint32_t aa = S::mul(a0, a0);
-                const uint32_t c0 = S::sub(0, S::mul(aa, r1[(g_len - 2) & mask]));
-                const uint32_t c1 = a0;  // 2 a0 - a0^2 r1h, with a0 r1h = 1 exactly
+                const uint32_t w = S::mul(a0, r1[(g_len - 2) & mask]);
                 // quotient coefficients t[idx] of t = (r2 - f0) * (c0 + c1 x) mod x^E - 1
                 const uint32_t right_n = E - g_len < 2 ? E - g_len : 2u;
                 const uint32_t idx0 = right_n >= 1 ? g_len : 0u;
@@ -732,8 +732,8 @@ __device__ inline uint32_t sugiyama(const Rsi16Dev &T, uint32_t *a1_out, uint32_
                     const uint32_t x = r2[at[u]];
                     v[u] = at[u] == 0 ? S::sub(x, r2l) : x;
                 }
-                const uint32_t q0 = S::add(S::mul(c0, v[0]), S::mul(c1, v[1]));
-                const uint32_t q1 = S::add(S::mul(c0, v[2]), S::mul(c1, v[3]));
+                const uint32_t q0 = S::mul(a0, S::sub(v[1], S::mul(w, v[0])));
+                const uint32_t q1 = S::mul(a0, S::sub(v[3], S::mul(w, v[2])));
                 wave_sync();
                 // a2 - q a1 and r2 - q r1 in place of a2 and r2
                 for (uint32_t i = G.gl; i < E; i += G.g) {
@@ -1343,11 +1343,11 @@ __device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, 
             PR<E> t, rn;
             if (f_len == g_len + 1) {  // q_len == 2 (g_len >= E / 2 >= 1 here)
                 q_len = 2;
+                // the reference's quotient factors c0 = -a0^2 r1s and c1 = 2 a0 - a0^2 r1h = a0
+                // (a0 r1h = 1 exactly), so q = c0 v + c1 v' = a0 (v' - w v) with w = a0 r1s
                 const uint32_t a0 = T.inv_tab[r1h];
-                const uint32_t aa = F::mul(a0, a0);
                 const uint32_t r1s = pr_get<E>(r1, (g_len - 2) & mask, L);
-                const uint32_t c0 = F::sub(0, F::mul(aa, r1s));
-                const uint32_t c1 = a0;  // 2 a0 - a0^2 r1h, with a0 r1h = 1 exactly
+                const uint32_t w = F::mul(a0, r1s);
                 // quotient coefficients t[idx] of t = (r2 - f0) * (c0 + c1 x) mod x^E - 1
                 const uint32_t right_n = (uint32_t)E - g_len < 2 ? (uint32_t)E - g_len : 2u;
                 const uint32_t idx0 = right_n >= 1 ? g_len : 0u;
@@ -1371,8 +1371,8 @@ __device__ inline uint32_t pr_sugiyama(const Rsi16Dev &T, PR<E> &a1, PR<E> &r1, 
                 h1 = r1h;
                 h2 = r1s;
                 carried = true;
-                const uint32_t q0 = F::add(F::mul(c0, v[0]), F::mul(c1, v[1]));
-                const uint32_t q1 = F::add(F::mul(c0, v[2]), F::mul(c1, v[3]));
+                const uint32_t q0 = F::mul(a0, F::sub(v[1], F::mul(w, v[0])));
+                const uint32_t q1 = F::mul(a0, F::sub(v[3], F::mul(w, v[2])));
                 const PR<E> pa = pr_prev<E>(a1, L), pr = pr_prev<E>(r1, L);
                 const uint32_t n0 = P16 - q0, n1 = P16 - q1;  // a2 - q0 a1 - q1 pa = a2 + n0 a1 + n1 pa
 #pragma unroll
